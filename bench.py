@@ -1,0 +1,202 @@
+"""Benchmark: CG solves/s to a 1e-12 FP64 residual on MM^dagger, mixed
+precision (BF16x3 inner CG + FP64 defect correction) against the plain FP64
+GPU path, on the 12x12 periodic honeycomb (Nx = 288), Nt = 64, beta = 8,
+U = 3: 16 seeded phi fields x 8 Gaussian right-hand sides = 128 solves per
+step, all resident in HBM before the timed region.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--inner bf16x3|f32]
+                    [--no-cpu] [--no-fp64]
+
+Prints one JSON line (see the keys below).  A step is one batched solve of
+all 128 systems to |b - MM^dag x| / |b| <= 1e-12 per right-hand side.
+"""
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "CG solves/s to 1e-12 resid, Nx=288 Nt=64: mixed vs FP64; p99 request latency"
+L, NT, BETA, U = 12, 64, 8.0, 3.0
+NFIELD, NRHS = 16, 8
+TOL = 1e-12
+PEAK = {"hbm": 8000.0, "f64": 78.6, "f32": 157.3, "bf16": 2500.0}   # GB/s, TFLOP/s
+
+
+def fields():
+    """Seeded inputs from the library's Philox generator: phi_f ~ N(0, sqrt(U~))
+    with seed 1000+f, b_{f,k} complex Gaussian with seed 2000+16f+k."""
+    import nsl_amd as nsl
+    Nx = 2 * L * L
+    Ut = BETA / NT * U
+    phi = np.stack([nsl.real_field(1000 + f, (NT, Nx), math.sqrt(Ut)) for f in range(NFIELD)])
+    b = np.stack([nsl.complex_normal(2000 + 16 * f + k, (NT, Nx))
+                  for f in range(NFIELD) for k in range(NRHS)])
+    return phi, b
+
+
+def kernel_bytes_flops(name, nb, Nx, Nt):
+    """Algorithmic bytes / flops of one hop launch over nb right-hand sides
+    (E and the phase table are < 2% and excluded)."""
+    prec = name.split("_")[2]
+    mode = name.split("_")[3]
+    w = 16 if prec == "f64" else 8
+    vec = {"plain": 2, "cga": 4, "cgb": 6, "cgi": 4}[mode]
+    bytes_ = vec * Nt * Nx * w * nb
+    flops = 4.0 * Nx * Nx * Nt * nb * (3 if prec == "bf16x3" else 1)
+    return bytes_, flops, prec
+
+
+def roofline_from_prof(text, nb, Nx, Nt, traffic=None):
+    rows = []
+    for line in text.strip().splitlines():
+        name, n, ms = line.split()
+        rows.append((name, int(n), float(ms)))
+    if not rows:
+        return None, []
+    name, n, ms = max(rows, key=lambda r: r[2])
+    avg_s = ms / n / 1e3
+    by, fl, prec = kernel_bytes_flops(name, nb, Nx, Nt)
+    if prec == "bf16x3":
+        bound, achieved, peak, unit = "hbm", by / avg_s / 1e9, PEAK["hbm"], "GB/s"
+    else:
+        bound, achieved, peak, unit = "mfma", fl / avg_s / 1e12, PEAK[prec], "TFLOP/s"
+    rf = {"bound": bound, "achieved": round(achieved, 2), "peak": peak, "unit": unit,
+          "frac": round(achieved / peak, 4), "traffic": traffic, "kernel": name,
+          "avg_us": round(avg_s * 1e6, 2), "launches": n,
+          "algorithmic_bytes": by, "algorithmic_flops": fl}
+    return rf, rows
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hop_pmc_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(phi, b, cores):
+    """The oracle (op-for-op libtorch complex128 restatement of the reference
+    CG on HubbardExp::MMdagger) on one field x 8 RHS: the reference's own
+    batched call, with eps = 1e-12 * |b| (its absolute criterion at the
+    relative target)."""
+    import torch
+    from oracle import fermion as F, lattice as olat, solvers as osol
+    torch.set_num_threads(cores)
+    lat = olat.honeycomb(L, L)
+    h = F.HubbardExp(lat, NT, BETA)
+    h.populate(torch.as_tensor(phi[0]))
+    bb = torch.as_tensor(b[:NRHS])
+    eps = TOL * float(bb.norm())
+    h.MMdagger(bb[:1])                        # warm-up (E memo, threads)
+    t0 = time.perf_counter()
+    x, info = osol.cg(h.MMdagger, bb, eps=eps, maxIter=20000)
+    dt = time.perf_counter() - t0
+    rel = float((bb - h.MMdagger(x)).norm() / bb.norm())
+    return {"value": round(NRHS / dt, 4), "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": f"1 field x {NRHS} RHS (one reference CG call), {info['iterations']} iterations, "
+                      f"{dt:.1f} s, true rel residual {rel:.2e}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--inner", default="bf16x3", choices=["bf16x3", "f32"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "the path shards by whole requests/fields with no communication, see DESIGN.md",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import nsl_amd as nsl
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a GPU")
+    phi_h, b_h = fields()
+    Nx = 2 * L * L
+    nb = NFIELD * NRHS
+    lat = nsl.Lattice.honeycomb(L, L)
+    fm = nsl.HubbardExp(lat, NT, BETA)
+    fm.populate(torch.as_tensor(phi_h).cuda())
+    b = torch.as_tensor(b_h).cuda()
+    x = torch.empty_like(b)
+    inner = nsl.BF16X3 if args.inner == "bf16x3" else nsl.F32
+    cg_mix = nsl.CG(fm, nsl.MMdagger, TOL, 20000, precision=inner)
+    cg_64 = nsl.CG(fm, nsl.MMdagger, TOL, 20000, tol_kind=nsl.TOL_REL_PER_RHS)
+    L_ = nsl._lib.lib()
+
+    def run(cg, k):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            cg(b, out=x)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    run(cg_mix, args.warmup)
+    L_.nslx_prof_reset()
+    L_.nslx_prof_enable(1)
+    t_mix = run(cg_mix, args.steps)
+    L_.nslx_prof_enable(0)
+    buf = ctypes.create_string_buffer(1 << 16)
+    L_.nslx_prof_read(buf, len(buf))
+    prof_text = buf.value.decode()
+    rep_mix = dict(cg_mix.report)
+    ok_mix = cg_mix.converged and rep_mix["max_rel_residual"] <= TOL
+    print(f"# mixed ({args.inner}) : {t_mix / args.steps * 1e3:.1f} ms/step, report {rep_mix}", file=sys.stderr)
+    print("# kernel timing (name launches total_ms):\n# " + prof_text.strip().replace("\n", "\n# "),
+          file=sys.stderr)
+
+    out = {"metric": METRIC, "value": round(nb * args.steps / t_mix, 3), "unit": "solves/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(t_mix / args.steps * 1e3, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": f"{args.inner}+f64",
+           "data": "synthetic (Philox-seeded phi ~ N(0, sqrt(U~)), Gaussian RHS)",
+           "config": {"workload": "precision form: honeycomb 12x12 (Nx=288), Nt=64, beta=8, U=3, "
+                                  "16 fields x 8 RHS, CG on MMdagger to 1e-12 FP64 true residual",
+                      "Nx": Nx, "Nt": NT, "beta": BETA, "U": U, "fields": NFIELD,
+                      "rhs_per_field": NRHS, "tol": TOL, "inner": args.inner},
+           "converged": bool(ok_mix), "max_rel_residual": rep_mix["max_rel_residual"],
+           "inner_iterations": rep_mix["iterations"], "outer_steps": rep_mix["outer_steps"]}
+    rf, rows = roofline_from_prof(prof_text, nb, Nx, NT)
+    if rf:
+        rf["traffic"] = read_traffic(rf["kernel"])
+    out["roofline"] = rf
+
+    if not args.no_fp64:
+        run(cg_64, 1)
+        t64 = run(cg_64, args.steps)
+        rep64 = dict(cg_64.report)
+        check = nsl.CG(fm, nsl.MMdagger, TOL, 20000, tol_kind=nsl.TOL_REL_PER_RHS, check_residual=True)
+        check(b, out=x)
+        v64 = nb * args.steps / t64
+        out["fp64_solves_per_s"] = round(v64, 3)
+        out["fp64_ms_per_step"] = round(t64 / args.steps * 1e3, 3)
+        out["fp64_iterations"] = rep64["iterations"]
+        out["fp64_true_rel_residual"] = check.report["max_rel_residual"]
+        out["speedup_vs_fp64"] = round(out["value"] / v64, 3)
+        print(f"# fp64: {t64 / args.steps * 1e3:.1f} ms/step, report {rep64}", file=sys.stderr)
+    out["p50_ms"] = None
+    out["p99_ms"] = None
+    if not args.no_cpu:
+        cores = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(phi_h, b_h, cores)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

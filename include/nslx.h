@@ -1,0 +1,212 @@
+/*
+ * nslx.h — C ABI of the MI355X-native HubbardExp fermion-matrix and
+ * Krylov-solver engine (library: nsl_amd/libnslx.so).
+ *
+ * The reference (Marcel-Rodekamp/NSL) is a header-only C++ template library
+ * on libtorch and has no C ABI; every entry point below replaces a C++
+ * member function or free function of the reference, cited on its
+ * declaration (paths relative to the reference's repository root).
+ *
+ * Conventions
+ *  - Complex fields are interleaved (re, im) pairs, row-major with shape
+ *    (nbatch, nt, nx), x fastest: the byte layout of a contiguous
+ *    NSL::Tensor<complex<double>> of shape (..., Nt, Nx)
+ *    (src/NSL/Tensor/Impl/base.tpp:42-59).  c128 = 2 x double, c64 = 2 x float.
+ *  - Field pointers are DEVICE pointers on the context's device unless a
+ *    parameter says "host".  `stream` is a hipStream_t passed as void*
+ *    (NULL = the null stream); work is stream-ordered.
+ *  - Every function returns an nslx_status.  On error nslx_last_error()
+ *    returns a message for the calling thread.  NSLX_NOT_CONVERGED is not a
+ *    hard error: the solution buffer holds the last iterate, as the
+ *    reference's solvers return x after logging (CG.tpp:90-93).
+ *  - A handle is not re-entrant (like the reference's objects, which hold
+ *    mutable scratch); distinct handles may be used from distinct threads.
+ */
+#ifndef NSLX_H
+#define NSLX_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status ------------------------------------------------------------ */
+typedef enum {
+    NSLX_OK = 0,
+    NSLX_E_ARG = 1,          /* invalid argument (shape, enum, null)         */
+    NSLX_E_DEVICE = 2,       /* HIP runtime error                            */
+    NSLX_E_OOM = 3,          /* device allocation failed                     */
+    NSLX_E_STATE = 4,        /* call order (e.g. apply before populate)      */
+    NSLX_NOT_CONVERGED = 5,  /* solver hit maxIter; x holds the last iterate */
+    NSLX_E_INTERNAL = 6
+} nslx_status;
+
+/* MatrixCombination (src/NSL/LinAlg/Solver/applicationType.tpp:18) */
+typedef enum { NSLX_OP_M = 0, NSLX_OP_MDAGGER = 1, NSLX_OP_MMDAGGER = 2,
+               NSLX_OP_MDAGGERM = 3 } nslx_op;
+
+/* NSL::Hubbard::Species (src/NSL/Action/Implementations/hubbard.tpp:7) */
+typedef enum { NSLX_PARTICLE = 0, NSLX_HOLE = 1 } nslx_species;
+
+/* arithmetic of an apply / of a solver's inner iteration */
+typedef enum {
+    NSLX_F64 = 0,      /* complex128 fields, FP64 MFMA                        */
+    NSLX_F32 = 1,      /* complex64 fields, FP32 MFMA, FP32 accumulate        */
+    NSLX_BF16X3 = 2    /* complex64 fields, E.X as 3 BF16 MFMA products
+                          (hi*hi + hi*lo + lo*hi), FP32 accumulate            */
+} nslx_precision;
+
+typedef enum { NSLX_CG = 0, NSLX_BICGSTAB = 1 } nslx_method;
+
+typedef enum {
+    /* the reference's criterion: one alpha/beta for the whole batch and an
+       absolute stop sum_all |r|^2 <= tol^2 on the recursive residual
+       (CG.hpp:34, CG.tpp:70, inner_product.tpp:31-33)                      */
+    NSLX_TOL_ABS_BATCH = 0,
+    /* per right-hand side alpha/beta; stop each RHS when
+       |r_k| <= tol * |b_k| (recursive residual for F64; for F32/BF16X3 the
+       FP64 true residual of the defect-correction loop)                     */
+    NSLX_TOL_REL_PER_RHS = 1
+} nslx_tol_kind;
+
+typedef struct nslx_ctx_s* nslx_ctx;
+typedef struct nslx_fm_s* nslx_fm;
+
+/* ---- library ------------------------------------------------------------ */
+int         nslx_version(void);
+const char* nslx_status_string(int status);
+const char* nslx_last_error(void);
+
+/* Device context: device ordinal, tuned tile table, scratch pool.
+ * Replaces NSL::Device (src/NSL/device.tpp:22-126) for this path. */
+int nslx_ctx_create(int device, nslx_ctx* out);
+int nslx_ctx_destroy(nslx_ctx ctx);
+
+/* ---- host RNG (Philox-4x32-10, Salmon et al. SC'11) -----------------------
+ * The reference draws from torch's global generator
+ * (src/NSL/Tensor/Impl/factory.tpp:38-41, src/NSL/MarkovChain/HMC.tpp:189-195);
+ * the engine uses a counter RNG so that CPU and GPU runs see the same numbers.
+ * Block i of stream (seed, offset) = philox(ctr=(i, off_lo, off_hi, 0),
+ * key=(seed_lo, seed_hi)).  Host memory. */
+int nslx_rng_philox4x32(uint64_t seed, uint64_t offset, int64_t nblocks, uint32_t* out4);
+int nslx_rng_uniform(uint64_t seed, uint64_t offset, int64_t n, double* out);
+int nslx_rng_normal(uint64_t seed, uint64_t offset, int64_t n, double* out);
+
+/* ---- lattices (host) -----------------------------------------------------
+ * K is written row-major nx*nx (real, symmetric); sites (optional, may be
+ * NULL) nx*2 (honeycomb) or nx*3 (ring, nanotube). */
+/* Honeycomb<T>(L, kappa): src/NSL/Lattice/Implementations/honeycomb.tpp:72-138 */
+int nslx_lattice_honeycomb(int L0, int L1, double kappa, double* K, double* sites);
+/* Ring<T>(n, kappa): src/NSL/Lattice/Implementations/ring.tpp:8-42 */
+int nslx_lattice_ring(int n, double kappa, double* K, double* sites);
+/* (n,n) armchair nanotube with open ends: no counterpart in the reference
+ * (project lattice for mixed-size batches); nx = 4*n*cells. */
+int nslx_lattice_armchair_nanotube(int n, int cells, double kappa, double* K, double* sites);
+
+/* ---- fermion matrix (HubbardExp) -----------------------------------------
+ * Replaces NSL::FermionMatrix::HubbardExp<complex<double>, Lattice>.
+ * ctor (species, lat, Nt, beta, mu): src/NSL/FermionMatrix/Impl/hubbardExp.hpp:64-76.
+ * K: host, nx*nx real (the lattice's hopping matrix, lattice.tpp:46-49).
+ * E(s*delta) = exp(s*delta*K) is computed in FP64 on the device (Taylor with
+ * scaling and squaring) and memoised per exponent, like
+ * SpatialLattice::exp_hopping_matrix (lattice.tpp:51-58).
+ * beta and mu are real here (the reference allows complex Type). */
+int nslx_fm_create(nslx_ctx ctx, int nx, int nt, const double* K, double beta, double mu,
+                   int species, nslx_fm* out);
+int nslx_fm_destroy(nslx_fm fm);
+
+/* populate(phi[, species]): hubbardExp.hpp:103-128.  phi: device c128,
+ * nfield fields of shape (nt, nx), contiguous.  RHS k of an apply/solve with
+ * nbatch RHS uses field k / (nbatch / nfield).  species < 0 keeps the current
+ * species. */
+int nslx_fm_populate(nslx_fm fm, const void* phi, int64_t nfield, int species, void* stream);
+
+/* M, Mdagger, MMdagger, MdaggerM applied to nbatch fields
+ * (fermionMatrix.hpp:40-56; hubbardExp.tpp:38-125).  in/out: device, shape
+ * (nbatch, nt, nx); c128 for NSLX_F64, c64 for NSLX_F32 / NSLX_BF16X3.
+ * in and out must not overlap.  nbatch must be a multiple of nfield. */
+int nslx_fm_apply(nslx_fm fm, int op, const void* in, void* out, int64_t nbatch,
+                  int precision, void* stream);
+
+/* logDetM() for field `field` (hubbardExp.tpp:128-150), FP64.  Computed
+ * as log det of the block-cyclic matrix of chunked slice products with
+ * partial pivoting (stable where the reference's det overflows); nchunks <= 0
+ * picks the count from the spectral growth.  out[0] = log|det M|, out[1] =
+ * arg det M in (-pi, pi]. Synchronous. */
+int nslx_fm_logdet(nslx_fm fm, int64_t field, int nchunks, double* out_host2, void* stream);
+
+/* gradLogDetM() for field `field` (hubbardExp.tpp:152-211), FP64.
+ * out: device c128 (nt, nx). */
+int nslx_fm_grad_logdet(nslx_fm fm, int64_t field, void* out, void* stream);
+
+/* ---- solvers --------------------------------------------------------------- */
+typedef struct {
+    int64_t iterations;        /* max over RHS of inner (Krylov) iterations   */
+    int64_t iterations_total;  /* sum of inner iterations over outer steps     */
+    int64_t outer_steps;       /* FP64 defect-correction steps (0 for F64)    */
+    int64_t converged_rhs;     /* RHS that met the criterion                   */
+    double  rsqr;              /* final |r|^2 (batch sum, recursive, FP64 run) */
+    double  max_rel_residual;  /* max_k |b_k - A x_k| / |b_k|, FP64, if checked */
+    double  seconds;           /* host wall time of the call                    */
+} nslx_solve_report;
+
+/* CG::operator()(b[, x0]) / BiCGStab::operator()(b[, x0]) on the operator
+ * `op` of fm (Solver.hpp:86-96; CG.tpp:14-113; BiCGStab.tpp:14-150).
+ * b, x: device c128 (nbatch, nt, nx).  x0: device c128 or NULL (NULL means
+ * x0 = b, as the reference).  `precision` != NSLX_F64 runs the inner solve
+ * in that precision inside an FP64 defect-correction loop (CG only,
+ * Hermitian ops only) driven to the FP64 true relative residual tol; it
+ * requires NSLX_TOL_REL_PER_RHS.  check_true_residual != 0 fills
+ * max_rel_residual. */
+int nslx_solve(nslx_fm fm, int op, int method, const void* b, void* x, const void* x0,
+               int64_t nbatch, double tol, int tol_kind, int64_t maxit, int precision,
+               int check_true_residual, nslx_solve_report* report, void* stream);
+
+/* ---- HMC ---------------------------------------------------------------------
+ * Hubbard action S = S_gauge + S_fermion and its force for one real field
+ * (src/NSL/Action/Implementations/hubbardGaugeAction.tpp:59-67,
+ * hubbardFermiAction.tpp:68-113; U~ = (beta/Nt) U, hubbard.tpp:10-19).
+ * phi: device c128 (nt, nx).  The fm handle is re-populated. */
+int nslx_hmc_action(nslx_fm fm, const void* phi, double U, int nchunks, double* out_host2,
+                    void* stream);
+int nslx_hmc_force(nslx_fm fm, const void* phi, double U, void* force_out, void* stream);
+
+/* One HMC trajectory with LeapfrogRealForce (leapfrogRealForce.tpp:35-74)
+ * and the Metropolis step of HMC::generate_ (HMC.tpp:185-241).  phi (in/out)
+ * and momentum (in): device c128 (nt, nx), momentum real-valued; S0 = action
+ * at phi (host, 2 doubles).  On accept phi is replaced and S0 updated. */
+typedef struct {
+    double dH_re, dH_im;       /* H1 - H0                                      */
+    double acc_prob;           /* exp(Re(H0 - H1))                             */
+    int    accepted;
+    double S_re, S_im;         /* action of the returned field                 */
+    double seconds;
+} nslx_hmc_report;
+
+int nslx_hmc_trajectory(nslx_fm fm, void* phi, const void* momentum, double* S0_host2,
+                        double U, double traj_length, int nmd, double uniform,
+                        int nchunks, nslx_hmc_report* report, void* stream);
+
+/* ntraj trajectories from phi with momenta and uniforms drawn from Philox:
+ * momentum of trajectory k = nslx_rng_normal(seed, 2k, nt*nx), uniform =
+ * nslx_rng_uniform(seed, 2k+1, 1).  reports: host array of ntraj. */
+int nslx_hmc_chain(nslx_fm fm, void* phi, double U, double traj_length, int nmd,
+                   int64_t ntraj, uint64_t seed, int nchunks, nslx_hmc_report* reports,
+                   void* stream);
+
+/* ---- kernel timing ----------------------------------------------------------
+ * No counterpart in the reference (its only profiling is a host stopwatch,
+ * src/NSL/logger.hpp:174-191).  While enabled, every hop launch is bracketed
+ * by HIP events on its own stream; nslx_prof_read writes one line per kernel
+ * class, "name launches total_ms\n", into buf (NUL terminated) and returns
+ * the number of classes.  Process-wide; not for concurrent use. */
+int nslx_prof_enable(int on);
+int nslx_prof_reset(void);
+int nslx_prof_read(char* buf, int64_t size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NSLX_H */

@@ -1,0 +1,343 @@
+"""nsl_amd: MI355X-native HubbardExp fermion matrix and Krylov solvers.
+
+Host-side mirror of the reference's interfaces for this path (Marcel-Rodekamp/
+NSL): ``HubbardExp`` (src/NSL/FermionMatrix/Impl/hubbardExp.hpp:33-233),
+``CG`` / ``BiCGStab`` (src/NSL/LinAlg/Solver/Impl/CG.hpp, BiCGStab.hpp) with
+``MatrixCombination`` (applicationType.tpp:18), the Hubbard action and HMC
+step (Action/, Integrator/, MarkovChain/HMC.tpp).  All arithmetic runs in
+libnslx.so (hand-written HIP kernels for gfx950) through the C ABI of
+include/nslx.h; torch tensors are used only to hold device memory.  There is
+no CPU fallback: without the built library or a GPU every call raises.
+"""
+
+import ctypes
+import logging
+
+import numpy as np
+
+from . import _lib
+from ._lib import (NslxError, SolveReport, HmcReport, F64, F32, BF16X3, TOL_ABS_BATCH,  # noqa: F401
+                   TOL_REL_PER_RHS)
+
+log = logging.getLogger("nsl_amd")
+
+Particle = _lib.PARTICLE
+Hole = _lib.HOLE
+
+
+class MatrixCombination:
+    """NSL::FermionMatrix::MatrixCombination (applicationType.tpp:18)."""
+    M = _lib.OP_M
+    Mdagger = _lib.OP_MDAGGER
+    MMdagger = _lib.OP_MMDAGGER
+    MdaggerM = _lib.OP_MDAGGERM
+
+
+M, Mdagger, MMdagger, MdaggerM = (MatrixCombination.M, MatrixCombination.Mdagger,
+                                  MatrixCombination.MMdagger, MatrixCombination.MdaggerM)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(device=None):
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+# ----------------------------------------------------------------- context
+class Context:
+    """Device context (nslx_ctx)."""
+
+    _default = {}
+
+    def __init__(self, device=0):
+        self.device = int(device)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().nslx_ctx_create(self.device, ctypes.byref(h)))
+        self._h = h
+
+    @classmethod
+    def default(cls, device=0):
+        if device not in cls._default:
+            cls._default[device] = Context(device)
+        return cls._default[device]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.lib().nslx_ctx_destroy(h)
+            self._h = None
+
+
+# ----------------------------------------------------------------- lattices
+class Lattice:
+    """Spatial lattice: hopping matrix K (numpy float64) built on the host by
+    the library (honeycomb.tpp:72-138, ring.tpp:8-42, project nanotube)."""
+
+    def __init__(self, name, K, sites=None):
+        self.name = name
+        self.K = np.ascontiguousarray(K, dtype=np.float64)
+        self.site_coords = sites
+
+    def sites(self):
+        return self.K.shape[0]
+
+    def hopping_matrix(self):
+        return self.K
+
+    @staticmethod
+    def _build(fn, nx, sdim, *args):
+        K = np.zeros((nx, nx))
+        S = np.zeros((nx, sdim))
+        _lib.check(fn(*args, K.ctypes.data_as(_lib.P_double), S.ctypes.data_as(_lib.P_double)))
+        return K, S
+
+    @classmethod
+    def honeycomb(cls, L0, L1=None, kappa=1.0):
+        L1 = L0 if L1 is None else L1
+        K, S = cls._build(_lib.lib().nslx_lattice_honeycomb, 2 * L0 * L1, 2, L0, L1, kappa)
+        return cls(f"Honeycomb({L0},{L1})", K, S)
+
+    @classmethod
+    def ring(cls, n, kappa=1.0):
+        K, S = cls._build(_lib.lib().nslx_lattice_ring, n, 3, n, kappa)
+        return cls(f"Ring({n})", K, S)
+
+    @classmethod
+    def armchair_nanotube(cls, n, cells, kappa=1.0):
+        K, S = cls._build(_lib.lib().nslx_lattice_armchair_nanotube, 4 * n * cells, 3, n, cells, kappa)
+        return cls(f"ArmchairNanotube({n},{n};{cells})", K, S)
+
+
+# ------------------------------------------------------------ fermion matrix
+class HubbardExp:
+    """NSL::FermionMatrix::HubbardExp on the device.
+
+    ctor (lat, Nt, beta, mu) / (species, lat, Nt, beta, mu): hubbardExp.hpp:48-76
+    (beta and mu real here).  populate(phi[, species]): hubbardExp.hpp:103-128;
+    phi is a complex128 CUDA tensor (Nt, Nx), or (nfield, Nt, Nx) to hold
+    several fields sharing the lattice (RHS k of a batch of nbatch uses field
+    k // (nbatch // nfield)).
+    """
+
+    def __init__(self, lat, Nt, beta=1.0, mu=0.0, species=Particle, ctx=None):
+        self.lat = lat
+        self.Nt = int(Nt)
+        self.Nx = lat.sites()
+        self.beta = float(beta)
+        self.mu = float(mu)
+        self._species = int(species)
+        self.ctx = ctx or Context.default()
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().nslx_fm_create(self.ctx._h, self.Nx, self.Nt,
+                                             lat.K.ctypes.data_as(_lib.P_double), self.beta, self.mu,
+                                             self._species, ctypes.byref(h)))
+        self._h = h
+        self.nfield = 0
+        self._phi = None
+
+    @classmethod
+    def with_species(cls, species, lat, Nt, beta=1.0, mu=0.0, ctx=None):
+        """The (species, lat, Nt, beta, mu) constructor (hubbardExp.hpp:64-76)."""
+        return cls(lat, Nt, beta, mu, species, ctx)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.lib().nslx_fm_destroy(h)
+            self._h = None
+
+    def species(self):
+        return self._species
+
+    def populate(self, phi, species=None):
+        torch = _torch()
+        phi = phi.to(dtype=torch.complex128).contiguous()
+        if phi.dim() == 2:
+            nf = 1
+        else:
+            nf = int(np.prod(phi.shape[:-2]))
+        if tuple(phi.shape[-2:]) != (self.Nt, self.Nx):
+            raise ValueError(f"phi must end in (Nt, Nx) = ({self.Nt}, {self.Nx}), got {tuple(phi.shape)}")
+        if species is not None:
+            self._species = int(species)
+        _lib.check(_lib.lib().nslx_fm_populate(self._h, _ptr(phi), nf,
+                                               -1 if species is None else int(species), _stream()))
+        self.nfield = nf
+        self._phi = phi
+        return self
+
+    # -- applications ------------------------------------------------------
+    def apply(self, op, psi, out=None, precision=F64):
+        torch = _torch()
+        dt = torch.complex128 if precision == F64 else torch.complex64
+        if psi.dtype != dt or not psi.is_contiguous():
+            psi = psi.to(dtype=dt).contiguous()
+        if tuple(psi.shape[-2:]) != (self.Nt, self.Nx):
+            raise ValueError(f"psi must end in (Nt, Nx) = ({self.Nt}, {self.Nx})")
+        nb = int(np.prod(psi.shape[:-2])) if psi.dim() > 2 else 1
+        if out is None:
+            out = torch.empty_like(psi)
+        _lib.check(_lib.lib().nslx_fm_apply(self._h, int(op), _ptr(psi), _ptr(out), nb, int(precision),
+                                            _stream()))
+        return out
+
+    def M(self, psi, precision=F64):
+        return self.apply(MatrixCombination.M, psi, precision=precision)
+
+    def Mdagger(self, psi, precision=F64):
+        return self.apply(MatrixCombination.Mdagger, psi, precision=precision)
+
+    def MMdagger(self, psi, precision=F64):
+        return self.apply(MatrixCombination.MMdagger, psi, precision=precision)
+
+    def MdaggerM(self, psi, precision=F64):
+        return self.apply(MatrixCombination.MdaggerM, psi, precision=precision)
+
+    # -- determinant and force --------------------------------------------
+    def logDetM(self, field=0, nchunks=0):
+        out = (ctypes.c_double * 2)()
+        _lib.check(_lib.lib().nslx_fm_logdet(self._h, int(field), int(nchunks),
+                                             ctypes.cast(out, _lib.P_double), _stream()))
+        return complex(out[0], out[1])
+
+    def gradLogDetM(self, field=0):
+        torch = _torch()
+        out = torch.empty((self.Nt, self.Nx), dtype=torch.complex128, device="cuda")
+        _lib.check(_lib.lib().nslx_fm_grad_logdet(self._h, int(field), _ptr(out), _stream()))
+        return out
+
+
+# ------------------------------------------------------------------ solvers
+class _Solver:
+    method = _lib.CG_METHOD
+
+    def __init__(self, fm, matrixCombination=MatrixCombination.M, eps=1e-12, maxIter=10000, precision=F64,
+                 tol_kind=None, check_residual=False):
+        self.fm = fm
+        self.op = int(matrixCombination)
+        self.eps = float(eps)
+        self.maxIter = int(maxIter)
+        self.precision = int(precision)
+        if tol_kind is None:
+            tol_kind = TOL_ABS_BATCH if precision == F64 else TOL_REL_PER_RHS
+        self.tol_kind = int(tol_kind)
+        self.check = bool(check_residual)
+        self.report = None
+        self.converged = None
+
+    def __call__(self, b, x0=None, out=None):
+        torch = _torch()
+        b = b.to(dtype=torch.complex128).contiguous()
+        nb = int(np.prod(b.shape[:-2])) if b.dim() > 2 else 1
+        x = torch.empty_like(b) if out is None else out
+        if x0 is not None:
+            x0 = x0.to(dtype=torch.complex128).contiguous()
+        rep = SolveReport()
+        st = _lib.lib().nslx_solve(self.fm._h, self.op, self.method, _ptr(b), _ptr(x),
+                                   _ptr(x0) if x0 is not None else None, nb, self.eps, self.tol_kind,
+                                   self.maxIter, self.precision, int(self.check), ctypes.byref(rep), _stream())
+        _lib.check(st, allow=(_lib.OK, _lib.NOT_CONVERGED))
+        self.report = rep.as_dict()
+        self.converged = st == _lib.OK
+        if not self.converged:
+            # the reference logs and returns the current x (CG.tpp:90-93)
+            log.error("%s did not converge within %d iterations", type(self).__name__, self.maxIter)
+        return x
+
+
+class CG(_Solver):
+    """NSL::LinAlg::CG (CG.hpp:31-186, CG.tpp:14-113): x0 = b by default,
+    batch-global alpha/beta and |r|^2 <= eps^2 stop (TOL_ABS_BATCH, the
+    default for FP64); TOL_REL_PER_RHS solves each RHS to |r|/|b| <= eps.
+    precision=F32/BF16X3 runs the FP64 defect-correction solver."""
+    method = _lib.CG_METHOD
+
+
+class BiCGStab(_Solver):
+    """NSL::LinAlg::BiCGStab (BiCGStab.hpp:31-200, BiCGStab.tpp:14-150)."""
+    method = _lib.BICGSTAB_METHOD
+
+
+# ------------------------------------------------------------------ HMC
+class HubbardAction:
+    """Gauge + fermion Hubbard action and the HMC step on one real field
+    (hubbardGaugeAction.tpp, hubbardFermiAction.tpp, leapfrogRealForce.tpp,
+    HMC.tpp:185-241)."""
+
+    def __init__(self, lat, Nt, beta, U, mu=0.0, ctx=None, nchunks=0):
+        self.fm = HubbardExp(lat, Nt, beta, mu, Particle, ctx)
+        self.U = float(U)
+        self.nchunks = int(nchunks)
+
+    def eval(self, phi):
+        phi = phi.to(dtype=_torch().complex128).contiguous()
+        out = (ctypes.c_double * 2)()
+        _lib.check(_lib.lib().nslx_hmc_action(self.fm._h, _ptr(phi), self.U, self.nchunks,
+                                              ctypes.cast(out, _lib.P_double), _stream()))
+        return complex(out[0], out[1])
+
+    def force(self, phi):
+        torch = _torch()
+        phi = phi.to(dtype=torch.complex128).contiguous()
+        out = torch.empty_like(phi)
+        _lib.check(_lib.lib().nslx_hmc_force(self.fm._h, _ptr(phi), self.U, _ptr(out), _stream()))
+        return out
+
+    def trajectory(self, phi, momentum, S0, traj_length, nmd, uniform):
+        """One trajectory; phi is updated in place on accept.  Returns
+        (report dict, new S0)."""
+        S = (ctypes.c_double * 2)(S0.real, S0.imag)
+        rep = HmcReport()
+        mom = momentum.to(dtype=_torch().complex128).contiguous()
+        _lib.check(_lib.lib().nslx_hmc_trajectory(self.fm._h, _ptr(phi), _ptr(mom), ctypes.cast(S, _lib.P_double),
+                                                  self.U, float(traj_length), int(nmd), float(uniform),
+                                                  self.nchunks, ctypes.byref(rep), _stream()))
+        return rep.as_dict(), complex(S[0], S[1])
+
+    def chain(self, phi, ntraj, traj_length, nmd, seed):
+        reps = (HmcReport * int(ntraj))()
+        _lib.check(_lib.lib().nslx_hmc_chain(self.fm._h, _ptr(phi), self.U, float(traj_length), int(nmd),
+                                             int(ntraj), int(seed), self.nchunks, reps, _stream()))
+        return [r.as_dict() for r in reps]
+
+
+# ------------------------------------------------------------------ RNG
+def rng_normal(seed, offset, n):
+    out = np.empty(int(n))
+    _lib.check(_lib.lib().nslx_rng_normal(int(seed), int(offset), int(n), out.ctypes.data_as(_lib.P_double)))
+    return out
+
+
+def rng_uniform(seed, offset, n):
+    out = np.empty(int(n))
+    _lib.check(_lib.lib().nslx_rng_uniform(int(seed), int(offset), int(n), out.ctypes.data_as(_lib.P_double)))
+    return out
+
+
+def real_field(seed, shape, std, offset=0):
+    """Real auxiliary field phi ~ N(0, std) (complex128 with Im = 0), numpy."""
+    n = int(np.prod(shape))
+    return (std * rng_normal(seed, offset, n)).reshape(shape).astype(np.complex128)
+
+
+def complex_normal(seed, shape, offset=0):
+    """Complex Gaussian with Re, Im each N(0,1) (the reference's complex randn,
+    src/NSL/Tensor/Impl/factory.tpp:38-41), numpy complex128."""
+    n = int(np.prod(shape))
+    z = rng_normal(seed, offset, 2 * n)
+    return (z[0::2] + 1j * z[1::2]).reshape(shape)
+
+
+def rng_philox(seed, offset, nblocks):
+    out = np.empty((int(nblocks), 4), dtype=np.uint32)
+    _lib.check(_lib.lib().nslx_rng_philox4x32(int(seed), int(offset), int(nblocks),
+                                              out.ctypes.data_as(_lib.P_uint32)))
+    return out

@@ -1,0 +1,103 @@
+"""ctypes binding of libnslx.so (the C ABI declared in include/nslx.h).
+
+Loading fails loudly: there is no CPU fallback anywhere in nsl_amd.
+"""
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnslx.so")
+
+c_int, c_int64, c_uint64, c_double, c_void_p = (ctypes.c_int, ctypes.c_int64, ctypes.c_uint64,
+                                                ctypes.c_double, ctypes.c_void_p)
+P_double = ctypes.POINTER(ctypes.c_double)
+P_uint32 = ctypes.POINTER(ctypes.c_uint32)
+
+OK, E_ARG, E_DEVICE, E_OOM, E_STATE, NOT_CONVERGED, E_INTERNAL = range(7)
+OP_M, OP_MDAGGER, OP_MMDAGGER, OP_MDAGGERM = range(4)
+PARTICLE, HOLE = 0, 1
+F64, F32, BF16X3 = 0, 1, 2
+CG_METHOD, BICGSTAB_METHOD = 0, 1
+TOL_ABS_BATCH, TOL_REL_PER_RHS = 0, 1
+
+
+class SolveReport(ctypes.Structure):
+    _fields_ = [("iterations", c_int64), ("iterations_total", c_int64), ("outer_steps", c_int64),
+                ("converged_rhs", c_int64), ("rsqr", c_double), ("max_rel_residual", c_double),
+                ("seconds", c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class HmcReport(ctypes.Structure):
+    _fields_ = [("dH_re", c_double), ("dH_im", c_double), ("acc_prob", c_double), ("accepted", c_int),
+                ("S_re", c_double), ("S_im", c_double), ("seconds", c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# name -> (restype, argtypes)
+PROTOS = {
+    "nslx_version": (c_int, []),
+    "nslx_status_string": (ctypes.c_char_p, [c_int]),
+    "nslx_last_error": (ctypes.c_char_p, []),
+    "nslx_ctx_create": (c_int, [c_int, ctypes.POINTER(c_void_p)]),
+    "nslx_ctx_destroy": (c_int, [c_void_p]),
+    "nslx_rng_philox4x32": (c_int, [c_uint64, c_uint64, c_int64, P_uint32]),
+    "nslx_rng_uniform": (c_int, [c_uint64, c_uint64, c_int64, P_double]),
+    "nslx_rng_normal": (c_int, [c_uint64, c_uint64, c_int64, P_double]),
+    "nslx_lattice_honeycomb": (c_int, [c_int, c_int, c_double, P_double, P_double]),
+    "nslx_lattice_ring": (c_int, [c_int, c_double, P_double, P_double]),
+    "nslx_lattice_armchair_nanotube": (c_int, [c_int, c_int, c_double, P_double, P_double]),
+    "nslx_fm_create": (c_int, [c_void_p, c_int, c_int, P_double, c_double, c_double, c_int,
+                               ctypes.POINTER(c_void_p)]),
+    "nslx_fm_destroy": (c_int, [c_void_p]),
+    "nslx_fm_populate": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_void_p]),
+    "nslx_fm_apply": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int64, c_int, c_void_p]),
+    "nslx_fm_logdet": (c_int, [c_void_p, c_int64, c_int, P_double, c_void_p]),
+    "nslx_fm_grad_logdet": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
+    "nslx_solve": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int64, c_double, c_int,
+                           c_int64, c_int, c_int, ctypes.POINTER(SolveReport), c_void_p]),
+    "nslx_hmc_action": (c_int, [c_void_p, c_void_p, c_double, c_int, P_double, c_void_p]),
+    "nslx_hmc_force": (c_int, [c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
+    "nslx_hmc_trajectory": (c_int, [c_void_p, c_void_p, c_void_p, P_double, c_double, c_double, c_int,
+                                    c_double, c_int, ctypes.POINTER(HmcReport), c_void_p]),
+    "nslx_hmc_chain": (c_int, [c_void_p, c_void_p, c_double, c_double, c_int, c_int64, c_uint64, c_int,
+                               ctypes.POINTER(HmcReport), c_void_p]),
+    "nslx_prof_enable": (c_int, [c_int]),
+    "nslx_prof_reset": (c_int, []),
+    "nslx_prof_read": (c_int, [ctypes.c_char_p, c_int64]),
+}
+
+_lib = None
+
+
+class NslxError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"nslx status {status}: {msg}")
+        self.status = status
+
+
+def lib():
+    """Load libnslx.so (built by nsl_amd/build.py).  Raises if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m nsl_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in PROTOS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status, allow=(OK,)):
+    if status not in allow:
+        msg = lib().nslx_last_error().decode(errors="replace")
+        raise NslxError(status, msg)
+    return status

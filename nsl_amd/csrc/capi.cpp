@@ -1,0 +1,275 @@
+// extern "C" boundary of libnslx (declarations and reference citations in
+// include/nslx.h).  Every entry point converts exceptions to status codes.
+#include <cmath>
+#include <cstring>
+
+#include <algorithm>
+
+#include "engine.hpp"
+#include "hmc.hpp"
+#include "prof.hpp"
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f) {
+    try {
+        return f();
+    } catch (const nslx::Error& e) {
+        g_err = e.what();
+        return e.status;
+    } catch (const std::bad_alloc& e) {
+        g_err = std::string("host allocation failed: ") + e.what();
+        return NSLX_E_OOM;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return NSLX_E_INTERNAL;
+    }
+}
+
+inline nslx::FermionMatrix& FM(nslx_fm fm) {
+    if (!fm) throw nslx::Error(NSLX_E_ARG, "null fermion-matrix handle");
+    return fm->m;
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        NSLX_HIP(hipGetDevice(&prev));
+        if (prev != dev) NSLX_HIP(hipSetDevice(dev));
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+}  // namespace
+
+extern "C" {
+
+int nslx_version(void) { return 10000; }
+
+const char* nslx_status_string(int s) {
+    switch (s) {
+        case NSLX_OK: return "ok";
+        case NSLX_E_ARG: return "invalid argument";
+        case NSLX_E_DEVICE: return "device error";
+        case NSLX_E_OOM: return "out of memory";
+        case NSLX_E_STATE: return "invalid state";
+        case NSLX_NOT_CONVERGED: return "not converged";
+        case NSLX_E_INTERNAL: return "internal error";
+        default: return "unknown status";
+    }
+}
+
+const char* nslx_last_error(void) { return g_err.c_str(); }
+
+int nslx_ctx_create(int device, nslx_ctx* out) {
+    return guard([&] {
+        NSLX_REQUIRE(out, "ctx_create: null out");
+        int n = 0;
+        NSLX_HIP(hipGetDeviceCount(&n));
+        NSLX_REQUIRE(device >= 0 && device < n, "ctx_create: no such device");
+        auto* c = new nslx_ctx_s();
+        c->c.device = device;
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, device) == hipSuccess) c->c.num_cu = p.multiProcessorCount;
+        *out = c;
+        return NSLX_OK;
+    });
+}
+
+int nslx_ctx_destroy(nslx_ctx ctx) {
+    delete ctx;
+    return NSLX_OK;
+}
+
+int nslx_rng_philox4x32(uint64_t seed, uint64_t offset, int64_t nblocks, uint32_t* out4) {
+    return guard([&] {
+        NSLX_REQUIRE(out4 && nblocks >= 0, "rng: bad arguments");
+        nslx::host_philox(seed, offset, nblocks, out4);
+        return NSLX_OK;
+    });
+}
+int nslx_rng_uniform(uint64_t seed, uint64_t offset, int64_t n, double* out) {
+    return guard([&] {
+        NSLX_REQUIRE(out && n >= 0, "rng: bad arguments");
+        nslx::host_uniform(seed, offset, n, out);
+        return NSLX_OK;
+    });
+}
+int nslx_rng_normal(uint64_t seed, uint64_t offset, int64_t n, double* out) {
+    return guard([&] {
+        NSLX_REQUIRE(out && n >= 0, "rng: bad arguments");
+        nslx::host_normal(seed, offset, n, out);
+        return NSLX_OK;
+    });
+}
+
+int nslx_lattice_honeycomb(int L0, int L1, double kappa, double* K, double* sites) {
+    return guard([&] { nslx::lattice_honeycomb(L0, L1, kappa, K, sites); return NSLX_OK; });
+}
+int nslx_lattice_ring(int n, double kappa, double* K, double* sites) {
+    return guard([&] { nslx::lattice_ring(n, kappa, K, sites); return NSLX_OK; });
+}
+int nslx_lattice_armchair_nanotube(int n, int cells, double kappa, double* K, double* sites) {
+    return guard([&] { nslx::lattice_nanotube(n, cells, kappa, K, sites); return NSLX_OK; });
+}
+
+int nslx_fm_create(nslx_ctx ctx, int nx, int nt, const double* K, double beta, double mu, int species, nslx_fm* out) {
+    return guard([&] {
+        NSLX_REQUIRE(ctx && out && K, "fm_create: null argument");
+        NSLX_REQUIRE(nx >= 1 && nt >= 2, "fm_create: need nx >= 1 and nt >= 2");
+        NSLX_REQUIRE(species == NSLX_PARTICLE || species == NSLX_HOLE, "fm_create: bad species");
+        NSLX_REQUIRE(std::isfinite(beta) && std::isfinite(mu), "fm_create: beta and mu must be finite");
+        DeviceGuard dg(ctx->c.device);
+        auto* f = new nslx_fm_s();
+        nslx::FermionMatrix& m = f->m;
+        m.ctx = &ctx->c;
+        m.nx = nx;
+        m.nt = nt;
+        m.beta = beta;
+        m.mu = mu;
+        m.species = species;
+        m.K_host.assign(K, K + (size_t)nx * nx);
+        double nk = 0.0;
+        for (int i = 0; i < nx; ++i) {
+            double rs = 0.0;
+            for (int j = 0; j < nx; ++j) rs += std::fabs(K[(size_t)i * nx + j]);
+            nk = std::max(nk, rs);
+        }
+        m.normK = nk;
+        try {
+            m.K.alloc((size_t)nx * nx);
+            NSLX_HIP(hipMemcpy(m.K.p, K, (size_t)nx * nx * sizeof(double), hipMemcpyHostToDevice));
+        } catch (...) {
+            delete f;
+            throw;
+        }
+        *out = f;
+        return NSLX_OK;
+    });
+}
+
+int nslx_fm_destroy(nslx_fm fm) {
+    if (fm) {
+        (void)hipDeviceSynchronize();
+        delete fm;
+    }
+    return NSLX_OK;
+}
+
+int nslx_fm_populate(nslx_fm fm, const void* phi, int64_t nfield, int species, void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        DeviceGuard dg(m.ctx->device);
+        m.populate(phi, nfield, species, nslx::as_stream(stream));
+        return NSLX_OK;
+    });
+}
+
+int nslx_fm_apply(nslx_fm fm, int op, const void* in, void* out, int64_t nbatch, int precision, void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        DeviceGuard dg(m.ctx->device);
+        m.apply(op, in, out, nbatch, precision, nslx::as_stream(stream));
+        return NSLX_OK;
+    });
+}
+
+int nslx_fm_logdet(nslx_fm fm, int64_t field, int nchunks, double* out_host2, void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        DeviceGuard dg(m.ctx->device);
+        nslx::fm_logdet(m, field, nchunks, out_host2, nslx::as_stream(stream));
+        return NSLX_OK;
+    });
+}
+
+int nslx_fm_grad_logdet(nslx_fm fm, int64_t field, void* out, void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        DeviceGuard dg(m.ctx->device);
+        nslx::fm_grad_logdet(m, field, out, nslx::as_stream(stream));
+        return NSLX_OK;
+    });
+}
+
+int nslx_solve(nslx_fm fm, int op, int method, const void* b, void* x, const void* x0, int64_t nbatch, double tol,
+               int tol_kind, int64_t maxit, int precision, int check_true_residual, nslx_solve_report* report,
+               void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        DeviceGuard dg(m.ctx->device);
+        return nslx::solve(m, op, method, b, x, x0, nbatch, tol, tol_kind, maxit, precision, check_true_residual,
+                           report, nslx::as_stream(stream));
+    });
+}
+
+int nslx_hmc_action(nslx_fm fm, const void* phi, double U, int nchunks, double* out_host2, void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        NSLX_REQUIRE(phi && out_host2, "hmc_action: null argument");
+        DeviceGuard dg(m.ctx->device);
+        nslx::hmc_action(m, phi, U, nchunks, out_host2, nslx::as_stream(stream));
+        return NSLX_OK;
+    });
+}
+
+int nslx_hmc_force(nslx_fm fm, const void* phi, double U, void* force_out, void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        NSLX_REQUIRE(phi && force_out, "hmc_force: null argument");
+        DeviceGuard dg(m.ctx->device);
+        nslx::hmc_force(m, phi, U, force_out, nslx::as_stream(stream));
+        return NSLX_OK;
+    });
+}
+
+int nslx_hmc_trajectory(nslx_fm fm, void* phi, const void* momentum, double* S0_host2, double U, double traj_length,
+                        int nmd, double uniform, int nchunks, nslx_hmc_report* report, void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        NSLX_REQUIRE(phi && momentum && S0_host2, "hmc_trajectory: null argument");
+        DeviceGuard dg(m.ctx->device);
+        nslx::hmc_trajectory(m, phi, momentum, S0_host2, U, traj_length, nmd, uniform, nchunks, report,
+                             nslx::as_stream(stream));
+        return NSLX_OK;
+    });
+}
+
+int nslx_hmc_chain(nslx_fm fm, void* phi, double U, double traj_length, int nmd, int64_t ntraj, uint64_t seed,
+                   int nchunks, nslx_hmc_report* reports, void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        NSLX_REQUIRE(phi && ntraj >= 0, "hmc_chain: bad argument");
+        DeviceGuard dg(m.ctx->device);
+        nslx::hmc_chain(m, phi, U, traj_length, nmd, ntraj, seed, nchunks, reports, nslx::as_stream(stream));
+        return NSLX_OK;
+    });
+}
+
+int nslx_prof_enable(int on) {
+    nslx::prof_enable(on != 0);
+    return NSLX_OK;
+}
+
+int nslx_prof_reset(void) {
+    return guard([&] { nslx::prof_reset(); return NSLX_OK; });
+}
+
+int nslx_prof_read(char* buf, int64_t size) {
+    int n = 0;
+    const int st = guard([&] {
+        NSLX_REQUIRE(buf && size > 0, "prof_read: bad buffer");
+        const std::string r = nslx::prof_report(&n);
+        const size_t m = std::min<size_t>(r.size(), (size_t)size - 1);
+        std::memcpy(buf, r.data(), m);
+        buf[m] = 0;
+        return NSLX_OK;
+    });
+    return st == NSLX_OK ? n : -st;
+}
+
+}  // extern "C"

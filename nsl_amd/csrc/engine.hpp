@@ -1,0 +1,108 @@
+// Engine objects behind the C ABI: device context, fermion matrix
+// (HubbardExp) with its memoised exponential hopping matrices, solvers.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <mutex>
+
+#include "common.hpp"
+#include "dense.hpp"
+#include "hop.hpp"
+#include "vec.hpp"
+
+namespace nslx {
+
+// Tuned tile choice per (precision, nx, nt, nb class); filled by the
+// autotuner (tune.cpp) or by default_tile().
+struct TileTable {
+    std::mutex mu;
+    std::map<std::tuple<int, int, int, int>, int> best;
+    int get(int prec, int nx, int nt, int64_t nb);
+    void set(int prec, int nx, int nt, int64_t nb, int tile);
+    static int nb_class(int64_t nb);
+};
+
+struct Ctx {
+    int device = 0;
+    int num_cu = 0;
+    TileTable tiles;
+};
+
+// E(a) = exp(a K) in the padded per-precision layouts (hop.hpp).
+struct EEntry {
+    DevBuf<double> e64;     // padded FP64
+    DevBuf<float> e32;      // padded FP32
+    DevBuf<uint16_t> ebf;   // padded BF16 hi + lo planes
+    DevBuf<double> dense;   // nx x nx FP64, ld nx (for the dense kernels)
+};
+
+// Device workspace of the solvers, kept on the fermion matrix between calls.
+struct SolverWork {
+    DevBuf<unsigned char> r, p0, p1, u, x;          // Krylov vectors (storage precision)
+    DevBuf<unsigned char> r64, t64, r32, x32;       // defect correction
+    DevBuf<unsigned char> v1, v2, v3, v4, v5;       // BiCGStab extra vectors
+    DevBuf<double> rr, alpha, beta, tol2, bn2, rn2, otol2, part, scal;
+    DevBuf<int> done, status;
+    DevBuf<unsigned> counter;
+    PinBuf<int> status_h;
+    PinBuf<double> dbl_h;
+    PinBuf<int> int_h;
+    void ensure_scalars(int64_t nb, int64_t nparts, hipStream_t s);
+};
+
+struct FermionMatrix {
+    Ctx* ctx = nullptr;
+    int nx = 0, nt = 0;
+    double beta = 1.0, mu = 0.0;
+    int species = NSLX_PARTICLE;
+    std::vector<double> K_host;
+    double normK = 0.0;              // max row sum of |K|
+    DevBuf<double> K;
+    std::map<double, std::unique_ptr<EEntry>> memo;
+
+    int64_t nfield = 0;
+    DevBuf<double2> phi;            // [nfield][nt][nx]
+    DevBuf<double2> P64, Pinv64;    // phases
+    DevBuf<float2> P32;
+
+    VecScratch vsc;
+    DevBuf<unsigned char> work;     // apply temporaries
+    SolverWork sw;
+
+    int sgn() const { return species == NSLX_PARTICLE ? +1 : -1; }
+    double delta() const { return beta / nt; }
+    int64_t len() const { return (int64_t)nt * nx; }
+
+    EEntry& E(double a, hipStream_t s);      // memoised exp(a K)
+    const void* Eptr(int prec, hipStream_t s);  // E(s*delta) for the hop
+    const void* Pptr(int prec) const { return prec == NSLX_F64 ? (const void*)P64.p : (const void*)P32.p; }
+
+    void populate(const void* phi_dev, int64_t nf, int sp, hipStream_t s);
+    void require_populated() const;
+    // one hop (PLAIN) with the fm's E and phases
+    void hop(int prec, int op, const void* in, void* out, int64_t nb, hipStream_t s);
+    void apply(int op, const void* in, void* out, int64_t nb, int prec, hipStream_t s);
+    HopArgs hop_args(int prec, int64_t nb, hipStream_t s);
+    int tile(int prec, int64_t nb) { return ctx->tiles.get(prec, nx, nt, nb); }
+};
+
+// Solvers (solver.cpp)
+int solve(FermionMatrix& fm, int op, int method, const void* b, void* x, const void* x0, int64_t nb, double tol,
+          int tol_kind, int64_t maxit, int prec, int check, nslx_solve_report* rep, hipStream_t s);
+
+// log det / force (logdet.cpp)
+void fm_logdet(FermionMatrix& fm, int64_t field, int nchunks, double* out2, hipStream_t s);
+void fm_grad_logdet(FermionMatrix& fm, int64_t field, void* out, hipStream_t s);
+
+// set by NSLX_INNER_TOL etc.; see solver.cpp
+double env_double(const char* name, double dflt);
+
+}  // namespace nslx
+
+struct nslx_ctx_s {
+    nslx::Ctx c;
+};
+struct nslx_fm_s {
+    nslx::FermionMatrix m;
+};

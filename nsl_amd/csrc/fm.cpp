@@ -1,0 +1,134 @@
+// HubbardExp fermion matrix on the device (reference:
+// src/NSL/FermionMatrix/Impl/hubbardExp.hpp:48-144, hubbardExp.tpp:12-125).
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <tuple>
+
+#include "engine.hpp"
+
+namespace nslx {
+
+int TileTable::nb_class(int64_t nb) {
+    int c = 0;
+    while ((int64_t(1) << c) < nb && c < 30) ++c;
+    return c;
+}
+
+int TileTable::get(int prec, int nx, int nt, int64_t nb) {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = best.find({prec, nx, nt, nb_class(nb)});
+    if (it != best.end()) return it->second;
+    return default_tile(prec, nx, nt, nb);
+}
+
+void TileTable::set(int prec, int nx, int nt, int64_t nb, int tile) {
+    std::lock_guard<std::mutex> g(mu);
+    best[{prec, nx, nt, nb_class(nb)}] = tile;
+}
+
+double env_double(const char* name, double dflt) {
+    const char* v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    return std::atof(v);
+}
+
+EEntry& FermionMatrix::E(double a, hipStream_t s) {
+    // memo keyed by the exponent, as SpatialLattice::exp_hopping_matrix
+    // (src/NSL/Lattice/lattice.tpp:51-58)
+    auto it = memo.find(a);
+    if (it != memo.end()) return *it->second;
+    auto e = std::make_unique<EEntry>();
+    const int n = nx;
+    e->dense.alloc((size_t)n * n);
+    DevBuf<double> w1((size_t)n * n), w2((size_t)n * n);
+    const double norm = std::fabs(a) * normK;
+    int sq = 0;
+    while (std::ldexp(norm, -sq) > 0.5) ++sq;
+    expm_real_scaled(n, K.p, a, sq, e->dense.p, w1.p, w2.p, s);
+    const size_t padded = (size_t)e_rows(n) * e_ld(n);
+    e->e64.alloc(padded);
+    e->e32.alloc(padded);
+    e->ebf.alloc(2 * padded);
+    pack_E(n, e->dense.p, e->e64.p, e->e32.p, e->ebf.p, s);
+    NSLX_HIP(hipStreamSynchronize(s));   // w1/w2 are freed on return
+    auto& ref = *e;
+    memo.emplace(a, std::move(e));
+    return ref;
+}
+
+const void* FermionMatrix::Eptr(int prec, hipStream_t s) {
+    EEntry& e = E(sgn() * delta(), s);
+    if (prec == NSLX_F64) return e.e64.p;
+    if (prec == NSLX_F32) return e.e32.p;
+    return e.ebf.p;
+}
+
+void FermionMatrix::populate(const void* phi_dev, int64_t nf, int sp, hipStream_t s) {
+    NSLX_REQUIRE(nf >= 1, "populate: nfield must be >= 1");
+    NSLX_REQUIRE(phi_dev != nullptr, "populate: phi is null");
+    if (sp >= 0) {
+        NSLX_REQUIRE(sp == NSLX_PARTICLE || sp == NSLX_HOLE, "populate: bad species");
+        species = sp;
+    }
+    const int64_t n = nf * len();
+    if (nf != nfield) {
+        phi.alloc(n);
+        P64.alloc(n);
+        Pinv64.alloc(n);
+        P32.alloc(n);
+        nfield = nf;
+    }
+    // deep copy of phi (the reference's populate copies into its own buffer)
+    NSLX_HIP(hipMemcpyAsync(phi.p, phi_dev, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    phase_tables(phi.p, n, sgn(), delta() * mu, P64.p, Pinv64.p, P32.p, s);
+}
+
+void FermionMatrix::require_populated() const {
+    if (nfield < 1) throw Error(NSLX_E_STATE, "fermion matrix used before populate()");
+}
+
+HopArgs FermionMatrix::hop_args(int prec, int64_t nb, hipStream_t s) {
+    require_populated();
+    NSLX_REQUIRE(nb >= 1, "nbatch must be >= 1");
+    NSLX_REQUIRE(nb % nfield == 0, "nbatch must be a multiple of the number of populated fields");
+    NSLX_REQUIRE(nb < (int64_t(1) << 31) / nt, "nbatch too large");
+    HopArgs a;
+    a.nx = nx;
+    a.nt = nt;
+    a.nb = (int)nb;
+    a.rpf = (int)(nb / nfield);
+    a.E = Eptr(prec, s);
+    a.P = Pptr(prec);
+    return a;
+}
+
+void FermionMatrix::hop(int prec, int op, const void* in, void* out, int64_t nb, hipStream_t s) {
+    HopArgs a = hop_args(prec, nb, s);
+    a.X = in;
+    a.Y = out;
+    launch_hop(prec, op, MODE_PLAIN, a, tile(prec, nb), s);
+}
+
+void FermionMatrix::apply(int op, const void* in, void* out, int64_t nb, int prec, hipStream_t s) {
+    NSLX_REQUIRE(prec == NSLX_F64 || prec == NSLX_F32 || prec == NSLX_BF16X3, "apply: bad precision");
+    NSLX_REQUIRE(in && out, "apply: null buffer");
+    NSLX_REQUIRE(in != out, "apply: in and out must not overlap");
+    switch (op) {
+        case NSLX_OP_M: hop(prec, HOP_M, in, out, nb, s); break;
+        case NSLX_OP_MDAGGER: hop(prec, HOP_MD, in, out, nb, s); break;
+        case NSLX_OP_MMDAGGER:
+        case NSLX_OP_MDAGGERM: {
+            const size_t esz = prec == NSLX_F64 ? 16 : 8;
+            work.ensure((size_t)nb * len() * esz);
+            const int first = op == NSLX_OP_MMDAGGER ? HOP_MD : HOP_M;
+            const int second = op == NSLX_OP_MMDAGGER ? HOP_M : HOP_MD;
+            hop(prec, first, in, work.p, nb, s);
+            hop(prec, second, work.p, out, nb, s);
+            break;
+        }
+        default: throw Error(NSLX_E_ARG, "apply: bad op");
+    }
+}
+
+}  // namespace nslx

@@ -1,0 +1,76 @@
+// HIP-event timing of kernel launches (enabled by nslx_prof_enable).
+#include <map>
+#include <mutex>
+#include <sstream>
+
+#include "prof.hpp"
+
+namespace nslx {
+namespace {
+struct Rec {
+    std::string name;
+    hipEvent_t a, b;
+};
+std::mutex g_mu;
+bool g_on = false;
+std::vector<Rec> g_recs;
+std::vector<hipEvent_t> g_pool;
+
+hipEvent_t take() {
+    if (!g_pool.empty()) {
+        hipEvent_t e = g_pool.back();
+        g_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    NSLX_HIP(hipEventCreate(&e));
+    return e;
+}
+}  // namespace
+
+bool prof_on() { return g_on; }
+
+ProfScope::ProfScope(const char* name, int prec, int mode, hipStream_t s) : s_(s), active_(g_on) {
+    if (!active_) return;
+    static const char* pn[] = {"f64", "f32", "bf16x3"};
+    static const char* mn[] = {"plain", "cga", "cgb", "cgi"};
+    std::lock_guard<std::mutex> g(g_mu);
+    name_ = std::string(name) + "_" + pn[prec % 3] + "_" + mn[mode % 4];
+    a_ = take();
+    b_ = take();
+    NSLX_HIP(hipEventRecord((hipEvent_t)a_, s_));
+}
+
+ProfScope::~ProfScope() {
+    if (!active_) return;
+    std::lock_guard<std::mutex> g(g_mu);
+    if (hipEventRecord((hipEvent_t)b_, s_) == hipSuccess)
+        g_recs.push_back({name_, (hipEvent_t)a_, (hipEvent_t)b_});
+}
+
+void prof_enable(bool on) { g_on = on; }
+
+void prof_reset() {
+    std::lock_guard<std::mutex> g(g_mu);
+    for (auto& r : g_recs) { g_pool.push_back(r.a); g_pool.push_back(r.b); }
+    g_recs.clear();
+}
+
+std::string prof_report(int* nclasses) {
+    std::lock_guard<std::mutex> g(g_mu);
+    std::map<std::string, std::pair<long, double>> acc;
+    for (auto& r : g_recs) {
+        NSLX_HIP(hipEventSynchronize(r.b));
+        float ms = 0.f;
+        NSLX_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        auto& e = acc[r.name];
+        e.first += 1;
+        e.second += ms;
+    }
+    std::ostringstream os;
+    for (auto& kv : acc) os << kv.first << " " << kv.second.first << " " << kv.second.second << "\n";
+    if (nclasses) *nclasses = (int)acc.size();
+    return os.str();
+}
+
+}  // namespace nslx

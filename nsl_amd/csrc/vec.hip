@@ -1,0 +1,370 @@
+// Vector kernels with deterministic per-RHS reductions (see vec.hpp).
+#include "vec.hpp"
+
+namespace nslx {
+namespace {
+
+constexpr int VNT = 256;
+constexpr int VEPT = 4;
+constexpr int VCH = VNT * VEPT;   // complex elements per block
+
+template <class S>
+struct CV { S re, im; };
+
+template <class S>
+__device__ inline CV<S> ld(const void* p, size_t i) {
+    if constexpr (sizeof(S) == 8) { const double2 v = reinterpret_cast<const double2*>(p)[i]; return {v.x, v.y}; }
+    else { const float2 v = reinterpret_cast<const float2*>(p)[i]; return {v.x, v.y}; }
+}
+template <class S>
+__device__ inline void st(void* p, size_t i, CV<S> v) {
+    if constexpr (sizeof(S) == 8) reinterpret_cast<double2*>(p)[i] = make_double2(v.re, v.im);
+    else reinterpret_cast<float2*>(p)[i] = make_float2(v.re, v.im);
+}
+
+__device__ inline double wsum(double v) {
+    #pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block sum of (v0, v1); result valid in all threads.
+__device__ inline void bsum2(double& v0, double& v1, double* red) {
+    v0 = wsum(v0);
+    v1 = wsum(v1);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[2 * w] = v0; red[2 * w + 1] = v1; }
+    __syncthreads();
+    double a = 0.0, b = 0.0;
+    #pragma unroll
+    for (int i = 0; i < VNT / 64; ++i) { a += red[2 * i]; b += red[2 * i + 1]; }
+    __syncthreads();
+    v0 = a; v1 = b;
+}
+
+// Publish this block's partial (2 doubles) and return true in the last block.
+__device__ inline bool publish(double* part, unsigned* counter, size_t slot, double v0, double v1, int* flag) {
+    if (threadIdx.x == 0) { part[2 * slot] = v0; part[2 * slot + 1] = v1; }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned prev = atomicAdd(counter, 1u);
+        *flag = (prev == gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (*flag) { __threadfence(); return true; }
+    return false;
+}
+
+__global__ void k_phase(const double2* phi, int64_t n, int sgn, double mut, double2* P, double2* Pinv, float2* P32) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double2 f = phi[i];
+    // exp(i s phi + s mu) with phi = fr + i fi: exponent = (s mu - s fi) + i s fr
+    const double er = sgn * mut - sgn * f.y, ei = sgn * f.x;
+    const double m = exp(er);
+    double sn, cs;
+    sincos(ei, &sn, &cs);
+    const double2 p = make_double2(m * cs, m * sn);
+    P[i] = p;
+    const double mi = exp(-er);
+    Pinv[i] = make_double2(mi * cs, -mi * sn);
+    P32[i] = make_float2((float)p.x, (float)p.y);
+}
+
+template <class S>
+__global__ void k_norm2(const void* v, int64_t len, int nchunk, double* part, unsigned* counter, double* out, int nb) {
+    __shared__ double red[2 * VNT / 64];
+    __shared__ int flag;
+    const int b = blockIdx.x / nchunk, c = blockIdx.x % nchunk;
+    const size_t base = (size_t)b * len;
+    double acc = 0.0;
+    for (int e = 0; e < VEPT; ++e) {
+        const int64_t i = (int64_t)c * VCH + e * VNT + threadIdx.x;
+        if (i < len) { const CV<S> z = ld<S>(v, base + i); acc += double(z.re) * z.re + double(z.im) * z.im; }
+    }
+    double dummy = 0.0;
+    bsum2(acc, dummy, red);
+    if (publish(part, counter, blockIdx.x, acc, 0.0, &flag)) {
+        for (int bb = threadIdx.x; bb < nb; bb += VNT) {
+            double s = 0.0;
+            for (int k = 0; k < nchunk; ++k) s += part[2 * ((size_t)bb * nchunk + k)];
+            out[bb] = s;
+        }
+        if (threadIdx.x == 0) *counter = 0u;
+    }
+}
+
+__global__ void k_set_tol2(const double* bn2, double* tol2, int64_t nb, double tol, int relative, int global) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (global) {
+        if (b == 0) tol2[0] = tol * tol;
+        return;
+    }
+    if (b < nb) tol2[b] = tol * tol * (relative ? bn2[b] : 1.0);
+}
+
+template <class S>
+__global__ void k_cg_init0(const void* bv, void* x, void* r, void* p, int64_t len, int nchunk, int nb,
+                           double* part, unsigned* counter, CgDev cg) {
+    __shared__ double red[2 * VNT / 64];
+    __shared__ int flag;
+    const int b = blockIdx.x / nchunk, c = blockIdx.x % nchunk;
+    const size_t base = (size_t)b * len;
+    double acc = 0.0;
+    for (int e = 0; e < VEPT; ++e) {
+        const int64_t i = (int64_t)c * VCH + e * VNT + threadIdx.x;
+        if (i < len) {
+            const CV<S> z = ld<S>(bv, base + i);
+            st<S>(x, base + i, CV<S>{S(0), S(0)});
+            st<S>(r, base + i, z);
+            st<S>(p, base + i, z);
+            acc += double(z.re) * z.re + double(z.im) * z.im;
+        }
+    }
+    double dummy = 0.0;
+    bsum2(acc, dummy, red);
+    if (publish(part, counter, blockIdx.x, acc, 0.0, &flag)) {
+        if (!cg.global) {
+            int active = 0;
+            for (int bb = threadIdx.x; bb < nb; bb += VNT) {
+                double s = 0.0;
+                for (int k = 0; k < nchunk; ++k) s += part[2 * ((size_t)bb * nchunk + k)];
+                cg.rr[bb] = s;
+                cg.beta[bb] = 0.0;
+                if (s <= cg.tol2[bb]) cg.done[bb] = 0; else { cg.done[bb] = -1; ++active; }
+            }
+            double a = active, d2 = 0.0;
+            bsum2(a, d2, red);
+            if (threadIdx.x == 0) { cg.status[0] = (int)a; cg.status[1] = 0; }
+        } else {
+            double s = 0.0;
+            for (int bb = threadIdx.x; bb < nb; bb += VNT)
+                for (int k = 0; k < nchunk; ++k) s += part[2 * ((size_t)bb * nchunk + k)];
+            double d2 = 0.0;
+            bsum2(s, d2, red);
+            if (threadIdx.x == 0) {
+                cg.rr[0] = s;
+                cg.beta[0] = 0.0;
+                const bool conv = s <= cg.tol2[0];
+                cg.done[0] = conv ? 0 : -1;
+                cg.status[0] = conv ? 0 : nb;
+                cg.status[1] = 0;
+            }
+        }
+        if (threadIdx.x == 0) *counter = 0u;
+    }
+}
+
+__global__ void k_resid64(const double2* bv, const double2* t, double2* r, int64_t len, int nchunk, int nb,
+                          double* part, unsigned* counter, double* rn2) {
+    __shared__ double red[2 * VNT / 64];
+    __shared__ int flag;
+    const int b = blockIdx.x / nchunk, c = blockIdx.x % nchunk;
+    const size_t base = (size_t)b * len;
+    double acc = 0.0;
+    for (int e = 0; e < VEPT; ++e) {
+        const int64_t i = (int64_t)c * VCH + e * VNT + threadIdx.x;
+        if (i < len) {
+            const double2 bb = bv[base + i], tt = t[base + i];
+            const double2 z = make_double2(bb.x - tt.x, bb.y - tt.y);
+            r[base + i] = z;
+            acc += z.x * z.x + z.y * z.y;
+        }
+    }
+    double dummy = 0.0;
+    bsum2(acc, dummy, red);
+    if (publish(part, counter, blockIdx.x, acc, 0.0, &flag)) {
+        for (int bb = threadIdx.x; bb < nb; bb += VNT) {
+            double s = 0.0;
+            for (int k = 0; k < nchunk; ++k) s += part[2 * ((size_t)bb * nchunk + k)];
+            rn2[bb] = s;
+        }
+        if (threadIdx.x == 0) *counter = 0u;
+    }
+}
+
+__global__ void k_scale_to32(const double2* r64, float2* r32, const double* rn2, const double* otol2, int64_t len, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t b = i / len;
+    const double nn = rn2[b];
+    const double sc = (nn > otol2[b] && nn > 0.0) ? 1.0 / sqrt(nn) : 0.0;
+    const double2 z = r64[i];
+    r32[i] = make_float2((float)(z.x * sc), (float)(z.y * sc));
+}
+
+__global__ void k_update64(double2* x64, const float2* d32, const double* rn2, const double* otol2, int64_t len, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t b = i / len;
+    const double nn = rn2[b];
+    if (!(nn > otol2[b])) return;
+    const double sc = sqrt(nn);
+    const float2 d = d32[i];
+    double2 x = x64[i];
+    x.x += sc * (double)d.x;
+    x.y += sc * (double)d.y;
+    x64[i] = x;
+}
+
+__global__ void k_c128_to_c64(const double2* in, float2* out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) { const double2 z = in[i]; out[i] = make_float2((float)z.x, (float)z.y); }
+}
+__global__ void k_c64_to_c128(const float2* in, double2* out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) { const float2 z = in[i]; out[i] = make_double2(z.x, z.y); }
+}
+
+template <class S>
+__global__ void k_dot(const void* a, const void* bv, int64_t len, int nchunk, int nb, double* part, unsigned* counter,
+                      double* out, int global) {
+    __shared__ double red[2 * VNT / 64];
+    __shared__ int flag;
+    const int b = blockIdx.x / nchunk, c = blockIdx.x % nchunk;
+    const size_t base = (size_t)b * len;
+    double sr = 0.0, si = 0.0;
+    for (int e = 0; e < VEPT; ++e) {
+        const int64_t i = (int64_t)c * VCH + e * VNT + threadIdx.x;
+        if (i < len) {
+            const CV<S> x = ld<S>(a, base + i), y = ld<S>(bv, base + i);
+            sr += double(x.re) * y.re + double(x.im) * y.im;
+            si += double(x.re) * y.im - double(x.im) * y.re;
+        }
+    }
+    bsum2(sr, si, red);
+    if (publish(part, counter, blockIdx.x, sr, si, &flag)) {
+        if (!global) {
+            for (int bb = threadIdx.x; bb < nb; bb += VNT) {
+                double r = 0.0, im = 0.0;
+                for (int k = 0; k < nchunk; ++k) {
+                    r += part[2 * ((size_t)bb * nchunk + k)];
+                    im += part[2 * ((size_t)bb * nchunk + k) + 1];
+                }
+                out[2 * bb] = r;
+                out[2 * bb + 1] = im;
+            }
+        } else {
+            double r = 0.0, im = 0.0;
+            for (int bb = threadIdx.x; bb < nb; bb += VNT)
+                for (int k = 0; k < nchunk; ++k) {
+                    r += part[2 * ((size_t)bb * nchunk + k)];
+                    im += part[2 * ((size_t)bb * nchunk + k) + 1];
+                }
+            bsum2(r, im, red);
+            if (threadIdx.x == 0) { out[0] = r; out[1] = im; }
+        }
+        if (threadIdx.x == 0) *counter = 0u;
+    }
+}
+
+template <class S>
+__global__ void k_axpy(const void* x, const void* y, void* z, const double* cs, double sign, int64_t len, int64_t n,
+                       int global, const int* done) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t b = i / len;
+    const int64_t slot = global ? 0 : b;
+    if (done && done[slot] >= 0) return;
+    const double cr = sign * cs[2 * slot], ci = sign * cs[2 * slot + 1];
+    const CV<S> xv = ld<S>(x, i), yv = ld<S>(y, i);
+    CV<S> out;
+    out.re = S(double(xv.re) + cr * yv.re - ci * yv.im);
+    out.im = S(double(xv.im) + cr * yv.im + ci * yv.re);
+    st<S>(z, i, out);
+}
+
+inline int nchunks(int64_t len) { return (int)((len + VCH - 1) / VCH); }
+inline unsigned grid1(int64_t n, int nt = 256) { return (unsigned)((n + nt - 1) / nt); }
+
+}  // namespace
+
+void VecScratch::ensure(int64_t nb, int64_t len, hipStream_t s) {
+    const size_t need = (size_t)nb * nchunks(len) * 2 + 16;
+    part.ensure(need);
+    if (!counter.p) {
+        counter.alloc(4);
+        NSLX_HIP(hipMemsetAsync(counter.p, 0, 4 * sizeof(unsigned), s));
+    }
+}
+
+void phase_tables(const void* phi, int64_t n, int sgn, double mut, void* P64, void* Pinv64, void* P32, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_phase, dim3(grid1(n)), dim3(256), 0, s, (const double2*)phi, n, sgn, mut, (double2*)P64,
+                       (double2*)Pinv64, (float2*)P32);
+    NSLX_LAUNCH_CHECK();
+}
+
+void norm2_perb(int f64, const void* v, int64_t nb, int64_t len, double* out, VecScratch& sc, hipStream_t s) {
+    sc.ensure(nb, len, s);
+    const int nc = nchunks(len);
+    if (f64) hipLaunchKernelGGL(k_norm2<double>, dim3(nb * nc), dim3(VNT), 0, s, v, len, nc, sc.part.p, sc.counter.p, out, (int)nb);
+    else hipLaunchKernelGGL(k_norm2<float>, dim3(nb * nc), dim3(VNT), 0, s, v, len, nc, sc.part.p, sc.counter.p, out, (int)nb);
+    NSLX_LAUNCH_CHECK();
+}
+
+void set_tol2(const double* bn2, double* tol2, int64_t nb, double tol, int relative, int global, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_tol2, dim3(grid1(nb)), dim3(256), 0, s, bn2, tol2, nb, tol, relative, global);
+    NSLX_LAUNCH_CHECK();
+}
+
+void cg_init_zero(int f64, const void* b, void* x, void* r, void* p, int64_t nb, int64_t len, const CgDev& cg,
+                  VecScratch& sc, hipStream_t s) {
+    sc.ensure(nb, len, s);
+    const int nc = nchunks(len);
+    if (f64) hipLaunchKernelGGL(k_cg_init0<double>, dim3(nb * nc), dim3(VNT), 0, s, b, x, r, p, len, nc, (int)nb, sc.part.p, sc.counter.p, cg);
+    else hipLaunchKernelGGL(k_cg_init0<float>, dim3(nb * nc), dim3(VNT), 0, s, b, x, r, p, len, nc, (int)nb, sc.part.p, sc.counter.p, cg);
+    NSLX_LAUNCH_CHECK();
+}
+
+void resid64(const void* b, const void* t, void* r, int64_t nb, int64_t len, double* rn2, VecScratch& sc, hipStream_t s) {
+    sc.ensure(nb, len, s);
+    const int nc = nchunks(len);
+    hipLaunchKernelGGL(k_resid64, dim3(nb * nc), dim3(VNT), 0, s, (const double2*)b, (const double2*)t, (double2*)r, len, nc,
+                       (int)nb, sc.part.p, sc.counter.p, rn2);
+    NSLX_LAUNCH_CHECK();
+}
+
+void scale_to32(const void* r64, void* r32, const double* rn2, const double* otol2, int64_t nb, int64_t len, hipStream_t s) {
+    const int64_t n = nb * len;
+    hipLaunchKernelGGL(k_scale_to32, dim3(grid1(n)), dim3(256), 0, s, (const double2*)r64, (float2*)r32, rn2, otol2, len, n);
+    NSLX_LAUNCH_CHECK();
+}
+
+void update64(void* x64, const void* d32, const double* rn2, const double* otol2, int64_t nb, int64_t len, hipStream_t s) {
+    const int64_t n = nb * len;
+    hipLaunchKernelGGL(k_update64, dim3(grid1(n)), dim3(256), 0, s, (double2*)x64, (const float2*)d32, rn2, otol2, len, n);
+    NSLX_LAUNCH_CHECK();
+}
+
+void c128_to_c64(const void* in, void* out, int64_t n, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_c128_to_c64, dim3(grid1(n)), dim3(256), 0, s, (const double2*)in, (float2*)out, n);
+    NSLX_LAUNCH_CHECK();
+}
+void c64_to_c128(const void* in, void* out, int64_t n, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_c64_to_c128, dim3(grid1(n)), dim3(256), 0, s, (const float2*)in, (double2*)out, n);
+    NSLX_LAUNCH_CHECK();
+}
+
+void dot_perb(int f64, const void* a, const void* bv, int64_t nb, int64_t len, double* out, int global, VecScratch& sc,
+              hipStream_t s) {
+    sc.ensure(nb, len, s);
+    const int nc = nchunks(len);
+    if (f64) hipLaunchKernelGGL(k_dot<double>, dim3(nb * nc), dim3(VNT), 0, s, a, bv, len, nc, (int)nb, sc.part.p, sc.counter.p, out, global);
+    else hipLaunchKernelGGL(k_dot<float>, dim3(nb * nc), dim3(VNT), 0, s, a, bv, len, nc, (int)nb, sc.part.p, sc.counter.p, out, global);
+    NSLX_LAUNCH_CHECK();
+}
+
+void axpy_perb(int f64, const void* x, const void* y, void* z, const double* cs, double sign, int64_t nb, int64_t len,
+               int global, const int* done, hipStream_t s) {
+    const int64_t n = nb * len;
+    if (!n) return;
+    if (f64) hipLaunchKernelGGL(k_axpy<double>, dim3(grid1(n)), dim3(256), 0, s, x, y, z, cs, sign, len, n, global, done);
+    else hipLaunchKernelGGL(k_axpy<float>, dim3(grid1(n)), dim3(256), 0, s, x, y, z, cs, sign, len, n, global, done);
+    NSLX_LAUNCH_CHECK();
+}
+
+}  // namespace nslx

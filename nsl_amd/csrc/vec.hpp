@@ -1,0 +1,61 @@
+// Vector (BLAS-1 style) kernels with per-right-hand-side deterministic
+// reductions.  A field batch is nb vectors of len = nt*nx complex elements.
+// Reductions use per-chunk partials and a last-block pass that sums them in
+// index order, so results do not depend on scheduling or on the rest of the
+// batch.
+#pragma once
+
+#include "common.hpp"
+#include "hop.hpp"
+
+namespace nslx {
+
+// Scratch for vec reductions: partial sums and a zeroed counter.
+struct VecScratch {
+    DevBuf<double> part;       // nb * nchunk * 2
+    DevBuf<unsigned> counter;  // 1, zero between launches
+    void ensure(int64_t nb, int64_t len, hipStream_t s);
+};
+
+// P = exp(i s phi + s mu~), Pinv = exp(-i s phi - s mu~) in FP64, plus the
+// FP32 copy of P.  phi: c128 [n].  (hubbardExp.hpp:116-128)
+void phase_tables(const void* phi, int64_t n, int sgn, double mut, void* P64, void* Pinv64,
+                  void* P32, hipStream_t s);
+
+// out[b] = sum |v_b|^2 (double), S = double (c128) or float (c64).
+void norm2_perb(int prec_is_f64, const void* v, int64_t nb, int64_t len, double* out,
+                VecScratch& sc, hipStream_t s);
+// tol2[b] = tol*tol*(relative ? bn2[b] : 1); in batch-global mode slot 0 only.
+void set_tol2(const double* bn2, double* tol2, int64_t nb, double tol, int relative, int global,
+              hipStream_t s);
+// CG start from x0 = 0: x = 0, r = b, p = b; |b|^2 partials; finalises like
+// MODE_CGI (rr, done, beta, status).
+void cg_init_zero(int prec_is_f64, const void* b, void* x, void* r, void* p, int64_t nb,
+                  int64_t len, const CgDev& cg, VecScratch& sc, hipStream_t s);
+
+// Defect correction helpers (FP64 outer loop around a low-precision solve).
+// r64 = b - t64; rn2[b] = |r64_b|^2.
+void resid64(const void* b, const void* t, void* r, int64_t nb, int64_t len, double* rn2,
+             VecScratch& sc, hipStream_t s);
+// r32 = r64 / |r64_b| for b with rn2[b] > otol2[b] (else 0).
+void scale_to32(const void* r64, void* r32, const double* rn2, const double* otol2, int64_t nb,
+                int64_t len, hipStream_t s);
+// x64 += |r64_b| * d32 for b with rn2[b] > otol2[b].
+void update64(void* x64, const void* d32, const double* rn2, const double* otol2, int64_t nb,
+              int64_t len, hipStream_t s);
+// Element type conversions.
+void c128_to_c64(const void* in, void* out, int64_t n, hipStream_t s);
+void c64_to_c128(const void* in, void* out, int64_t n, hipStream_t s);
+
+// Generic per-RHS complex dots: out[2b], out[2b+1] = sum conj(a_b) b_b.
+// If global, out[0..1] = sum over all b (fixed order).
+void dot_perb(int prec_is_f64, const void* a, const void* bv, int64_t nb, int64_t len, double* out,
+              int global, VecScratch& sc, hipStream_t s);
+
+// z = x + c_b * y  with complex per-b scalar c (cs[2b], cs[2b+1]); sign = +1/-1
+// applied to c; if global, c = cs[0..1] for all b.  x, y, z may alias
+// elementwise (same index).
+void axpy_perb(int prec_is_f64, const void* x, const void* y, void* z, const double* cs, double sign,
+               int64_t nb, int64_t len, int global, const int* done, hipStream_t s);
+
+}  // namespace nslx

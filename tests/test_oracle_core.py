@@ -1,0 +1,278 @@
+"""CPU pins of the oracle (no GPU): every oracle function is checked against a
+known-answer vector, a closed form, brute force or an invariant that a
+plausible mistake (sign, shift direction, transposed operand, dropped term)
+would break.  Mirrors the reference's own tests where they exist
+(Tests/FermionMatrix/test_fermionMatrixHubbardExp.cpp, Tests/Action/
+test_force.cpp, Tests/LinAlg/Solver/test_CG.cpp, Tests/Lattice/test_honeycomb.cpp).
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import correlator, fermion as F, hmc, lattice, rng, solvers
+
+
+# ------------------------------------------------------------------ RNG
+def test_philox_known_answers():
+    # Random123 kat_vectors, philox4x32 with 10 rounds
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+           ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+           ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+            (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+    for c, k, want in kat:
+        got = rng.philox4x32_10(np.array([c], np.uint32), np.array([k], np.uint32))[0]
+        assert tuple(int(v) for v in got) == want
+
+
+def test_normal_moments():
+    z = rng.normal(7, 3, 200000)
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
+    u = rng.uniform(7, 4, 100000)
+    assert u.min() >= 0 and u.max() < 1 and abs(u.mean() - 0.5) < 0.01
+
+
+# -------------------------------------------------------------- lattices
+def honeycomb_bloch(L0, L1):
+    """Closed-form spectrum of the periodic honeycomb: +-|1 + e^{ik.a0} + e^{ik.a1}|
+    with k.a_i = 2 pi m_i / L_i."""
+    ev = []
+    for m0 in range(L0):
+        for m1 in range(L1):
+            f = abs(1 + np.exp(2j * np.pi * m0 / L0) + np.exp(2j * np.pi * m1 / L1))
+            ev += [f, -f]
+    return np.sort(ev)
+
+
+@pytest.mark.parametrize("L", [(3, 3), (4, 3), (6, 6), (5, 7)])
+def test_honeycomb_closed_form(L):
+    lat = lattice.honeycomb(*L)
+    K = lat.hops.real.numpy()
+    assert K.shape == (2 * L[0] * L[1],) * 2
+    assert np.allclose(K, K.T)
+    assert (K.sum(1) == 3).all()                              # valence 3 (test_honeycomb.cpp)
+    ev = np.linalg.eigvalsh(K)
+    assert np.allclose(ev, honeycomb_bloch(*L), atol=1e-12)
+    zero = int((np.abs(ev) < 1e-10).sum())
+    assert zero == (4 if L[0] % 3 == 0 and L[1] % 3 == 0 else 0) or not (L[0] % 3 == 0 and L[1] % 3 == 0)
+
+
+def test_ring_matches_reference_corner_order():
+    K = lattice.ring(5).hops.real.numpy()
+    assert (K.sum(1) == 2).all() and K[0, 4] == 1 and K[4, 0] == 1
+    assert np.allclose(np.linalg.eigvalsh(K), np.sort(2 * np.cos(2 * np.pi * np.arange(5) / 5)))
+
+
+@pytest.mark.parametrize("n,cells", [(6, 4), (6, 7), (3, 2)])
+def test_armchair_nanotube(n, cells):
+    lat = lattice.armchair_nanotube(n, cells)
+    K = lat.hops.real.numpy()
+    N = K.shape[0]
+    assert N == 4 * n * cells
+    deg = K.sum(1)
+    assert set(np.unique(deg)) <= {2.0, 3.0}
+    assert (deg == 2).sum() == 4 * n                            # 2n edge sites at each open end
+    ev = np.linalg.eigvalsh(K)
+    assert np.allclose(np.sort(ev), np.sort(-ev), atol=1e-10)  # bipartite: symmetric spectrum
+    assert np.abs(ev).max() <= 3 + 1e-12
+
+
+def test_exp_hopping_matrix():
+    lat = lattice.honeycomb(3, 3)
+    for a in (0.25, -0.25, 0.75):
+        E = lat.exp_hopping_matrix(a)
+        w, U = np.linalg.eigh(lat.hops.real.numpy())
+        Eref = (U * np.exp(a * w)) @ U.T
+        assert np.abs(E.numpy() - Eref).max() < 1e-13
+    I = lat.exp_hopping_matrix(0.3) @ lat.exp_hopping_matrix(-0.3)
+    assert torch.allclose(I, torch.eye(18, dtype=torch.complex128), atol=1e-14)
+
+
+# ---------------------------------------------------------- fermion matrix
+def _setup(lat, Nt, beta, seed=3, std=0.7, mu=0.0, species=F.PARTICLE, complex_phi=False):
+    phi = torch.as_tensor(rng.real_field(seed, 0, (Nt, lat.sites()), std))
+    if complex_phi:
+        phi = phi + 0.3j * torch.as_tensor(rng.normal(seed, 1, Nt * lat.sites()).reshape(Nt, -1))
+    h = F.HubbardExp(lat, Nt, beta, mu, species)
+    h.populate(phi)
+    return h, phi
+
+
+@pytest.mark.parametrize("species", [F.PARTICLE, F.HOLE])
+@pytest.mark.parametrize("lat,Nt", [(lattice.ring(4), 4), (lattice.ring(6), 8), (lattice.honeycomb(2, 2), 6)])
+def test_M_against_index_formula(lat, Nt, species):
+    """brute force: dense matrix from the index formula of hubbardExp.hpp:22-31"""
+    h, phi = _setup(lat, Nt, 2.0, mu=0.1, species=species, complex_phi=True)
+    D = h.M_dense()
+    D2 = F.index_formula_dense(lat, Nt, 2.0, phi, mu=0.1, species=species)
+    assert (D - D2).abs().max() < 1e-14
+    N = Nt * lat.sites()
+    psi = torch.as_tensor(rng.complex_normal(9, 0, (3, Nt, lat.sites())))
+    Dm = D.reshape(N, N)
+    md = h.Mdagger(psi).reshape(3, N)
+    assert (md - (Dm.conj().T @ psi.reshape(3, N).T).T).abs().max() < 1e-13
+    assert (h.MMdagger(psi) - h.M(h.Mdagger(psi))).abs().max() < 1e-13
+    assert (h.MdaggerM(psi) - h.Mdagger(h.M(psi))).abs().max() < 1e-13
+    A = (Dm @ Dm.conj().T)
+    assert (A - A.conj().T).abs().max() < 1e-13                 # hermiticity
+
+
+def test_batched_equals_single():
+    lat = lattice.honeycomb(2, 2)
+    h, _ = _setup(lat, 6, 2.0)
+    psi = torch.as_tensor(rng.complex_normal(11, 0, (5, 6, 8)))
+    for op in (h.M, h.Mdagger, h.MMdagger, h.MdaggerM):
+        b = op(psi)
+        for k in range(5):
+            assert (b[k] - op(psi[k])).abs().max() < 1e-14
+
+
+def test_free_matsubara_spectrum():
+    """phi = 0: eigenvalues of M are 1 - e^{s delta eps_k} e^{i w_n}, w_n = (2n+1) pi / Nt."""
+    lat = lattice.ring(5)
+    Nt, beta = 6, 2.0
+    h = F.HubbardExp(lat, Nt, beta)
+    h.populate(torch.zeros(Nt, 5))
+    ev = torch.linalg.eigvals(h.M_dense().reshape(30, 30)).numpy()
+    eps = np.linalg.eigvalsh(lat.hops.real.numpy())
+    w = (2 * np.arange(Nt) + 1) * np.pi / Nt
+    ref = (1 - np.exp(beta / Nt * eps)[:, None] * np.exp(1j * w)[None, :]).ravel()
+    for e in ref:
+        assert np.min(np.abs(ev - e)) < 1e-12
+
+
+@pytest.mark.parametrize("species", [F.PARTICLE, F.HOLE])
+def test_logdet_pins(species):
+    lat = lattice.honeycomb(2, 3)
+    Nt, beta = 8, 3.0
+    h, phi = _setup(lat, Nt, beta, species=species, complex_phi=True)
+    ld = h.logDetM()
+    sign, la = torch.linalg.slogdet(h.M_dense().reshape(Nt * 12, Nt * 12))
+    ref = complex(float(la), math.atan2(float(sign.imag), float(sign.real)))
+    assert abs(ld - ref) < 1e-10
+    for nc in (1, 2, 3, 8):
+        assert abs(h.logdet_stable(nc) - ref) < 1e-10
+    # time-shift and 2 pi invariance (test_fermionMatrixHubbardExp.cpp:475-545)
+    h2 = F.HubbardExp(lat, Nt, beta, species=species)
+    h2.populate(torch.roll(phi, 3, 0))
+    assert abs(h2.logDetM() - ld) < 1e-10
+    h2.populate(phi + 2 * math.pi * torch.randint(0, 5, phi.shape).to(torch.float64))
+    d = h2.logDetM() - ld
+    assert abs(d.real) < 1e-10 and abs(math.remainder(d.imag, 2 * math.pi)) < 1e-10
+
+
+def test_logdet_closed_forms_and_large_beta():
+    lat = lattice.honeycomb(3, 3)
+    eps = np.linalg.eigvalsh(lat.hops.real.numpy())
+    for Nt, beta in ((16, 4.0), (32, 40.0)):
+        h = F.HubbardExp(lat, Nt, beta)
+        h.populate(torch.zeros(Nt, 18))
+        exact = float(np.sum(np.logaddexp(0.0, beta * eps)))   # logdet(1 + e^{beta K})
+        assert abs(h.logdet_stable() - exact) < 1e-11 * max(1, exact)
+        if beta < 10:
+            assert abs(h.logDetM() - exact) < 1e-9
+        # uniform time slices: logdet(1 + e^{i sum phi_t} e^{beta K})
+        th = torch.as_tensor(rng.uniform(5, 0, Nt))
+        phi = th[:, None].repeat(1, 18).to(torch.complex128)
+        h.populate(phi)
+        Phi = float(th.sum())
+        exact_u = complex(np.sum(np.log(1 + np.exp(1j * Phi) * np.exp(beta * eps))))
+        got = h.logdet_stable()
+        assert abs(got.real - exact_u.real) < 1e-9 * max(1, abs(exact_u.real))
+        assert abs(math.remainder(got.imag - exact_u.imag, 2 * math.pi)) < 1e-8
+
+
+def test_grad_logdet_relation():
+    """gradLogDetM = i - s d(logdet)/d(phi) per species (the reference's
+    convention; checked by central differences)."""
+    lat = lattice.ring(4)
+    Nt, beta = 6, 2.0
+    for species, s in ((F.PARTICLE, 1), (F.HOLE, -1)):
+        h, phi = _setup(lat, Nt, beta, species=species)
+        g = h.gradLogDetM()
+        for (t, x) in ((0, 0), (2, 3), (5, 1)):
+            e = 1e-6
+            pp, pm = phi.clone(), phi.clone()
+            pp[t, x] += e
+            pm[t, x] -= e
+            hp = F.HubbardExp(lat, Nt, beta, species=species); hp.populate(pp)
+            hm = F.HubbardExp(lat, Nt, beta, species=species); hm.populate(pm)
+            fd = (hp.logDetM() - hm.logDetM()) / (2 * e)
+            assert abs(g[t, x] - (1j - s * fd)) < 1e-7
+
+
+def test_action_force_finite_differences():
+    """Tests/Action/test_force.cpp: Nt=8, ring(2), beta=2, U=3, real phi."""
+    lat = lattice.ring(2)
+    A = hmc.HubbardAction(lat, 8, 2.0, 3.0)
+    phi = torch.as_tensor(rng.real_field(21, 0, (8, 2), 1.0)) * A.Utilde
+    g = A.grad(phi)
+    assert g.imag.abs().max() < 1e-12                          # force is real for real phi
+    for t in range(8):
+        for x in range(2):
+            pp, pm = phi.clone(), phi.clone()
+            pp[t, x] += 1e-6
+            pm[t, x] -= 1e-6
+            fd = (A.eval(pp) - A.eval(pm)) / 2e-6
+            assert abs(fd - g[t, x]) < 1e-6
+
+
+def test_leapfrog_reversible_and_energy():
+    lat = lattice.ring(3)
+    A = hmc.HubbardAction(lat, 6, 2.0, 2.0)
+    phi = torch.as_tensor(rng.real_field(2, 0, (6, 3), 0.5))
+    p = hmc.momentum_draw(4, 0, (6, 3))
+    q1, p1 = hmc.leapfrog_real_force(A, phi, p, 1.0, 10)
+    q0, p0 = hmc.leapfrog_real_force(A, q1, -p1, 1.0, 10)
+    assert (q0 - phi).abs().max() < 1e-10 and (p0 + p).abs().max() < 1e-10
+    dH = []
+    for nmd in (10, 20):
+        q1, p1 = hmc.leapfrog_real_force(A, phi, p, 1.0, nmd)
+        H0 = A.eval(phi) + 0.5 * complex((p * p).sum())
+        H1 = A.eval(q1) + 0.5 * complex((p1 * p1).sum())
+        dH.append(abs((H1 - H0).real))
+    assert dH[1] < dH[0] / 3                                   # O(eps^2)
+
+
+# ------------------------------------------------------------------ solvers
+def test_cg_random_hpd():
+    g = torch.Generator().manual_seed(0)
+    n = 40
+    X = torch.randn(n, n, dtype=torch.complex128, generator=g)
+    A = X @ X.conj().T + n * torch.eye(n, dtype=torch.complex128)
+    b = torch.randn(n, dtype=torch.complex128, generator=g)
+    for solver in (solvers.cg, solvers.bicgstab):
+        x, info = solver(lambda v: A @ v, b, eps=1e-12)
+        assert info["converged"]
+        assert (x - torch.linalg.solve(A, b)).abs().max() < 1e-10
+
+
+def test_cg_fermion_matrix_and_not_converged():
+    lat = lattice.honeycomb(3, 3)
+    h, _ = _setup(lat, 16, 4.0, std=math.sqrt(0.5))
+    b = torch.as_tensor(rng.complex_normal(5, 0, (2, 16, 18)))
+    x, info = solvers.cg(h.MMdagger, b, eps=1e-9)
+    assert info["converged"]
+    assert ((h.MMdagger(x) - b).norm() / b.norm()) < 1e-9
+    x1, info1 = solvers.cg(h.MMdagger, b, eps=1e-12, maxIter=3)
+    assert not info1["converged"] and info1["iterations"] == 3
+
+
+def test_correlator_free_propagator():
+    lat = lattice.ring(4)
+    Nt, beta = 8, 2.0
+    h = F.HubbardExp(lat, Nt, beta)
+    h.populate(torch.zeros(Nt, 4))
+    corr, _ = correlator.two_point(h, Nt, 4, 2)
+    Minv = torch.linalg.inv(h.M_dense().reshape(Nt * 4, Nt * 4)).reshape(Nt, 4, Nt, 4)
+    # free propagator depends on t - tsrc with the antiperiodic sign
+    ref = torch.zeros(Nt, 4, 4, dtype=torch.complex128)
+    for tsrc in (0, 4):
+        for t in range(Nt):
+            tt = (t + tsrc) % Nt
+            sgn = -1.0 if t + tsrc >= Nt else 1.0
+            ref[t] += sgn * Minv[tt, :, tsrc, :]
+    ref /= 2
+    assert (corr - ref).abs().max() < 1e-10
