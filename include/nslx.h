@@ -150,6 +150,11 @@ typedef struct {
     double  rsqr;              /* final |r|^2 (batch sum, recursive, FP64 run) */
     double  max_rel_residual;  /* max_k |b_k - A x_k| / |b_k|, FP64, if checked */
     double  seconds;           /* host wall time of the call                    */
+    /* NSLX_TOL_REL_PER_RHS only: RHS that stopped at the attainable accuracy
+     * of an FP64 residual (|r| <= c eps ||A|| |x|, above tol |b|), and the
+     * largest such bound relative to |b| over the RHS                        */
+    int64_t floor_stops;
+    double  max_floor_rel;
 } nslx_solve_report;
 
 /* CG::operator()(b[, x0]) / BiCGStab::operator()(b[, x0]) on the operator

@@ -71,8 +71,8 @@ class Context:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            _lib.lib().nslx_ctx_destroy(h)
+        if h and _lib is not None and _lib._lib is not None:
+            _lib._lib.nslx_ctx_destroy(h)
             self._h = None
 
 
@@ -150,8 +150,8 @@ class HubbardExp:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            _lib.lib().nslx_fm_destroy(h)
+        if h and _lib is not None and _lib._lib is not None:
+            _lib._lib.nslx_fm_destroy(h)
             self._h = None
 
     def species(self):

@@ -14,16 +14,21 @@ thread_local std::string g_err;
 
 template <class F>
 int guard(F&& f) {
+    // A failed runtime call leaves its code as the thread's last error; clear
+    // it so that the next call's launch check does not report it again.
     try {
         return f();
     } catch (const nslx::Error& e) {
         g_err = e.what();
+        (void)hipGetLastError();
         return e.status;
     } catch (const std::bad_alloc& e) {
         g_err = std::string("host allocation failed: ") + e.what();
+        (void)hipGetLastError();
         return NSLX_E_OOM;
     } catch (const std::exception& e) {
         g_err = e.what();
+        (void)hipGetLastError();
         return NSLX_E_INTERNAL;
     }
 }

@@ -43,11 +43,15 @@ struct SolverWork {
     DevBuf<unsigned char> r64, t64, r32, x32;       // defect correction
     DevBuf<unsigned char> v1, v2, v3, v4, v5;       // BiCGStab extra vectors
     DevBuf<double> rr, alpha, beta, tol2, bn2, rn2, otol2, part, scal;
-    DevBuf<int> done, status;
+    DevBuf<int> done, status, want;
+    DevBuf<double> upart, rpart0, rpart1;           // fused-CG tile partials
+    std::vector<double> last_rr;                    // |r|^2 per slot after the last iteration
+    DevBuf<double> rr_ref, rr_prev, xn2, best;
+    DevBuf<int> stall;
     DevBuf<unsigned> counter;
     PinBuf<int> status_h;
     PinBuf<double> dbl_h;
-    PinBuf<int> int_h;
+    PinBuf<int> int_h, status_h2;
     void ensure_scalars(int64_t nb, int64_t nparts, hipStream_t s);
 };
 
@@ -62,6 +66,7 @@ struct FermionMatrix {
     std::map<double, std::unique_ptr<EEntry>> memo;
 
     int64_t nfield = 0;
+    double max_im_phi = 0.0;        // max |Im phi| over the populated fields
     DevBuf<double2> phi;            // [nfield][nt][nx]
     DevBuf<double2> P64, Pinv64;    // phases
     DevBuf<float2> P32;

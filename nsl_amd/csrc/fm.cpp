@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <tuple>
 
 #include "engine.hpp"
@@ -16,6 +17,8 @@ int TileTable::nb_class(int64_t nb) {
 }
 
 int TileTable::get(int prec, int nx, int nt, int64_t nb) {
+    static const int forced = (int)env_double("NSLX_TILE", -1.0);
+    if (forced >= 0 && forced < (int)tile_menu(prec).size()) return forced;
     std::lock_guard<std::mutex> g(mu);
     auto it = best.find({prec, nx, nt, nb_class(nb)});
     if (it != best.end()) return it->second;
@@ -82,6 +85,13 @@ void FermionMatrix::populate(const void* phi_dev, int64_t nf, int sp, hipStream_
     // deep copy of phi (the reference's populate copies into its own buffer)
     NSLX_HIP(hipMemcpyAsync(phi.p, phi_dev, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
     phase_tables(phi.p, n, sgn(), delta() * mu, P64.p, Pinv64.p, P32.p, s);
+    // max |Im phi| bounds |P| (used by the solvers' attainable-accuracy test)
+    std::vector<double2> h((size_t)n);
+    NSLX_HIP(hipMemcpyAsync(h.data(), phi.p, n * sizeof(double2), hipMemcpyDeviceToHost, s));
+    NSLX_HIP(hipStreamSynchronize(s));
+    double m = 0.0;
+    for (const double2& z : h) m = std::max(m, std::fabs(z.y));
+    max_im_phi = m;
 }
 
 void FermionMatrix::require_populated() const {

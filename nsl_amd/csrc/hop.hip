@@ -19,6 +19,8 @@
 #include "hop.hpp"
 #include "prof.hpp"
 
+#include <algorithm>
+
 namespace nslx {
 namespace {
 
@@ -37,18 +39,21 @@ struct PF64 {
     using L = double;      // LDS / MFMA operand scalar
     using Acc = d4;
     static constexpr int KSTEP = 4, PAD = 2, NPA = 2, NPB = 1, VEC = 2;
+    static constexpr bool ROW4V = true;
 };
 struct PF32 {
     using S = float;
     using L = float;
     using Acc = f4;
     static constexpr int KSTEP = 4, PAD = 4, NPA = 2, NPB = 1, VEC = 4;
+    static constexpr bool ROW4V = false;
 };
 struct PB3 {
     using S = float;
     using L = uint16_t;    // bf16 bits
     using Acc = f4;
     static constexpr int KSTEP = 32, PAD = 8, NPA = 4, NPB = 2, VEC = 4;
+    static constexpr bool ROW4V = false;
 };
 
 __device__ inline uint16_t f2bf(float x) {   // round to nearest even
@@ -108,108 +113,36 @@ __device__ inline int block_sum_int(int v, int* red) {
     return s;
 }
 
-// Last-block finalisation of the fused CG scalars (fixed summation order).
-template <int NT>
-__device__ void cg_finalize(const HopArgs& a, int mode, int nparts, double* red, int* redi) {
-    const CgDev& c = a.cg;
-    if (!c.global) {
-        int active = 0;
-        for (int b = threadIdx.x; b < a.nb; b += NT) {
-            if (mode != MODE_CGI && c.done[b] >= 0) continue;
-            double s = 0.0;
-            for (int k = 0; k < nparts; ++k) s += c.part[(size_t)b * nparts + k];
-            if (mode == MODE_CGA) {
-                c.alpha[b] = (s > 0.0) ? c.rr[b] / s : 0.0;
-            } else if (mode == MODE_CGB) {
-                const double r0 = c.rr[b];
-                c.beta[b] = (r0 > 0.0) ? s / r0 : 0.0;
-                c.rr[b] = s;
-                if (s <= c.tol2[b]) c.done[b] = a.iter; else ++active;
-            } else {
-                c.rr[b] = s;
-                c.beta[b] = 0.0;
-                if (s <= c.tol2[b]) c.done[b] = 0; else { c.done[b] = -1; ++active; }
-            }
-        }
-        if (mode != MODE_CGA) {
-            const int tot = block_sum_int<NT>(active, redi);
-            if (threadIdx.x == 0) { c.status[0] = tot; c.status[1] = a.iter; }
-        }
-    } else {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < a.nb; b += NT)
-            for (int k = 0; k < nparts; ++k) s += c.part[(size_t)b * nparts + k];
-        const double S = block_sum<NT>(s, red);
-        if (threadIdx.x == 0) {
-            if (mode == MODE_CGA) {
-                c.alpha[0] = (S > 0.0) ? c.rr[0] / S : 0.0;
-            } else if (mode == MODE_CGB) {
-                const double r0 = c.rr[0];
-                c.beta[0] = (r0 > 0.0) ? S / r0 : 0.0;
-                c.rr[0] = S;
-                const bool conv = S <= c.tol2[0];
-                if (conv) c.done[0] = a.iter;
-                c.status[0] = conv ? 0 : a.nb;
-                c.status[1] = a.iter;
-            } else {
-                c.rr[0] = S;
-                c.beta[0] = 0.0;
-                const bool conv = S <= c.tol2[0];
-                c.done[0] = conv ? 0 : -1;
-                c.status[0] = conv ? 0 : a.nb;
-                c.status[1] = a.iter;
-            }
-        }
-    }
-}
+// Pointers of one tile job (the field block of one problem).
+struct JobPtrs {
+    const void* E; const void* P;
+    const void* X; void* Y; const void* Pold; void* Pnew; void* Xs; void* R; const void* Bv;
+};
 
+template <class TL, class PR>
+struct Smem {
+    typename PR::L A[PR::NPA][TL::BM][TL::BK + PR::PAD];
+    typename PR::L B[PR::NPB][TL::BN][TL::BK + PR::PAD];
+};
+
+// One BM x BN output tile of rhs b (rows t0 = rt*BM.., sites y0 = ct*BN..).
+// Called by the whole workgroup; returns this thread's share of the tile's
+// reduction (|u|^2 or |r|^2 in the CG modes).
 template <class TL, class PR, int OP, int MODE>
-__global__ void __launch_bounds__(TL::NT) hop_kernel(HopArgs a, int nrt, int nct, int swz, int ldE, size_t planeE) {
+__device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, size_t planeE, int b, int rt, int ct,
+                           double alpha, double beta, Smem<TL, PR>& sm) {
     using S = typename PR::S;
     using L = typename PR::L;
     using Acc = typename PR::Acc;
     constexpr int BM = TL::BM, BN = TL::BN, BK = TL::BK, NT = TL::NT;
     constexpr int FM = TL::FM, FN = TL::FN;
-    constexpr int LDK = BK + PR::PAD;
     constexpr int RUN = BK / 4;                       // k values per lane per chunk
     static_assert(BK % 4 == 0 && RUN % (PR::KSTEP / 4) == 0, "BK");
-
-    __shared__ __attribute__((aligned(16))) L sA[PR::NPA][BM][LDK];
-    __shared__ __attribute__((aligned(16))) L sB[PR::NPB][BN][LDK];
-    __shared__ double red[NT / 64];
-    __shared__ int redi[NT / 64];
-    __shared__ int amLast;
-
-    // ---- tile decode ----
-    const int G = a.nb * nrt;
-    int g, ct;
-    if (swz) {
-        const int xcd = blockIdx.x & 7, s = blockIdx.x >> 3;
-        ct = s % nct;
-        g = (s / nct) * 8 + xcd;
-    } else {
-        ct = blockIdx.x % nct;
-        g = blockIdx.x / nct;
-    }
-    const bool vblk = g < G;
-    const int b = vblk ? g / nrt : 0;
-    const int rt = vblk ? g % nrt : 0;
-    const int nt = a.nt, nx = a.nx;
+    auto& sA = sm.A;
+    auto& sB = sm.B;
     const int t0 = rt * BM, y0 = ct * BN;
-    const int f = b / a.rpf;
-    const int nparts = nrt * nct;
-
-    bool skip = !vblk;
-    double alpha = 0.0, beta = 0.0;
-    if (MODE == MODE_CGA || MODE == MODE_CGB) {
-        const int slot = a.cg.global ? 0 : b;
-        if (vblk && a.cg.done[slot] >= 0) skip = true;
-        if (MODE == MODE_CGA) beta = a.cg.beta[slot];
-        else alpha = a.cg.alpha[slot];
-    }
-
+    const int f = b / rpf;
     double part = 0.0;
-    if (!skip) {
         const int tid = threadIdx.x;
         const int lane = tid & 63, wave = tid >> 6;
         const int wm = wave / TL::WN, wn = wave % TL::WN;
@@ -221,7 +154,7 @@ __global__ void __launch_bounds__(TL::NT) hop_kernel(HopArgs a, int nrt, int nct
             #pragma unroll
             for (int j = 0; j < FN; ++j) { accR[i][j] = Acc{0, 0, 0, 0}; accI[i][j] = Acc{0, 0, 0, 0}; }
 
-        const S* Pf = reinterpret_cast<const S*>(a.P) + (size_t)f * nt * nx * 2;
+        const S* Pf = reinterpret_cast<const S*>(J.P) + (size_t)f * nt * nx * 2;
         const size_t bbase = (size_t)b * nt * nx;
 
         // A loader geometry
@@ -252,12 +185,12 @@ __global__ void __launch_bounds__(TL::NT) hop_kernel(HopArgs a, int nrt, int nct
                         const int x = k0 + kk + e;
                         CS<S> z{S(0), S(0)};
                         if (vrow && x < nx) {
-                            z = ldc<S>(a.X, rowoff + x);
+                            z = ldc<S>(J.X, rowoff + x);
                             if (MODE == MODE_CGA) {
-                                const CS<S> po = ldc<S>(a.Pold, rowoff + x);
+                                const CS<S> po = ldc<S>(J.Pold, rowoff + x);
                                 z.re = z.re + S(beta) * po.re;
                                 z.im = z.im + S(beta) * po.im;
-                                if (ct == 0) stc<S>(a.Pnew, rowoff + x, z);
+                                if (ct == 0) stc<S>(J.Pnew, rowoff + x, z);
                             }
                             if (OP == HOP_M) {
                                 const CS<S> ph = ldc<S>(Pf, (size_t)tin * nx + x);
@@ -296,7 +229,7 @@ __global__ void __launch_bounds__(TL::NT) hop_kernel(HopArgs a, int nrt, int nct
                     #pragma unroll
                     for (int pl = 0; pl < PR::NPB; ++pl) {
                         const uint4 w = *reinterpret_cast<const uint4*>(
-                            reinterpret_cast<const L*>(a.E) + (size_t)pl * planeE + eoff);
+                            reinterpret_cast<const L*>(J.E) + (size_t)pl * planeE + eoff);
                         *reinterpret_cast<uint4*>(&sB[pl][j][kk]) = w;
                     }
                 }
@@ -370,7 +303,10 @@ __global__ void __launch_bounds__(TL::NT) hop_kernel(HopArgs a, int nrt, int nct
         for (int i = 0; i < FM; ++i) {
             #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int t = t0 + (wm * FM + i) * 16 + 4 * q + v;
+                // accumulator register v of lane (q, r16) holds row 4q+v of the
+                // 16x16 block for the f32/bf16 MFMAs, row 4v+q for f64 (its
+                // output groups have size 1)
+                const int t = t0 + (wm * FM + i) * 16 + (PR::ROW4V ? 4 * v + q : 4 * q + v);
                 if (t >= nt) continue;
                 const size_t rowoff = bbase + (size_t)t * nx;
                 #pragma unroll
@@ -390,56 +326,177 @@ __global__ void __launch_bounds__(TL::NT) hop_kernel(HopArgs a, int nrt, int nct
                     }
                     CS<S> id;
                     if (MODE == MODE_CGA) {
-                        const CS<S> rv = ldc<S>(a.X, idx);
-                        const CS<S> po = ldc<S>(a.Pold, idx);
+                        const CS<S> rv = ldc<S>(J.X, idx);
+                        const CS<S> po = ldc<S>(J.Pold, idx);
                         id.re = rv.re + S(beta) * po.re;
                         id.im = rv.im + S(beta) * po.im;
                     } else {
-                        id = ldc<S>(a.X, idx);
+                        id = ldc<S>(J.X, idx);
                     }
                     const CS<S> out{id.re - fr, id.im - fi};
                     if (MODE == MODE_PLAIN) {
-                        stc<S>(a.Y, idx, out);
+                        stc<S>(J.Y, idx, out);
                     } else if (MODE == MODE_CGA) {
-                        stc<S>(a.Y, idx, out);
+                        stc<S>(J.Y, idx, out);
                         part += double(out.re) * out.re + double(out.im) * out.im;
                     } else if (MODE == MODE_CGB) {
-                        const CS<S> pv = ldc<S>(a.Pold, idx);
-                        CS<S> xv = ldc<S>(a.Xs, idx);
-                        CS<S> rv = ldc<S>(a.R, idx);
+                        const CS<S> pv = ldc<S>(J.Pold, idx);
+                        CS<S> xv = ldc<S>(J.Xs, idx);
+                        CS<S> rv = ldc<S>(J.R, idx);
                         xv.re += S(alpha) * pv.re; xv.im += S(alpha) * pv.im;
                         rv.re -= S(alpha) * out.re; rv.im -= S(alpha) * out.im;
-                        stc<S>(a.Xs, idx, xv);
-                        stc<S>(a.R, idx, rv);
+                        stc<S>(J.Xs, idx, xv);
+                        stc<S>(J.R, idx, rv);
                         part += double(rv.re) * rv.re + double(rv.im) * rv.im;
                     } else {  // CGI
-                        const CS<S> bv = ldc<S>(a.Bv, idx);
+                        const CS<S> bv = ldc<S>(J.Bv, idx);
                         const CS<S> rv{bv.re - out.re, bv.im - out.im};
-                        stc<S>(a.R, idx, rv);
-                        stc<S>(a.Pnew, idx, rv);
+                        stc<S>(J.R, idx, rv);
+                        stc<S>(J.Pnew, idx, rv);
                         part += double(rv.re) * rv.re + double(rv.im) * rv.im;
                     }
                 }
             }
         }
+    
+    return part;
+}
+
+// Sum of n consecutive tile partials (fixed order).
+__device__ inline double sum_parts(const double* p, size_t off, size_t n) {
+    double s = 0.0;
+    for (size_t k = 0; k < n; ++k) s += p[off + k];
+    return s;
+}
+
+// Block-wide fixed-order sum of n partials (all threads get the result).
+template <int NT>
+__device__ inline double block_sum_parts(const double* p, size_t n, double* red) {
+    double s = 0.0;
+    for (size_t k = threadIdx.x; k < n; k += NT) s += p[k];
+    return block_sum<NT>(s, red);
+}
+
+// CG decisions shared by the single and the grouped kernels.  Returns true if
+// the tile's right-hand side is skipped; sets beta (CGA) or alpha (CGB).
+// rr1 = |r_{i-1}|^2, rr2 = |r_{i-2}|^2, uu = |u_i|^2 of the slot.
+template <int MODE>
+__device__ inline bool cg_decide(const CgDev& c, int slot, int iter, bool designated, double rr1, double rr2,
+                                 double uu, double& alpha, double& beta) {
+    if (c.done[slot] >= 0) return true;
+    if (MODE == MODE_CGA) {
+        if (c.ru) {
+            if (designated) c.want[slot] = (rr1 <= c.delta2 * c.rr_ref[slot] || rr1 <= c.tol2[slot]) ? 1 : 0;
+        } else if (rr1 <= c.tol2[slot]) {
+            if (designated) c.done[slot] = iter - 1;
+            return true;
+        }
+        beta = (iter >= 2 && rr2 > 0.0) ? rr1 / rr2 : 0.0;
+    } else if (MODE == MODE_CGB) {
+        alpha = (uu > 0.0) ? rr1 / uu : 0.0;
     }
+    return false;
+}
 
-    if (MODE == MODE_PLAIN) return;
+template <class TL, class PR, int OP, int MODE>
+__global__ void __launch_bounds__(TL::NT) hop_kernel(HopArgs a, int nrt, int nct, int swz, int ldE, size_t planeE) {
+    constexpr int NT = TL::NT;
+    __shared__ __attribute__((aligned(16))) Smem<TL, PR> sm;
+    __shared__ double red[NT / 64];
 
-    // ---- partial sum of this tile, then last-block finalisation ----
+    // ---- tile decode ----
+    const int G = a.nb * nrt;
+    int g, ct;
+    if (swz) {
+        const int xcd = blockIdx.x & 7, s = blockIdx.x >> 3;
+        ct = s % nct;
+        g = (s / nct) * 8 + xcd;
+    } else {
+        ct = blockIdx.x % nct;
+        g = blockIdx.x / nct;
+    }
+    const bool vblk = g < G;
+    if (!vblk) return;
+    const int b = g / nrt;
+    const int rt = g % nrt;
+    const int nparts = nrt * nct;
+
+    bool skip = false;
+    double alpha = 0.0, beta = 0.0;
+    if (MODE == MODE_CGA || MODE == MODE_CGB) {
+        const CgDev& c = a.cg;
+        const int it = a.iter;
+        double rr1, rr2 = 0.0, uu = 0.0;
+        if (c.global) {
+            const size_t n = (size_t)a.nb * nparts;
+            rr1 = block_sum_parts<NT>(c.rpart[(it - 1) & 1], n, red);
+            if (MODE == MODE_CGA && it >= 2) rr2 = block_sum_parts<NT>(c.rpart[(it - 2) & 1], n, red);
+            if (MODE == MODE_CGB) uu = block_sum_parts<NT>(c.upart, n, red);
+        } else {
+            const size_t off = (size_t)b * nparts;
+            rr1 = sum_parts(c.rpart[(it - 1) & 1], off, nparts);
+            if (MODE == MODE_CGA && it >= 2) rr2 = sum_parts(c.rpart[(it - 2) & 1], off, nparts);
+            if (MODE == MODE_CGB) uu = sum_parts(c.upart, off, nparts);
+        }
+        const int slot = c.global ? 0 : b;
+        const bool designated = rt == 0 && ct == 0 && (!c.global || b == 0);
+        skip = cg_decide<MODE>(c, slot, it, designated, rr1, rr2, uu, alpha, beta);
+    }
+    double part = 0.0;
+    if (!skip) {
+        const JobPtrs J{a.E, a.P, a.X, a.Y, a.Pold, a.Pnew, a.Xs, a.R, a.Bv};
+        part = hop_tile<TL, PR, OP, MODE>(J, a.nx, a.nt, a.rpf, ldE, planeE, b, rt, ct, alpha, beta, sm);
+    }
+    if (MODE == MODE_PLAIN || skip) return;
+    // ---- this tile's partial sum, reduced by the next launch ----
     const double tot = block_sum<NT>(part, red);
-    if (threadIdx.x == 0 && vblk) a.cg.part[(size_t)b * nparts + rt * nct + ct] = tot;
-    __threadfence();
-    __syncthreads();
     if (threadIdx.x == 0) {
-        const unsigned prev = atomicAdd(a.cg.counter, 1u);
-        amLast = (prev == gridDim.x - 1) ? 1 : 0;
+        double* dst = MODE == MODE_CGA ? a.cg.upart : a.cg.rpart[MODE == MODE_CGB ? (a.iter & 1) : 0];
+        dst[(size_t)b * nparts + rt * nct + ct] = tot;
     }
-    __syncthreads();
-    if (amLast) {
-        __threadfence();
-        cg_finalize<NT>(a, MODE, nparts, red, redi);
-        if (threadIdx.x == 0) *a.cg.counter = 0u;
+}
+
+// Persistent grouped launch: workgroups stride over a device tile list that
+// spans many problems (each with its own nx, nt, E, phases and vectors).
+// Per-RHS decisions only (no batch-global mode).
+template <class TL, class PR, int OP, int MODE>
+__global__ void __launch_bounds__(TL::NT) hop_grouped(GroupArgs g) {
+    constexpr int NT = TL::NT;
+    __shared__ __attribute__((aligned(16))) Smem<TL, PR> sm;
+    __shared__ double red[NT / 64];
+    for (int t = blockIdx.x; t < g.ntiles; t += gridDim.x) {
+        const int4 T = g.tiles[t];
+        const HopProblem& P = g.probs[T.x];
+        const int b = T.y, rt = T.z, ct = T.w;
+        const int slot = P.rhs0 + b;
+        const int nparts = P.nrt * P.nct;
+        const size_t off = (size_t)P.part0 + (size_t)b * nparts;
+        bool skip = false;
+        double alpha = 0.0, beta = 0.0;
+        if (MODE == MODE_CGA || MODE == MODE_CGB) {
+            const CgDev& c = g.cg;
+            const double rr1 = sum_parts(c.rpart[(g.iter - 1) & 1], off, nparts);
+            const double rr2 = (MODE == MODE_CGA && g.iter >= 2) ? sum_parts(c.rpart[(g.iter - 2) & 1], off, nparts) : 0.0;
+            const double uu = MODE == MODE_CGB ? sum_parts(c.upart, off, nparts) : 0.0;
+            skip = cg_decide<MODE>(c, slot, g.iter, rt == 0 && ct == 0, rr1, rr2, uu, alpha, beta);
+        }
+        double part = 0.0;
+        if (!skip) {
+            auto v = [&](int r) -> void* { return r >= 0 ? P.v[r] : nullptr; };
+            const JobPtrs J{P.E, P.P, v(g.role[0]), v(g.role[1]), v(g.role[2]), v(g.role[3]),
+                            v(g.role[4]), v(g.role[5]), v(g.role[6])};
+            part = hop_tile<TL, PR, OP, MODE>(J, P.nx, P.nt, P.rpf, P.ldE, (size_t)P.planeE, b, rt, ct, alpha,
+                                              beta, sm);
+        }
+        if (MODE != MODE_PLAIN && !skip) {
+            const double tot = block_sum<NT>(part, red);
+            if (threadIdx.x == 0) {
+                double* dst = MODE == MODE_CGA ? g.cg.upart : g.cg.rpart[MODE == MODE_CGB ? (g.iter & 1) : 0];
+                dst[off + rt * P.nct + ct] = tot;
+            }
+        } else {
+            __syncthreads();   // LDS reuse by the next tile
+        }
     }
 }
 
@@ -498,7 +555,80 @@ void dispatch4(int tile, int op, int mode, const HopArgs& a, hipStream_t s) {
     }
 }
 
+template <class TL, class PR>
+void launch_g(int op, int mode, const GroupArgs& g, int grid, hipStream_t s) {
+    dim3 gr(grid), bl(TL::NT);
+#define NSLX_HOPG_CASE(OPV, MV)                                                    \
+    if (op == OPV && mode == MV) {                                                 \
+        hipLaunchKernelGGL((hop_grouped<TL, PR, OPV, MV>), gr, bl, 0, s, g);        \
+        return;                                                                    \
+    }
+    NSLX_HOPG_CASE(HOP_M, MODE_PLAIN)
+    NSLX_HOPG_CASE(HOP_MD, MODE_PLAIN)
+    NSLX_HOPG_CASE(HOP_M, MODE_CGA)
+    NSLX_HOPG_CASE(HOP_MD, MODE_CGA)
+    NSLX_HOPG_CASE(HOP_M, MODE_CGB)
+    NSLX_HOPG_CASE(HOP_MD, MODE_CGB)
+    NSLX_HOPG_CASE(HOP_M, MODE_CGI)
+    NSLX_HOPG_CASE(HOP_MD, MODE_CGI)
+#undef NSLX_HOPG_CASE
+    throw Error(NSLX_E_ARG, "hop: bad op/mode");
+}
+
+template <class PR, class T0, class T1, class T2, class T3>
+void dispatch4g(int tile, int op, int mode, const GroupArgs& g, int grid, hipStream_t s) {
+    switch (tile) {
+        case 0: launch_g<T0, PR>(op, mode, g, grid, s); break;
+        case 1: launch_g<T1, PR>(op, mode, g, grid, s); break;
+        case 2: launch_g<T2, PR>(op, mode, g, grid, s); break;
+        case 3: launch_g<T3, PR>(op, mode, g, grid, s); break;
+        default: throw Error(NSLX_E_ARG, "hop: bad tile id");
+    }
+}
+
 }  // namespace
+
+void launch_hop_grouped(int prec, int op, int mode, int tile, const GroupArgs& g, int grid, hipStream_t s) {
+    NSLX_REQUIRE(g.nprob > 0 && g.ntiles > 0 && grid > 0, "hop: empty grouped launch");
+    ProfScope prof(op == HOP_M ? "grp_M" : "grp_Mdag", prec, mode, s);
+    if (prec == NSLX_F64) dispatch4g<PF64, T64_a, T64_b, T64_c, T64_d>(tile, op, mode, g, grid, s);
+    else if (prec == NSLX_F32) dispatch4g<PF32, T32_a, T32_b, T32_c, T32_d>(tile, op, mode, g, grid, s);
+    else if (prec == NSLX_BF16X3) dispatch4g<PB3, TB3_a, TB3_b, TB3_c, TB3_d>(tile, op, mode, g, grid, s);
+    else throw Error(NSLX_E_ARG, "hop: bad precision");
+    NSLX_LAUNCH_CHECK();
+}
+
+std::vector<int4> build_tile_list(std::vector<HopProblem>& probs, int prec, int tile) {
+    const TileInfo& ti = tile_menu(prec).at(tile);
+    std::vector<int> order(probs.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return probs[a].nx > probs[b].nx; });
+    // row groups in cost order, each group's nct column tiles
+    std::vector<std::vector<int4>> groups;
+    for (int p : order) {
+        HopProblem& P = probs[p];
+        P.nrt = (P.nt + ti.bm - 1) / ti.bm;
+        P.nct = (P.nx + ti.bn - 1) / ti.bn;
+        P.ldE = e_ld(P.nx);
+        P.planeE = (long long)e_rows(P.nx) * P.ldE;
+        for (int b = 0; b < P.nb; ++b)
+            for (int rt = 0; rt < P.nrt; ++rt) {
+                std::vector<int4> gtiles;
+                for (int ct = 0; ct < P.nct; ++ct) gtiles.push_back(make_int4(p, b, rt, ct));
+                groups.push_back(std::move(gtiles));
+            }
+    }
+    // interleave 8 groups at a time: entry 8*(j*maxct + ct) + i for group i of a block of 8
+    std::vector<int4> out;
+    for (size_t g0 = 0; g0 < groups.size(); g0 += 8) {
+        size_t maxct = 0;
+        for (size_t i = g0; i < std::min(groups.size(), g0 + 8); ++i) maxct = std::max(maxct, groups[i].size());
+        for (size_t ct = 0; ct < maxct; ++ct)
+            for (size_t i = g0; i < std::min(groups.size(), g0 + 8); ++i)
+                if (ct < groups[i].size()) out.push_back(groups[i][ct]);
+    }
+    return out;
+}
 
 const std::vector<TileInfo>& tile_menu(int prec) {
     static const std::vector<TileInfo> m64 = {info<T64_a>(0), info<T64_b>(1), info<T64_c>(2), info<T64_d>(3)};

@@ -17,27 +17,35 @@ namespace nslx {
 
 enum HopOp { HOP_M = 0, HOP_MD = 1 };
 
+// Iteration i of the fused CG on A = outer(inner(.)) is two launches:
+// CGA_i: rr_{i-1} = sum of the |r|^2 tile partials of CGB_{i-1} (rpart[(i-1)&1]),
+//        rr_{i-2} likewise; stop b if rr_{i-1} <= tol2 (not in reliable-update
+//        mode); beta = rr_{i-1}/rr_{i-2} (0 for i = 1); p_new = X(=r) +
+//        beta p_old (written once, by column tile 0); Y(=u) = inner(p_new);
+//        |u|^2 tile partials -> upart.
+// CGB_i: alpha = rr_{i-1} / sum(upart); t = outer(X(=u)) in registers;
+//        x += alpha p; r -= alpha t; |r|^2 tile partials -> rpart[i&1].
+// CGI:   t = outer(X(=u)); r = b - t; p = r; |r|^2 partials -> rpart[0].
 // PLAIN: Y = op(X).
-// CGA:   p_new = X(=r) + beta_b * p_old (written once, by column tile 0),
-//        Y(=u) = op(p_new), partial |u|^2; last block: alpha_b = rr_b/|u_b|^2.
-// CGB:   t = op(X(=u)) kept in registers; x += alpha_b p; r -= alpha_b t;
-//        partial |r|^2; last block: beta_b, rr_b, done_b.
-// CGI:   t = op(X(=u)); r = b - t; p = r; partial |r|^2; last block: rr_b,
-//        done_b, beta_b = 0.
+// Each workgroup reduces the few partials of its own right-hand side in a
+// fixed order, so no cross-workgroup synchronisation is needed inside a launch.
 enum HopMode { MODE_PLAIN = 0, MODE_CGA = 1, MODE_CGB = 2, MODE_CGI = 3 };
 
-// Device-side scalar state of the fused CG (one slot per right-hand side; in
-// batch-global mode slot 0 carries the batch value).
+// Device-side state of the fused CG (one slot per right-hand side; in
+// batch-global mode slot 0 carries the batch decision).
 struct CgDev {
-    double* rr = nullptr;       // current |r|^2
-    double* alpha = nullptr;
-    double* beta = nullptr;
-    double* tol2 = nullptr;     // stop when rr <= tol2
-    int* done = nullptr;        // -1 active, else iteration count at convergence
-    int* status = nullptr;      // [0] active rhs, [1] last iteration finalised
-    double* part = nullptr;     // per-tile partial sums [nb][nparts]
-    unsigned* counter = nullptr;// last-block counter, zero between launches
+    double* upart = nullptr;                 // [nb][nparts] |u|^2 tile partials
+    double* rpart[2] = {nullptr, nullptr};   // ping-pong |r|^2 tile partials
+    double* tol2 = nullptr;                  // stop when rr <= tol2
+    int* done = nullptr;                     // -1 active, else the converging iteration
     int global = 0;
+    // reliable-update mode: CGA does not stop a right-hand side; it sets
+    // want[b] = 1 once rr_{i-1} <= delta2 * rr_ref[b] (rr_ref: |r|^2 of the last
+    // FP64 residual replacement) or rr_{i-1} <= tol2[b].
+    int ru = 0;
+    double delta2 = 0.01;
+    double* rr_ref = nullptr;
+    int* want = nullptr;
 };
 
 struct HopArgs {
@@ -51,6 +59,34 @@ struct HopArgs {
     void* Xs = nullptr;
     void* R = nullptr;
     const void* Bv = nullptr;
+    CgDev cg;
+    int iter = 0;
+};
+
+// One problem of a grouped launch: a field block with its own lattice size,
+// E, phases and Krylov vectors (vector slots v[], selected per launch by
+// GroupArgs::role).  nrt/nct must match the launch's tile shape.  rhs0 and
+// part0 offset this problem's right-hand sides in the CG scalar arrays and
+// its tiles in the partial-sum array.
+struct HopProblem {
+    int nx = 0, nt = 0, nb = 0, rpf = 1;
+    int nrt = 0, nct = 0, ldE = 0, rhs0 = 0;
+    long long planeE = 0, part0 = 0;
+    const void* E = nullptr;
+    const void* P = nullptr;
+    void* v[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+};
+
+// Vector slots of HopProblem::v used by the batched solver.
+enum VecSlot { VS_R = 0, VS_P0 = 1, VS_P1 = 2, VS_U = 3, VS_X = 4, VS_B = 5, VS_IN = 6, VS_OUT = 7 };
+
+struct GroupArgs {
+    const HopProblem* probs = nullptr;   // device array
+    int nprob = 0;
+    const int4* tiles = nullptr;         // device array of (problem, b, rt, ct)
+    int ntiles = 0;
+    // slot index (or -1) for the roles X, Y, Pold, Pnew, Xs, R, Bv
+    signed char role[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
     CgDev cg;
     int iter = 0;
 };
@@ -70,6 +106,13 @@ struct TileInfo {
 // Launch one hop.  prec: NSLX_F64 / NSLX_F32 / NSLX_BF16X3; op: HOP_M/HOP_MD;
 // tile: index into tile_menu(prec).  Throws on bad arguments.
 void launch_hop(int prec, int op, int mode, const HopArgs& a, int tile, hipStream_t s);
+// Grouped persistent launch over g.ntiles tiles with `grid` workgroups.
+void launch_hop_grouped(int prec, int op, int mode, int tile, const GroupArgs& g, int grid, hipStream_t s);
+// Host-side tile list for problems (nrt/nct filled in from the tile shape):
+// problems with larger nx first (longer K loops), and the column tiles of a
+// row block 8 entries apart so that a persistent grid that is a multiple of 8
+// keeps them on one XCD.
+std::vector<int4> build_tile_list(std::vector<HopProblem>& probs, int prec, int tile);
 const std::vector<TileInfo>& tile_menu(int prec);
 int hop_nparts(int prec, int tile, int nx, int nt);
 // Heuristic tile choice when no tuned entry exists.

@@ -6,10 +6,11 @@
 //   the per-RHS stop flags stay on the device; the host polls a status word
 //   every few iterations.  Iterations after convergence are no-ops, so the
 //   returned x is the iterate of the converging step, as in CG.tpp:70-73.
-// * Defect correction (precision != F64): the FP64 true residual drives an
-//   inner CG in FP32 or BF16x3 on the normalised residual; x is accumulated
-//   in FP64 (the inner/outer pattern of CGpreconditioned.tpp:14-120 with a
-//   low-precision inner solve of the same operator).
+// * Per-RHS relative tolerance (every precision): CG with FP64 reliable
+//   updates -- the recurrence runs in FP32 / BF16x3 / FP64 storage while x
+//   and the true residual are periodically recomputed in FP64 (the
+//   inner/outer correction pattern of CGpreconditioned.tpp:14-120, without
+//   restarting the Krylov process).  The stop test is the FP64 TRUE residual.
 // * BiCGStab and CG on the non-Hermitian single hops: device dots with the
 //   scalars formed on the host (as the reference, one sync per inner product).
 #include <algorithm>
@@ -21,11 +22,15 @@
 namespace nslx {
 
 void SolverWork::ensure_scalars(int64_t nb, int64_t nparts, hipStream_t s) {
+    // complex scalars of the generic solvers need 2 doubles per slot
     const size_t n = (size_t)std::max<int64_t>(nb, 1);
-    rr.ensure(n); alpha.ensure(n); beta.ensure(n); tol2.ensure(n); bn2.ensure(n); rn2.ensure(n);
+    rr.ensure(2 * n); alpha.ensure(2 * n); beta.ensure(2 * n); tol2.ensure(n); bn2.ensure(n); rn2.ensure(n);
     otol2.ensure(n); scal.ensure(16 * n + 16);
     part.ensure(n * std::max<int64_t>(nparts, 1) + 16);
-    done.ensure(n); status.ensure(4);
+    upart.ensure(n * std::max<int64_t>(nparts, 1) + 16);
+    rpart0.ensure(n * std::max<int64_t>(nparts, 1) + 16);
+    rpart1.ensure(n * std::max<int64_t>(nparts, 1) + 16);
+    done.ensure(n); status.ensure(4); want.ensure(n);
     if (!counter.p) {
         counter.alloc(4);
         NSLX_HIP(hipMemsetAsync(counter.p, 0, 4 * sizeof(unsigned), s));
@@ -44,28 +49,39 @@ inline int poll_every() {
 
 CgDev make_cg(SolverWork& w, int global) {
     CgDev c;
-    c.rr = w.rr.p; c.alpha = w.alpha.p; c.beta = w.beta.p; c.tol2 = w.tol2.p; c.done = w.done.p;
-    c.status = w.status.p; c.part = w.part.p; c.counter = w.counter.p; c.global = global;
+    c.upart = w.upart.p;
+    c.rpart[0] = w.rpart0.p;
+    c.rpart[1] = w.rpart1.p;
+    c.tol2 = w.tol2.p;
+    c.done = w.done.p;
+    c.want = w.want.p;
+    c.global = global;
     return c;
 }
 
-// Fused CG on A = outer(inner(.)).  tol2 must be set.  x holds x0 unless
-// x0_zero.  Buffers r, p0, p1, u: nb*len elements of the storage type.
-// Returns the number of launched iterations; per-RHS iteration counts in done_h.
-int64_t cg_fused(FermionMatrix& fm, int prec, int inner, int outer, const void* b, void* x, bool x0_zero,
-                 int64_t nb, int global, int64_t maxit, void* r, void* p0, void* p1, void* u, hipStream_t s,
-                 std::vector<int>& done_h) {
+// Number of RHS (or batch slots) still active: done < 0 on the device.
+int64_t count_active(SolverWork& w, int64_t nslot, hipStream_t s) {
+    NSLX_HIP(hipMemcpyAsync(w.int_h.p, w.done.p, nslot * sizeof(int), hipMemcpyDeviceToHost, s));
+    NSLX_HIP(hipStreamSynchronize(s));
+    int64_t a = 0;
+    for (int64_t i = 0; i < nslot; ++i) a += w.int_h.p[i] < 0 ? 1 : 0;
+    return a;
+}
+
+// Fused CG on A = outer(inner(.)) with the reference's recurrence from x0
+// (held in x).  tol2 must be set.  Buffers r, p0, p1, u: nb*len elements.
+// Returns the number of launched iterations; per-slot iteration counts in done_h.
+int64_t cg_fused(FermionMatrix& fm, int prec, int inner, int outer, const void* b, void* x, int64_t nb, int global,
+                 int64_t maxit, void* r, void* p0, void* p1, void* u, hipStream_t s, std::vector<int>& done_h) {
     SolverWork& w = fm.sw;
     const int tile = fm.tile(prec, nb);
     const int nparts = hop_nparts(prec, tile, fm.nx, fm.nt);
     w.ensure_scalars(nb, nparts, s);
     HopArgs a = fm.hop_args(prec, nb, s);
     a.cg = make_cg(w, global);
-    const int f64 = prec == NSLX_F64;
-
-    if (x0_zero) {
-        cg_init_zero(f64, b, x, r, p0, nb, fm.len(), a.cg, fm.vsc, s);
-    } else {
+    const int64_t nslot = global ? 1 : nb;
+    NSLX_HIP(hipMemsetAsync(w.done.p, 0xFF, nslot * sizeof(int), s));   // all active (-1)
+    {
         HopArgs h = a;
         h.X = x;
         h.Y = u;
@@ -82,10 +98,6 @@ int64_t cg_fused(FermionMatrix& fm, int prec, int inner, int outer, const void* 
     const int chk = poll_every();
     int64_t it = 0;
     bool finished = false;
-    // status after init
-    NSLX_HIP(hipMemcpyAsync(w.status_h.p, w.status.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-    NSLX_HIP(hipStreamSynchronize(s));
-    if (w.status_h.p[0] == 0) finished = true;
     while (!finished && it < maxit) {
         ++it;
         HopArgs A = a;
@@ -102,17 +114,24 @@ int64_t cg_fused(FermionMatrix& fm, int prec, int inner, int outer, const void* 
         B.R = r;
         B.iter = (int)it;
         launch_hop(prec, outer, MODE_CGB, B, tile, s);
-        if (it % chk == 0 || it == maxit) {
-            NSLX_HIP(hipMemcpyAsync(w.status_h.p, w.status.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-            NSLX_HIP(hipStreamSynchronize(s));
-            if (w.status_h.p[0] == 0) finished = true;
-        }
+        if (it % chk == 0 || it == maxit) finished = count_active(w, nslot, s) == 0;
     }
-    const int64_t nslot = global ? 1 : nb;
     done_h.assign(nslot, -1);
     NSLX_HIP(hipMemcpyAsync(w.int_h.p, w.done.p, nslot * sizeof(int), hipMemcpyDeviceToHost, s));
     NSLX_HIP(hipStreamSynchronize(s));
     for (int64_t i = 0; i < nslot; ++i) done_h[i] = w.int_h.p[i];
+    // the stop test of iteration `it` is made by the next CGA; make it here
+    const size_t np = (size_t)nb * nparts;
+    std::vector<double> rp(np), t2(nslot);
+    NSLX_HIP(hipMemcpyAsync(rp.data(), (it & 1) ? w.rpart1.p : w.rpart0.p, np * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+    NSLX_HIP(hipMemcpyAsync(t2.data(), w.tol2.p, nslot * sizeof(double), hipMemcpyDeviceToHost, s));
+    NSLX_HIP(hipStreamSynchronize(s));
+    w.last_rr.assign(nslot, 0.0);
+    for (int64_t b2 = 0; b2 < nb; ++b2)
+        for (int k = 0; k < nparts; ++k) w.last_rr[global ? 0 : b2] += rp[(size_t)b2 * nparts + k];
+    for (int64_t i = 0; i < nslot; ++i)
+        if (done_h[i] < 0 && w.last_rr[i] <= t2[i]) done_h[i] = (int)it;
     return it;
 }
 
@@ -120,14 +139,6 @@ void copy_dev(void* dst, const void* src, size_t bytes, hipStream_t s) {
     NSLX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
 }
 
-double host_rsqr(SolverWork& w, int64_t nslot, hipStream_t s, double* maxv = nullptr) {
-    NSLX_HIP(hipMemcpyAsync(w.dbl_h.p, w.rr.p, nslot * sizeof(double), hipMemcpyDeviceToHost, s));
-    NSLX_HIP(hipStreamSynchronize(s));
-    double t = 0.0, m = 0.0;
-    for (int64_t i = 0; i < nslot; ++i) { t += w.dbl_h.p[i]; m = std::max(m, w.dbl_h.p[i]); }
-    if (maxv) *maxv = m;
-    return t;
-}
 
 // max_k |b_k - A x_k| / |b_k| in FP64
 double true_rel_residual(FermionMatrix& fm, int op, const void* b, const void* x, int64_t nb, hipStream_t s) {
@@ -171,8 +182,8 @@ int solve_cg_f64(FermionMatrix& fm, int op, const void* b, void* x, const void* 
     const int inner = op == NSLX_OP_MMDAGGER ? HOP_MD : HOP_M;
     const int outer = op == NSLX_OP_MMDAGGER ? HOP_M : HOP_MD;
     std::vector<int> done;
-    const int64_t it = cg_fused(fm, NSLX_F64, inner, outer, b, x, false, nb, global, maxit, w.r.p, w.p0.p, w.p1.p,
-                                w.u.p, s, done);
+    const int64_t it = cg_fused(fm, NSLX_F64, inner, outer, b, x, nb, global, maxit, w.r.p, w.p0.p, w.p1.p, w.u.p, s,
+                                done);
     int64_t conv = 0, maxi = 0;
     for (int d : done) {
         if (d >= 0) { ++conv; maxi = std::max<int64_t>(maxi, d); }
@@ -184,78 +195,201 @@ int solve_cg_f64(FermionMatrix& fm, int op, const void* b, void* x, const void* 
         rep->iterations_total = rep->iterations;
         rep->outer_steps = 0;
         rep->converged_rhs = conv;
-        rep->rsqr = host_rsqr(w, global ? 1 : nb, s);
+        double t = 0.0;
+        for (double v : w.last_rr) t += v;
+        rep->rsqr = t;
     }
     return all ? NSLX_OK : NSLX_NOT_CONVERGED;
 }
 
-// ------------------------------------------------- defect correction (mixed)
-int solve_cg_mixed(FermionMatrix& fm, int op, int prec, const void* b, void* x, const void* x0, int64_t nb,
-                   double tol, int64_t maxit, nslx_solve_report* rep, hipStream_t s) {
-    SolverWork& w = fm.sw;
-    const int64_t len = fm.len();
-    const size_t b64 = (size_t)nb * len * 16, b32 = (size_t)nb * len * 8;
-    w.r64.ensure(b64); w.t64.ensure(b64);
-    w.r32.ensure(b32); w.x32.ensure(b32);
-    w.r.ensure(b32); w.p0.ensure(b32); w.p1.ensure(b32); w.u.ensure(b32);
-    w.ensure_scalars(nb, 1, s);
-    const double tau = env_double(prec == NSLX_F32 ? "NSLX_INNER_TOL_F32" : "NSLX_INNER_TOL_BF16", prec == NSLX_F32 ? 3e-6 : 3e-5);
-    const int max_outer = (int)env_double("NSLX_MAX_OUTER", 30.0);
-    const int inner = op == NSLX_OP_MMDAGGER ? HOP_MD : HOP_M;
-    const int outer = op == NSLX_OP_MMDAGGER ? HOP_M : HOP_MD;
+// ------------------------------- CG with FP64 reliable updates (per RHS)
+// The CG recurrence runs in the "low" storage (c64 for F32/BF16X3, c128 for
+// F64) on the fused hop kernels; x and the true residual live in FP64.  When
+// a fraction frac of the active RHS has reduced its recursive |r|^2 by
+// delta^2 since the last replacement (or reached tol^2), or maxgap iterations
+// passed, the low-storage increment is folded into x, r = b - A x is
+// recomputed in FP64 and replaces the recursive residual, while the search
+// direction p is kept (residual replacement: Sleijpen & van der Vorst 1996;
+// van der Vorst & Ye 2000), so the Krylov process is not restarted.  A RHS
+// also stops at the attainable accuracy of an FP64 residual,
+// |r| <= c eps ||A|| |x| (no FP64 vector x has a smaller true residual; the
+// near-singular MM^dagger of random fields at large beta puts this above
+// 1e-12 |b|).  Otherwise a RHS stops once its FP64 TRUE relative
+// residual is <= tol.
+struct RuPhase {
+    FermionMatrix& fm;
+    int op;
+    const void* b;
+    void* x;
+    int64_t nb;
+    double floor2;
+    double delta, frac;
+    int64_t maxgap;
+    int chk;
+    hipStream_t s;
+    int64_t it = 0;      // iterations so far (all phases)
+    int64_t nupd = 0;    // FP64 residual replacements so far
+    int64_t active = 0, waiting = 0;
 
-    norm2_perb(1, b, nb, len, w.bn2.p, fm.vsc, s);
-    set_tol2(w.bn2.p, w.otol2.p, nb, tol, 1, 0, s);
-    bool xzero = x0 == nullptr;
-    if (xzero) NSLX_HIP(hipMemsetAsync(x, 0, b64, s));
-    else copy_dev(x, x0, b64, s);
+    // host copy of done[] / want[]: active RHS and those not yet asking for
+    // a residual replacement
+    void read_flags() {
+        SolverWork& w = fm.sw;
+        NSLX_HIP(hipMemcpyAsync(w.int_h.p, w.done.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        NSLX_HIP(hipMemcpyAsync(w.status_h2.p, w.want.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        NSLX_HIP(hipStreamSynchronize(s));
+        active = waiting = 0;
+        for (int64_t i = 0; i < nb; ++i)
+            if (w.int_h.p[i] < 0) { ++active; if (!w.status_h2.p[i]) ++waiting; }
+    }
 
-    int64_t total_inner = 0, outer_steps = 0, conv = 0;
-    double maxrel = 0.0;
-    std::vector<int> done;
-    std::vector<double> bn2h(nb), rn2h(nb);
-    NSLX_HIP(hipMemcpyAsync(w.dbl_h.p, w.bn2.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
-    NSLX_HIP(hipStreamSynchronize(s));
-    for (int64_t i = 0; i < nb; ++i) bn2h[i] = w.dbl_h.p[i];
-    for (int step = 0;; ++step) {
-        if (xzero) {
-            copy_dev(w.r64.p, b, b64, s);
-            copy_dev(w.rn2.p, w.bn2.p, nb * sizeof(double), s);
+    // FP64 true residual of the current x for every RHS (and |x|^2).
+    void true_residual(bool x_is_zero) {
+        SolverWork& w = fm.sw;
+        const int64_t len = fm.len();
+        if (x_is_zero) {
+            NSLX_HIP(hipMemcpyAsync(w.r64.p, b, (size_t)nb * len * 16, hipMemcpyDeviceToDevice, s));
+            NSLX_HIP(hipMemcpyAsync(w.rn2.p, w.bn2.p, nb * sizeof(double), hipMemcpyDeviceToDevice, s));
         } else {
             fm.apply(op, x, w.t64.p, nb, NSLX_F64, s);
             resid64(b, w.t64.p, w.r64.p, nb, len, w.rn2.p, fm.vsc, s);
         }
-        NSLX_HIP(hipMemcpyAsync(w.dbl_h.p, w.rn2.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
-        NSLX_HIP(hipStreamSynchronize(s));
-        conv = 0;
-        maxrel = 0.0;
-        for (int64_t i = 0; i < nb; ++i) {
-            rn2h[i] = w.dbl_h.p[i];
-            const double rel = bn2h[i] > 0 ? std::sqrt(rn2h[i] / bn2h[i]) : std::sqrt(rn2h[i]);
-            maxrel = std::max(maxrel, rel);
-            if (rn2h[i] <= tol * tol * bn2h[i]) ++conv;
+        norm2_perb(1, x, nb, len, w.xn2.p, fm.vsc, s);
+    }
+
+    // One reliable-update CG phase in precision prec over the RHS with
+    // done < 0 or done == RU_ESCALATE (a fresh recurrence from the current x).
+    void run(int prec, bool x_is_zero, int stall_max, int64_t maxit) {
+        SolverWork& w = fm.sw;
+        const int64_t len = fm.len();
+        const int low64 = prec == NSLX_F64;
+        const size_t blo = (size_t)nb * len * (low64 ? 16 : 8);
+        w.r.ensure(blo); w.p0.ensure(blo); w.p1.ensure(blo); w.u.ensure(blo); w.x32.ensure(blo);
+        const int tile = fm.tile(prec, nb);
+        const int nparts = hop_nparts(prec, tile, fm.nx, fm.nt);
+        w.ensure_scalars(nb, nparts, s);
+        w.status_h2.ensure(nb + 4);
+        CgDev cg = make_cg(w, 0);
+        cg.ru = 1;
+        cg.delta2 = delta * delta;
+        cg.rr_ref = w.rr_ref.p;
+        const int inner = op == NSLX_OP_MMDAGGER ? HOP_MD : HOP_M;
+        const int outer = op == NSLX_OP_MMDAGGER ? HOP_M : HOP_MD;
+
+        true_residual(x_is_zero);
+        ru_residual_to_low(low64, w.r64.p, w.r.p, w.p0.p, w.x32.p, 1, nb, len, s);
+        ru_scalars(w.rn2.p, w.xn2.p, floor2, cg, nb, nparts, 0, (int)it, 1, w.best.p, w.stall.p, stall_max, s);
+        read_flags();
+        bool finished = active == 0;
+        HopArgs a = fm.hop_args(prec, nb, s);
+        a.cg = cg;
+        void* pb[2] = {w.p0.p, w.p1.p};
+        int64_t k = 0, last_upd = 0;
+        while (!finished && k < maxit) {
+            ++k;
+            ++it;
+            HopArgs A = a;
+            A.X = w.r.p;
+            A.Pold = pb[(k - 1) & 1];
+            A.Pnew = pb[k & 1];
+            A.Y = w.u.p;
+            A.iter = (int)k;
+            launch_hop(prec, inner, MODE_CGA, A, tile, s);
+            HopArgs B = a;
+            B.X = w.u.p;
+            B.Pold = pb[k & 1];
+            B.Xs = w.x32.p;
+            B.R = w.r.p;
+            B.iter = (int)k;
+            launch_hop(prec, outer, MODE_CGB, B, tile, s);
+            if (k % chk == 0 || k == maxit) {
+                read_flags();
+                if (waiting <= (1.0 - frac) * active || k - last_upd >= maxgap || k == maxit) {
+                    ru_accumulate(low64, x, w.x32.p, nb, len, s);
+                    true_residual(false);
+                    ru_scalars(w.rn2.p, w.xn2.p, floor2, cg, nb, nparts, (int)(k & 1), (int)it, 0, w.best.p,
+                               w.stall.p, stall_max, s);
+                    ru_residual_to_low(low64, w.r64.p, w.r.p, nullptr, nullptr, 0, nb, len, s);
+                    ++nupd;
+                    last_upd = k;
+                    read_flags();
+                    finished = active == 0;
+                }
+            }
         }
-        if (conv == nb || step >= max_outer) break;
-        ++outer_steps;
-        scale_to32(w.r64.p, w.r32.p, w.rn2.p, w.otol2.p, nb, len, s);
-        set_tol2(nullptr, w.tol2.p, nb, tau, 0, 0, s);
-        const int64_t it = cg_fused(fm, prec, inner, outer, w.r32.p, w.x32.p, true, nb, 0, maxit, w.r.p, w.p0.p,
-                                    w.p1.p, w.u.p, s, done);
-        int64_t mi = 0;
-        for (int d : done) mi = std::max<int64_t>(mi, d >= 0 ? d : it);
-        total_inner += mi;
-        update64(x, w.x32.p, w.rn2.p, w.otol2.p, nb, len, s);
-        xzero = false;
+    }
+};
+
+int solve_cg_ru(FermionMatrix& fm, int op, int prec, const void* b, void* x, const void* x0, int64_t nb, double tol,
+                int64_t maxit, nslx_solve_report* rep, hipStream_t s) {
+    SolverWork& w = fm.sw;
+    const int64_t len = fm.len();
+    const size_t b64 = (size_t)nb * len * 16;
+    w.r64.ensure(b64); w.t64.ensure(b64);
+    w.ensure_scalars(nb, 1, s);
+    w.rr_ref.ensure(nb); w.rr_prev.ensure(nb); w.want.ensure(nb); w.xn2.ensure(nb);
+    w.best.ensure(nb); w.stall.ensure(nb);
+    // ||MM^dag|| = ||M||^2 <= (1 + ||E|| max|P|)^2, ||E|| <= e^{|delta| ||K||},
+    // |P| <= e^{|mu~| + max|Im phi|}
+    const double nM = 1.0 + std::exp(std::fabs(fm.delta()) * fm.normK + std::fabs(fm.delta() * fm.mu) + fm.max_im_phi);
+    const double fc = env_double("NSLX_FLOOR_C", 1.0) * 1.1102230246251565e-16 * nM * nM;
+    RuPhase ph{fm, op, b, x, nb, fc * fc, env_double("NSLX_RU_DELTA", 0.1), env_double("NSLX_RU_FRAC", 0.5),
+               (int64_t)env_double("NSLX_RU_MAXGAP", 200.0), poll_every(), s};
+    const int stall_max = (int)env_double("NSLX_RU_STALL", 8.0);
+
+    norm2_perb(1, b, nb, len, w.bn2.p, fm.vsc, s);
+    set_tol2(w.bn2.p, w.tol2.p, nb, tol, 1, 0, s);
+    if (x0) NSLX_HIP(hipMemcpyAsync(x, x0, b64, hipMemcpyDeviceToDevice, s));
+    else NSLX_HIP(hipMemsetAsync(x, 0, b64, s));
+    NSLX_HIP(hipMemsetAsync(w.done.p, 0xFF, nb * sizeof(int), s));   // all active (-1)
+    ph.run(prec, x0 == nullptr, prec == NSLX_F64 ? 0 : stall_max, maxit);
+    int64_t escalated = 0;
+    if (prec != NSLX_F64) {
+        // RHS whose true residual stagnated in low precision: finish in FP64
+        std::vector<int> d(nb);
+        NSLX_HIP(hipMemcpyAsync(d.data(), w.done.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        NSLX_HIP(hipStreamSynchronize(s));
+        for (int v : d) escalated += (v == RU_ESCALATE || v < 0) ? 1 : 0;
+        if (escalated > 0 && ph.it < maxit) {
+            // not-converged low-precision RHS (hit maxit) are escalated too
+            for (auto& v : d) if (v < 0) v = RU_ESCALATE;
+            NSLX_HIP(hipMemcpyAsync(w.done.p, d.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
+            ph.run(NSLX_F64, false, 0, maxit - ph.it);
+        }
+    }
+    const double floor2 = ph.floor2;
+    const int64_t it = ph.it, nupd = ph.nupd;
+    // final bookkeeping
+    std::vector<double> rn2(nb), bn2(nb), xn2(nb, 0.0);
+    std::vector<int> done(nb);
+    NSLX_HIP(hipMemcpyAsync(rn2.data(), w.rn2.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+    NSLX_HIP(hipMemcpyAsync(bn2.data(), w.bn2.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+    NSLX_HIP(hipMemcpyAsync(done.data(), w.done.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+    if (nupd > 0) NSLX_HIP(hipMemcpyAsync(xn2.data(), w.xn2.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+    NSLX_HIP(hipStreamSynchronize(s));
+    int64_t conv = 0, maxi = 0, floor_stops = 0;
+    double maxrel = 0.0, tot = 0.0, maxfloor = 0.0;
+    for (int64_t i = 0; i < nb; ++i) {
+        const double fl2 = floor2 * xn2[i];
+        if (done[i] >= 0 && done[i] != RU_ESCALATE) {
+            ++conv;
+            maxi = std::max<int64_t>(maxi, done[i]);
+            if (rn2[i] > tol * tol * bn2[i]) ++floor_stops;
+        }
+        const double rel = bn2[i] > 0 ? std::sqrt(rn2[i] / bn2[i]) : std::sqrt(rn2[i]);
+        maxrel = std::max(maxrel, rel);
+        if (bn2[i] > 0) maxfloor = std::max(maxfloor, std::sqrt(fl2 / bn2[i]));
+        tot += rn2[i];
     }
     if (rep) {
-        rep->iterations = total_inner;
-        rep->iterations_total = total_inner;
-        rep->outer_steps = outer_steps;
+        rep->iterations = conv == nb ? maxi : it;
+        rep->iterations_total = it;
+        rep->outer_steps = nupd;
         rep->converged_rhs = conv;
-        double t = 0.0;
-        for (double v : rn2h) t += v;
-        rep->rsqr = t;
+        rep->rsqr = tot;
         rep->max_rel_residual = maxrel;
+        rep->floor_stops = floor_stops;
+        rep->max_floor_rel = maxfloor;
     }
     return conv == nb ? NSLX_OK : NSLX_NOT_CONVERGED;
 }
@@ -436,7 +570,9 @@ int solve(FermionMatrix& fm, int op, int method, const void* b, void* x, const v
     if (prec != NSLX_F64) {
         NSLX_REQUIRE(method == NSLX_CG && herm, "solve: low-precision inner solves need CG on MMdagger/MdaggerM");
         NSLX_REQUIRE(tol_kind == NSLX_TOL_REL_PER_RHS, "solve: low-precision inner solves need NSLX_TOL_REL_PER_RHS");
-        st = solve_cg_mixed(fm, op, prec, b, x, x0, nb, tol, maxit, rep, s);
+        st = solve_cg_ru(fm, op, prec, b, x, x0, nb, tol, maxit, rep, s);
+    } else if (method == NSLX_CG && herm && tol_kind == NSLX_TOL_REL_PER_RHS) {
+        st = solve_cg_ru(fm, op, NSLX_F64, b, x, x0, nb, tol, maxit, rep, s);
     } else if (method == NSLX_CG && herm) {
         st = solve_cg_f64(fm, op, b, x, x0, nb, tol, tol_kind, maxit, rep, s);
     } else {

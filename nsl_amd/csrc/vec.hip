@@ -104,58 +104,6 @@ __global__ void k_set_tol2(const double* bn2, double* tol2, int64_t nb, double t
     if (b < nb) tol2[b] = tol * tol * (relative ? bn2[b] : 1.0);
 }
 
-template <class S>
-__global__ void k_cg_init0(const void* bv, void* x, void* r, void* p, int64_t len, int nchunk, int nb,
-                           double* part, unsigned* counter, CgDev cg) {
-    __shared__ double red[2 * VNT / 64];
-    __shared__ int flag;
-    const int b = blockIdx.x / nchunk, c = blockIdx.x % nchunk;
-    const size_t base = (size_t)b * len;
-    double acc = 0.0;
-    for (int e = 0; e < VEPT; ++e) {
-        const int64_t i = (int64_t)c * VCH + e * VNT + threadIdx.x;
-        if (i < len) {
-            const CV<S> z = ld<S>(bv, base + i);
-            st<S>(x, base + i, CV<S>{S(0), S(0)});
-            st<S>(r, base + i, z);
-            st<S>(p, base + i, z);
-            acc += double(z.re) * z.re + double(z.im) * z.im;
-        }
-    }
-    double dummy = 0.0;
-    bsum2(acc, dummy, red);
-    if (publish(part, counter, blockIdx.x, acc, 0.0, &flag)) {
-        if (!cg.global) {
-            int active = 0;
-            for (int bb = threadIdx.x; bb < nb; bb += VNT) {
-                double s = 0.0;
-                for (int k = 0; k < nchunk; ++k) s += part[2 * ((size_t)bb * nchunk + k)];
-                cg.rr[bb] = s;
-                cg.beta[bb] = 0.0;
-                if (s <= cg.tol2[bb]) cg.done[bb] = 0; else { cg.done[bb] = -1; ++active; }
-            }
-            double a = active, d2 = 0.0;
-            bsum2(a, d2, red);
-            if (threadIdx.x == 0) { cg.status[0] = (int)a; cg.status[1] = 0; }
-        } else {
-            double s = 0.0;
-            for (int bb = threadIdx.x; bb < nb; bb += VNT)
-                for (int k = 0; k < nchunk; ++k) s += part[2 * ((size_t)bb * nchunk + k)];
-            double d2 = 0.0;
-            bsum2(s, d2, red);
-            if (threadIdx.x == 0) {
-                cg.rr[0] = s;
-                cg.beta[0] = 0.0;
-                const bool conv = s <= cg.tol2[0];
-                cg.done[0] = conv ? 0 : -1;
-                cg.status[0] = conv ? 0 : nb;
-                cg.status[1] = 0;
-            }
-        }
-        if (threadIdx.x == 0) *counter = 0u;
-    }
-}
-
 __global__ void k_resid64(const double2* bv, const double2* t, double2* r, int64_t len, int nchunk, int nb,
                           double* part, unsigned* counter, double* rn2) {
     __shared__ double red[2 * VNT / 64];
@@ -275,6 +223,59 @@ __global__ void k_axpy(const void* x, const void* y, void* z, const double* cs, 
     st<S>(z, i, out);
 }
 
+template <class S>
+__global__ void k_ru_to_low(const double2* r64, void* rs, void* p, void* xs, int init, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double2 z = r64[i];
+    const CV<S> v{S(z.x), S(z.y)};
+    st<S>(rs, i, v);
+    if (init) {
+        st<S>(p, i, v);
+        st<S>(xs, i, CV<S>{S(0), S(0)});
+    }
+}
+
+template <class S>
+__global__ void k_ru_accum(double2* x64, void* xs, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const CV<S> d = ld<S>(xs, i);
+    double2 x = x64[i];
+    x.x += double(d.re);
+    x.y += double(d.im);
+    x64[i] = x;
+    st<S>(xs, i, CV<S>{S(0), S(0)});
+}
+
+__global__ void k_ru_scalars(const double* rn2, const double* xn2, double floor2, CgDev cg, int64_t nb, int nparts,
+                             int cur, int iter, int init, double* best, int* stall, int stall_max) {
+    for (int64_t b = threadIdx.x; b < nb; b += VNT) {
+        const int d = cg.done[b];
+        if (init ? (d >= 0 && d != RU_ESCALATE) : d >= 0) continue;
+        const double s = rn2[b];
+        cg.want[b] = 0;
+        if (s <= cg.tol2[b] || (xn2 && s <= floor2 * xn2[b])) {
+            cg.done[b] = iter;
+            continue;
+        }
+        if (init) {
+            best[b] = s;
+            stall[b] = 0;
+        } else if (stall_max > 0) {
+            // stagnation of the true residual: hand the RHS to a higher precision
+            if (s < 0.25 * best[b]) { best[b] = s; stall[b] = 0; }
+            else if (++stall[b] >= stall_max) { cg.done[b] = RU_ESCALATE; continue; }
+        }
+        cg.done[b] = -1;
+        cg.rr_ref[b] = s;
+        // the replaced residual's |r|^2 becomes the partial sum the next CGA reads
+        double* rp = cg.rpart[cur] + (size_t)b * nparts;
+        rp[0] = s;
+        for (int k = 1; k < nparts; ++k) rp[k] = 0.0;
+    }
+}
+
 inline int nchunks(int64_t len) { return (int)((len + VCH - 1) / VCH); }
 inline unsigned grid1(int64_t n, int nt = 256) { return (unsigned)((n + nt - 1) / nt); }
 
@@ -309,15 +310,6 @@ void set_tol2(const double* bn2, double* tol2, int64_t nb, double tol, int relat
     NSLX_LAUNCH_CHECK();
 }
 
-void cg_init_zero(int f64, const void* b, void* x, void* r, void* p, int64_t nb, int64_t len, const CgDev& cg,
-                  VecScratch& sc, hipStream_t s) {
-    sc.ensure(nb, len, s);
-    const int nc = nchunks(len);
-    if (f64) hipLaunchKernelGGL(k_cg_init0<double>, dim3(nb * nc), dim3(VNT), 0, s, b, x, r, p, len, nc, (int)nb, sc.part.p, sc.counter.p, cg);
-    else hipLaunchKernelGGL(k_cg_init0<float>, dim3(nb * nc), dim3(VNT), 0, s, b, x, r, p, len, nc, (int)nb, sc.part.p, sc.counter.p, cg);
-    NSLX_LAUNCH_CHECK();
-}
-
 void resid64(const void* b, const void* t, void* r, int64_t nb, int64_t len, double* rn2, VecScratch& sc, hipStream_t s) {
     sc.ensure(nb, len, s);
     const int nc = nchunks(len);
@@ -335,6 +327,28 @@ void scale_to32(const void* r64, void* r32, const double* rn2, const double* oto
 void update64(void* x64, const void* d32, const double* rn2, const double* otol2, int64_t nb, int64_t len, hipStream_t s) {
     const int64_t n = nb * len;
     hipLaunchKernelGGL(k_update64, dim3(grid1(n)), dim3(256), 0, s, (double2*)x64, (const float2*)d32, rn2, otol2, len, n);
+    NSLX_LAUNCH_CHECK();
+}
+
+void ru_residual_to_low(int low_f64, const void* r64, void* rs, void* p, void* xs, int init, int64_t nb, int64_t len,
+                        hipStream_t s) {
+    const int64_t n = nb * len;
+    if (low_f64) hipLaunchKernelGGL(k_ru_to_low<double>, dim3(grid1(n)), dim3(256), 0, s, (const double2*)r64, rs, p, xs, init, n);
+    else hipLaunchKernelGGL(k_ru_to_low<float>, dim3(grid1(n)), dim3(256), 0, s, (const double2*)r64, rs, p, xs, init, n);
+    NSLX_LAUNCH_CHECK();
+}
+
+void ru_accumulate(int low_f64, void* x64, void* xs, int64_t nb, int64_t len, hipStream_t s) {
+    const int64_t n = nb * len;
+    if (low_f64) hipLaunchKernelGGL(k_ru_accum<double>, dim3(grid1(n)), dim3(256), 0, s, (double2*)x64, xs, n);
+    else hipLaunchKernelGGL(k_ru_accum<float>, dim3(grid1(n)), dim3(256), 0, s, (double2*)x64, xs, n);
+    NSLX_LAUNCH_CHECK();
+}
+
+void ru_scalars(const double* rn2, const double* xn2, double floor2, const CgDev& cg, int64_t nb, int nparts, int cur,
+                int iter, int init, double* best, int* stall, int stall_max, hipStream_t s) {
+    hipLaunchKernelGGL(k_ru_scalars, dim3(1), dim3(VNT), 0, s, rn2, xn2, floor2, cg, nb, nparts, cur, iter, init,
+                       best, stall, stall_max);
     NSLX_LAUNCH_CHECK();
 }
 

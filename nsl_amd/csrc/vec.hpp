@@ -28,11 +28,6 @@ void norm2_perb(int prec_is_f64, const void* v, int64_t nb, int64_t len, double*
 // tol2[b] = tol*tol*(relative ? bn2[b] : 1); in batch-global mode slot 0 only.
 void set_tol2(const double* bn2, double* tol2, int64_t nb, double tol, int relative, int global,
               hipStream_t s);
-// CG start from x0 = 0: x = 0, r = b, p = b; |b|^2 partials; finalises like
-// MODE_CGI (rr, done, beta, status).
-void cg_init_zero(int prec_is_f64, const void* b, void* x, void* r, void* p, int64_t nb,
-                  int64_t len, const CgDev& cg, VecScratch& sc, hipStream_t s);
-
 // Defect correction helpers (FP64 outer loop around a low-precision solve).
 // r64 = b - t64; rn2[b] = |r64_b|^2.
 void resid64(const void* b, const void* t, void* r, int64_t nb, int64_t len, double* rn2,
@@ -43,6 +38,27 @@ void scale_to32(const void* r64, void* r32, const double* rn2, const double* oto
 // x64 += |r64_b| * d32 for b with rn2[b] > otol2[b].
 void update64(void* x64, const void* d32, const double* rn2, const double* otol2, int64_t nb,
               int64_t len, hipStream_t s);
+// Reliable-update CG (FP64 residual replacement around a CG recurrence in
+// the "low" storage: c64 for F32/BF16X3, c128 for F64).
+// rs = low(r64) for every RHS; with init also p = rs and xs = 0.
+void ru_residual_to_low(int low_is_f64, const void* r64, void* rs, void* p, void* xs, int init, int64_t nb,
+                        int64_t len, hipStream_t s);
+// x64 += xs, xs = 0.
+void ru_accumulate(int low_is_f64, void* x64, void* xs, int64_t nb, int64_t len, hipStream_t s);
+// Per-RHS bookkeeping after a true residual rn2 (single block): a RHS is
+// converged when rn2 <= tol2 or rn2 <= floor2 * xn2 (the attainable accuracy
+// of an FP64 residual, (c eps ||A||)^2 |x|^2; xn2 may be null); converged RHS
+// get done = iter, the others restart their reference |r|^2 and
+// their |r|^2 partials (rpart[cur], nparts per RHS) are replaced by rn2 so
+// that the next CGA continues the recurrence from the true residual.
+// With stall_max > 0 a RHS whose true |r|^2 has not fallen below a quarter
+// of its best value for stall_max replacements gets done = RU_ESCALATE (the
+// kernels skip it; the solver finishes it in FP64).  init re-initialises the
+// RHS with done < 0 or done == RU_ESCALATE.  best/stall: nb entries.
+constexpr int RU_ESCALATE = 1 << 30;
+void ru_scalars(const double* rn2, const double* xn2, double floor2, const CgDev& cg, int64_t nb, int nparts,
+                int cur, int iter, int init, double* best, int* stall, int stall_max, hipStream_t s);
+
 // Element type conversions.
 void c128_to_c64(const void* in, void* out, int64_t n, hipStream_t s);
 void c64_to_c128(const void* in, void* out, int64_t n, hipStream_t s);
