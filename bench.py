@@ -9,6 +9,12 @@ step, all resident in HBM before the timed region.
 
 Prints one JSON line (see the keys below).  A step is one batched solve of
 all 128 systems to |b - MM^dag x| / |b| <= 1e-12 per right-hand side.
+
+Profiling helpers (no JSON contract line):
+    python bench.py --hops N        N plain M and M^dag applies per precision
+                                    on the same 128 x 64 x 288 block
+    python bench.py --copy-calib M  three device copies of M MiB (scales the
+                                    FETCH_SIZE / WRITE_SIZE counters)
 """
 
 import argparse
@@ -114,6 +120,50 @@ def cpu_baseline(phi, b, cores, iters_needed, budget_s=20.0):
                       f"the FP64 GPU solve needed for 1e-12"}
 
 
+def prof_text(L_):
+    buf = ctypes.create_string_buffer(1 << 16)
+    L_.nslx_prof_read(buf, len(buf))
+    return buf.value.decode()
+
+
+def hop_profile(n):
+    """n plain M and M^dag applies at each precision on the bench block;
+    prints the per-kernel average duration from HIP events."""
+    import torch
+    import nsl_amd as nsl
+    phi_h, b_h = fields()
+    fm = nsl.HubbardExp(nsl.Lattice.honeycomb(L, L), NT, BETA)
+    fm.populate(torch.as_tensor(phi_h).cuda())
+    b = torch.as_tensor(b_h).cuda()
+    L_ = nsl._lib.lib()
+    for prec in (nsl.F64, nsl.F32, nsl.BF16X3):
+        x = b.to(torch.complex128 if prec == nsl.F64 else torch.complex64)
+        y = torch.empty_like(x)
+        fm.apply(nsl.M, x, y, precision=prec)
+        torch.cuda.synchronize()
+        L_.nslx_prof_reset()
+        L_.nslx_prof_enable(1)
+        for _ in range(n):
+            fm.apply(nsl.M, x, y, precision=prec)
+            fm.apply(nsl.Mdagger, x, y, precision=prec)
+        torch.cuda.synchronize()
+        L_.nslx_prof_enable(0)
+        for line in prof_text(L_).strip().splitlines():
+            name, k, ms = line.split()
+            print(f"{name} launches={k} avg_us={float(ms) / int(k) * 1e3:.2f}", flush=True)
+
+
+def copy_calib(mib):
+    import torch
+    n = mib << 20
+    x = torch.ones(n, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,7 +172,13 @@ def main():
     ap.add_argument("--inner", default="bf16x3", choices=["bf16x3", "f32"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--hops", type=int, default=0)
+    ap.add_argument("--copy-calib", type=int, default=0)
     args = ap.parse_args()
+    if args.copy_calib:
+        return copy_calib(args.copy_calib)
+    if args.hops:
+        return hop_profile(args.hops)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the path shards by whole requests/fields with no communication, see DESIGN.md",

@@ -201,6 +201,22 @@ int nslx_hmc_chain(nslx_fm fm, void* phi, double U, double traj_length, int nmd,
                    int64_t ntraj, uint64_t seed, int nchunks, nslx_hmc_report* reports,
                    void* stream);
 
+/* ---- hop tiles and autotuning ------------------------------------------------
+ * No counterpart in the reference (its GEMMs are whatever torch::matmul
+ * dispatches, Tensor/Impl/matmul.tpp:15-17).  The hop kernels come in tile
+ * variants (ids 0 .. count-1 per precision); the context keeps one choice per
+ * (precision, nx, nt, power-of-two class of nbatch), filled by nslx_fm_tune
+ * or nslx_ctx_set_tile, else a built-in heuristic.
+ * nslx_fm_tune times reps x (M + Mdagger) plain applies of nbatch vectors on
+ * fm's populated fields for every tile that fits nx, records the fastest in
+ * fm's context and returns it (best_us: mean time of one M + Mdagger pair).
+ * Synchronous; allocates two temporary vectors of nbatch fields. */
+int nslx_tile_count(int precision, int* count);
+int nslx_ctx_set_tile(nslx_ctx ctx, int precision, int nx, int nt, int64_t nbatch, int tile);
+int nslx_ctx_get_tile(nslx_ctx ctx, int precision, int nx, int nt, int64_t nbatch, int* tile);
+int nslx_fm_tune(nslx_fm fm, int precision, int64_t nbatch, int reps, int* best_tile, double* best_us,
+                 void* stream);
+
 /* ---- kernel timing ----------------------------------------------------------
  * No counterpart in the reference (its only profiling is a host stopwatch,
  * src/NSL/logger.hpp:174-191).  While enabled, every hop launch is bracketed

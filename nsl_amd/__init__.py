@@ -75,6 +75,24 @@ class Context:
             _lib._lib.nslx_ctx_destroy(h)
             self._h = None
 
+    # -- hop tile table (nslx_ctx_set_tile / nslx_ctx_get_tile) -------------
+    @staticmethod
+    def tile_count(precision=F64):
+        n = ctypes.c_int()
+        _lib.check(_lib.lib().nslx_tile_count(int(precision), ctypes.byref(n)))
+        return n.value
+
+    def set_tile(self, precision, nx, nt, nbatch, tile):
+        """Pin the hop tile for (precision, nx, nt, nbatch class); tile=None clears."""
+        _lib.check(_lib.lib().nslx_ctx_set_tile(self._h, int(precision), int(nx), int(nt), int(nbatch),
+                                                -1 if tile is None else int(tile)))
+
+    def get_tile(self, precision, nx, nt, nbatch):
+        t = ctypes.c_int()
+        _lib.check(_lib.lib().nslx_ctx_get_tile(self._h, int(precision), int(nx), int(nt), int(nbatch),
+                                                ctypes.byref(t)))
+        return t.value
+
 
 # ----------------------------------------------------------------- lattices
 class Lattice:
@@ -173,6 +191,15 @@ class HubbardExp:
         self.nfield = nf
         self._phi = phi
         return self
+
+    def tune(self, precision=F64, nbatch=None, reps=10):
+        """Autotune the hop tile for this lattice and nbatch (nslx_fm_tune);
+        returns (tile, microseconds per M + Mdagger pair)."""
+        nb = int(nbatch or self.nfield)
+        t, us = ctypes.c_int(), ctypes.c_double()
+        _lib.check(_lib.lib().nslx_fm_tune(self._h, int(precision), nb, int(reps), ctypes.byref(t),
+                                           ctypes.byref(us), _stream()))
+        return t.value, us.value
 
     # -- applications ------------------------------------------------------
     def apply(self, op, psi, out=None, precision=F64):

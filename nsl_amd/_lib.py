@@ -67,6 +67,10 @@ PROTOS = {
                                     c_double, c_int, ctypes.POINTER(HmcReport), c_void_p]),
     "nslx_hmc_chain": (c_int, [c_void_p, c_void_p, c_double, c_double, c_int, c_int64, c_uint64, c_int,
                                ctypes.POINTER(HmcReport), c_void_p]),
+    "nslx_tile_count": (c_int, [c_int, ctypes.POINTER(c_int)]),
+    "nslx_ctx_set_tile": (c_int, [c_void_p, c_int, c_int, c_int, c_int64, c_int]),
+    "nslx_ctx_get_tile": (c_int, [c_void_p, c_int, c_int, c_int, c_int64, ctypes.POINTER(c_int)]),
+    "nslx_fm_tune": (c_int, [c_void_p, c_int, c_int64, c_int, ctypes.POINTER(c_int), P_double, c_void_p]),
     "nslx_prof_enable": (c_int, [c_int]),
     "nslx_prof_reset": (c_int, []),
     "nslx_prof_read": (c_int, [ctypes.c_char_p, c_int64]),

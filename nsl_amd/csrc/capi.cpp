@@ -255,6 +255,49 @@ int nslx_hmc_chain(nslx_fm fm, void* phi, double U, double traj_length, int nmd,
     });
 }
 
+int nslx_tile_count(int precision, int* count) {
+    return guard([&] {
+        NSLX_REQUIRE(count, "tile_count: null output");
+        NSLX_REQUIRE(precision == NSLX_F64 || precision == NSLX_F32 || precision == NSLX_BF16X3,
+                     "tile_count: bad precision");
+        *count = (int)nslx::tile_menu(precision).size();
+        return NSLX_OK;
+    });
+}
+
+int nslx_ctx_set_tile(nslx_ctx ctx, int precision, int nx, int nt, int64_t nbatch, int tile) {
+    return guard([&] {
+        NSLX_REQUIRE(ctx, "null context handle");
+        NSLX_REQUIRE(precision == NSLX_F64 || precision == NSLX_F32 || precision == NSLX_BF16X3,
+                     "set_tile: bad precision");
+        NSLX_REQUIRE(nx > 0 && nt > 0 && nbatch > 0, "set_tile: bad shape");
+        if (tile >= 0) {
+            NSLX_REQUIRE(tile < (int)nslx::tile_menu(precision).size(), "set_tile: bad tile id");
+            NSLX_REQUIRE(nslx::tile_fits(precision, tile, nx), "set_tile: tile does not fit this lattice");
+        }
+        ctx->c.tiles.set(precision, nx, nt, nbatch, tile);
+        return NSLX_OK;
+    });
+}
+
+int nslx_ctx_get_tile(nslx_ctx ctx, int precision, int nx, int nt, int64_t nbatch, int* tile) {
+    return guard([&] {
+        NSLX_REQUIRE(ctx && tile, "get_tile: null argument");
+        NSLX_REQUIRE(nx > 0 && nt > 0 && nbatch > 0, "get_tile: bad shape");
+        *tile = ctx->c.tiles.get(precision, nx, nt, nbatch);
+        return NSLX_OK;
+    });
+}
+
+int nslx_fm_tune(nslx_fm fm, int precision, int64_t nbatch, int reps, int* best_tile, double* best_us,
+                 void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        DeviceGuard dg(m.ctx->device);
+        return nslx::fm_tune(m, precision, nbatch, reps, best_tile, best_us, nslx::as_stream(stream));
+    });
+}
+
 int nslx_prof_enable(int on) {
     nslx::prof_enable(on != 0);
     return NSLX_OK;

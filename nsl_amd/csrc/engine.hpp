@@ -19,7 +19,8 @@ struct TileTable {
     std::mutex mu;
     std::map<std::tuple<int, int, int, int>, int> best;
     int get(int prec, int nx, int nt, int64_t nb);
-    void set(int prec, int nx, int nt, int64_t nb, int tile);
+    void set(int prec, int nx, int nt, int64_t nb, int tile);   // tile < 0 clears
+    bool lookup(int prec, int nx, int nt, int64_t nb, int* tile);
     static int nb_class(int64_t nb);
 };
 
@@ -99,6 +100,9 @@ int solve(FermionMatrix& fm, int op, int method, const void* b, void* x, const v
 // log det / force (logdet.cpp)
 void fm_logdet(FermionMatrix& fm, int64_t field, int nchunks, double* out2, hipStream_t s);
 void fm_grad_logdet(FermionMatrix& fm, int64_t field, void* out, hipStream_t s);
+
+// hop tile autotuner (tune.cpp)
+int fm_tune(FermionMatrix& fm, int prec, int64_t nb, int reps, int* best_tile, double* best_us, hipStream_t s);
 
 // set by NSLX_INNER_TOL etc.; see solver.cpp
 double env_double(const char* name, double dflt);

@@ -18,16 +18,24 @@ int TileTable::nb_class(int64_t nb) {
 
 int TileTable::get(int prec, int nx, int nt, int64_t nb) {
     static const int forced = (int)env_double("NSLX_TILE", -1.0);
-    if (forced >= 0 && forced < (int)tile_menu(prec).size()) return forced;
+    if (forced >= 0 && forced < (int)tile_menu(prec).size() && tile_fits(prec, forced, nx)) return forced;
+    int t;
+    if (lookup(prec, nx, nt, nb, &t)) return t;
+    return default_tile(prec, nx, nt, nb);
+}
+
+bool TileTable::lookup(int prec, int nx, int nt, int64_t nb, int* tile) {
     std::lock_guard<std::mutex> g(mu);
     auto it = best.find({prec, nx, nt, nb_class(nb)});
-    if (it != best.end()) return it->second;
-    return default_tile(prec, nx, nt, nb);
+    if (it == best.end()) return false;
+    *tile = it->second;
+    return true;
 }
 
 void TileTable::set(int prec, int nx, int nt, int64_t nb, int tile) {
     std::lock_guard<std::mutex> g(mu);
-    best[{prec, nx, nt, nb_class(nb)}] = tile;
+    if (tile < 0) best.erase({prec, nx, nt, nb_class(nb)});
+    else best[{prec, nx, nt, nb_class(nb)}] = tile;
 }
 
 double env_double(const char* name, double dflt) {

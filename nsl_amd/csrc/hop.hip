@@ -190,7 +190,6 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
                                 const CS<S> po = ldc<S>(J.Pold, rowoff + x);
                                 z.re = z.re + S(beta) * po.re;
                                 z.im = z.im + S(beta) * po.im;
-                                if (ct == 0) stc<S>(J.Pnew, rowoff + x, z);
                             }
                             if (OP == HOP_M) {
                                 const CS<S> ph = ldc<S>(Pf, (size_t)tin * nx + x);
@@ -338,6 +337,7 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
                         stc<S>(J.Y, idx, out);
                     } else if (MODE == MODE_CGA) {
                         stc<S>(J.Y, idx, out);
+                        stc<S>(J.Pnew, idx, id);   // p_new = r + beta p_old, each element once
                         part += double(out.re) * out.re + double(out.im) * out.im;
                     } else if (MODE == MODE_CGB) {
                         const CS<S> pv = ldc<S>(J.Pold, idx);
@@ -500,6 +500,440 @@ __global__ void __launch_bounds__(TL::NT) hop_grouped(GroupArgs g) {
     }
 }
 
+// ------------------------------------------------ slice-resident kernels
+// A workgroup keeps the column slice E[y0 .. y0+BN)[0 .. kp) in LDS for its
+// whole life (loaded once, one barrier) and each of its W waves computes
+// 16-row output blocks with the full K loop on its own: the wave streams its
+// 16 input rows (time-shifted, phase-multiplied) straight from global memory
+// into MFMA operand registers, one K chunk ahead of the MFMAs, with no LDS
+// staging of the field and no barrier inside the K loop.  Lane (r16, q) feeds
+// row r16 and the contiguous k run [k0 + q*VK, k0 + (q+1)*VK) of each chunk
+// (the same k permutation for the E operand, read from LDS as 16-byte
+// vectors).  The LDS row stride is padded so that the 16-byte reads of eight
+// consecutive lanes cover all 32 banks once.
+template <class PR>
+struct SliceP;
+template <>
+struct SliceP<PF64> { static constexpr int KC = 16, VK = 4, LPAD = 2; };
+template <>
+struct SliceP<PF32> { static constexpr int KC = 16, VK = 4, LPAD = 4; };
+template <>
+struct SliceP<PB3> { static constexpr int KC = 32, VK = 8, LPAD = 8; };
+
+template <class S, int VK>
+struct Raw {
+    CS<S> x[VK];    // input row chunk
+    CS<S> ph[VK];   // phase chunk (M)
+    CS<S> po[VK];   // previous search direction (CGA)
+};
+
+// 64 zero bytes: the source of every run that lies outside the field (a run
+// is at most 64 bytes).
+// Not const: a const array would live in the constant address space, and a
+// select between it and a global pointer would turn the loads into flat loads
+// (whose waits cannot be counted per load).  Never written.
+__device__ __attribute__((aligned(16))) double g_zero_run[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+// VK complex (64 bytes) at element `off` of base, 16-byte aligned.  Branch-
+// and select-free on the loaded data, so that the compiler can keep several
+// chunks of loads in flight across the K loop: a run that is not valid (row
+// past nt or chunk past kp) reads g_zero_run instead.
+template <class S, int VK>
+__device__ inline void load_run(const void* base, size_t off, bool ok, CS<S>* out) {
+    static_assert(VK * 2 * sizeof(S) <= sizeof(g_zero_run), "zero run size");
+    if constexpr (sizeof(S) == 8) {
+        const double2* p = ok ? reinterpret_cast<const double2*>(base) + off
+                              : reinterpret_cast<const double2*>(g_zero_run);
+        #pragma unroll
+        for (int e = 0; e < VK; ++e) {
+            const double2 v = p[e];
+            out[e] = CS<S>{v.x, v.y};
+        }
+    } else {
+        const float4* p = ok ? reinterpret_cast<const float4*>(reinterpret_cast<const float2*>(base) + off)
+                             : reinterpret_cast<const float4*>(g_zero_run);
+        #pragma unroll
+        for (int e = 0; e < VK / 2; ++e) {
+            const float4 v = p[e];
+            out[2 * e] = CS<S>{v.x, v.y};
+            out[2 * e + 1] = CS<S>{v.z, v.w};
+        }
+    }
+}
+
+typedef unsigned short us8 __attribute__((ext_vector_type(8)));
+
+// One 16 x BN output block (rows rt*16.., columns y0..) of rhs b by one wave.
+// Returns the lane's share of the block's reduction (CG modes).
+template <class PR, int FN, int OP, int MODE>
+__device__ double slice_task(const JobPtrs& J, const typename PR::L* __restrict__ sE, int ld, int kp, int nx, int nt,
+                             int rpf, int b, int rt, int s, double alpha, double beta) {
+    using S = typename PR::S;
+    using Acc = typename PR::Acc;
+    constexpr int KC = SliceP<PR>::KC, VK = SliceP<PR>::VK, BN = FN * 16;
+    const int lane = threadIdx.x & 63;
+    const int r16 = lane & 15, q = lane >> 4;
+    const int f = b / rpf;
+    const S* Pf = reinterpret_cast<const S*>(J.P) + (size_t)f * nt * nx * 2;
+    const size_t bbase = (size_t)b * nt * nx;
+    const int ta = rt * 16 + r16;
+    const bool vrow = ta < nt;
+    const int tin = (OP == HOP_M) ? (ta == 0 ? nt - 1 : ta - 1) : (ta + 1 >= nt ? 0 : ta + 1);
+    const size_t xoff = bbase + (size_t)tin * nx;
+    const size_t poff = (size_t)tin * nx;
+    const int y0 = s * BN;
+
+    Acc accR[FN], accI[FN];
+    #pragma unroll
+    for (int j = 0; j < FN; ++j) { accR[j] = Acc{0, 0, 0, 0}; accI[j] = Acc{0, 0, 0, 0}; }
+
+    // nx is a multiple of VK for these tiles, so a lane's run is entirely
+    // inside the row (k < nx) or entirely past it; kp is a multiple of 3 KC
+    // and the LDS slice is zero past nx
+    const int nch = kp / KC;
+    auto load = [&](Raw<S, VK>& r, int c) {
+        const int k = c * KC + q * VK;
+        const bool ok = vrow && k < nx;
+        load_run<S, VK>(J.X, xoff + k, ok, r.x);
+        if (OP == HOP_M) load_run<S, VK>(Pf, poff + k, ok, r.ph);
+        if (MODE == MODE_CGA) load_run<S, VK>(J.Pold, xoff + k, ok, r.po);
+    };
+    auto compute = [&](const Raw<S, VK>& r, int c) {
+        const int k = c * KC + q * VK;
+        CS<S> z[VK];
+        #pragma unroll
+        for (int e = 0; e < VK; ++e) {
+            z[e] = r.x[e];
+            if (MODE == MODE_CGA) {
+                z[e].re = z[e].re + S(beta) * r.po[e].re;
+                z[e].im = z[e].im + S(beta) * r.po[e].im;
+            }
+            if (OP == HOP_M) {
+                const S re = r.ph[e].re * z[e].re - r.ph[e].im * z[e].im;
+                const S im = r.ph[e].re * z[e].im + r.ph[e].im * z[e].re;
+                z[e].re = re; z[e].im = im;
+            }
+        }
+        if constexpr (PR::KSTEP == 4) {
+            #pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                const S* bp = sE + (size_t)(j * 16 + r16) * ld + k;
+                S bv[VK];
+                if constexpr (sizeof(S) == 8) {
+                    #pragma unroll
+                    for (int e = 0; e < VK; e += 2) {
+                        const double2 w = *reinterpret_cast<const double2*>(bp + e);
+                        bv[e] = w.x; bv[e + 1] = w.y;
+                    }
+                } else {
+                    const float4 w = *reinterpret_cast<const float4*>(bp);
+                    bv[0] = w.x; bv[1] = w.y; bv[2] = w.z; bv[3] = w.w;
+                }
+                #pragma unroll
+                for (int e = 0; e < VK; ++e) {
+                    if constexpr (sizeof(S) == 8) {
+                        accR[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(z[e].re, bv[e], accR[j], 0, 0, 0);
+                        accI[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(z[e].im, bv[e], accI[j], 0, 0, 0);
+                    } else {
+                        accR[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(z[e].re, bv[e], accR[j], 0, 0, 0);
+                        accI[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(z[e].im, bv[e], accI[j], 0, 0, 0);
+                    }
+                }
+            }
+        } else {
+            us8 rh, rl, ih, il;
+            #pragma unroll
+            for (int e = 0; e < VK; ++e) {
+                const uint16_t a = f2bf(z[e].re), c2 = f2bf(z[e].im);
+                rh[e] = a; rl[e] = f2bf(z[e].re - bf2f(a));
+                ih[e] = c2; il[e] = f2bf(z[e].im - bf2f(c2));
+            }
+            const bf8 arh = __builtin_bit_cast(bf8, rh), arl = __builtin_bit_cast(bf8, rl);
+            const bf8 aih = __builtin_bit_cast(bf8, ih), ail = __builtin_bit_cast(bf8, il);
+            #pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                const uint16_t* bp = reinterpret_cast<const uint16_t*>(sE) + (size_t)(j * 16 + r16) * ld + k;
+                const bf8 bh = *reinterpret_cast<const bf8*>(bp);
+                const bf8 bl = *reinterpret_cast<const bf8*>(bp + (size_t)BN * ld);
+                accR[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arl, bh, accR[j], 0, 0, 0);
+                accR[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, bl, accR[j], 0, 0, 0);
+                accR[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, bh, accR[j], 0, 0, 0);
+                accI[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ail, bh, accI[j], 0, 0, 0);
+                accI[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, bl, accI[j], 0, 0, 0);
+                accI[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, bh, accI[j], 0, 0, 0);
+            }
+        }
+    };
+
+    // three chunk buffers in rotation: the loads of chunks c+1 and c+2 are in
+    // flight while chunk c feeds the MFMAs; no branch inside the loop, so the
+    // waits before each chunk only cover that chunk's loads (the two loads
+    // past the end read the zero run)
+    // (the scheduling barriers keep the compiler from sinking the prefetch
+    // loads next to their uses)
+    Raw<S, VK> r0, r1, r2;
+    load(r0, 0);
+    load(r1, 1);
+    for (int c = 0; c < nch; c += 3) {
+        load(r2, c + 2);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(r0, c);
+        __builtin_amdgcn_sched_barrier(0);
+        load(r0, c + 3);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(r1, c + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        load(r1, c + 4);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(r2, c + 2);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+
+    // epilogue (as in hop_tile): shift sign, identity, conj(P), fused recurrences
+    double part = 0.0;
+    #pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int t = rt * 16 + (PR::ROW4V ? 4 * v + q : 4 * q + v);
+        if (t >= nt) continue;
+        const size_t rowoff = bbase + (size_t)t * nx;
+        #pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int y = y0 + j * 16 + r16;
+            if (y >= nx) continue;
+            const size_t idx = rowoff + y;
+            S fr = S(accR[j][v]), fi = S(accI[j][v]);
+            if (OP == HOP_M) {
+                if (t == 0) { fr = -fr; fi = -fi; }
+            } else {
+                const CS<S> ph = ldc<S>(Pf, (size_t)t * nx + y);
+                const S re = ph.re * fr + ph.im * fi;
+                const S im = ph.re * fi - ph.im * fr;
+                fr = re; fi = im;
+                if (t == nt - 1) { fr = -fr; fi = -fi; }
+            }
+            CS<S> id;
+            if (MODE == MODE_CGA) {
+                const CS<S> rv = ldc<S>(J.X, idx);
+                const CS<S> po = ldc<S>(J.Pold, idx);
+                id.re = rv.re + S(beta) * po.re;
+                id.im = rv.im + S(beta) * po.im;
+            } else {
+                id = ldc<S>(J.X, idx);
+            }
+            const CS<S> out{id.re - fr, id.im - fi};
+            if (MODE == MODE_PLAIN) {
+                stc<S>(J.Y, idx, out);
+            } else if (MODE == MODE_CGA) {
+                stc<S>(J.Y, idx, out);
+                stc<S>(J.Pnew, idx, id);   // p_new = r + beta p_old, each element once
+                part += double(out.re) * out.re + double(out.im) * out.im;
+            } else if (MODE == MODE_CGB) {
+                const CS<S> pv = ldc<S>(J.Pold, idx);
+                CS<S> xv = ldc<S>(J.Xs, idx);
+                CS<S> rv = ldc<S>(J.R, idx);
+                xv.re += S(alpha) * pv.re; xv.im += S(alpha) * pv.im;
+                rv.re -= S(alpha) * out.re; rv.im -= S(alpha) * out.im;
+                stc<S>(J.Xs, idx, xv);
+                stc<S>(J.R, idx, rv);
+                part += double(rv.re) * rv.re + double(rv.im) * rv.im;
+            } else {  // CGI
+                const CS<S> bv = ldc<S>(J.Bv, idx);
+                const CS<S> rv{bv.re - out.re, bv.im - out.im};
+                stc<S>(J.R, idx, rv);
+                stc<S>(J.Pnew, idx, rv);
+                part += double(rv.re) * rv.re + double(rv.im) * rv.im;
+            }
+        }
+    }
+    return part;
+}
+
+// Grid: nct column slices x P row groups (XCD-swizzled like hop_kernel: the
+// nct slices of a row group are 8 block ids apart).  Row group gi owns the
+// row blocks [gi*R*W, (gi+1)*R*W); wave w takes blocks (gi*R + i)*W + w.
+template <class PR, int FN, int W, int OP, int MODE>
+__global__ void __launch_bounds__(64 * W) hop_slice(HopArgs a, int nrt, int nct, int P, int R, int swz, int kp,
+                                                    int ldE, size_t planeE) {
+    using L = typename PR::L;
+    constexpr int NT = 64 * W, BN = FN * 16;
+    extern __shared__ __attribute__((aligned(16))) unsigned char slice_lds[];
+    __shared__ double red[W];
+    L* sE = reinterpret_cast<L*>(slice_lds);
+    const int ld = kp + SliceP<PR>::LPAD;
+
+    int gi, s;
+    if (swz) {
+        const int xcd = blockIdx.x & 7, l = blockIdx.x >> 3;
+        s = l % nct;
+        gi = (l / nct) * 8 + xcd;
+    } else {
+        s = blockIdx.x % nct;
+        gi = blockIdx.x / nct;
+    }
+    if (gi >= P) return;
+
+    // the E slice, once (columns past the padded global row are zero)
+    constexpr int VL = 16 / sizeof(L);
+    const int vpr = kp / VL;
+    const int y0 = s * BN;
+    #pragma unroll
+    for (int pl = 0; pl < PR::NPB; ++pl) {
+        const L* src = reinterpret_cast<const L*>(a.E) + (size_t)pl * planeE;
+        for (int i = threadIdx.x; i < BN * vpr; i += NT) {
+            const int j = i / vpr, kv = (i - j * vpr) * VL;
+            const uint4 w = kv < ldE ? *reinterpret_cast<const uint4*>(src + (size_t)(y0 + j) * ldE + kv)
+                                     : make_uint4(0, 0, 0, 0);
+            *reinterpret_cast<uint4*>(sE + (size_t)(pl * BN + j) * ld + kv) = w;
+        }
+    }
+    const int nparts = nrt * nct;
+    double g1 = 0.0, g2 = 0.0, gu = 0.0;
+    if ((MODE == MODE_CGA || MODE == MODE_CGB) && a.cg.global) {
+        const size_t n = (size_t)a.nb * nparts;
+        g1 = block_sum_parts<NT>(a.cg.rpart[(a.iter - 1) & 1], n, red);
+        if (MODE == MODE_CGA && a.iter >= 2) g2 = block_sum_parts<NT>(a.cg.rpart[(a.iter - 2) & 1], n, red);
+        if (MODE == MODE_CGB) gu = block_sum_parts<NT>(a.cg.upart, n, red);
+    }
+    __syncthreads();
+
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int G = a.nb * nrt;
+    const JobPtrs J{a.E, a.P, a.X, a.Y, a.Pold, a.Pnew, a.Xs, a.R, a.Bv};
+    for (int i = 0; i < R; ++i) {
+        const int rb = (gi * R + i) * W + wave;
+        if (rb >= G) break;
+        const int b = rb / nrt, rt = rb - (rb / nrt) * nrt;
+        double alpha = 0.0, beta = 0.0;
+        if (MODE == MODE_CGA || MODE == MODE_CGB) {
+            const CgDev& c = a.cg;
+            const int it = a.iter;
+            double rr1 = g1, rr2 = g2, uu = gu;
+            if (!c.global) {
+                const size_t off = (size_t)b * nparts;
+                rr1 = sum_parts(c.rpart[(it - 1) & 1], off, nparts);
+                rr2 = (MODE == MODE_CGA && it >= 2) ? sum_parts(c.rpart[(it - 2) & 1], off, nparts) : 0.0;
+                uu = MODE == MODE_CGB ? sum_parts(c.upart, off, nparts) : 0.0;
+            }
+            const int slot = c.global ? 0 : b;
+            const bool designated = rt == 0 && s == 0 && (!c.global || b == 0);
+            if (cg_decide<MODE>(c, slot, it, designated, rr1, rr2, uu, alpha, beta)) continue;
+        }
+        double part = slice_task<PR, FN, OP, MODE>(J, sE, ld, kp, a.nx, a.nt, a.rpf, b, rt, s, alpha, beta);
+        if (MODE == MODE_PLAIN) continue;
+        #pragma unroll
+        for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+        if (lane == 0) {
+            double* dst = MODE == MODE_CGA ? a.cg.upart : a.cg.rpart[MODE == MODE_CGB ? (a.iter & 1) : 0];
+            dst[(size_t)b * nparts + rt * nct + s] = part;
+        }
+    }
+}
+
+// K extent of the LDS slice: nx rounded up to three chunks (the K loop's unroll).
+template <class PR>
+int slice_kp(int nx) {
+    constexpr int K3 = 3 * SliceP<PR>::KC;
+    return (nx + K3 - 1) / K3 * K3;
+}
+
+template <class PR, int FN>
+size_t slice_lds_bytes(int nx) {
+    return (size_t)PR::NPB * FN * 16 * (slice_kp<PR>(nx) + SliceP<PR>::LPAD) * sizeof(typename PR::L);
+}
+
+template <class PR, int FN, int W, int OP, int MODE>
+void launch_slice_k(const HopArgs& a, int R, hipStream_t s) {
+    constexpr int BN = FN * 16;
+    NSLX_REQUIRE(a.nx % SliceP<PR>::VK == 0, "hop: slice tiles need nx to be a multiple of the lane run");
+    const int nrt = (a.nt + 15) / 16, nct = (a.nx + BN - 1) / BN;
+    const int kp = slice_kp<PR>(a.nx);
+    const size_t lds = slice_lds_bytes<PR, FN>(a.nx);
+    NSLX_REQUIRE(lds + W * sizeof(double) <= 160 * 1024, "hop: lattice too large for this slice tile");
+    const int G = a.nb * nrt;
+    const int P = (G + R * W - 1) / (R * W);
+    const int swz = P >= 8 ? 1 : 0;
+    const int grid = swz ? ((P + 7) / 8) * 8 * nct : P * nct;
+    const int ldE = e_ld(a.nx);
+    const size_t planeE = (size_t)e_rows(a.nx) * ldE;
+    auto kern = hop_slice<PR, FN, W, OP, MODE>;
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  150 * 1024);
+        (void)hipGetLastError();
+        attr = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * W), lds, s, a, nrt, nct, P, R, swz, kp, ldE, planeE);
+}
+
+template <class PR, int FN, int W>
+void launch_slice(int op, int mode, const HopArgs& a, int R, hipStream_t s) {
+#define NSLX_SLICE_CASE(OPV, MV)                           \
+    if (op == OPV && mode == MV) {                         \
+        launch_slice_k<PR, FN, W, OPV, MV>(a, R, s);       \
+        return;                                            \
+    }
+    NSLX_SLICE_CASE(HOP_M, MODE_PLAIN)
+    NSLX_SLICE_CASE(HOP_MD, MODE_PLAIN)
+    NSLX_SLICE_CASE(HOP_M, MODE_CGA)
+    NSLX_SLICE_CASE(HOP_MD, MODE_CGA)
+    NSLX_SLICE_CASE(HOP_M, MODE_CGB)
+    NSLX_SLICE_CASE(HOP_MD, MODE_CGB)
+    NSLX_SLICE_CASE(HOP_M, MODE_CGI)
+    NSLX_SLICE_CASE(HOP_MD, MODE_CGI)
+#undef NSLX_SLICE_CASE
+    throw Error(NSLX_E_ARG, "hop: bad op/mode");
+}
+
+// Slice tile menus per precision, tile ids SLICE_BASE + index:
+// (FN: slice width / 16, W: waves per workgroup, R: row blocks per wave).
+constexpr int SLICE_BASE = 4;
+struct SliceEntry { int fn, w, r; };
+constexpr int SLICE_N = 4;
+constexpr SliceEntry SLICE_MENU64[SLICE_N] = {{3, 8, 1}, {2, 4, 1}, {2, 8, 1}, {1, 8, 1}};
+constexpr SliceEntry SLICE_MENU32[SLICE_N] = {{3, 8, 1}, {6, 8, 1}, {3, 4, 1}, {2, 8, 1}};
+constexpr SliceEntry SLICE_MENUB3[SLICE_N] = {{6, 8, 1}, {3, 8, 1}, {3, 4, 1}, {2, 8, 1}};
+
+const SliceEntry* slice_menu(int prec) {
+    return prec == NSLX_F64 ? SLICE_MENU64 : prec == NSLX_F32 ? SLICE_MENU32 : SLICE_MENUB3;
+}
+
+template <class PR, int W, int FA, int FB, int FC>
+void dispatch_slice_w(const SliceEntry& e, int op, int mode, const HopArgs& a, hipStream_t s) {
+    if (e.fn == FA) launch_slice<PR, FA, W>(op, mode, a, e.r, s);
+    else if (e.fn == FB) launch_slice<PR, FB, W>(op, mode, a, e.r, s);
+    else if (e.fn == FC) launch_slice<PR, FC, W>(op, mode, a, e.r, s);
+    else throw Error(NSLX_E_ARG, "hop: bad slice tile");
+}
+
+template <class PR, int FA, int FB, int FC>
+void dispatch_slice(int id, int op, int mode, const HopArgs& a, hipStream_t s, const SliceEntry* menu) {
+    const SliceEntry e = menu[id];
+    if (e.w == 4) dispatch_slice_w<PR, 4, FA, FB, FC>(e, op, mode, a, s);
+    else if (e.w == 8) dispatch_slice_w<PR, 8, FA, FB, FC>(e, op, mode, a, s);
+    else throw Error(NSLX_E_ARG, "hop: bad slice tile");
+}
+
+template <class PR>
+size_t slice_lds_fn(int fn, int nx) {
+    switch (fn) {
+        case 1: return slice_lds_bytes<PR, 1>(nx);
+        case 2: return slice_lds_bytes<PR, 2>(nx);
+        case 3: return slice_lds_bytes<PR, 3>(nx);
+        default: return slice_lds_bytes<PR, 6>(nx);
+    }
+}
+
+bool slice_fits(int prec, int id, int nx) {
+    const SliceEntry e = slice_menu(prec)[id];
+    size_t lds = 0;
+    int vk = 0;
+    if (prec == NSLX_F64) { lds = slice_lds_fn<PF64>(e.fn, nx); vk = SliceP<PF64>::VK; }
+    else if (prec == NSLX_F32) { lds = slice_lds_fn<PF32>(e.fn, nx); vk = SliceP<PF32>::VK; }
+    else { lds = slice_lds_fn<PB3>(e.fn, nx); vk = SliceP<PB3>::VK; }
+    return nx % vk == 0 && lds + e.w * sizeof(double) <= 160 * 1024;
+}
+
 // ---------------------------------------------------------------- tile menus
 using T64_a = Tile<2, 2, 2, 3, 16>;   // 64 x 96
 using T64_b = Tile<2, 2, 1, 3, 16>;   // 32 x 96
@@ -590,6 +1024,7 @@ void dispatch4g(int tile, int op, int mode, const GroupArgs& g, int grid, hipStr
 
 void launch_hop_grouped(int prec, int op, int mode, int tile, const GroupArgs& g, int grid, hipStream_t s) {
     NSLX_REQUIRE(g.nprob > 0 && g.ntiles > 0 && grid > 0, "hop: empty grouped launch");
+    NSLX_REQUIRE(tile >= 0 && tile < SLICE_BASE, "hop: grouped launches take tiles 0-3");
     ProfScope prof(op == HOP_M ? "grp_M" : "grp_Mdag", prec, mode, s);
     if (prec == NSLX_F64) dispatch4g<PF64, T64_a, T64_b, T64_c, T64_d>(tile, op, mode, g, grid, s);
     else if (prec == NSLX_F32) dispatch4g<PF32, T32_a, T32_b, T32_c, T32_d>(tile, op, mode, g, grid, s);
@@ -599,6 +1034,7 @@ void launch_hop_grouped(int prec, int op, int mode, int tile, const GroupArgs& g
 }
 
 std::vector<int4> build_tile_list(std::vector<HopProblem>& probs, int prec, int tile) {
+    NSLX_REQUIRE(tile >= 0 && tile < SLICE_BASE, "hop: grouped launches take tiles 0-3");
     const TileInfo& ti = tile_menu(prec).at(tile);
     std::vector<int> order(probs.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
@@ -631,12 +1067,24 @@ std::vector<int4> build_tile_list(std::vector<HopProblem>& probs, int prec, int 
 }
 
 const std::vector<TileInfo>& tile_menu(int prec) {
-    static const std::vector<TileInfo> m64 = {info<T64_a>(0), info<T64_b>(1), info<T64_c>(2), info<T64_d>(3)};
-    static const std::vector<TileInfo> m32 = {info<T32_a>(0), info<T32_b>(1), info<T32_c>(2), info<T32_d>(3)};
-    static const std::vector<TileInfo> mb3 = {info<TB3_a>(0), info<TB3_b>(1), info<TB3_c>(2), info<TB3_d>(3)};
+    auto build = [](std::vector<TileInfo> m, const SliceEntry* menu, int kc) {
+        for (int i = 0; i < SLICE_N; ++i) m.push_back({SLICE_BASE + i, 16, menu[i].fn * 16, kc, 64 * menu[i].w});
+        return m;
+    };
+    static const std::vector<TileInfo> m64 =
+        build({info<T64_a>(0), info<T64_b>(1), info<T64_c>(2), info<T64_d>(3)}, SLICE_MENU64, SliceP<PF64>::KC);
+    static const std::vector<TileInfo> m32 =
+        build({info<T32_a>(0), info<T32_b>(1), info<T32_c>(2), info<T32_d>(3)}, SLICE_MENU32, SliceP<PF32>::KC);
+    static const std::vector<TileInfo> mb3 =
+        build({info<TB3_a>(0), info<TB3_b>(1), info<TB3_c>(2), info<TB3_d>(3)}, SLICE_MENUB3, SliceP<PB3>::KC);
     if (prec == NSLX_F64) return m64;
     if (prec == NSLX_F32) return m32;
     return mb3;
+}
+
+bool tile_fits(int prec, int tile, int nx) {
+    if (tile < SLICE_BASE) return true;
+    return slice_fits(prec, tile - SLICE_BASE, nx);
 }
 
 int hop_nparts(int prec, int tile, int nx, int nt) {
@@ -681,8 +1129,15 @@ void pack_E(int nx, const double* E, void* E64, void* E32, void* Ebf, hipStream_
 
 void launch_hop(int prec, int op, int mode, const HopArgs& a, int tile, hipStream_t s) {
     NSLX_REQUIRE(a.nx > 0 && a.nt > 0 && a.nb > 0, "hop: empty shape");
+    NSLX_REQUIRE(tile >= 0 && tile < SLICE_BASE + SLICE_N, "hop: bad tile id");
     ProfScope prof(op == HOP_M ? "hop_M" : "hop_Mdag", prec, mode, s);
-    if (prec == NSLX_F64) dispatch4<PF64, T64_a, T64_b, T64_c, T64_d>(tile, op, mode, a, s);
+    if (tile >= SLICE_BASE) {
+        const int id = tile - SLICE_BASE;
+        if (prec == NSLX_F64) dispatch_slice<PF64, 1, 2, 3>(id, op, mode, a, s, SLICE_MENU64);
+        else if (prec == NSLX_F32) dispatch_slice<PF32, 2, 3, 6>(id, op, mode, a, s, SLICE_MENU32);
+        else if (prec == NSLX_BF16X3) dispatch_slice<PB3, 2, 3, 6>(id, op, mode, a, s, SLICE_MENUB3);
+        else throw Error(NSLX_E_ARG, "hop: bad precision");
+    } else if (prec == NSLX_F64) dispatch4<PF64, T64_a, T64_b, T64_c, T64_d>(tile, op, mode, a, s);
     else if (prec == NSLX_F32) dispatch4<PF32, T32_a, T32_b, T32_c, T32_d>(tile, op, mode, a, s);
     else if (prec == NSLX_BF16X3) dispatch4<PB3, TB3_a, TB3_b, TB3_c, TB3_d>(tile, op, mode, a, s);
     else throw Error(NSLX_E_ARG, "hop: bad precision");

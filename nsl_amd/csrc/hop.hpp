@@ -114,6 +114,9 @@ void launch_hop_grouped(int prec, int op, int mode, int tile, const GroupArgs& g
 // keeps them on one XCD.
 std::vector<int4> build_tile_list(std::vector<HopProblem>& probs, int prec, int tile);
 const std::vector<TileInfo>& tile_menu(int prec);
+// Whether tile `tile` of tile_menu(prec) can run a lattice of nx sites (the
+// slice tiles keep a column slice of E in LDS, which bounds nx).
+bool tile_fits(int prec, int tile, int nx);
 int hop_nparts(int prec, int tile, int nx, int nt);
 // Heuristic tile choice when no tuned entry exists.
 int default_tile(int prec, int nx, int nt, int64_t nb);

@@ -335,7 +335,11 @@ int solve_cg_ru(FermionMatrix& fm, int op, int prec, const void* b, void* x, con
     const double fc = env_double("NSLX_FLOOR_C", 1.0) * 1.1102230246251565e-16 * nM * nM;
     RuPhase ph{fm, op, b, x, nb, fc * fc, env_double("NSLX_RU_DELTA", 0.1), env_double("NSLX_RU_FRAC", 0.5),
                (int64_t)env_double("NSLX_RU_MAXGAP", 200.0), poll_every(), s};
-    const int stall_max = (int)env_double("NSLX_RU_STALL", 8.0);
+    // A low-precision RHS is finished in FP64 only after its true residual
+    // has made no 4x progress over this many replacements.  CG on these
+    // matrices plateaus for thousands of iterations before dropping again,
+    // so a short window escalates RHS that FP32 would still solve.
+    const int stall_max = (int)env_double("NSLX_RU_STALL", 64.0);
 
     norm2_perb(1, b, nb, len, w.bn2.p, fm.vsc, s);
     set_tol2(w.bn2.p, w.tol2.p, nb, tol, 1, 0, s);

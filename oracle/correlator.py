@@ -8,7 +8,7 @@ MM^dagger (:139, CG defaults eps=1e-12), G = M^dagger X (:143),
 G.shift(-tsrc, -2, -1) (antiperiodic re-centring, shift.tpp:63-68),
 corr[t, y, x] += G[x, t, y] (:152), corr /= N (:160).
 
-Parity pin (tests/test_oracle_correlator.py): at phi = 0 the correlator is
+Parity pin (tests/test_oracle_core.py::test_correlator_free_propagator): at phi = 0 the correlator is
 the free propagator, computed independently as (M_dense)^{-1} and as the
 Matsubara sum over the eigenbasis of K.
 """

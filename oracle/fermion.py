@@ -10,7 +10,7 @@ Fields have shape (..., Nt, Nx), x fastest.  s = +1 particle, -1 hole;
 delta = beta/Nt; mu~ = delta*mu; P = exp(i s phi + s mu~), Pinv = exp(-i s phi
 - s mu~) (hubbardExp.hpp:116-128).
 
-Parity pins (tests/test_oracle_fermion.py): M, M^dagger against the dense
+Parity pins (tests/test_oracle_core.py): M, M^dagger against the dense
 matrix built from the index formula of hubbardExp.hpp:22-31 on tiny lattices
 (brute force), M^dagger = (M_dense)^H, MMdagger = M(Mdagger(.)), MdaggerM =
 Mdagger(M(.)), Hermiticity, the phi = 0 Matsubara spectrum; logDetM against

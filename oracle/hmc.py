@@ -13,7 +13,7 @@ TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
   acceptance exp(Re(H0 - H1)), accept iff uniform <= acceptance.  Momenta and
   the uniform are injected (drawn by oracle.rng) instead of torch's global RNG.
 
-Parity pins (tests/test_oracle_hmc.py): force against finite differences of
+Parity pins (tests/test_oracle_core.py): force against finite differences of
 the action (Tests/Action/test_force.cpp:75-99), leapfrog reversibility and
 O(eps^2) energy violation (Tests/Integrator/test_leapfrog.cpp), exact
 arithmetic for the gauge pieces.

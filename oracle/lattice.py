@@ -18,7 +18,7 @@ Follows:
   in x with period 3n, open along y over `cells` translation periods of
   length sqrt3; 4n sites per cell.
 
-Parity: pinned by closed forms in tests/test_oracle_lattice.py (honeycomb:
+Parity: pinned by closed forms in tests/test_oracle_core.py (honeycomb:
 valence 3, spectrum in [-3,3], Bloch spectrum |1 + e^{ik.a0} + e^{ik.a1}|,
 4 zero modes iff L0 and L1 are multiples of 3; E vs eigendecomposition;
 E(a)E(-a) = I).  Nanotube: site count, bipartiteness, valence (3 inside,

@@ -12,7 +12,7 @@ easy as 1, 2, 3", SC'11) and injects identical numbers into both paths.  The
 C++ host library implements the same generator (nslx_rng_* in include/nslx.h).
 
 Pinned by: the Random123 known-answer vectors for philox4x32_10
-(tests/test_oracle_rng.py) and by equality with the C++ implementation.
+(tests/test_oracle_core.py) and by equality with the C++ implementation.
 
 Stream layout (shared with the C++ side): block i of stream (seed, offset) is
 philox(counter=(i, offset & 0xffffffff, offset >> 32, 0),

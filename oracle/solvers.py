@@ -11,7 +11,7 @@ TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
 
 Each solver returns (x, info) with info = {"iterations", "rsqr", "converged"}.
 
-Parity pins (tests/test_oracle_solvers.py): random HPD systems against a
+Parity pins (tests/test_oracle_core.py): random HPD systems against a
 direct solve; the fermion-matrix systems through the FP64 true residual
 |b - A x| computed independently; the iteration recurrences are the
 reference's (not pinned beyond that: the reference cannot be run here).
