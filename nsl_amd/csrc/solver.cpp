@@ -335,28 +335,39 @@ int solve_cg_ru(FermionMatrix& fm, int op, int prec, const void* b, void* x, con
     const double fc = env_double("NSLX_FLOOR_C", 1.0) * 1.1102230246251565e-16 * nM * nM;
     RuPhase ph{fm, op, b, x, nb, fc * fc, env_double("NSLX_RU_DELTA", 0.1), env_double("NSLX_RU_FRAC", 0.5),
                (int64_t)env_double("NSLX_RU_MAXGAP", 200.0), poll_every(), s};
-    // A low-precision RHS is finished in FP64 only after its true residual
-    // has made no 4x progress over this many replacements.  CG on these
-    // matrices plateaus for thousands of iterations before dropping again,
-    // so a short window escalates RHS that FP32 would still solve.
-    const int stall_max = (int)env_double("NSLX_RU_STALL", 64.0);
+    // A low-precision RHS is also finished in FP64 after its true residual
+    // has made no 4x progress over this many replacements (the main test is
+    // the true/recursive residual gap, see ru_scalars).
+    const int stall_max = (int)env_double("NSLX_RU_STALL", 16.0);
 
     norm2_perb(1, b, nb, len, w.bn2.p, fm.vsc, s);
     set_tol2(w.bn2.p, w.tol2.p, nb, tol, 1, 0, s);
     if (x0) NSLX_HIP(hipMemcpyAsync(x, x0, b64, hipMemcpyDeviceToDevice, s));
     else NSLX_HIP(hipMemsetAsync(x, 0, b64, s));
     NSLX_HIP(hipMemsetAsync(w.done.p, 0xFF, nb * sizeof(int), s));   // all active (-1)
-    ph.run(prec, x0 == nullptr, prec == NSLX_F64 ? 0 : stall_max, maxit);
+    // Low-precision iteration budget: a system that low precision has not
+    // solved by then is ill-conditioned beyond what its rounding allows
+    // (eps_low * cond >> 1), and is solved in FP64 instead.
+    const int64_t lowmax = (int64_t)env_double("NSLX_RU_LOWMAX", 2000.0);
+    ph.run(prec, x0 == nullptr, prec == NSLX_F64 ? 0 : stall_max, prec == NSLX_F64 ? maxit : std::min(maxit, lowmax));
     int64_t escalated = 0;
     if (prec != NSLX_F64) {
-        // RHS whose true residual stagnated in low precision: finish in FP64
+        // RHS whose low-precision recurrence lost track, stagnated or ran out
+        // of budget: solve again in FP64 from x0 (their low-precision iterate
+        // carries rounding errors of order eps_low * cond and is discarded)
         std::vector<int> d(nb);
         NSLX_HIP(hipMemcpyAsync(d.data(), w.done.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
         NSLX_HIP(hipStreamSynchronize(s));
-        for (int v : d) escalated += (v == RU_ESCALATE || v < 0) ? 1 : 0;
+        for (auto& v : d)
+            if (v == RU_ESCALATE || v < 0) { v = RU_ESCALATE; ++escalated; }
         if (escalated > 0 && ph.it < maxit) {
-            // not-converged low-precision RHS (hit maxit) are escalated too
-            for (auto& v : d) if (v < 0) v = RU_ESCALATE;
+            const size_t rb = (size_t)len * 16;
+            for (int64_t i = 0; i < nb; ++i) {
+                if (d[i] != RU_ESCALATE) continue;
+                char* xi = static_cast<char*>(x) + i * rb;
+                if (x0) NSLX_HIP(hipMemcpyAsync(xi, static_cast<const char*>(x0) + i * rb, rb, hipMemcpyDeviceToDevice, s));
+                else NSLX_HIP(hipMemsetAsync(xi, 0, rb, s));
+            }
             NSLX_HIP(hipMemcpyAsync(w.done.p, d.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
             ph.run(NSLX_F64, false, 0, maxit - ph.it);
         }

@@ -259,18 +259,24 @@ __global__ void k_ru_scalars(const double* rn2, const double* xn2, double floor2
             cg.done[b] = iter;
             continue;
         }
+        double* rp = cg.rpart[cur] + (size_t)b * nparts;
         if (init) {
             best[b] = s;
             stall[b] = 0;
         } else if (stall_max > 0) {
-            // stagnation of the true residual: hand the RHS to a higher precision
+            // the low-precision recurrence has lost track of the true residual
+            // (true |r| 10x above the recursive one): its rounding errors
+            // dominate, so hand the RHS to FP64 now
+            double rec = 0.0;
+            for (int k = 0; k < nparts; ++k) rec += rp[k];
+            if (s > 100.0 * rec) { cg.done[b] = RU_ESCALATE; continue; }
+            // stagnation of the true residual over stall_max replacements
             if (s < 0.25 * best[b]) { best[b] = s; stall[b] = 0; }
             else if (++stall[b] >= stall_max) { cg.done[b] = RU_ESCALATE; continue; }
         }
         cg.done[b] = -1;
         cg.rr_ref[b] = s;
         // the replaced residual's |r|^2 becomes the partial sum the next CGA reads
-        double* rp = cg.rpart[cur] + (size_t)b * nparts;
         rp[0] = s;
         for (int k = 1; k < nparts; ++k) rp[k] = 0.0;
     }

@@ -51,9 +51,12 @@ void ru_accumulate(int low_is_f64, void* x64, void* xs, int64_t nb, int64_t len,
 // get done = iter, the others restart their reference |r|^2 and
 // their |r|^2 partials (rpart[cur], nparts per RHS) are replaced by rn2 so
 // that the next CGA continues the recurrence from the true residual.
-// With stall_max > 0 a RHS whose true |r|^2 has not fallen below a quarter
-// of its best value for stall_max replacements gets done = RU_ESCALATE (the
-// kernels skip it; the solver finishes it in FP64).  init re-initialises the
+// With stall_max > 0 (a low-precision phase) a RHS gets done = RU_ESCALATE
+// (the kernels skip it; the solver finishes it in FP64) when its true |r|^2
+// exceeds 100x the recursive |r|^2 held in rpart[cur] (the low-precision
+// recurrence has lost track of the residual), or when its true |r|^2 has not
+// fallen below a quarter of its best value for stall_max replacements.
+// init re-initialises the
 // RHS with done < 0 or done == RU_ESCALATE.  best/stall: nb entries.
 constexpr int RU_ESCALATE = 1 << 30;
 void ru_scalars(const double* rn2, const double* xn2, double floor2, const CgDev& cg, int64_t nb, int nparts,
