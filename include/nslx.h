@@ -155,6 +155,11 @@ typedef struct {
      * largest such bound relative to |b| over the RHS                        */
     int64_t floor_stops;
     double  max_floor_rel;
+    /* reliable-update solvers: sum over iterations of the right-hand sides
+     * still active, in the low precision and in FP64 (counted at the
+     * solver's polls of the device flags, every few iterations)             */
+    int64_t rhs_iters_low;
+    int64_t rhs_iters_f64;
 } nslx_solve_report;
 
 /* CG::operator()(b[, x0]) / BiCGStab::operator()(b[, x0]) on the operator

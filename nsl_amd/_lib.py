@@ -25,7 +25,8 @@ TOL_ABS_BATCH, TOL_REL_PER_RHS = 0, 1
 class SolveReport(ctypes.Structure):
     _fields_ = [("iterations", c_int64), ("iterations_total", c_int64), ("outer_steps", c_int64),
                 ("converged_rhs", c_int64), ("rsqr", c_double), ("max_rel_residual", c_double),
-                ("seconds", c_double), ("floor_stops", c_int64), ("max_floor_rel", c_double)]
+                ("seconds", c_double), ("floor_stops", c_int64), ("max_floor_rel", c_double),
+                ("rhs_iters_low", c_int64), ("rhs_iters_f64", c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

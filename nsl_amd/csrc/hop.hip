@@ -863,7 +863,7 @@ void launch_slice_k(const HopArgs& a, int R, hipStream_t s) {
         (void)hipGetLastError();
         attr = true;
     }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * W), lds, s, a, nrt, nct, P, R, swz, kp, ldE, planeE);
+    launch_timed(kern, dim3(grid), dim3(64 * W), lds, s, a, nrt, nct, P, R, swz, kp, ldE, planeE);
 }
 
 template <class PR, int FN, int W>
@@ -963,7 +963,7 @@ void launch_t(int op, int mode, const HopArgs& a, hipStream_t s) {
     dim3 gr(grid), bl(TL::NT);
 #define NSLX_HOP_CASE(OPV, MV)                                                                  \
     if (op == OPV && mode == MV) {                                                              \
-        hipLaunchKernelGGL((hop_kernel<TL, PR, OPV, MV>), gr, bl, 0, s, a, nrt, nct, swz, ldE, planeE); \
+        launch_timed(hop_kernel<TL, PR, OPV, MV>, gr, bl, 0, s, a, nrt, nct, swz, ldE, planeE);       \
         return;                                                                                 \
     }
     NSLX_HOP_CASE(HOP_M, MODE_PLAIN)
@@ -994,7 +994,7 @@ void launch_g(int op, int mode, const GroupArgs& g, int grid, hipStream_t s) {
     dim3 gr(grid), bl(TL::NT);
 #define NSLX_HOPG_CASE(OPV, MV)                                                    \
     if (op == OPV && mode == MV) {                                                 \
-        hipLaunchKernelGGL((hop_grouped<TL, PR, OPV, MV>), gr, bl, 0, s, g);        \
+        launch_timed(hop_grouped<TL, PR, OPV, MV>, gr, bl, 0, s, g);                \
         return;                                                                    \
     }
     NSLX_HOPG_CASE(HOP_M, MODE_PLAIN)

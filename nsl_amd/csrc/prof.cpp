@@ -30,7 +30,16 @@ hipEvent_t take() {
 
 bool prof_on() { return g_on; }
 
-ProfScope::ProfScope(const char* name, int prec, int mode, hipStream_t s) : s_(s), active_(g_on) {
+namespace {
+thread_local ProfScope* t_scope = nullptr;
+}
+
+ProfScope* prof_current() { return t_scope; }
+
+ProfScope::ProfScope(const char* name, int prec, int mode, hipStream_t s) : active_(g_on) {
+    (void)s;
+    outer_ = t_scope;
+    t_scope = this;
     if (!active_) return;
     static const char* pn[] = {"f64", "f32", "bf16x3"};
     static const char* mn[] = {"plain", "cga", "cgb", "cgi"};
@@ -38,14 +47,26 @@ ProfScope::ProfScope(const char* name, int prec, int mode, hipStream_t s) : s_(s
     name_ = std::string(name) + "_" + pn[prec % 3] + "_" + mn[mode % 4];
     a_ = take();
     b_ = take();
-    NSLX_HIP(hipEventRecord((hipEvent_t)a_, s_));
+}
+
+bool ProfScope::take_events(hipEvent_t* start, hipEvent_t* stop) {
+    if (!active_ || used_) return false;
+    used_ = true;
+    *start = (hipEvent_t)a_;
+    *stop = (hipEvent_t)b_;
+    return true;
 }
 
 ProfScope::~ProfScope() {
+    t_scope = outer_;
     if (!active_) return;
     std::lock_guard<std::mutex> g(g_mu);
-    if (hipEventRecord((hipEvent_t)b_, s_) == hipSuccess)
+    if (used_) {
         g_recs.push_back({name_, (hipEvent_t)a_, (hipEvent_t)b_});
+    } else {   // no kernel was launched in the scope (an error path)
+        g_pool.push_back((hipEvent_t)a_);
+        g_pool.push_back((hipEvent_t)b_);
+    }
 }
 
 void prof_enable(bool on) { g_on = on; }

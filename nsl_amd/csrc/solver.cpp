@@ -230,6 +230,7 @@ struct RuPhase {
     int64_t it = 0;      // iterations so far (all phases)
     int64_t nupd = 0;    // FP64 residual replacements so far
     int64_t active = 0, waiting = 0;
+    int64_t rhs_iters[2] = {0, 0};   // active RHS x iterations: [0] low precision, [1] FP64
 
     // host copy of done[] / want[]: active RHS and those not yet asking for
     // a residual replacement
@@ -284,7 +285,7 @@ struct RuPhase {
         HopArgs a = fm.hop_args(prec, nb, s);
         a.cg = cg;
         void* pb[2] = {w.p0.p, w.p1.p};
-        int64_t k = 0, last_upd = 0;
+        int64_t k = 0, last_upd = 0, kprev = 0, act_prev = active;
         while (!finished && k < maxit) {
             ++k;
             ++it;
@@ -304,6 +305,9 @@ struct RuPhase {
             launch_hop(prec, outer, MODE_CGB, B, tile, s);
             if (k % chk == 0 || k == maxit) {
                 read_flags();
+                rhs_iters[low64] += act_prev * (k - kprev);   // active RHS x iterations since the last poll
+                kprev = k;
+                act_prev = active;
                 if (waiting <= (1.0 - frac) * active || k - last_upd >= maxgap || k == maxit) {
                     ru_accumulate(low64, x, w.x32.p, nb, len, s);
                     true_residual(false);
@@ -313,6 +317,7 @@ struct RuPhase {
                     ++nupd;
                     last_upd = k;
                     read_flags();
+                    act_prev = active;
                     finished = active == 0;
                 }
             }
@@ -405,6 +410,8 @@ int solve_cg_ru(FermionMatrix& fm, int op, int prec, const void* b, void* x, con
         rep->max_rel_residual = maxrel;
         rep->floor_stops = floor_stops;
         rep->max_floor_rel = maxfloor;
+        rep->rhs_iters_low = ph.rhs_iters[0];
+        rep->rhs_iters_f64 = ph.rhs_iters[1];
     }
     return conv == nb ? NSLX_OK : NSLX_NOT_CONVERGED;
 }
