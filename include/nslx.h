@@ -174,6 +174,23 @@ int nslx_solve(nslx_fm fm, int op, int method, const void* b, void* x, const voi
                int64_t nbatch, double tol, int tol_kind, int64_t maxit, int precision,
                int check_true_residual, nslx_solve_report* report, void* stream);
 
+/* ---- measurements ----------------------------------------------------------------
+ * TwoPointCorrelator::measure(N)
+ * (src/NSL/Measurements/Hubbard/twoPointCorrelationFunction.tpp:116-162) on
+ * fm's populated field (nfield must be 1; species as populated): point
+ * sources S[x][tsrc][x] = 1 at tsrc = 0, Nt/N, 2 Nt/N, ... < Nt, X = CG on
+ * MMdagger to eps, G = Mdagger X, G shifted by -tsrc in time with the
+ * antiperiodic sign, corr[t][y][x] += G[x][t][y], corr /= N.
+ * corr: device c128 (nt, nx, nx).  tol_kind NSLX_TOL_ABS_BATCH reproduces
+ * the reference (one CG per time source over its nx sources, x0 = b,
+ * sum |r|^2 <= eps^2); NSLX_TOL_REL_PER_RHS solves up to max_rhs sources per
+ * batch (<= 0: 4096) with per-RHS relative stops and allows `precision`
+ * F32 / BF16X3 (reliable updates, FP64 true residual).  report: aggregate
+ * over the solves (iterations = max).  Synchronous. */
+int nslx_correlator_two_point(nslx_fm fm, int64_t n_tsrc, double eps, int64_t maxit, int tol_kind,
+                              int precision, int64_t max_rhs, void* corr, nslx_solve_report* report,
+                              void* stream);
+
 /* ---- HMC ---------------------------------------------------------------------
  * Hubbard action S = S_gauge + S_fermion and its force for one real field
  * (src/NSL/Action/Implementations/hubbardGaugeAction.tpp:59-67,

@@ -284,13 +284,50 @@ class CG(_Solver):
     """NSL::LinAlg::CG (CG.hpp:31-186, CG.tpp:14-113): x0 = b by default,
     batch-global alpha/beta and |r|^2 <= eps^2 stop (TOL_ABS_BATCH, the
     default for FP64); TOL_REL_PER_RHS solves each RHS to |r|/|b| <= eps.
-    precision=F32/BF16X3 runs the FP64 defect-correction solver."""
+    precision=F32/BF16X3 runs the reliable-update solver (low-precision
+    recurrence, FP64 true residual)."""
     method = _lib.CG_METHOD
 
 
 class BiCGStab(_Solver):
     """NSL::LinAlg::BiCGStab (BiCGStab.hpp:31-200, BiCGStab.tpp:14-150)."""
     method = _lib.BICGSTAB_METHOD
+
+
+# ------------------------------------------------------------- measurements
+class TwoPointCorrelator:
+    """NSL::Measure::Hubbard::TwoPointCorrelator::measure(N)
+    (twoPointCorrelationFunction.tpp:116-162) on a populated HubbardExp with
+    one field.  measure() returns corr (Nt, Nx, Nx) complex128 on the device,
+    corr[t, y, x].  The default (TOL_ABS_BATCH, eps 1e-12) is the reference's
+    CG per time source; TOL_REL_PER_RHS batches time sources and allows
+    precision=F32/BF16X3."""
+
+    def __init__(self, fm, eps=1e-12, maxIter=10000, tol_kind=TOL_ABS_BATCH, precision=F64, max_rhs=0):
+        self.fm = fm
+        self.eps = float(eps)
+        self.maxIter = int(maxIter)
+        self.tol_kind = int(tol_kind)
+        self.precision = int(precision)
+        self.max_rhs = int(max_rhs)
+        self.report = None
+        self.converged = None
+
+    def measure(self, NumberTimeSources=None, out=None):
+        torch = _torch()
+        n = int(NumberTimeSources or self.fm.Nt)
+        corr = out if out is not None else torch.empty((self.fm.Nt, self.fm.Nx, self.fm.Nx),
+                                                       dtype=torch.complex128, device="cuda")
+        rep = SolveReport()
+        st = _lib.lib().nslx_correlator_two_point(self.fm._h, n, self.eps, self.maxIter, self.tol_kind,
+                                                  self.precision, self.max_rhs, _ptr(corr), ctypes.byref(rep),
+                                                  _stream())
+        _lib.check(st, allow=(_lib.OK, _lib.NOT_CONVERGED))
+        self.report = rep.as_dict()
+        self.converged = st == _lib.OK
+        if not self.converged:
+            log.error("TwoPointCorrelator: CG did not converge within %d iterations", self.maxIter)
+        return corr
 
 
 # ------------------------------------------------------------------ HMC

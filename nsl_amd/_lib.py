@@ -62,6 +62,8 @@ PROTOS = {
     "nslx_fm_grad_logdet": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
     "nslx_solve": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int64, c_double, c_int,
                            c_int64, c_int, c_int, ctypes.POINTER(SolveReport), c_void_p]),
+    "nslx_correlator_two_point": (c_int, [c_void_p, c_int64, c_double, c_int64, c_int, c_int, c_int64, c_void_p,
+                                          ctypes.POINTER(SolveReport), c_void_p]),
     "nslx_hmc_action": (c_int, [c_void_p, c_void_p, c_double, c_int, P_double, c_void_p]),
     "nslx_hmc_force": (c_int, [c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
     "nslx_hmc_trajectory": (c_int, [c_void_p, c_void_p, c_void_p, P_double, c_double, c_double, c_int,

@@ -6,6 +6,7 @@
 #include <algorithm>
 
 #include "engine.hpp"
+#include "corr.hpp"
 #include "hmc.hpp"
 #include "prof.hpp"
 
@@ -209,6 +210,16 @@ int nslx_solve(nslx_fm fm, int op, int method, const void* b, void* x, const voi
         DeviceGuard dg(m.ctx->device);
         return nslx::solve(m, op, method, b, x, x0, nbatch, tol, tol_kind, maxit, precision, check_true_residual,
                            report, nslx::as_stream(stream));
+    });
+}
+
+int nslx_correlator_two_point(nslx_fm fm, int64_t n_tsrc, double eps, int64_t maxit, int tol_kind, int precision,
+                              int64_t max_rhs, void* corr, nslx_solve_report* report, void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        DeviceGuard dg(m.ctx->device);
+        return nslx::fm_two_point(m, n_tsrc, eps, maxit, tol_kind, precision, max_rhs, corr, report,
+                                  nslx::as_stream(stream));
     });
 }
 
