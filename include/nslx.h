@@ -174,6 +174,41 @@ int nslx_solve(nslx_fm fm, int op, int method, const void* b, void* x, const voi
                int64_t nbatch, double tol, int tol_kind, int64_t maxit, int precision,
                int check_true_residual, nslx_solve_report* report, void* stream);
 
+/* ---- batched ragged solve ----------------------------------------------------------
+ * No single counterpart in the reference (it solves one HubbardExp at a
+ * time, CG.tpp:14-113); this is the reference's CG on MMdagger for many
+ * fermion matrices at once.  Each item is a populated fm (own lattice, Nt,
+ * beta, fields) with its own b and x (device c128, nbatch x nt x nx); all fms
+ * must belong to ctx.  Every Krylov iteration is one grouped persistent hop
+ * launch per phase over the tiles of all items (per precision / tile class);
+ * with use_graph the iteration loop is captured once as a HIP graph and
+ * replayed.  Per-RHS stop: |b - A x| <= tol |b| in FP64 or the FP64 attainable
+ * accuracy (c eps ||A|| |x|).  precision F32 / BF16X3 runs restarted defect
+ * correction (low-precision inner CG to 1e-5 relative, FP64 residual),
+ * switching an item to FP64 when a low-precision step gains less than 10x.
+ * An item's iterates do not depend on the other items.  Synchronous. */
+typedef struct {
+    nslx_fm     fm;
+    const void* b;
+    void*       x;
+    int64_t     nbatch;
+} nslx_batch_item;
+
+typedef struct {
+    int64_t iterations;         /* global Krylov iterations run               */
+    int64_t replays;            /* graph replays (or launch rounds)           */
+    int64_t restarts;           /* FP64 residual restarts over all items      */
+    int64_t escalations;        /* items switched from low precision to FP64  */
+    int64_t converged_items;
+    int64_t floor_stops;        /* RHS stopped at the attainable accuracy     */
+    double  max_rel_residual;   /* max over RHS of |b - A x| / |b|, FP64      */
+    double  max_floor_rel;
+    double  seconds;
+} nslx_batch_report;
+
+int nslx_batch_solve(nslx_ctx ctx, int64_t nitems, const nslx_batch_item* items, double tol, int precision,
+                     int64_t maxit, int use_graph, nslx_batch_report* report, void* stream);
+
 /* ---- measurements ----------------------------------------------------------------
  * TwoPointCorrelator::measure(N)
  * (src/NSL/Measurements/Hubbard/twoPointCorrelationFunction.tpp:116-162) on

@@ -294,6 +294,36 @@ class BiCGStab(_Solver):
     method = _lib.BICGSTAB_METHOD
 
 
+def batch_solve(problems, tol=1e-12, precision=F64, maxIter=100000, use_graph=True, ctx=None):
+    """Solve MMdagger x = b for many fermion matrices of different Nx, Nt
+    and fields at once (nslx_batch_solve): problems is a list of (fm, b)
+    with b (nbatch, Nt, Nx) complex128 on the device.  Returns (list of x,
+    report dict); raises on errors, reports non-convergence in the dict."""
+    torch = _torch()
+    if not problems:
+        raise ValueError("batch_solve: no problems")
+    ctx = ctx or problems[0][0].ctx
+    items = (_lib.BatchItem * len(problems))()
+    keep, xs = [], []
+    for i, (fm, b) in enumerate(problems):
+        b = b.to(dtype=torch.complex128).contiguous()
+        nb = int(np.prod(b.shape[:-2])) if b.dim() > 2 else 1
+        x = torch.empty_like(b)
+        keep.append(b)
+        xs.append(x)
+        items[i].fm = fm._h
+        items[i].b = _ptr(b)
+        items[i].x = _ptr(x)
+        items[i].nbatch = nb
+    rep = _lib.BatchReport()
+    st = _lib.lib().nslx_batch_solve(ctx._h, len(problems), items, float(tol), int(precision), int(maxIter),
+                                     int(bool(use_graph)), ctypes.byref(rep), _stream())
+    _lib.check(st, allow=(_lib.OK, _lib.NOT_CONVERGED))
+    d = rep.as_dict()
+    d["converged"] = st == _lib.OK
+    return xs, d
+
+
 # ------------------------------------------------------------- measurements
 class TwoPointCorrelator:
     """NSL::Measure::Hubbard::TwoPointCorrelator::measure(N)

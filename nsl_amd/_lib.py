@@ -32,6 +32,19 @@ class SolveReport(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchItem(ctypes.Structure):
+    _fields_ = [("fm", c_void_p), ("b", c_void_p), ("x", c_void_p), ("nbatch", c_int64)]
+
+
+class BatchReport(ctypes.Structure):
+    _fields_ = [("iterations", c_int64), ("replays", c_int64), ("restarts", c_int64), ("escalations", c_int64),
+                ("converged_items", c_int64), ("floor_stops", c_int64), ("max_rel_residual", c_double),
+                ("max_floor_rel", c_double), ("seconds", c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class HmcReport(ctypes.Structure):
     _fields_ = [("dH_re", c_double), ("dH_im", c_double), ("acc_prob", c_double), ("accepted", c_int),
                 ("S_re", c_double), ("S_im", c_double), ("seconds", c_double)]
@@ -62,6 +75,8 @@ PROTOS = {
     "nslx_fm_grad_logdet": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
     "nslx_solve": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int64, c_double, c_int,
                            c_int64, c_int, c_int, ctypes.POINTER(SolveReport), c_void_p]),
+    "nslx_batch_solve": (c_int, [c_void_p, c_int64, ctypes.POINTER(BatchItem), c_double, c_int, c_int64, c_int,
+                                 ctypes.POINTER(BatchReport), c_void_p]),
     "nslx_correlator_two_point": (c_int, [c_void_p, c_int64, c_double, c_int64, c_int, c_int, c_int64, c_void_p,
                                           ctypes.POINTER(SolveReport), c_void_p]),
     "nslx_hmc_action": (c_int, [c_void_p, c_void_p, c_double, c_int, P_double, c_void_p]),

@@ -7,6 +7,7 @@
 
 #include "engine.hpp"
 #include "corr.hpp"
+#include "pool.hpp"
 #include "hmc.hpp"
 #include "prof.hpp"
 
@@ -210,6 +211,18 @@ int nslx_solve(nslx_fm fm, int op, int method, const void* b, void* x, const voi
         DeviceGuard dg(m.ctx->device);
         return nslx::solve(m, op, method, b, x, x0, nbatch, tol, tol_kind, maxit, precision, check_true_residual,
                            report, nslx::as_stream(stream));
+    });
+}
+
+int nslx_batch_solve(nslx_ctx ctx, int64_t nitems, const nslx_batch_item* items, double tol, int precision,
+                     int64_t maxit, int use_graph, nslx_batch_report* report, void* stream) {
+    return guard([&] {
+        NSLX_REQUIRE(ctx, "null context handle");
+        NSLX_REQUIRE(precision == NSLX_F64 || precision == NSLX_F32 || precision == NSLX_BF16X3,
+                     "batch_solve: bad precision");
+        DeviceGuard dg(ctx->c.device);
+        return nslx::batch_solve(ctx->c, nitems, items, tol, precision, maxit, use_graph, report,
+                                 nslx::as_stream(stream));
     });
 }
 

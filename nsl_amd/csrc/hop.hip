@@ -464,7 +464,9 @@ __global__ void __launch_bounds__(TL::NT) hop_grouped(GroupArgs g) {
     constexpr int NT = TL::NT;
     __shared__ __attribute__((aligned(16))) Smem<TL, PR> sm;
     __shared__ double red[NT / 64];
-    for (int t = blockIdx.x; t < g.ntiles; t += gridDim.x) {
+    const int it = g.iter_dev ? *g.iter_dev : g.iter;
+    const int ntiles = g.ntiles_dev ? *g.ntiles_dev : g.ntiles;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int4 T = g.tiles[t];
         const HopProblem& P = g.probs[T.x];
         const int b = T.y, rt = T.z, ct = T.w;
@@ -475,10 +477,10 @@ __global__ void __launch_bounds__(TL::NT) hop_grouped(GroupArgs g) {
         double alpha = 0.0, beta = 0.0;
         if (MODE == MODE_CGA || MODE == MODE_CGB) {
             const CgDev& c = g.cg;
-            const double rr1 = sum_parts(c.rpart[(g.iter - 1) & 1], off, nparts);
-            const double rr2 = (MODE == MODE_CGA && g.iter >= 2) ? sum_parts(c.rpart[(g.iter - 2) & 1], off, nparts) : 0.0;
+            const double rr1 = sum_parts(c.rpart[(it - 1) & 1], off, nparts);
+            const double rr2 = (MODE == MODE_CGA && it >= 2) ? sum_parts(c.rpart[(it - 2) & 1], off, nparts) : 0.0;
             const double uu = MODE == MODE_CGB ? sum_parts(c.upart, off, nparts) : 0.0;
-            skip = cg_decide<MODE>(c, slot, g.iter, rt == 0 && ct == 0, rr1, rr2, uu, alpha, beta);
+            skip = cg_decide<MODE>(c, slot, it, rt == 0 && ct == 0, rr1, rr2, uu, alpha, beta);
         }
         double part = 0.0;
         if (!skip) {
@@ -491,7 +493,7 @@ __global__ void __launch_bounds__(TL::NT) hop_grouped(GroupArgs g) {
         if (MODE != MODE_PLAIN && !skip) {
             const double tot = block_sum<NT>(part, red);
             if (threadIdx.x == 0) {
-                double* dst = MODE == MODE_CGA ? g.cg.upart : g.cg.rpart[MODE == MODE_CGB ? (g.iter & 1) : 0];
+                double* dst = MODE == MODE_CGA ? g.cg.upart : g.cg.rpart[MODE == MODE_CGB ? (it & 1) : 0];
                 dst[off + rt * P.nct + ct] = tot;
             }
         } else {

@@ -89,6 +89,10 @@ struct GroupArgs {
     signed char role[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
     CgDev cg;
     int iter = 0;
+    // when set, the iteration number and the tile count are read from device
+    // memory (so that one captured launch sequence can be replayed)
+    const int* iter_dev = nullptr;
+    const int* ntiles_dev = nullptr;
 };
 
 // Layout of E in device memory: row y (output site), column x, row stride

@@ -1,0 +1,467 @@
+// Ragged batched CG engine (see pool.hpp).
+//
+// Every job owns a contiguous range of RHS slots (per-RHS tol2 / done flags)
+// and of tile-partial slots.  A global iteration counter k lives on the
+// device: iteration k is one grouped CGA launch (M^dag hop with p = r + beta
+// p, |u|^2 partials) and one grouped CGB launch (M hop with x += alpha p,
+// r -= alpha t, |r|^2 partials) per (precision, tile) class, followed by a
+// one-thread kernel that advances k.  The kernels read k and the tile count
+// from device memory, so the launch sequence of P iterations is captured once
+// as a HIP graph and replayed; jobs join and leave between replays by
+// rewriting the device tile lists.  A job joining before iteration k puts its
+// first search direction in p[(k-1)&1], its |r|^2 in rpart[(k-1)&1] and zeros
+// in rpart[k&1], so that its first beta is 0; P is even, so every replay
+// starts at the same parity and the captured roles stay valid.
+//
+// When all RHS of a job have met their recursive tolerance the job is
+// finalised in FP64: x (+= the low-precision increment), the true residual
+// b - A x, and the per-RHS test |r| <= tol |b| or |r| <= c eps ||A|| |x| (the
+// FP64 attainable accuracy).  Otherwise the job restarts from x: FP64 jobs with
+// a fresh recurrence, low-precision jobs as the next defect-correction step,
+// or in FP64 when the last low-precision step gained less than 10x.
+#include "pool.hpp"
+#include "prof.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <chrono>
+#include <cstring>
+
+namespace nslx {
+namespace {
+
+__global__ void k_inc(int* it) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *it += 1;
+}
+
+// Slot initialisation of a job joining before iteration k.
+__global__ void k_join(int64_t nb, int64_t rhs0, int64_t part0, int nparts, const double* __restrict__ rn2,
+                       const double* __restrict__ bn2, double tol, double delta, double* tol2, int* done, int* want,
+                       double* rp_cur, double* rp_prev) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb * nparts) return;
+    const int64_t b = i / nparts;
+    const int k = (int)(i - b * nparts);
+    rp_cur[part0 + i] = k == 0 ? rn2[b] : 0.0;
+    rp_prev[part0 + i] = 0.0;
+    if (k == 0) {
+        tol2[rhs0 + b] = std::max(tol * tol * bn2[b], delta * delta * rn2[b]);
+        done[rhs0 + b] = -1;
+        want[rhs0 + b] = 0;
+    }
+}
+
+inline unsigned g1(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+size_t PoolJob::bytes() const {
+    const size_t len = fm ? (size_t)fm->len() : 0;
+    const size_t v = (size_t)nb * len;
+    const size_t low = prec == NSLX_F64 ? 16 : 8;
+    return v * (4 * low + (prec == NSLX_F64 ? 0 : low) + 2 * 16);
+}
+
+Pool::Pool(Ctx* ctx, hipStream_t s, int64_t cap_rhs, int use_graph, int iters_per_replay)
+    : ctx_(ctx), s_(s), cap_rhs_(cap_rhs), use_graph_(use_graph), P_(std::max(2, iters_per_replay & ~1)) {
+    NSLX_REQUIRE(cap_rhs >= 1, "pool: capacity must be >= 1");
+    cap_parts_ = cap_rhs * 128;
+    tol2_.alloc(cap_rhs);
+    rr_ref_.alloc(cap_rhs);
+    done_.alloc(cap_rhs);
+    want_.alloc(cap_rhs);
+    upart_.alloc(cap_parts_);
+    rpart0_.alloc(cap_parts_);
+    rpart1_.alloc(cap_parts_);
+    it_dev_.alloc(1);
+    const int one = 1;
+    NSLX_HIP(hipMemcpyAsync(it_dev_.p, &one, sizeof(int), hipMemcpyHostToDevice, s_));
+    rhs_used_.assign(cap_rhs, 0);
+    part_used_.assign(cap_parts_, 0);
+    done_h_.assign(cap_rhs, 0);
+    const int ncu = ctx->num_cu > 0 ? ctx->num_cu : 256;
+    for (int c = 0; c < NCLS; ++c) {
+        const int tile = c % 4;
+        const int per_cu = tile == 2 ? 8 : 4;
+        cls_[c].grid = (ncu * per_cu + 7) / 8 * 8;
+        cls_[c].ntiles.alloc(1);
+        const int zero = 0;
+        NSLX_HIP(hipMemcpyAsync(cls_[c].ntiles.p, &zero, sizeof(int), hipMemcpyHostToDevice, s_));
+    }
+    NSLX_HIP(hipStreamSynchronize(s_));
+}
+
+Pool::~Pool() {
+    (void)hipStreamSynchronize(s_);
+    drop_graph();
+}
+
+void Pool::drop_graph() {
+    if (exec_) { (void)hipGraphExecDestroy(exec_); exec_ = nullptr; }
+    if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+    cls_mask_ = 0;
+}
+
+int64_t Pool::free_rhs() const {
+    return (int64_t)std::count(rhs_used_.begin(), rhs_used_.end(), (char)0);
+}
+
+int64_t Pool::alloc_range(std::vector<char>& used, int64_t n) {
+    int64_t run = 0;
+    for (int64_t i = 0; i < (int64_t)used.size(); ++i) {
+        run = used[i] ? 0 : run + 1;
+        if (run == n) {
+            const int64_t s = i - n + 1;
+            std::fill(used.begin() + s, used.begin() + s + n, (char)1);
+            return s;
+        }
+    }
+    return -1;
+}
+
+void Pool::free_range(std::vector<char>& used, int64_t start, int64_t n) {
+    if (start >= 0) std::fill(used.begin() + start, used.begin() + start + n, (char)0);
+}
+
+CgDev Pool::cgdev() const {
+    CgDev c;
+    c.upart = upart_.p;
+    c.rpart[0] = rpart0_.p;
+    c.rpart[1] = rpart1_.p;
+    c.tol2 = tol2_.p;
+    c.done = done_.p;
+    c.global = 0;
+    c.ru = 0;
+    c.rr_ref = rr_ref_.p;
+    c.want = want_.p;
+    return c;
+}
+
+// (Re)start the recurrence of job j before global iteration iter_: fresh
+// (x = 0, r = b) or from the current x (r = b - A x in FP64).
+void Pool::start_recurrence(PoolJob& j, bool fresh) {
+    FermionMatrix& fm = *j.fm;
+    const int64_t len = fm.len();
+    const size_t n = (size_t)j.nb * len;
+    const size_t esz = j.prec == NSLX_F64 ? 16 : 8;
+    j.r.ensure(n * esz); j.p0.ensure(n * esz); j.p1.ensure(n * esz); j.u.ensure(n * esz);
+    if (j.prec != NSLX_F64) j.xs.ensure(n * esz);
+    j.r64.ensure(n); j.t64.ensure(n);
+    j.rn2.ensure(j.nb); j.xn2.ensure(j.nb);
+    if (fresh) {
+        NSLX_HIP(hipMemsetAsync(j.x, 0, n * 16, s_));
+        NSLX_HIP(hipMemcpyAsync(j.r64.p, j.b, n * 16, hipMemcpyDeviceToDevice, s_));
+        NSLX_HIP(hipMemcpyAsync(j.rn2.p, j.bn2.p, j.nb * sizeof(double), hipMemcpyDeviceToDevice, s_));
+    } else {
+        fm.apply(j.op, j.x, j.t64.p, j.nb, NSLX_F64, s_);
+        resid64(j.b, j.t64.p, j.r64.p, j.nb, len, j.rn2.p, fm.vsc, s_);
+    }
+    if (j.prec == NSLX_F64) NSLX_HIP(hipMemcpyAsync(j.r.p, j.r64.p, n * 16, hipMemcpyDeviceToDevice, s_));
+    else {
+        c128_to_c64(j.r64.p, j.r.p, (int64_t)n, s_);
+        NSLX_HIP(hipMemsetAsync(j.xs.p, 0, n * esz, s_));
+    }
+    void* pprev = ((iter_ - 1) & 1) ? j.p1.p : j.p0.p;
+    NSLX_HIP(hipMemcpyAsync(pprev, j.r.p, n * esz, hipMemcpyDeviceToDevice, s_));
+
+    // class, tile and partial slots
+    int tile = fm.tile(j.prec, j.nb);
+    if (tile >= 4 || !tile_fits(j.prec, tile, fm.nx)) tile = default_tile(j.prec, fm.nx, fm.nt, j.nb);
+    j.tile = tile;
+    j.cls = cls_of(j.prec, tile);
+    j.nparts = hop_nparts(j.prec, tile, fm.nx, fm.nt);
+    j.part0 = alloc_range(part_used_, j.nb * j.nparts);
+    NSLX_REQUIRE(j.part0 >= 0, "pool: out of partial-sum slots");
+    double* cur = ((iter_ - 1) & 1) ? rpart1_.p : rpart0_.p;
+    double* prev = (iter_ & 1) ? rpart1_.p : rpart0_.p;
+    const double delta = j.prec == NSLX_F64 ? 0.0 : env_double("NSLX_POOL_INNER", 1e-5);
+    const int64_t m = j.nb * j.nparts;
+    hipLaunchKernelGGL(k_join, dim3(g1(m)), dim3(256), 0, s_, j.nb, j.rhs0, j.part0, j.nparts, j.rn2.p, j.bn2.p, j.tol,
+                       delta, tol2_.p, done_.p, want_.p, cur, prev);
+    NSLX_LAUNCH_CHECK();
+    // relative residual at the start of this recurrence (host)
+    std::vector<double> rn(j.nb), bn(j.nb);
+    NSLX_HIP(hipMemcpyAsync(rn.data(), j.rn2.p, j.nb * sizeof(double), hipMemcpyDeviceToHost, s_));
+    NSLX_HIP(hipMemcpyAsync(bn.data(), j.bn2.p, j.nb * sizeof(double), hipMemcpyDeviceToHost, s_));
+    NSLX_HIP(hipStreamSynchronize(s_));
+    double rel = 0.0;
+    for (int64_t i = 0; i < j.nb; ++i) rel = std::max(rel, bn[i] > 0 ? std::sqrt(rn[i] / bn[i]) : 0.0);
+    j.rel_start = rel;
+    j.join_iter = iter_;
+}
+
+void Pool::admit(std::unique_ptr<PoolJob> jp) {
+    PoolJob& j = *jp;
+    NSLX_REQUIRE(j.fm && j.b && j.x && j.nb >= 1, "pool: bad job");
+    j.fm->require_populated();
+    NSLX_REQUIRE(j.nb % j.fm->nfield == 0, "pool: nbatch must be a multiple of the populated fields");
+    NSLX_REQUIRE(j.op == NSLX_OP_MMDAGGER, "pool: only MMdagger systems are batched");
+    NSLX_REQUIRE(j.prec == NSLX_F64 || j.prec == NSLX_F32 || j.prec == NSLX_BF16X3, "pool: bad precision");
+    j.rhs0 = alloc_range(rhs_used_, j.nb);
+    NSLX_REQUIRE(j.rhs0 >= 0, "pool: out of RHS slots");
+    j.bn2.ensure(j.nb);
+    norm2_perb(1, j.b, j.nb, j.fm->len(), j.bn2.p, j.fm->vsc, s_);
+    start_recurrence(j, true);
+    const int c = j.cls;
+    jobs_.push_back(std::move(jp));
+    rebuild_class(c);
+}
+
+void Pool::rebuild_class(int c) {
+    Cls& K = cls_[c];
+    std::vector<HopProblem> probs;
+    std::vector<PoolJob*> owners;
+    for (auto& jp : jobs_) {
+        PoolJob& j = *jp;
+        if (j.cls != c || j.finished) continue;
+        FermionMatrix& fm = *j.fm;
+        HopProblem P;
+        P.nx = fm.nx;
+        P.nt = fm.nt;
+        P.nb = (int)j.nb;
+        P.rpf = (int)(j.nb / fm.nfield);
+        P.rhs0 = (int)j.rhs0;
+        P.part0 = j.part0;
+        P.E = fm.Eptr(j.prec, s_);
+        P.P = fm.Pptr(j.prec);
+        P.v[VS_R] = j.r.p;
+        P.v[VS_P0] = j.p0.p;
+        P.v[VS_P1] = j.p1.p;
+        P.v[VS_U] = j.u.p;
+        P.v[VS_X] = j.prec == NSLX_F64 ? j.x : (void*)j.xs.p;
+        P.v[VS_B] = const_cast<void*>(j.b);
+        probs.push_back(P);
+        owners.push_back(&j);
+    }
+    K.njobs = (int)probs.size();
+    std::vector<int4> tiles;
+    if (!probs.empty()) {
+        const int tile = c % 4;
+        const int prec = c / 4;
+        tiles = build_tile_list(probs, prec, tile);
+    }
+    if (probs.size() > K.cap_probs) {
+        K.cap_probs = std::max<size_t>(probs.size(), 2 * K.cap_probs + 4);
+        K.probs.alloc(K.cap_probs);
+        drop_graph();
+    }
+    if (tiles.size() > K.cap_tiles) {
+        K.cap_tiles = std::max<size_t>(tiles.size(), 2 * K.cap_tiles + 1024);
+        K.tiles.alloc(K.cap_tiles);
+        drop_graph();
+    }
+    if (!probs.empty()) {
+        NSLX_HIP(hipMemcpyAsync(K.probs.p, probs.data(), probs.size() * sizeof(HopProblem), hipMemcpyHostToDevice, s_));
+        NSLX_HIP(hipMemcpyAsync(K.tiles.p, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, s_));
+    }
+    const int nt = (int)tiles.size();
+    NSLX_HIP(hipMemcpyAsync(K.ntiles.p, &nt, sizeof(int), hipMemcpyHostToDevice, s_));
+    NSLX_HIP(hipStreamSynchronize(s_));   // host vectors are freed on return
+}
+
+// P iterations over the active classes (captured or launched directly).
+void Pool::run_iterations() {
+    unsigned mask = 0;
+    for (int c = 0; c < NCLS; ++c)
+        if (cls_[c].njobs > 0) mask |= 1u << c;
+    if (!mask) return;
+    auto launch_all = [&]() {
+        const CgDev cg = cgdev();
+        for (int k = 0; k < P_; ++k) {
+            const int64_t K = iter_ + k;   // parity only (the kernels read the counter)
+            const int pold = ((K - 1) & 1) ? VS_P1 : VS_P0;
+            const int pnew = ((K - 1) & 1) ? VS_P0 : VS_P1;
+            for (int pass = 0; pass < 2; ++pass) {
+                for (int c = 0; c < NCLS; ++c) {
+                    if (!(mask & (1u << c))) continue;
+                    GroupArgs g;
+                    g.probs = cls_[c].probs.p;
+                    g.nprob = cls_[c].njobs;
+                    g.tiles = cls_[c].tiles.p;
+                    g.ntiles = (int)std::max<size_t>(1, cls_[c].cap_tiles);
+                    g.ntiles_dev = cls_[c].ntiles.p;
+                    g.iter_dev = it_dev_.p;
+                    g.cg = cg;
+                    if (pass == 0) {   // CGA: Y = M^dag(p_new), p_new = r + beta p_old
+                        g.role[0] = VS_R; g.role[1] = VS_U; g.role[2] = (signed char)pold; g.role[3] = (signed char)pnew;
+                        launch_hop_grouped(c / 4, HOP_MD, MODE_CGA, c % 4, g, cls_[c].grid, s_);
+                    } else {           // CGB: t = M u, x += alpha p_new, r -= alpha t
+                        g.role[0] = VS_U; g.role[2] = (signed char)pnew; g.role[4] = VS_X; g.role[5] = VS_R;
+                        launch_hop_grouped(c / 4, HOP_M, MODE_CGB, c % 4, g, cls_[c].grid, s_);
+                    }
+                }
+            }
+            hipLaunchKernelGGL(k_inc, dim3(1), dim3(64), 0, s_, it_dev_.p);
+            NSLX_LAUNCH_CHECK();
+        }
+    };
+    if (use_graph_) {
+        if (!exec_ || mask != cls_mask_) {
+            drop_graph();
+            const bool was = prof_on();
+            prof_enable(false);
+            NSLX_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+            try {
+                launch_all();
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(s_, &g);
+                if (g) (void)hipGraphDestroy(g);
+                prof_enable(was);
+                throw;
+            }
+            NSLX_HIP(hipStreamEndCapture(s_, &graph_));
+            prof_enable(was);
+            NSLX_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+            cls_mask_ = mask;
+        }
+        NSLX_HIP(hipGraphLaunch(exec_, s_));
+    } else {
+        launch_all();
+    }
+    iter_ += P_;
+    st_.iterations += P_;
+    st_.replays += 1;
+}
+
+bool Pool::finalize(PoolJob& j) {
+    FermionMatrix& fm = *j.fm;
+    const int64_t len = fm.len();
+    j.iters += iter_ - j.join_iter;
+    if (j.prec != NSLX_F64) ru_accumulate(0, j.x, j.xs.p, j.nb, len, s_);
+    fm.apply(j.op, j.x, j.t64.p, j.nb, NSLX_F64, s_);
+    resid64(j.b, j.t64.p, j.r64.p, j.nb, len, j.rn2.p, fm.vsc, s_);
+    norm2_perb(1, j.x, j.nb, len, j.xn2.p, fm.vsc, s_);
+    std::vector<double> rn(j.nb), bn(j.nb), xn(j.nb);
+    NSLX_HIP(hipMemcpyAsync(rn.data(), j.rn2.p, j.nb * sizeof(double), hipMemcpyDeviceToHost, s_));
+    NSLX_HIP(hipMemcpyAsync(bn.data(), j.bn2.p, j.nb * sizeof(double), hipMemcpyDeviceToHost, s_));
+    NSLX_HIP(hipMemcpyAsync(xn.data(), j.xn2.p, j.nb * sizeof(double), hipMemcpyDeviceToHost, s_));
+    NSLX_HIP(hipStreamSynchronize(s_));
+    const double nM = 1.0 + std::exp(std::fabs(fm.delta()) * fm.normK + std::fabs(fm.delta() * fm.mu) + fm.max_im_phi);
+    const double fc = env_double("NSLX_FLOOR_C", 1.0) * 1.1102230246251565e-16 * nM * nM;
+    bool all = true;
+    double rel = 0.0, maxrel = 0.0, maxfloor = 0.0;
+    int64_t fstops = 0;
+    for (int64_t i = 0; i < j.nb; ++i) {
+        const double fl2 = fc * fc * xn[i];
+        const bool ok = rn[i] <= j.tol * j.tol * bn[i] || rn[i] <= fl2;
+        if (ok && rn[i] > j.tol * j.tol * bn[i]) ++fstops;
+        const double r = bn[i] > 0 ? std::sqrt(rn[i] / bn[i]) : std::sqrt(rn[i]);
+        maxrel = std::max(maxrel, r);
+        if (bn[i] > 0) maxfloor = std::max(maxfloor, std::sqrt(fl2 / bn[i]));
+        if (!ok) { all = false; rel = std::max(rel, r); }
+    }
+    j.max_rel = maxrel;
+    j.max_floor_rel = maxfloor;
+    j.floor_stops = fstops;
+    if (all) { j.status = NSLX_OK; return true; }
+    if (j.iters >= j.maxit || j.restarts >= 200) { j.status = NSLX_NOT_CONVERGED; return true; }
+    // restart: the next defect-correction step, or FP64 when low precision stalls
+    ++j.restarts;
+    ++st_.restarts;
+    if (j.prec != NSLX_F64 && !(rel < 0.1 * j.rel_start)) {
+        j.prec = NSLX_F64;
+        j.escalated = true;
+        ++st_.escalations;
+        j.xs.reset();
+    }
+    return false;
+}
+
+std::vector<std::unique_ptr<PoolJob>> Pool::step() {
+    std::vector<std::unique_ptr<PoolJob>> out;
+    if (jobs_.empty()) return out;
+    run_iterations();
+    int64_t hi = 0;
+    for (auto& jp : jobs_) hi = std::max<int64_t>(hi, jp->rhs0 + jp->nb);
+    NSLX_HIP(hipMemcpyAsync(done_h_.data(), done_.p, hi * sizeof(int), hipMemcpyDeviceToHost, s_));
+    NSLX_HIP(hipStreamSynchronize(s_));
+    const int64_t lowmax = (int64_t)env_double("NSLX_RU_LOWMAX", 2000.0);
+    unsigned dirty = 0;
+    std::vector<std::unique_ptr<PoolJob>> keep;
+    for (auto& jp : jobs_) {
+        PoolJob& j = *jp;
+        bool all_done = true;
+        for (int64_t i = 0; i < j.nb && all_done; ++i) all_done = done_h_[j.rhs0 + i] >= 0;
+        const int64_t ran = iter_ - j.join_iter;
+        const bool budget = ran >= (j.prec == NSLX_F64 ? j.maxit - j.iters : std::min(lowmax, j.maxit - j.iters));
+        if (!all_done && !budget) { keep.push_back(std::move(jp)); continue; }
+        const int old_cls = j.cls;
+        dirty |= 1u << old_cls;
+        free_range(part_used_, j.part0, j.nb * j.nparts);
+        j.part0 = -1;
+        if (finalize(j)) {
+            j.finished = true;
+            free_range(rhs_used_, j.rhs0, j.nb);
+            j.rhs0 = -1;
+            out.push_back(std::move(jp));
+        } else {
+            start_recurrence(j, false);
+            dirty |= 1u << j.cls;
+            keep.push_back(std::move(jp));
+        }
+    }
+    jobs_ = std::move(keep);
+    for (int c = 0; c < NCLS; ++c)
+        if (dirty & (1u << c)) rebuild_class(c);
+    return out;
+}
+
+}  // namespace nslx
+
+namespace nslx {
+
+int batch_solve(Ctx& ctx, int64_t nitems, const nslx_batch_item* items, double tol, int prec, int64_t maxit,
+                int use_graph, nslx_batch_report* rep, hipStream_t user) {
+    NSLX_REQUIRE(nitems >= 1 && items, "batch_solve: no items");
+    NSLX_REQUIRE(tol > 0.0 && maxit >= 1, "batch_solve: bad tolerance or maxit");
+    int64_t cap = 0;
+    for (int64_t i = 0; i < nitems; ++i) {
+        NSLX_REQUIRE(items[i].fm && items[i].b && items[i].x && items[i].nbatch >= 1, "batch_solve: bad item");
+        NSLX_REQUIRE(&items[i].fm->m.ctx->tiles == &ctx.tiles, "batch_solve: every fm must belong to ctx");
+        cap += items[i].nbatch;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    NSLX_HIP(hipStreamSynchronize(user));
+    hipStream_t s;
+    NSLX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    nslx_batch_report agg{};
+    try {
+        Pool pool(&ctx, s, cap, use_graph, (int)env_double("NSLX_POOL_P", 8.0));
+        for (int64_t i = 0; i < nitems; ++i) {
+            auto j = std::make_unique<PoolJob>();
+            j->id = i;
+            j->fm = &items[i].fm->m;
+            j->b = items[i].b;
+            j->x = items[i].x;
+            j->nb = items[i].nbatch;
+            j->tol = tol;
+            j->prec = prec;
+            j->maxit = maxit;
+            pool.admit(std::move(j));
+        }
+        while (!pool.idle()) {
+            for (auto& j : pool.step()) {
+                if (j->status == NSLX_OK) ++agg.converged_items;
+                agg.floor_stops += j->floor_stops;
+                agg.max_rel_residual = std::max(agg.max_rel_residual, j->max_rel);
+                agg.max_floor_rel = std::max(agg.max_floor_rel, j->max_floor_rel);
+            }
+        }
+        agg.iterations = pool.stats().iterations;
+        agg.replays = pool.stats().replays;
+        agg.restarts = pool.stats().restarts;
+        agg.escalations = pool.stats().escalations;
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+        throw;
+    }
+    NSLX_HIP(hipStreamSynchronize(s));
+    NSLX_HIP(hipStreamDestroy(s));
+    agg.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (rep) *rep = agg;
+    return agg.converged_items == nitems ? NSLX_OK : NSLX_NOT_CONVERGED;
+}
+
+}  // namespace nslx

@@ -1,0 +1,103 @@
+"""Ragged batched CG (nslx_batch_solve): many fermion matrices of different
+Nx, Nt and fields in one grouped launch per Krylov iteration, with and
+without HIP-graph replay.  Tolerances: every RHS to its FP64 true residual
+target max(tol, attainable accuracy), checked with the oracle's own
+MMdagger; an item's solution is bit-identical alone and in a mix, and with
+and without the graph."""
+
+import math
+
+import pytest
+import torch
+
+import nsl_amd as nsl
+from oracle import fermion as F, lattice as olat, rng
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def make(kind, size, Nt, beta, U, nfield, nrhs, seed, ctx):
+    if kind == "honeycomb":
+        lat_d, lat_o = nsl.Lattice.honeycomb(*size), olat.honeycomb(*size)
+    elif kind == "ring":
+        lat_d, lat_o = nsl.Lattice.ring(size), olat.ring(size)
+    else:
+        lat_d, lat_o = nsl.Lattice.armchair_nanotube(*size), olat.armchair_nanotube(*size)
+    Nx = lat_d.sites()
+    phis = [torch.as_tensor(rng.real_field(seed + f, 0, (Nt, Nx), math.sqrt(beta / Nt * U))) for f in range(nfield)]
+    fm = nsl.HubbardExp(lat_d, Nt, beta, 0.0, ctx=ctx)
+    fm.populate(torch.stack(phis).to(DEV))
+    ofs = []
+    for p in phis:
+        o = F.HubbardExp(lat_o, Nt, beta)
+        o.populate(p)
+        ofs.append(o)
+    b = torch.as_tensor(rng.complex_normal(seed + 500, 0, (nfield * nrhs, Nt, Nx)))
+    return fm, ofs, b
+
+
+SPECS = [
+    ("honeycomb", (3, 3), 16, 4.0, 2.0, 2, 2, 100),   # Nx 18
+    ("honeycomb", (6, 6), 32, 6.0, 2.0, 1, 3, 200),   # Nx 72
+    ("nanotube", (3, 4), 20, 3.0, 1.5, 1, 2, 300),    # Nx 48
+    ("ring", 10, 12, 2.0, 3.0, 2, 1, 400),            # Nx 10
+]
+
+
+def check_residuals(ofs, b, x, target):
+    rpf = b.shape[0] // len(ofs)
+    for k in range(b.shape[0]):
+        r = b[k] - ofs[k // rpf].MMdagger(x[k].cpu())
+        assert float(r.norm() / b[k].norm()) < 2 * target, k
+
+
+@pytest.mark.parametrize("prec", [nsl.F64, nsl.F32, nsl.BF16X3])
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_true_residual(prec, use_graph):
+    ctx = nsl.Context(0)
+    probs = [make(*s, ctx=ctx) for s in SPECS]
+    xs, rep = nsl.batch_solve([(fm, b.to(DEV)) for fm, _, b in probs], tol=1e-10, precision=prec,
+                              use_graph=use_graph, ctx=ctx)
+    assert rep["converged"], rep
+    assert rep["converged_items"] == len(SPECS)
+    assert rep["iterations"] > 0 and rep["replays"] > 0
+    target = max(1e-10, rep["max_floor_rel"])
+    assert rep["max_rel_residual"] <= target
+    for (fm, ofs, b), x in zip(probs, xs):
+        check_residuals(ofs, b, x, target)
+
+
+def test_batch_alone_equals_in_mix():
+    ctx = nsl.Context(0)
+    probs = [make(*s, ctx=ctx) for s in SPECS]
+    mine = probs[1]
+    alone, _ = nsl.batch_solve([(mine[0], mine[2].to(DEV))], tol=1e-10, ctx=ctx)
+    mix, _ = nsl.batch_solve([(fm, b.to(DEV)) for fm, _, b in probs], tol=1e-10, ctx=ctx)
+    nograph, _ = nsl.batch_solve([(fm, b.to(DEV)) for fm, _, b in probs[::-1]], tol=1e-10, use_graph=False, ctx=ctx)
+    assert torch.equal(alone[0], mix[1])
+    assert torch.equal(alone[0], nograph[2])
+
+
+def test_batch_matches_single_solver():
+    """Against the single-problem per-RHS solver (reliable-update FP64):
+    both reach the FP64 true residual target, so the solutions agree to
+    target x cond; here 1e-6 relative."""
+    ctx = nsl.Context(0)
+    fm, ofs, b = make(*SPECS[1], ctx=ctx)
+    xs, rep = nsl.batch_solve([(fm, b.to(DEV))], tol=1e-11, ctx=ctx)
+    cg = nsl.CG(fm, nsl.MMdagger, 1e-11, 100000, tol_kind=nsl.TOL_REL_PER_RHS)
+    x1 = cg(b.to(DEV))
+    err = float((xs[0] - x1).abs().max() / x1.abs().max())
+    assert err < 1e-6, err
+
+
+def test_batch_errors():
+    ctx = nsl.Context(0)
+    fm, ofs, b = make(*SPECS[0], ctx=ctx)
+    with pytest.raises(nsl.NslxError):                       # nbatch not a multiple of the fields
+        nsl.batch_solve([(fm, b[:3].to(DEV))], ctx=ctx)
+    other = nsl.Context(0)
+    with pytest.raises(nsl.NslxError):                       # fm of another context
+        nsl.batch_solve([(fm, b.to(DEV))], ctx=other)
