@@ -226,6 +226,69 @@ int nslx_correlator_two_point(nslx_fm fm, int64_t n_tsrc, double eps, int64_t ma
                               int precision, int64_t max_rhs, void* corr, nslx_solve_report* report,
                               void* stream);
 
+/* ---- correlator server (continuous batching) ----------------------------------
+ * No counterpart in the reference (its measurements run one configuration
+ * at a time, twoPointCorrelationFunction.tpp:116-162).  A server owns a
+ * worker thread and a batched CG pool on its own stream.  A request is a
+ * lattice (K, Nx), Nt, beta, mu, species, one field phi (host c128 (nt, nx),
+ * copied at submit) and a number of time sources N; its result is the
+ * two-point correlator of TwoPointCorrelator::measure(N) (N x Nx point
+ * sources, MMdagger solves to tol per RHS, M^dag back-multiply, antiperiodic
+ * re-centring, average), kept on the device until fetched.
+ * Requests wait in FIFO order and are admitted while their device footprint
+ * fits hbm_budget and their RHS fit max_rhs; admitted requests join the
+ * running batch at the next Krylov-iteration boundary (every
+ * iters_per_replay iterations) and leave it as they finish, so requests that
+ * arrive while others are on the GPU share every hop launch with them.  A
+ * request's result does not depend on what else is in the batch.
+ * Latency = completion - submit, host clock.  Thread-safe. */
+typedef struct nslx_server_s* nslx_server;
+
+typedef struct {
+    int64_t max_rhs;            /* RHS slots of the pool (<= 0: 65536)          */
+    double  hbm_budget;         /* bytes of device memory for admitted work     */
+    int     use_graph;          /* replay the iteration loop as a HIP graph     */
+    int     iters_per_replay;   /* even; <= 0: 8                                */
+} nslx_server_config;
+
+typedef struct {
+    int           nx, nt;
+    const double* K;            /* host, nx*nx, copied at submit                */
+    double        beta, mu;
+    int           species;
+    const void*   phi;          /* host c128 (nt, nx), copied at submit         */
+    int64_t       n_tsrc;       /* time sources N (1 <= N <= nt)                */
+    double        tol;          /* per-RHS relative FP64 residual               */
+    int           precision;    /* NSLX_F64 / NSLX_F32 / NSLX_BF16X3            */
+    int64_t       maxit;
+} nslx_request;
+
+typedef struct {
+    int     status;             /* NSLX_OK, NSLX_NOT_CONVERGED or an error      */
+    int64_t iterations;         /* Krylov iterations of the request             */
+    int     escalated;          /* low precision gave way to FP64               */
+    double  max_rel_residual;   /* FP64 true residual, max over its RHS         */
+    double  queue_ms;           /* submit -> admitted                           */
+    double  latency_ms;         /* submit -> result ready                       */
+} nslx_result;
+
+typedef struct {
+    int64_t submitted, completed, rhs_solved, admitted_peak;
+    double  p50_ms, p99_ms, max_ms;   /* latency over completed requests          */
+    double  busy_seconds;             /* worker time with work on the device      */
+    double  bytes_peak;               /* largest admitted footprint               */
+    int64_t iterations, replays;
+} nslx_server_stats;
+
+int nslx_server_create(nslx_ctx ctx, const nslx_server_config* cfg, nslx_server* out);
+int nslx_server_destroy(nslx_server srv);     /* waits for admitted work, drops the rest */
+int nslx_server_submit(nslx_server srv, const nslx_request* req, int64_t* id);
+/* Block until request id is done (timeout_ms < 0: no limit); on success copy
+ * the correlator (device c128 (nt, nx, nx), corr[t][y][x]) to corr (may be
+ * NULL) and release it.  NSLX_E_STATE on timeout. */
+int nslx_server_wait(nslx_server srv, int64_t id, double timeout_ms, void* corr, nslx_result* result);
+int nslx_server_stats_get(nslx_server srv, nslx_server_stats* stats);
+
 /* ---- HMC ---------------------------------------------------------------------
  * Hubbard action S = S_gauge + S_fermion and its force for one real field
  * (src/NSL/Action/Implementations/hubbardGaugeAction.tpp:59-67,

@@ -324,6 +324,58 @@ def batch_solve(problems, tol=1e-12, precision=F64, maxIter=100000, use_graph=Tr
     return xs, d
 
 
+class Server:
+    """Correlator server with continuous batching (nslx_server_*): submit
+    requests (lattice, Nt, beta, mu, species, phi, time sources, tol,
+    precision), wait for their two-point correlators; admission by device
+    footprint, requests share every Krylov iteration with what is running."""
+
+    def __init__(self, ctx=None, max_rhs=0, hbm_budget=0.0, use_graph=True, iters_per_replay=8):
+        self.ctx = ctx or Context.default()
+        cfg = _lib.ServerConfig(int(max_rhs), float(hbm_budget), int(bool(use_graph)), int(iters_per_replay))
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().nslx_server_create(self.ctx._h, ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self._shapes = {}
+
+    def submit(self, lat, Nt, beta, phi, n_tsrc=1, mu=0.0, species=Particle, tol=1e-10, precision=F64,
+               maxIter=100000):
+        phi = np.ascontiguousarray(np.asarray(phi.cpu() if hasattr(phi, "cpu") else phi, dtype=np.complex128))
+        Nx = lat.sites()
+        if phi.shape != (Nt, Nx):
+            raise ValueError(f"phi must be (Nt, Nx) = ({Nt}, {Nx})")
+        req = _lib.Request(Nx, int(Nt), lat.K.ctypes.data_as(_lib.P_double), float(beta), float(mu), int(species),
+                           phi.ctypes.data, int(n_tsrc), float(tol), int(precision), int(maxIter))
+        rid = ctypes.c_int64()
+        _lib.check(_lib.lib().nslx_server_submit(self._h, ctypes.byref(req), ctypes.byref(rid)))
+        self._shapes[rid.value] = (int(Nt), Nx)
+        return rid.value
+
+    def wait(self, rid, timeout_ms=-1.0):
+        """(corr (Nt, Nx, Nx) complex128 on the device, result dict)."""
+        torch = _torch()
+        Nt, Nx = self._shapes.pop(rid)
+        corr = torch.empty((Nt, Nx, Nx), dtype=torch.complex128, device="cuda")
+        res = _lib.Result()
+        st = _lib.lib().nslx_server_wait(self._h, int(rid), float(timeout_ms), _ptr(corr), ctypes.byref(res))
+        _lib.check(st, allow=(_lib.OK, _lib.NOT_CONVERGED))
+        return corr, res.as_dict()
+
+    def stats(self):
+        st = _lib.ServerStats()
+        _lib.check(_lib.lib().nslx_server_stats_get(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None and _lib._lib is not None:
+            _lib._lib.nslx_server_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
 # ------------------------------------------------------------- measurements
 class TwoPointCorrelator:
     """NSL::Measure::Hubbard::TwoPointCorrelator::measure(N)

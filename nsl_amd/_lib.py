@@ -45,6 +45,33 @@ class BatchReport(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("max_rhs", c_int64), ("hbm_budget", c_double), ("use_graph", c_int), ("iters_per_replay", c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("nx", c_int), ("nt", c_int), ("K", P_double), ("beta", c_double), ("mu", c_double),
+                ("species", c_int), ("phi", c_void_p), ("n_tsrc", c_int64), ("tol", c_double),
+                ("precision", c_int), ("maxit", c_int64)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", c_int), ("iterations", c_int64), ("escalated", c_int), ("max_rel_residual", c_double),
+                ("queue_ms", c_double), ("latency_ms", c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", c_int64), ("completed", c_int64), ("rhs_solved", c_int64), ("admitted_peak", c_int64),
+                ("p50_ms", c_double), ("p99_ms", c_double), ("max_ms", c_double), ("busy_seconds", c_double),
+                ("bytes_peak", c_double), ("iterations", c_int64), ("replays", c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class HmcReport(ctypes.Structure):
     _fields_ = [("dH_re", c_double), ("dH_im", c_double), ("acc_prob", c_double), ("accepted", c_int),
                 ("S_re", c_double), ("S_im", c_double), ("seconds", c_double)]
@@ -77,6 +104,11 @@ PROTOS = {
                            c_int64, c_int, c_int, ctypes.POINTER(SolveReport), c_void_p]),
     "nslx_batch_solve": (c_int, [c_void_p, c_int64, ctypes.POINTER(BatchItem), c_double, c_int, c_int64, c_int,
                                  ctypes.POINTER(BatchReport), c_void_p]),
+    "nslx_server_create": (c_int, [c_void_p, ctypes.POINTER(ServerConfig), ctypes.POINTER(c_void_p)]),
+    "nslx_server_destroy": (c_int, [c_void_p]),
+    "nslx_server_submit": (c_int, [c_void_p, ctypes.POINTER(Request), ctypes.POINTER(c_int64)]),
+    "nslx_server_wait": (c_int, [c_void_p, c_int64, c_double, c_void_p, ctypes.POINTER(Result)]),
+    "nslx_server_stats_get": (c_int, [c_void_p, ctypes.POINTER(ServerStats)]),
     "nslx_correlator_two_point": (c_int, [c_void_p, c_int64, c_double, c_int64, c_int, c_int, c_int64, c_void_p,
                                           ctypes.POINTER(SolveReport), c_void_p]),
     "nslx_hmc_action": (c_int, [c_void_p, c_void_p, c_double, c_int, P_double, c_void_p]),

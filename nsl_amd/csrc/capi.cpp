@@ -8,6 +8,7 @@
 #include "engine.hpp"
 #include "corr.hpp"
 #include "pool.hpp"
+#include "serve.hpp"
 #include "hmc.hpp"
 #include "prof.hpp"
 
@@ -131,30 +132,9 @@ int nslx_fm_create(nslx_ctx ctx, int nx, int nt, const double* K, double beta, d
         NSLX_REQUIRE(species == NSLX_PARTICLE || species == NSLX_HOLE, "fm_create: bad species");
         NSLX_REQUIRE(std::isfinite(beta) && std::isfinite(mu), "fm_create: beta and mu must be finite");
         DeviceGuard dg(ctx->c.device);
-        auto* f = new nslx_fm_s();
-        nslx::FermionMatrix& m = f->m;
-        m.ctx = &ctx->c;
-        m.nx = nx;
-        m.nt = nt;
-        m.beta = beta;
-        m.mu = mu;
-        m.species = species;
-        m.K_host.assign(K, K + (size_t)nx * nx);
-        double nk = 0.0;
-        for (int i = 0; i < nx; ++i) {
-            double rs = 0.0;
-            for (int j = 0; j < nx; ++j) rs += std::fabs(K[(size_t)i * nx + j]);
-            nk = std::max(nk, rs);
-        }
-        m.normK = nk;
-        try {
-            m.K.alloc((size_t)nx * nx);
-            NSLX_HIP(hipMemcpy(m.K.p, K, (size_t)nx * nx * sizeof(double), hipMemcpyHostToDevice));
-        } catch (...) {
-            delete f;
-            throw;
-        }
-        *out = f;
+        auto f = std::make_unique<nslx_fm_s>();
+        nslx::fm_init(f->m, &ctx->c, nx, nt, K, beta, mu, species);
+        *out = f.release();
         return NSLX_OK;
     });
 }
@@ -223,6 +203,49 @@ int nslx_batch_solve(nslx_ctx ctx, int64_t nitems, const nslx_batch_item* items,
         DeviceGuard dg(ctx->c.device);
         return nslx::batch_solve(ctx->c, nitems, items, tol, precision, maxit, use_graph, report,
                                  nslx::as_stream(stream));
+    });
+}
+
+int nslx_server_create(nslx_ctx ctx, const nslx_server_config* cfg, nslx_server* out) {
+    return guard([&] {
+        NSLX_REQUIRE(ctx && out, "server_create: null argument");
+        nslx_server_config c{};
+        if (cfg) c = *cfg;
+        DeviceGuard dg(ctx->c.device);
+        auto s = std::make_unique<nslx_server_s>();
+        s->s = std::make_unique<nslx::Server>(&ctx->c, c);
+        *out = s.release();
+        return NSLX_OK;
+    });
+}
+
+int nslx_server_destroy(nslx_server srv) {
+    return guard([&] {
+        delete srv;
+        return NSLX_OK;
+    });
+}
+
+int nslx_server_submit(nslx_server srv, const nslx_request* req, int64_t* id) {
+    return guard([&] {
+        NSLX_REQUIRE(srv && req && id, "server_submit: null argument");
+        *id = srv->s->submit(*req);
+        return NSLX_OK;
+    });
+}
+
+int nslx_server_wait(nslx_server srv, int64_t id, double timeout_ms, void* corr, nslx_result* result) {
+    return guard([&] {
+        NSLX_REQUIRE(srv, "server_wait: null server");
+        return srv->s->wait(id, timeout_ms, corr, result);
+    });
+}
+
+int nslx_server_stats_get(nslx_server srv, nslx_server_stats* stats) {
+    return guard([&] {
+        NSLX_REQUIRE(srv && stats, "server_stats: null argument");
+        srv->s->stats(stats);
+        return NSLX_OK;
     });
 }
 

@@ -69,6 +69,31 @@ inline unsigned grid_for(int64_t n) { return (unsigned)std::min<int64_t>((n + 25
 
 }  // namespace
 
+void corr_sources(void* S, int ng, int tsrc0, int step, int nt, int nx, hipStream_t s) {
+    const int64_t n = (int64_t)ng * nx * nt * nx;
+    hipLaunchKernelGGL(k_sources, dim3(grid_for(n)), dim3(256), 0, s, (double2*)S, ng, tsrc0, step, nt, nx);
+    NSLX_LAUNCH_CHECK();
+}
+
+void corr_zero(void* corr, int nt, int nx, hipStream_t s) {
+    const int64_t n = (int64_t)nt * nx * nx;
+    hipLaunchKernelGGL(k_fill0, dim3(grid_for(n)), dim3(256), 0, s, (double2*)corr, n);
+    NSLX_LAUNCH_CHECK();
+}
+
+void corr_accumulate(void* corr, const void* G, int ng, int tsrc0, int step, int nt, int nx, hipStream_t s) {
+    const int64_t n = (int64_t)nt * nx * nx;
+    hipLaunchKernelGGL(k_accumulate, dim3(grid_for(n)), dim3(256), 0, s, (double2*)corr, (const double2*)G, ng, tsrc0,
+                       step, nt, nx, 1.0);
+    NSLX_LAUNCH_CHECK();
+}
+
+void corr_scale(void* corr, int nt, int nx, double f, hipStream_t s) {
+    const int64_t n = (int64_t)nt * nx * nx;
+    hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(256), 0, s, (double2*)corr, n, f);
+    NSLX_LAUNCH_CHECK();
+}
+
 int fm_two_point(FermionMatrix& fm, int64_t ntsrc, double eps, int64_t maxit, int tol_kind, int prec, int64_t max_rhs,
                  void* corr_out, nslx_solve_report* rep, hipStream_t s) {
     fm.require_populated();

@@ -10,4 +10,12 @@ namespace nslx {
 int fm_two_point(FermionMatrix& fm, int64_t ntsrc, double eps, int64_t maxit, int tol_kind, int prec, int64_t max_rhs,
                  void* corr_out, nslx_solve_report* rep, hipStream_t s);
 
+// Pieces of the measurement for other drivers (the correlator server):
+// S (c128, ng*nx RHS of nt*nx) = point sources at tsrc0 + g*step;
+// corr (c128 nt*nx*nx) = 0; corr += re-centred G of those sources; corr *= f.
+void corr_sources(void* S, int ng, int tsrc0, int step, int nt, int nx, hipStream_t s);
+void corr_zero(void* corr, int nt, int nx, hipStream_t s);
+void corr_accumulate(void* corr, const void* G, int ng, int tsrc0, int step, int nt, int nx, hipStream_t s);
+void corr_scale(void* corr, int nt, int nx, double f, hipStream_t s);
+
 }  // namespace nslx

@@ -93,6 +93,9 @@ struct FermionMatrix {
     int tile(int prec, int64_t nb) { return ctx->tiles.get(prec, nx, nt, nb); }
 };
 
+// HubbardExp(species, lat, Nt, beta, mu) on ctx's device (fm.cpp)
+void fm_init(FermionMatrix& m, Ctx* ctx, int nx, int nt, const double* K, double beta, double mu, int species);
+
 // Solvers (solver.cpp)
 int solve(FermionMatrix& fm, int op, int method, const void* b, void* x, const void* x0, int64_t nb, double tol,
           int tol_kind, int64_t maxit, int prec, int check, nslx_solve_report* rep, hipStream_t s);

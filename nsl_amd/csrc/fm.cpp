@@ -38,6 +38,29 @@ void TileTable::set(int prec, int nx, int nt, int64_t nb, int tile) {
     else best[{prec, nx, nt, nb_class(nb)}] = tile;
 }
 
+void fm_init(FermionMatrix& m, Ctx* ctx, int nx, int nt, const double* K, double beta, double mu, int species) {
+    NSLX_REQUIRE(K, "fm_create: null K");
+    NSLX_REQUIRE(nx >= 1 && nt >= 2, "fm_create: need nx >= 1 and nt >= 2");
+    NSLX_REQUIRE(species == NSLX_PARTICLE || species == NSLX_HOLE, "fm_create: bad species");
+    NSLX_REQUIRE(std::isfinite(beta) && std::isfinite(mu), "fm_create: beta and mu must be finite");
+    m.ctx = ctx;
+    m.nx = nx;
+    m.nt = nt;
+    m.beta = beta;
+    m.mu = mu;
+    m.species = species;
+    m.K_host.assign(K, K + (size_t)nx * nx);
+    double nk = 0.0;
+    for (int i = 0; i < nx; ++i) {
+        double rs = 0.0;
+        for (int j = 0; j < nx; ++j) rs += std::fabs(K[(size_t)i * nx + j]);
+        nk = std::max(nk, rs);
+    }
+    m.normK = nk;
+    m.K.alloc((size_t)nx * nx);
+    NSLX_HIP(hipMemcpy(m.K.p, K, (size_t)nx * nx * sizeof(double), hipMemcpyHostToDevice));
+}
+
 double env_double(const char* name, double dflt) {
     const char* v = std::getenv(name);
     if (!v || !*v) return dflt;

@@ -199,9 +199,15 @@ void Pool::admit(std::unique_ptr<PoolJob> jp) {
     NSLX_REQUIRE(j.prec == NSLX_F64 || j.prec == NSLX_F32 || j.prec == NSLX_BF16X3, "pool: bad precision");
     j.rhs0 = alloc_range(rhs_used_, j.nb);
     NSLX_REQUIRE(j.rhs0 >= 0, "pool: out of RHS slots");
-    j.bn2.ensure(j.nb);
-    norm2_perb(1, j.b, j.nb, j.fm->len(), j.bn2.p, j.fm->vsc, s_);
-    start_recurrence(j, true);
+    try {
+        j.bn2.ensure(j.nb);
+        norm2_perb(1, j.b, j.nb, j.fm->len(), j.bn2.p, j.fm->vsc, s_);
+        start_recurrence(j, true);
+    } catch (...) {
+        free_range(rhs_used_, j.rhs0, j.nb);
+        if (j.part0 >= 0) free_range(part_used_, j.part0, j.nb * j.nparts);
+        throw;
+    }
     const int c = j.cls;
     jobs_.push_back(std::move(jp));
     rebuild_class(c);

@@ -1,0 +1,122 @@
+"""Correlator server (continuous batching): results against the oracle's
+TwoPointCorrelator, identical alone and in a mix of concurrent requests
+(bit for bit: a request's recurrence does not depend on the batch), mixed
+precision to its FP64 true-residual target, admission by memory budget, and
+latency statistics."""
+
+import math
+import time
+
+import numpy as np
+import pytest
+import torch
+
+import nsl_amd as nsl
+from oracle import correlator as ocorr, fermion as F, lattice as olat, rng
+
+pytestmark = pytest.mark.gpu
+
+
+def relerr(a, b):
+    a = a.detach().to("cpu", torch.complex128)
+    b = torch.as_tensor(b).detach().to("cpu", torch.complex128)
+    return float((a - b).abs().max() / b.abs().max())
+
+
+def field(lat, Nt, beta, U, seed):
+    return torch.as_tensor(rng.real_field(seed, 0, (Nt, lat.sites()), math.sqrt(beta / Nt * U)))
+
+
+def test_server_vs_oracle_cfg0():
+    """Honeycomb 3x3, Nt = 16, beta = 4, U = 2; 2 time sources; per-RHS
+    1e-12: against the oracle's reference-mode correlator to 1e-9."""
+    lat = nsl.Lattice.honeycomb(3, 3)
+    phi = field(lat, 16, 4.0, 2.0, 1000)
+    srv = nsl.Server()
+    try:
+        rid = srv.submit(lat, 16, 4.0, phi, n_tsrc=2, tol=1e-12)
+        corr, res = srv.wait(rid)
+    finally:
+        srv.close()
+    assert res["status"] == 0, res
+    o = F.HubbardExp(olat.honeycomb(3, 3), 16, 4.0)
+    o.populate(phi)
+    ref, _ = ocorr.two_point(o, 16, 18, 2)
+    assert relerr(corr, ref) < 1e-9
+
+
+def test_server_alone_equals_in_mix():
+    lats = [nsl.Lattice.honeycomb(3, 3), nsl.Lattice.honeycomb(6, 6), nsl.Lattice.armchair_nanotube(3, 4),
+            nsl.Lattice.ring(10)]
+    reqs = [(lats[0], 16, 4.0, 2.0, 1), (lats[1], 32, 6.0, 2.0, 2), (lats[2], 20, 3.0, 1.5, 1),
+            (lats[3], 12, 2.0, 3.0, 3), (lats[1], 16, 3.0, 1.0, 1)]
+    mine = 1
+    lat, Nt, beta, U, nts = reqs[mine]
+    phi_mine = field(lat, Nt, beta, U, 77)
+    srv = nsl.Server()
+    try:
+        corr_alone, res_alone = srv.wait(srv.submit(lat, Nt, beta, phi_mine, n_tsrc=nts, tol=1e-11))
+    finally:
+        srv.close()
+    srv = nsl.Server(iters_per_replay=8)
+    try:
+        ids = {}
+        for k, (lt, nt, bt, u, n) in enumerate(reqs):
+            phi = phi_mine if k == mine else field(lt, nt, bt, u, 100 + k)
+            ids[k] = srv.submit(lt, nt, bt, phi, n_tsrc=n, tol=1e-11)
+            time.sleep(0.002)        # arrivals while earlier requests are on the GPU
+        out = {k: srv.wait(i) for k, i in ids.items()}
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert torch.equal(out[mine][0], corr_alone)
+    assert out[mine][1]["iterations"] == res_alone["iterations"]
+    assert all(r[1]["status"] == 0 for r in out.values())
+    assert st["completed"] == len(reqs) and st["p99_ms"] >= st["p50_ms"] > 0
+
+
+@pytest.mark.parametrize("prec", [nsl.F32, nsl.BF16X3])
+def test_server_mixed_precision(prec):
+    lat = nsl.Lattice.honeycomb(6, 6)
+    phi = field(lat, 16, 4.0, 2.0, 5)
+    srv = nsl.Server()
+    try:
+        c64, r64 = srv.wait(srv.submit(lat, 16, 4.0, phi, n_tsrc=2, tol=1e-10))
+        cm, rm = srv.wait(srv.submit(lat, 16, 4.0, phi, n_tsrc=2, tol=1e-10, precision=prec))
+    finally:
+        srv.close()
+    assert r64["status"] == 0 and rm["status"] == 0, (r64, rm)
+    assert rm["max_rel_residual"] <= 1e-10 or rm["escalated"]
+    assert relerr(cm, c64) < 1e-7
+
+
+def test_server_admission_budget():
+    """A budget that holds about two requests: all complete, never more than
+    the budget admitted at once."""
+    lat = nsl.Lattice.honeycomb(6, 6)
+    Nt = 16
+    # the footprint the server charges for one request (serve.cpp footprint())
+    one = 11 * 72 * Nt * 72 * 16 + 72 * 72 * 16 + 192 * 128 * 16 + Nt * 72 * 40 + Nt * 72 * 72 * 16 + (64 << 20)
+    srv = nsl.Server(hbm_budget=2.5 * one)
+    try:
+        ids = [srv.submit(lat, Nt, 4.0, field(lat, Nt, 4.0, 2.0, 300 + k), tol=1e-10) for k in range(6)]
+        res = [srv.wait(i)[1] for i in ids]
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert all(r["status"] == 0 for r in res)
+    assert 1 <= st["admitted_peak"] <= 2
+    assert st["bytes_peak"] <= 2.5 * one
+    assert max(r["queue_ms"] for r in res) > 0
+
+
+def test_server_errors():
+    lat = nsl.Lattice.honeycomb(3, 3)
+    srv = nsl.Server(hbm_budget=1e6)
+    try:
+        with pytest.raises(nsl.NslxError):                  # more time sources than Nt
+            srv.submit(lat, 16, 4.0, np.zeros((16, 18)), n_tsrc=17)
+        with pytest.raises(nsl.NslxError):                  # larger than the memory budget
+            srv.submit(lat, 16, 4.0, np.zeros((16, 18)))
+    finally:
+        srv.close()
