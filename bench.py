@@ -1,14 +1,18 @@
 """Benchmark: CG solves/s to a 1e-12 FP64 residual on MM^dagger, mixed
-precision (BF16x3 inner CG + FP64 defect correction) against the plain FP64
-GPU path, on the 12x12 periodic honeycomb (Nx = 288), Nt = 64, beta = 8,
-U = 3: 16 seeded phi fields x 8 Gaussian right-hand sides = 128 solves per
-step, all resident in HBM before the timed region.
+precision (BF16x3 recurrence with FP64 reliable updates) against the plain
+FP64 GPU path, on the 12x12 periodic honeycomb (Nx = 288), Nt = 64,
+beta = 8, U = 3: 16 seeded phi fields x 8 Gaussian right-hand sides = 128
+solves per step, all resident in HBM before the timed region.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--inner bf16x3|f32]
-                    [--no-cpu] [--no-fp64]
+                    [--no-cpu] [--no-fp64] [--no-free] [--no-batch] [--no-serve]
 
 Prints one JSON line (see the keys below).  A step is one batched solve of
-all 128 systems to |b - MM^dag x| / |b| <= 1e-12 per right-hand side.
+all 128 systems to |b - MM^dag x| / |b| <= 1e-12 per right-hand side (or,
+where cond(MM^dag) puts 1e-12 out of FP64's reach, to the attainable
+accuracy c eps ||A|| |x|).  Extra keys: the FP64 path, the same shapes at
+phi = 0, the batching form (ragged problems, grouped launches, HIP graph)
+and the serving form (Poisson correlator requests: p50/p99 latency).
 
 Profiling helpers (no JSON contract line):
     python bench.py --hops N        N plain M and M^dag applies per precision
@@ -155,6 +159,117 @@ def free_field_check(fm, b, x, inner, nb):
     return res
 
 
+def batching_leg(prec, nprob=64, seed=7000, tol=1e-10):
+    """The batching form: nprob independent fermion matrices of uneven size
+    (honeycomb Nx 18/72, (6,6) nanotubes Nx 96-144), Nt 16/32, beta 2-6,
+    U 1-4, one Philox field each with 1-16 Gaussian RHS, solved to 1e-10
+    per RHS (a) in one grouped launch per Krylov phase, HIP-graph replayed,
+    (b) the same without the graph, (c) one problem at a time with the
+    single-problem solver.  Returns solves/s of each."""
+    import torch
+    import nsl_amd as nsl
+    u = nsl.rng_uniform(seed, 0, nprob * 6).reshape(nprob, 6)
+    lats = {}
+    probs = []
+    for k in range(nprob):
+        kind = int(u[k, 0] * 4)
+        key = [("h", 3), ("h", 6), ("n", 4), ("n", 6)][kind]
+        if key not in lats:
+            lats[key] = (nsl.Lattice.honeycomb(key[1], key[1]) if key[0] == "h"
+                         else nsl.Lattice.armchair_nanotube(6, key[1]))
+        lat = lats[key]
+        Nt = 16 if u[k, 1] < 0.5 else 32
+        beta = 2.0 + 4.0 * u[k, 2]
+        U = 1.0 + 3.0 * u[k, 3]
+        nrhs = 1 + int(u[k, 4] * 16)
+        fm = nsl.HubbardExp(lat, Nt, beta)
+        fm.populate(torch.as_tensor(nsl.real_field(seed + 1 + k, (Nt, lat.sites()),
+                                                   math.sqrt(beta / Nt * U))).cuda())
+        b = torch.as_tensor(nsl.complex_normal(seed + 500 + k, (nrhs, Nt, lat.sites()))).cuda()
+        probs.append((fm, b))
+    nsolves = sum(b.shape[0] for _, b in probs)
+    out = {"problems": nprob, "rhs": nsolves}
+    for name, graph in (("grouped_graph", True), ("grouped", False)):
+        nsl.batch_solve(probs[:4], tol=tol, precision=prec, use_graph=graph)        # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, rep = nsl.batch_solve(probs, tol=tol, precision=prec, use_graph=graph)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[name] = {"solves_per_s": round(nsolves / dt, 1), "iterations": rep["iterations"],
+                     "converged": rep["converged"], "max_rel_residual": rep["max_rel_residual"],
+                     "escalations": rep["escalations"]}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for fm, b in probs:
+        nsl.CG(fm, nsl.MMdagger, tol, 100000, precision=prec, tol_kind=nsl.TOL_REL_PER_RHS)(b)
+    torch.cuda.synchronize()
+    out["one_at_a_time"] = {"solves_per_s": round(nsolves / (time.perf_counter() - t0), 1)}
+    out["speedup_graph_vs_one_at_a_time"] = round(out["grouped_graph"]["solves_per_s"] /
+                                                 out["one_at_a_time"]["solves_per_s"], 2)
+    return out
+
+
+SERVE_RATE = 40.0      # requests / s (Poisson)
+SERVE_N = 32           # requests per serving run
+
+
+def serve_requests(seed=9000):
+    """The serving mix: honeycomb flakes (Nx 18, 72) and (6,6) armchair
+    nanotubes of 2-4 cells (Nx 48-96), Nt in {16, 32}, beta in [2, 6],
+    U in [1, 4], a Philox-seeded phi ~ N(0, sqrt(U beta / Nt)) and Nx point
+    sources at t = 0 each."""
+    import nsl_amd as nsl
+    u = nsl.rng_uniform(seed, 0, SERVE_N * 6).reshape(SERVE_N, 6)
+    lats = {}
+    out = []
+    for k in range(SERVE_N):
+        kind = int(u[k, 0] * 3)
+        if kind == 0:
+            key = ("h", 3)
+        elif kind == 1:
+            key = ("h", 6)
+        else:
+            key = ("n", 2 + int(u[k, 1] * 3))
+        if key not in lats:
+            lats[key] = (nsl.Lattice.honeycomb(key[1], key[1]) if key[0] == "h"
+                         else nsl.Lattice.armchair_nanotube(6, key[1]))
+        lat = lats[key]
+        Nt = 16 if u[k, 2] < 0.5 else 32
+        beta = 2.0 + 4.0 * u[k, 3]
+        U = 1.0 + 3.0 * u[k, 4]
+        phi = nsl.real_field(seed + 1 + k, (Nt, lat.sites()), math.sqrt(beta / Nt * U))
+        out.append((lat, Nt, beta, phi))
+    arrivals = -np.log(1.0 - nsl.rng_uniform(seed + 999, 0, SERVE_N)) / SERVE_RATE
+    return out, arrivals
+
+
+def serving_leg(prec, tol=1e-10):
+    """One Poisson request stream through the correlator server; returns
+    latency percentiles and sustained solves/s (RHS solved / wall time)."""
+    import nsl_amd as nsl
+    reqs, gaps = serve_requests()
+    srv = nsl.Server()
+    try:
+        warm = srv.submit(reqs[0][0], reqs[0][1], reqs[0][2], reqs[0][3], tol=tol, precision=prec)
+        srv.wait(warm)
+        t0 = time.perf_counter()
+        ids = []
+        for (lat, Nt, beta, phi), gap in zip(reqs, gaps):
+            time.sleep(float(gap))
+            ids.append(srv.submit(lat, Nt, beta, phi, tol=tol, precision=prec))
+        res = [srv.wait(i)[1] for i in ids]
+        wall = time.perf_counter() - t0
+    finally:
+        srv.close()
+    lat_ms = np.array([r["latency_ms"] for r in res])
+    rhs = sum(lat.sites() for lat, _, _, _ in reqs)
+    return {"p50_ms": round(float(np.percentile(lat_ms, 50)), 3), "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+            "solves_per_s": round(rhs / wall, 1), "requests": len(res), "rate_per_s": SERVE_RATE,
+            "ok": sum(int(r["status"] == 0) for r in res), "escalated": sum(r["escalated"] for r in res),
+            "max_rel_residual": max(r["max_rel_residual"] for r in res)}
+
+
 def prof_text_of(L_):
     buf = ctypes.create_string_buffer(1 << 16)
     L_.nslx_prof_read(buf, len(buf))
@@ -208,6 +323,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-free", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--hops", type=int, default=0)
     ap.add_argument("--copy-calib", type=int, default=0)
     args = ap.parse_args()
@@ -292,8 +409,20 @@ def main():
         print(f"# fp64: {t64 / args.steps * 1e3:.1f} ms/step, report {rep64}", file=sys.stderr)
     if not args.no_free:
         out["free_field"] = free_field_check(fm, b, x, inner, nb)
-    out["p50_ms"] = None
-    out["p99_ms"] = None
+    if not args.no_batch:
+        out["batching"] = batching_leg(nsl.F32)
+    if not args.no_serve:
+        sv_mix = serving_leg(inner)
+        sv_64 = serving_leg(nsl.F64)
+        out["p50_ms"] = sv_mix["p50_ms"]
+        out["p99_ms"] = sv_mix["p99_ms"]
+        out["serving"] = {"mixed": sv_mix, "fp64": sv_64,
+                          "workload": f"{SERVE_N} correlator requests, Poisson {SERVE_RATE}/s, honeycomb Nx 18/72 "
+                                      "and (6,6) nanotubes Nx 48-96, Nt 16/32, beta 2-6, U 1-4, Nx point "
+                                      "sources each, per-RHS 1e-10"}
+    else:
+        out["p50_ms"] = None
+        out["p99_ms"] = None
     if not args.no_cpu:
         cores = min(16, os.cpu_count() or 1)
         iters = out.get("fp64_iterations") or rep_mix["iterations"]
