@@ -38,21 +38,21 @@ struct PF64 {
     using S = double;      // field scalar
     using L = double;      // LDS / MFMA operand scalar
     using Acc = d4;
-    static constexpr int KSTEP = 4, PAD = 2, NPA = 2, NPB = 1, VEC = 2;
+    static constexpr int KSTEP = 4, PAD = 2, NPA = 2, NPB = 1, VEC = 2, NBUF = 2;
     static constexpr bool ROW4V = true;
 };
 struct PF32 {
     using S = float;
     using L = float;
     using Acc = f4;
-    static constexpr int KSTEP = 4, PAD = 4, NPA = 2, NPB = 1, VEC = 4;
+    static constexpr int KSTEP = 4, PAD = 4, NPA = 2, NPB = 1, VEC = 4, NBUF = 2;
     static constexpr bool ROW4V = false;
 };
 struct PB3 {
     using S = float;
     using L = uint16_t;    // bf16 bits
     using Acc = f4;
-    static constexpr int KSTEP = 32, PAD = 8, NPA = 4, NPB = 2, VEC = 4;
+    static constexpr int KSTEP = 32, PAD = 8, NPA = 4, NPB = 2, VEC = 4, NBUF = 1;
     static constexpr bool ROW4V = false;
 };
 
@@ -119,10 +119,36 @@ struct JobPtrs {
     const void* X; void* Y; const void* Pold; void* Pnew; void* Xs; void* R; const void* Bv;
 };
 
+// 64 zero bytes: the source of every load that lies outside the field (a run
+// is at most 64 bytes).  Not const: a const array would live in the constant
+// address space, and a select between it and a global pointer would turn the
+// loads into flat loads (whose waits cannot be counted per load).  Never
+// written.
+__device__ __attribute__((aligned(16))) double g_zero_run[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+// One complex element, or zero when !ok, without a branch on the data.
+template <class S>
+__device__ inline CS<S> ldc_or0(const void* base, size_t idx, bool ok) {
+    if constexpr (sizeof(S) == 8) {
+        const double2* p = ok ? reinterpret_cast<const double2*>(base) + idx
+                              : reinterpret_cast<const double2*>(g_zero_run);
+        const double2 v = *p;
+        return {v.x, v.y};
+    } else {
+        const float2* p = ok ? reinterpret_cast<const float2*>(base) + idx
+                             : reinterpret_cast<const float2*>(g_zero_run);
+        const float2 v = *p;
+        return {v.x, v.y};
+    }
+}
+
+// LDS of a classic tile: NBUF copies of the A (field, split planes) and B (E)
+// chunks; with two the next chunk is stored while the current one feeds the
+// MFMAs (one barrier per chunk).
 template <class TL, class PR>
 struct Smem {
-    typename PR::L A[PR::NPA][TL::BM][TL::BK + PR::PAD];
-    typename PR::L B[PR::NPB][TL::BN][TL::BK + PR::PAD];
+    typename PR::L A[PR::NBUF][PR::NPA][TL::BM][TL::BK + PR::PAD];
+    typename PR::L B[PR::NBUF][PR::NPB][TL::BN][TL::BK + PR::PAD];
 };
 
 // One BM x BN output tile of rhs b (rows t0 = rt*BM.., sites y0 = ct*BN..).
@@ -168,74 +194,86 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
         constexpr int RPPB = NT / TPRB;
         constexpr int BPASS = (BN + RPPB - 1) / RPPB;
 
-        for (int k0 = 0; k0 < nx; k0 += BK) {
-            // ---- stage A: rows of the (shifted, phased) input field ----
+        static_assert(APASS * RPP == BM && BPASS * RPPB == BN, "loader passes must tile BM and BN exactly");
+
+        // ---- staging of one K chunk: global loads into registers (branch-
+        // free: out-of-field elements read g_zero_run), then LDS stores ----
+        struct Stage {
+            CS<S> x[APASS][VEC];
+            CS<S> po[APASS][VEC];   // CGA: previous search direction
+            CS<S> ph[APASS][VEC];   // M: phase
+            uint4 e[BPASS][PR::NPB];
+        };
+        auto load = [&](Stage& g, int k0) {
             #pragma unroll
             for (int ps = 0; ps < APASS; ++ps) {
                 const int i = ps * RPP + tid / TPR;
                 const int kk = (tid % TPR) * VEC;
-                if (i < BM) {
-                    const int t = t0 + i;
-                    const bool vrow = t < nt;
-                    const int tin = (OP == HOP_M) ? (t == 0 ? nt - 1 : t - 1) : (t + 1 == nt ? 0 : t + 1);
-                    const size_t rowoff = (bbase + (size_t)tin * nx);
-                    CS<S> v[VEC];
-                    #pragma unroll
-                    for (int e = 0; e < VEC; ++e) {
-                        const int x = k0 + kk + e;
-                        CS<S> z{S(0), S(0)};
-                        if (vrow && x < nx) {
-                            z = ldc<S>(J.X, rowoff + x);
-                            if (MODE == MODE_CGA) {
-                                const CS<S> po = ldc<S>(J.Pold, rowoff + x);
-                                z.re = z.re + S(beta) * po.re;
-                                z.im = z.im + S(beta) * po.im;
-                            }
-                            if (OP == HOP_M) {
-                                const CS<S> ph = ldc<S>(Pf, (size_t)tin * nx + x);
-                                const S re = ph.re * z.re - ph.im * z.im;
-                                const S im = ph.re * z.im + ph.im * z.re;
-                                z.re = re; z.im = im;
-                            }
-                        }
-                        v[e] = z;
-                    }
-                    if constexpr (PR::NPA == 2) {
-                        #pragma unroll
-                        for (int e = 0; e < VEC; ++e) {
-                            sA[0][i][kk + e] = v[e].re;
-                            sA[1][i][kk + e] = v[e].im;
-                        }
-                    } else {
-                        #pragma unroll
-                        for (int e = 0; e < VEC; ++e) {
-                            const uint16_t rh = f2bf(v[e].re), ih = f2bf(v[e].im);
-                            sA[0][i][kk + e] = rh;
-                            sA[1][i][kk + e] = f2bf(v[e].re - bf2f(rh));
-                            sA[2][i][kk + e] = ih;
-                            sA[3][i][kk + e] = f2bf(v[e].im - bf2f(ih));
-                        }
-                    }
+                const int t = t0 + i;
+                const bool vrow = t < nt;
+                const int tin = (OP == HOP_M) ? (t == 0 ? nt - 1 : t - 1) : (t + 1 >= nt ? 0 : t + 1);
+                const size_t rowoff = bbase + (size_t)tin * nx;
+                #pragma unroll
+                for (int e = 0; e < VEC; ++e) {
+                    const int x = k0 + kk + e;
+                    const bool ok = vrow && x < nx;
+                    g.x[ps][e] = ldc_or0<S>(J.X, rowoff + x, ok);
+                    if (MODE == MODE_CGA) g.po[ps][e] = ldc_or0<S>(J.Pold, rowoff + x, ok);
+                    if (OP == HOP_M) g.ph[ps][e] = ldc_or0<S>(Pf, (size_t)tin * nx + x, ok);
                 }
             }
-            // ---- stage B: rows y of E (zero padded to ldE columns / rows) ----
+            // rows y of E, zero padded to ldE columns / e_rows rows: always in bounds
             #pragma unroll
             for (int ps = 0; ps < BPASS; ++ps) {
                 const int j = ps * RPPB + tid / TPRB;
                 const int kk = (tid % TPRB) * VB;
-                if (j < BN) {
-                    const size_t eoff = (size_t)(y0 + j) * ldE + k0 + kk;
-                    #pragma unroll
-                    for (int pl = 0; pl < PR::NPB; ++pl) {
-                        const uint4 w = *reinterpret_cast<const uint4*>(
-                            reinterpret_cast<const L*>(J.E) + (size_t)pl * planeE + eoff);
-                        *reinterpret_cast<uint4*>(&sB[pl][j][kk]) = w;
+                const size_t eoff = (size_t)(y0 + j) * ldE + k0 + kk;
+                #pragma unroll
+                for (int pl = 0; pl < PR::NPB; ++pl)
+                    g.e[ps][pl] = *reinterpret_cast<const uint4*>(reinterpret_cast<const L*>(J.E) + (size_t)pl * planeE + eoff);
+            }
+        };
+        auto store = [&](const Stage& g, int buf) {
+            #pragma unroll
+            for (int ps = 0; ps < APASS; ++ps) {
+                const int i = ps * RPP + tid / TPR;
+                const int kk = (tid % TPR) * VEC;
+                #pragma unroll
+                for (int e = 0; e < VEC; ++e) {
+                    CS<S> z = g.x[ps][e];
+                    if (MODE == MODE_CGA) {
+                        z.re = z.re + S(beta) * g.po[ps][e].re;
+                        z.im = z.im + S(beta) * g.po[ps][e].im;
+                    }
+                    if (OP == HOP_M) {
+                        const CS<S> ph = g.ph[ps][e];
+                        const S re = ph.re * z.re - ph.im * z.im;
+                        const S im = ph.re * z.im + ph.im * z.re;
+                        z.re = re; z.im = im;
+                    }
+                    if constexpr (PR::NPA == 2) {
+                        sA[buf][0][i][kk + e] = z.re;
+                        sA[buf][1][i][kk + e] = z.im;
+                    } else {
+                        const uint16_t rh = f2bf(z.re), ih = f2bf(z.im);
+                        sA[buf][0][i][kk + e] = rh;
+                        sA[buf][1][i][kk + e] = f2bf(z.re - bf2f(rh));
+                        sA[buf][2][i][kk + e] = ih;
+                        sA[buf][3][i][kk + e] = f2bf(z.im - bf2f(ih));
                     }
                 }
             }
-            __syncthreads();
+            #pragma unroll
+            for (int ps = 0; ps < BPASS; ++ps) {
+                const int j = ps * RPPB + tid / TPRB;
+                const int kk = (tid % TPRB) * VB;
+                #pragma unroll
+                for (int pl = 0; pl < PR::NPB; ++pl) *reinterpret_cast<uint4*>(&sB[buf][pl][j][kk]) = g.e[ps][pl];
+            }
+        };
 
-            // ---- MFMA over the chunk ----
+        // ---- MFMA over the chunk in LDS buffer cur ----
+        auto mfma = [&](const int cur) {
             if constexpr (PR::KSTEP == 4) {
                 #pragma unroll
                 for (int s = 0; s < RUN; ++s) {
@@ -244,11 +282,11 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
                     #pragma unroll
                     for (int i = 0; i < FM; ++i) {
                         const int row = (wm * FM + i) * 16 + r16;
-                        ar[i] = sA[0][row][k];
-                        ai[i] = sA[1][row][k];
+                        ar[i] = sA[cur][0][row][k];
+                        ai[i] = sA[cur][1][row][k];
                     }
                     #pragma unroll
-                    for (int j = 0; j < FN; ++j) bv[j] = sB[0][(wn * FN + j) * 16 + r16][k];
+                    for (int j = 0; j < FN; ++j) bv[j] = sB[cur][0][(wn * FN + j) * 16 + r16][k];
                     #pragma unroll
                     for (int i = 0; i < FM; ++i)
                         #pragma unroll
@@ -270,16 +308,16 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
                     #pragma unroll
                     for (int i = 0; i < FM; ++i) {
                         const int row = (wm * FM + i) * 16 + r16;
-                        arh[i] = *reinterpret_cast<const bf8*>(&sA[0][row][k]);
-                        arl[i] = *reinterpret_cast<const bf8*>(&sA[1][row][k]);
-                        aih[i] = *reinterpret_cast<const bf8*>(&sA[2][row][k]);
-                        ail[i] = *reinterpret_cast<const bf8*>(&sA[3][row][k]);
+                        arh[i] = *reinterpret_cast<const bf8*>(&sA[cur][0][row][k]);
+                        arl[i] = *reinterpret_cast<const bf8*>(&sA[cur][1][row][k]);
+                        aih[i] = *reinterpret_cast<const bf8*>(&sA[cur][2][row][k]);
+                        ail[i] = *reinterpret_cast<const bf8*>(&sA[cur][3][row][k]);
                     }
                     #pragma unroll
                     for (int j = 0; j < FN; ++j) {
                         const int col = (wn * FN + j) * 16 + r16;
-                        bh[j] = *reinterpret_cast<const bf8*>(&sB[0][col][k]);
-                        bl[j] = *reinterpret_cast<const bf8*>(&sB[1][col][k]);
+                        bh[j] = *reinterpret_cast<const bf8*>(&sB[cur][0][col][k]);
+                        bl[j] = *reinterpret_cast<const bf8*>(&sB[cur][1][col][k]);
                     }
                     #pragma unroll
                     for (int i = 0; i < FM; ++i)
@@ -294,8 +332,28 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
                         }
                 }
             }
+        };
+
+        const int nch = (nx + BK - 1) / BK;
+        Stage stg;
+        load(stg, 0);
+        store(stg, 0);
+        __syncthreads();
+        for (int c = 0; c + 1 < nch; ++c) {
+            // the next chunk's global loads are in flight during this chunk's
+            // MFMAs: the scheduling barriers keep global loads above and LDS
+            // writes and MFMAs out of the other phases (VALU, SALU and LDS
+            // reads may cross; mask 0x146)
+            load(stg, (c + 1) * BK);
+            __builtin_amdgcn_sched_barrier(0x146);
+            mfma(PR::NBUF == 2 ? (c & 1) : 0);
+            __builtin_amdgcn_sched_barrier(0x146);
+            if (PR::NBUF == 1) __syncthreads();   // all waves done reading the only buffer
+            store(stg, PR::NBUF == 2 ? ((c + 1) & 1) : 0);
             __syncthreads();
         }
+        mfma(PR::NBUF == 2 ? ((nch - 1) & 1) : 0);
+        __syncthreads();   // the LDS is reused by the next tile of a grouped launch
 
         // ---- epilogue: shift sign, identity, conj(P), fused recurrences ----
         #pragma unroll
@@ -529,12 +587,6 @@ struct Raw {
     CS<S> po[VK];   // previous search direction (CGA)
 };
 
-// 64 zero bytes: the source of every run that lies outside the field (a run
-// is at most 64 bytes).
-// Not const: a const array would live in the constant address space, and a
-// select between it and a global pointer would turn the loads into flat loads
-// (whose waits cannot be counted per load).  Never written.
-__device__ __attribute__((aligned(16))) double g_zero_run[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
 // VK complex (64 bytes) at element `off` of base, 16-byte aligned.  Branch-
 // and select-free on the loaded data, so that the compiler can keep several
