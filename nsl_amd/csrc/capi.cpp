@@ -346,7 +346,7 @@ int nslx_fm_tune(nslx_fm fm, int precision, int64_t nbatch, int reps, int* best_
 }
 
 int nslx_prof_enable(int on) {
-    nslx::prof_enable(on != 0);
+    nslx::prof_enable(on);
     return NSLX_OK;
 }
 

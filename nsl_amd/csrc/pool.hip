@@ -304,8 +304,8 @@ void Pool::run_iterations() {
     if (use_graph_) {
         if (!exec_ || mask != cls_mask_) {
             drop_graph();
-            const bool was = prof_on();
-            prof_enable(false);
+            const int was = prof_every();
+            prof_enable(0);
             NSLX_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
             try {
                 launch_all();

@@ -12,7 +12,8 @@ struct Rec {
     hipEvent_t a, b;
 };
 std::mutex g_mu;
-bool g_on = false;
+int g_every = 0;                       // 0: off; n: time every n-th launch of a class
+std::map<std::string, long> g_count;   // launches per class while on
 std::vector<Rec> g_recs;
 std::vector<hipEvent_t> g_pool;
 
@@ -28,7 +29,8 @@ hipEvent_t take() {
 }
 }  // namespace
 
-bool prof_on() { return g_on; }
+bool prof_on() { return g_every > 0; }
+int prof_every() { return g_every; }
 
 namespace {
 thread_local ProfScope* t_scope = nullptr;
@@ -36,7 +38,7 @@ thread_local ProfScope* t_scope = nullptr;
 
 ProfScope* prof_current() { return t_scope; }
 
-ProfScope::ProfScope(const char* name, int prec, int mode, hipStream_t s) : active_(g_on) {
+ProfScope::ProfScope(const char* name, int prec, int mode, hipStream_t s) : active_(g_every > 0) {
     (void)s;
     outer_ = t_scope;
     t_scope = this;
@@ -45,6 +47,12 @@ ProfScope::ProfScope(const char* name, int prec, int mode, hipStream_t s) : acti
     static const char* mn[] = {"plain", "cga", "cgb", "cgi"};
     std::lock_guard<std::mutex> g(g_mu);
     name_ = std::string(name) + "_" + pn[prec % 3] + "_" + mn[mode % 4];
+    // every g_every-th launch of a class is timed
+    const long n = g_count[name_]++;
+    if (n % g_every != 0) {
+        active_ = false;
+        return;
+    }
     a_ = take();
     b_ = take();
 }
@@ -69,12 +77,13 @@ ProfScope::~ProfScope() {
     }
 }
 
-void prof_enable(bool on) { g_on = on; }
+void prof_enable(int every) { g_every = every > 0 ? every : 0; }
 
 void prof_reset() {
     std::lock_guard<std::mutex> g(g_mu);
     for (auto& r : g_recs) { g_pool.push_back(r.a); g_pool.push_back(r.b); }
     g_recs.clear();
+    g_count.clear();
 }
 
 std::string prof_report(int* nclasses) {
@@ -89,7 +98,11 @@ std::string prof_report(int* nclasses) {
         e.second += ms;
     }
     std::ostringstream os;
-    for (auto& kv : acc) os << kv.first << " " << kv.second.first << " " << kv.second.second << "\n";
+    for (auto& kv : acc) {
+        const auto c = g_count.find(kv.first);
+        os << kv.first << " " << (c == g_count.end() ? kv.second.first : c->second) << " " << kv.second.second << " "
+           << kv.second.first << "\n";
+    }
     if (nclasses) *nclasses = (int)acc.size();
     return os.str();
 }

@@ -36,7 +36,8 @@ class ProfScope {
 ProfScope* prof_current();
 
 bool prof_on();
-void prof_enable(bool on);
+int prof_every();
+void prof_enable(int every);
 void prof_reset();
 std::string prof_report(int* nclasses);
 

@@ -365,9 +365,11 @@ int solve_cg_ru(FermionMatrix& fm, int op, int prec, const void* b, void* x, con
         NSLX_HIP(hipStreamSynchronize(s));
         for (auto& v : d)
             if (v == RU_ESCALATE || v < 0) { v = RU_ESCALATE; ++escalated; }
+        // NSLX_RU_ESC_KEEP=1 continues from the FP64-accumulated x instead
+        const bool keep = env_double("NSLX_RU_ESC_KEEP", 0.0) != 0.0;
         if (escalated > 0 && ph.it < maxit) {
             const size_t rb = (size_t)len * 16;
-            for (int64_t i = 0; i < nb; ++i) {
+            for (int64_t i = 0; i < nb && !keep; ++i) {
                 if (d[i] != RU_ESCALATE) continue;
                 char* xi = static_cast<char*>(x) + i * rb;
                 if (x0) NSLX_HIP(hipMemcpyAsync(xi, static_cast<const char*>(x0) + i * rb, rb, hipMemcpyDeviceToDevice, s));
