@@ -174,6 +174,26 @@ int nslx_solve(nslx_fm fm, int op, int method, const void* b, void* x, const voi
                int64_t nbatch, double tol, int tol_kind, int64_t maxit, int precision,
                int check_true_residual, nslx_solve_report* report, void* stream);
 
+/* Preconditioned CG.  Replaces NSL::LinAlg::CGpreconditioned<Type>
+ * (src/NSL/LinAlg/Solver/Impl/CGpreconditioned.hpp:34-41 (std::function M,
+ * Mprec), :83-91 (two fermion matrices + MatrixCombination), :133-141,
+ * :183-190; solve_ CGpreconditioned.tpp:14-100, operator() :103-118): outer CG
+ * on op of fm, each z = P^-1 r an inner CG on op of fm_prec started from r
+ * with the same tol and maxit (the reference's innerCG_).  x0 NULL means
+ * x0 = b.  tol_kind NSLX_TOL_ABS_BATCH reproduces the reference (batch-global
+ * inner products, |r|^2 <= tol^2, both levels); NSLX_TOL_REL_PER_RHS runs
+ * both levels per RHS.  inner_precision NSLX_F32 / NSLX_BF16X3 runs the inner
+ * solves with the reliable-update solver (then op must be MMDAGGER or
+ * MDAGGERM and tol_kind NSLX_TOL_REL_PER_RHS).  fm_prec may be fm itself; it
+ * must have fm's Nx, Nt and device.  Report: iterations = outer iterations,
+ * iterations_total = inner iterations summed over all inner solves,
+ * outer_steps = number of inner solves, max_rel_residual = FP64 true
+ * residual max_k |b_k - A x_k| / |b_k| (always computed).  Returns
+ * NSLX_NOT_CONVERGED (x written) like nslx_solve. */
+int nslx_solve_precond(nslx_fm fm, nslx_fm fm_prec, int op, const void* b, void* x, const void* x0,
+                       int64_t nbatch, double tol, int tol_kind, int64_t maxit, int inner_precision,
+                       nslx_solve_report* report, void* stream);
+
 /* ---- batched ragged solve ----------------------------------------------------------
  * No single counterpart in the reference (it solves one HubbardExp at a
  * time, CG.tpp:14-113); this is the reference's CG on MMdagger for many

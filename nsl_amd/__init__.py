@@ -294,6 +294,46 @@ class BiCGStab(_Solver):
     method = _lib.BICGSTAB_METHOD
 
 
+class CGpreconditioned:
+    """NSL::LinAlg::CGpreconditioned (CGpreconditioned.hpp:83-91, .tpp:14-118):
+    CG on matrixCombination of M, each preconditioner step z = P^-1 r an
+    inner CG on the same combination of Mprec started from r, with the same
+    eps and maxIter.  The reference's example preconditions with the
+    non-interacting matrix (phi = 0).  tol_kind TOL_ABS_BATCH (default) is
+    the reference's batch-global criterion at both levels; inner_precision
+    F32/BF16X3 runs the inner solves in low precision (TOL_REL_PER_RHS,
+    MMdagger/MdaggerM only).  Mprec may be M itself."""
+
+    def __init__(self, M, Mprec, matrixCombination=MatrixCombination.M, eps=1e-12, maxIter=10000,
+                 tol_kind=TOL_ABS_BATCH, inner_precision=F64):
+        self.fm, self.fp = M, Mprec
+        self.op = int(matrixCombination)
+        self.eps = float(eps)
+        self.maxIter = int(maxIter)
+        self.tol_kind = int(tol_kind)
+        self.inner_precision = int(inner_precision)
+        self.report = None
+        self.converged = None
+
+    def __call__(self, b, x0=None, out=None):
+        torch = _torch()
+        b = b.to(dtype=torch.complex128).contiguous()
+        nb = int(np.prod(b.shape[:-2])) if b.dim() > 2 else 1
+        x = torch.empty_like(b) if out is None else out
+        if x0 is not None:
+            x0 = x0.to(dtype=torch.complex128).contiguous()
+        rep = SolveReport()
+        st = _lib.lib().nslx_solve_precond(self.fm._h, self.fp._h, self.op, _ptr(b), _ptr(x),
+                                           _ptr(x0) if x0 is not None else None, nb, self.eps, self.tol_kind,
+                                           self.maxIter, self.inner_precision, ctypes.byref(rep), _stream())
+        _lib.check(st, allow=(_lib.OK, _lib.NOT_CONVERGED))
+        self.report = rep.as_dict()
+        self.converged = st == _lib.OK
+        if not self.converged:
+            log.error("CGpreconditioned did not converge within %d iterations", self.maxIter)
+        return x
+
+
 def batch_solve(problems, tol=1e-12, precision=F64, maxIter=100000, use_graph=True, ctx=None):
     """Solve MMdagger x = b for many fermion matrices of different Nx, Nt
     and fields at once (nslx_batch_solve): problems is a list of (fm, b)

@@ -194,6 +194,18 @@ int nslx_solve(nslx_fm fm, int op, int method, const void* b, void* x, const voi
     });
 }
 
+int nslx_solve_precond(nslx_fm fm, nslx_fm fm_prec, int op, const void* b, void* x, const void* x0, int64_t nbatch,
+                       double tol, int tol_kind, int64_t maxit, int inner_precision, nslx_solve_report* report,
+                       void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        auto& mp = FM(fm_prec);
+        DeviceGuard dg(m.ctx->device);
+        return nslx::solve_precond(m, mp, op, b, x, x0, nbatch, tol, tol_kind, maxit, inner_precision, report,
+                                   nslx::as_stream(stream));
+    });
+}
+
 int nslx_batch_solve(nslx_ctx ctx, int64_t nitems, const nslx_batch_item* items, double tol, int precision,
                      int64_t maxit, int use_graph, nslx_batch_report* report, void* stream) {
     return guard([&] {

@@ -99,6 +99,9 @@ void fm_init(FermionMatrix& m, Ctx* ctx, int nx, int nt, const double* K, double
 // Solvers (solver.cpp)
 int solve(FermionMatrix& fm, int op, int method, const void* b, void* x, const void* x0, int64_t nb, double tol,
           int tol_kind, int64_t maxit, int prec, int check, nslx_solve_report* rep, hipStream_t s);
+// Outer CG on fm preconditioned by an inner CG on fp (CGpreconditioned).
+int solve_precond(FermionMatrix& fm, FermionMatrix& fp, int op, const void* b, void* x, const void* x0, int64_t nb,
+                  double tol, int tol_kind, int64_t maxit, int inner_prec, nslx_solve_report* rep, hipStream_t s);
 
 // log det / force (logdet.cpp)
 void fm_logdet(FermionMatrix& fm, int64_t field, int nchunks, double* out2, hipStream_t s);

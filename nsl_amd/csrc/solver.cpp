@@ -608,4 +608,122 @@ int solve(FermionMatrix& fm, int op, int method, const void* b, void* x, const v
     return st;
 }
 
+// ------------------------------------ CG with an inner-CG preconditioner
+// The outer recurrence of CGpreconditioned.tpp:14-100 on fm's operator: x0 = b
+// (or the given x0, :103-118), r = b - A x, z = P^{-1} r, p = z; per step
+// t = A p, alpha = <r,z>/<p,t>, x += alpha p, r -= alpha t, stop on |r|^2,
+// z = P^{-1} r, beta = <r,z>_new/<r,z>_old, p = z + beta p.  P^{-1} r is an
+// inner CG on fp's operator with the same eps and maxIter started from r
+// (the reference's innerCG_, CGpreconditioned.hpp:34-41,83-91); it may run in
+// a low precision (reliable updates), which makes the preconditioner inexact.
+// The outer vectors have their own buffers, so fp may be fm itself.  The
+// reference computes z before its first stop test; here a RHS that is already
+// converged skips that inner solve (x is the same).
+int solve_precond(FermionMatrix& fm, FermionMatrix& fp, int op, const void* b, void* x, const void* x0, int64_t nb,
+                  double tol, int tol_kind, int64_t maxit, int inner_prec, nslx_solve_report* rep, hipStream_t s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    NSLX_REQUIRE(b && x, "solve_precond: null buffer");
+    NSLX_REQUIRE(b != x && x0 != x, "solve_precond: x must not alias b or x0");
+    NSLX_REQUIRE(op >= NSLX_OP_M && op <= NSLX_OP_MDAGGERM, "solve_precond: bad op");
+    NSLX_REQUIRE(tol_kind == NSLX_TOL_ABS_BATCH || tol_kind == NSLX_TOL_REL_PER_RHS, "solve_precond: bad tol_kind");
+    NSLX_REQUIRE(tol >= 0.0 && maxit >= 0, "solve_precond: bad tol/maxit");
+    NSLX_REQUIRE(inner_prec == NSLX_F64 || inner_prec == NSLX_F32 || inner_prec == NSLX_BF16X3,
+                 "solve_precond: bad inner precision");
+    fm.require_populated();
+    fp.require_populated();
+    NSLX_REQUIRE(fp.nx == fm.nx && fp.nt == fm.nt, "solve_precond: the preconditioner's Nx and Nt must match");
+    NSLX_REQUIRE(fp.ctx->device == fm.ctx->device, "solve_precond: both fermion matrices must be on one device");
+    NSLX_REQUIRE(nb >= 1 && nb % fm.nfield == 0 && nb % fp.nfield == 0,
+                 "solve_precond: nbatch must be a positive multiple of both nfield");
+    nslx_solve_report local{};
+    if (!rep) rep = &local;
+    *rep = nslx_solve_report{};
+    const int64_t len = fm.len();
+    const size_t bytes = (size_t)nb * len * 16;
+    const int global = tol_kind == NSLX_TOL_ABS_BATCH;
+    const int64_t nslot = global ? 1 : nb;
+    fm.sw.ensure_scalars(nb, 1, s);
+    DevBuf<char> r(bytes), t(bytes), p(bytes), z(bytes);
+    DevBuf<int> done_d(nslot);
+    HostDots hd{fm, nb, global, s};
+    std::vector<double> t2(nslot);
+    if (global) t2[0] = tol * tol;
+    else {
+        auto bb = hd.dot(b, b);
+        for (int64_t i = 0; i < nslot; ++i) t2[i] = tol * tol * bb[i].first;
+    }
+    std::vector<int> done(nslot, -1);
+    std::vector<int> done_h(nslot);
+    auto push_done = [&]() {
+        for (int64_t i = 0; i < nslot; ++i) done_h[i] = done[i];
+        NSLX_HIP(hipMemcpyAsync(done_d.p, done_h.data(), nslot * sizeof(int), hipMemcpyHostToDevice, s));
+        NSLX_HIP(hipStreamSynchronize(s));
+    };
+    int64_t inner_its = 0, nprec = 0;
+    auto precond = [&]() {
+        nslx_solve_report ir{};
+        // a non-converged inner solve still returns its current iterate (CG.tpp:90-93)
+        (void)solve(fp, op, NSLX_CG, r.p, z.p, r.p, nb, tol, tol_kind, maxit, inner_prec, 0, &ir, s);
+        inner_its += ir.iterations_total;
+        ++nprec;
+    };
+    const std::vector<cd> one(nslot, cd{1.0, 0.0});
+    copy_dev(x, x0 ? x0 : b, bytes, s);
+    fm.apply(op, x, t.p, nb, NSLX_F64, s);
+    push_done();
+    hd.axpy(b, t.p, r.p, one, -1.0, nullptr);   // r = b - A x
+    auto rr = hd.dot(r.p, r.p);
+    int64_t active = 0;
+    for (int64_t i = 0; i < nslot; ++i) { if (rr[i].first <= t2[i]) done[i] = 0; else ++active; }
+    push_done();
+    std::vector<double> rz(nslot, 0.0);
+    int64_t it = 0;
+    if (active > 0) {
+        precond();
+        copy_dev(p.p, z.p, bytes, s);            // p = z
+        auto d = hd.dot(r.p, z.p);
+        for (int64_t i = 0; i < nslot; ++i) rz[i] = d[i].first;
+    }
+    while (active > 0 && it < maxit) {
+        ++it;
+        fm.apply(op, p.p, t.p, nb, NSLX_F64, s);
+        auto pt = hd.dot(p.p, t.p);
+        std::vector<cd> al(nslot);
+        for (int64_t i = 0; i < nslot; ++i) al[i] = cdiv(cd{rz[i], 0.0}, pt[i]);
+        hd.axpy(x, p.p, x, al, 1.0, done_d.p);
+        hd.axpy(r.p, t.p, r.p, al, -1.0, done_d.p);
+        auto rn = hd.dot(r.p, r.p);
+        for (int64_t i = 0; i < nslot; ++i) {
+            if (done[i] >= 0) continue;
+            rr[i] = rn[i];
+            if (rn[i].first <= t2[i]) { done[i] = (int)it; --active; }
+        }
+        push_done();
+        if (active == 0) break;
+        precond();
+        auto rzn = hd.dot(r.p, z.p);
+        std::vector<cd> be(nslot, cd{0.0, 0.0});
+        for (int64_t i = 0; i < nslot; ++i) {
+            if (done[i] >= 0) continue;
+            be[i] = {rzn[i].first / rz[i], 0.0};
+            rz[i] = rzn[i].first;
+        }
+        hd.axpy(z.p, p.p, p.p, be, 1.0, done_d.p);   // p = z + beta p
+    }
+    int64_t conv = 0, maxi = 0;
+    for (int d : done) if (d >= 0) { ++conv; maxi = std::max<int64_t>(maxi, d); }
+    const bool all = conv == nslot;
+    rep->iterations = all ? maxi : it;
+    rep->iterations_total = inner_its;
+    rep->outer_steps = nprec;
+    rep->converged_rhs = global ? (all ? nb : 0) : conv;
+    double tot = 0.0;
+    for (auto& v : rr) tot += v.first;
+    rep->rsqr = tot;
+    rep->max_rel_residual = true_rel_residual(fm, op, b, x, nb, s);
+    NSLX_HIP(hipStreamSynchronize(s));
+    rep->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return all ? NSLX_OK : NSLX_NOT_CONVERGED;
+}
+
 }  // namespace nslx

@@ -260,6 +260,39 @@ def test_cg_fermion_matrix_and_not_converged():
     assert not info1["converged"] and info1["iterations"] == 3
 
 
+def test_cg_preconditioned_pins():
+    """CGpreconditioned.tpp:14-118.  P = I: the inner CG from x0 = r stops at
+    once with z = r, so the outer recurrence must be plain CG iterate for
+    iterate; P = A: z = A^-1 r, one or two outer steps; a random HPD P: the
+    direct solve; a fermion-matrix system preconditioned by the free
+    (phi = 0) matrix, as in the reference's example: its FP64 true residual."""
+    g = torch.Generator().manual_seed(1)
+    n = 40
+    X = torch.randn(n, n, dtype=torch.complex128, generator=g)
+    A = X @ X.conj().T + n * torch.eye(n, dtype=torch.complex128)
+    Y = torch.randn(n, n, dtype=torch.complex128, generator=g)
+    P = Y @ Y.conj().T + n * torch.eye(n, dtype=torch.complex128)
+    b = torch.randn(n, dtype=torch.complex128, generator=g)
+    mat = lambda M: (lambda v: M @ v)
+    x_cg, i_cg = solvers.cg(mat(A), b, eps=1e-12)
+    x_pi, i_pi = solvers.cg_preconditioned(mat(A), lambda v: v, b, eps=1e-12)
+    assert i_pi["iterations"] == i_cg["iterations"] and i_pi["inner_iterations"] == 0
+    assert torch.equal(x_pi, x_cg)
+    x_pa, i_pa = solvers.cg_preconditioned(mat(A), mat(A), b, eps=1e-12)
+    assert i_pa["converged"] and i_pa["iterations"] <= 2
+    x_pp, i_pp = solvers.cg_preconditioned(mat(A), mat(P), b, eps=1e-12)
+    assert i_pp["converged"]
+    assert (x_pp - torch.linalg.solve(A, b)).abs().max() < 1e-10
+    lat = lattice.honeycomb(3, 3)
+    h, _ = _setup(lat, 16, 4.0, std=math.sqrt(0.5))
+    h0 = F.HubbardExp(lat, 16, 4.0)
+    h0.populate(torch.zeros(16, 18))
+    bf = torch.as_tensor(rng.complex_normal(6, 0, (2, 16, 18)))
+    x, info = solvers.cg_preconditioned(h.MMdagger, h0.MMdagger, bf, eps=1e-10)
+    assert info["converged"] and info["inner_iterations"] > 0
+    assert ((h.MMdagger(x) - bf).norm() / bf.norm()) < 1e-9
+
+
 def test_correlator_free_propagator():
     lat = lattice.ring(4)
     Nt, beta = 8, 2.0
