@@ -115,6 +115,21 @@ int nslx_lattice_armchair_nanotube(int n, int cells, double kappa, double* K, do
  * beta and mu are real here (the reference allows complex Type). */
 int nslx_fm_create(nslx_ctx ctx, int nx, int nt, const double* K, double beta, double mu,
                    int species, nslx_fm* out);
+
+/* Replaces NSL::FermionMatrix::HubbardDiag<complex<double>, Lattice>, ctor
+ * (species, lat, Nt, beta, mu): src/NSL/FermionMatrix/Impl/hubbardDiag.hpp.
+ * The hop's dense matrix is s*delta*K on the same time slice and the phases
+ * sit on the time shift: (M psi)_t = psi_t - s delta K psi_t
+ * - b_t P_{t-1} psi_{t-1} (hubbardDiag.tpp:10-36), M^dagger :39-72, MMdagger
+ * and MdaggerM as M(M^dagger .) and M^dagger(M .) (:75-165 expand to exactly
+ * these).  Every other nslx_fm_* call, nslx_solve, nslx_solve_precond,
+ * nslx_correlator_two_point and the HMC entries take the handle; logdet
+ * follows hubbardDiag.tpp:166-190 (i s sum(phi) + log det(1 + prod_t
+ * (1 - s delta K) diag(Pinv_{t+1})), stable chunked form, no mu term like the
+ * reference) and grad_logdet :193-252.  The batching engine
+ * (nslx_batch_solve, nslx_server_*) takes HubbardExp matrices only. */
+int nslx_fm_create_diag(nslx_ctx ctx, int nx, int nt, const double* K, double beta, double mu,
+                        int species, nslx_fm* out);
 int nslx_fm_destroy(nslx_fm fm);
 
 /* populate(phi[, species]): hubbardExp.hpp:103-128.  phi: device c128,

@@ -1,7 +1,8 @@
-"""nsl_amd: MI355X-native HubbardExp fermion matrix and Krylov solvers.
+"""nsl_amd: MI355X-native HubbardExp / HubbardDiag fermion matrices and Krylov solvers.
 
 Host-side mirror of the reference's interfaces for this path (Marcel-Rodekamp/
 NSL): ``HubbardExp`` (src/NSL/FermionMatrix/Impl/hubbardExp.hpp:33-233),
+``HubbardDiag`` (src/NSL/FermionMatrix/Impl/hubbardDiag.hpp),
 ``CG`` / ``BiCGStab`` (src/NSL/LinAlg/Solver/Impl/CG.hpp, BiCGStab.hpp) with
 ``MatrixCombination`` (applicationType.tpp:18), the Hubbard action and HMC
 step (Action/, Integrator/, MarkovChain/HMC.tpp).  All arithmetic runs in
@@ -145,6 +146,8 @@ class HubbardExp:
     k // (nbatch // nfield)).
     """
 
+    _create = "nslx_fm_create"
+
     def __init__(self, lat, Nt, beta=1.0, mu=0.0, species=Particle, ctx=None):
         self.lat = lat
         self.Nt = int(Nt)
@@ -154,9 +157,9 @@ class HubbardExp:
         self._species = int(species)
         self.ctx = ctx or Context.default()
         h = ctypes.c_void_p()
-        _lib.check(_lib.lib().nslx_fm_create(self.ctx._h, self.Nx, self.Nt,
-                                             lat.K.ctypes.data_as(_lib.P_double), self.beta, self.mu,
-                                             self._species, ctypes.byref(h)))
+        _lib.check(getattr(_lib.lib(), self._create)(self.ctx._h, self.Nx, self.Nt,
+                                                     lat.K.ctypes.data_as(_lib.P_double), self.beta, self.mu,
+                                                     self._species, ctypes.byref(h)))
         self._h = h
         self.nfield = 0
         self._phi = None
@@ -240,6 +243,16 @@ class HubbardExp:
         out = torch.empty((self.Nt, self.Nx), dtype=torch.complex128, device="cuda")
         _lib.check(_lib.lib().nslx_fm_grad_logdet(self._h, int(field), _ptr(out), _stream()))
         return out
+
+
+class HubbardDiag(HubbardExp):
+    """NSL::FermionMatrix::HubbardDiag on the device (hubbardDiag.hpp,
+    hubbardDiag.tpp): (M psi)_t = psi_t - s delta K psi_t - B_t P_{t-1} psi_{t-1}.
+    Same constructors, populate, M/Mdagger/MMdagger/MdaggerM, logDetM and
+    gradLogDetM as HubbardExp; solvers, the correlator and the HMC entries
+    take it too.  The batching engine (batch_solve, Server) does not."""
+
+    _create = "nslx_fm_create_diag"
 
 
 # ------------------------------------------------------------------ solvers
@@ -458,8 +471,10 @@ class HubbardAction:
     (hubbardGaugeAction.tpp, hubbardFermiAction.tpp, leapfrogRealForce.tpp,
     HMC.tpp:185-241)."""
 
-    def __init__(self, lat, Nt, beta, U, mu=0.0, ctx=None, nchunks=0):
-        self.fm = HubbardExp(lat, Nt, beta, mu, Particle, ctx)
+    def __init__(self, lat, Nt, beta, U, mu=0.0, ctx=None, nchunks=0, fermion_matrix=None):
+        # HubbardFermionAction<..., FermionMatrixType> (hubbardFermiAction.tpp:23-60):
+        # HubbardExp by default, HubbardDiag if passed
+        self.fm = (fermion_matrix or HubbardExp)(lat, Nt, beta, mu, Particle, ctx)
         self.U = float(U)
         self.nchunks = int(nchunks)
 

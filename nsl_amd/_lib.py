@@ -95,6 +95,8 @@ PROTOS = {
     "nslx_lattice_armchair_nanotube": (c_int, [c_int, c_int, c_double, P_double, P_double]),
     "nslx_fm_create": (c_int, [c_void_p, c_int, c_int, P_double, c_double, c_double, c_int,
                                ctypes.POINTER(c_void_p)]),
+    "nslx_fm_create_diag": (c_int, [c_void_p, c_int, c_int, P_double, c_double, c_double, c_int,
+                                    ctypes.POINTER(c_void_p)]),
     "nslx_fm_destroy": (c_int, [c_void_p]),
     "nslx_fm_populate": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_void_p]),
     "nslx_fm_apply": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int64, c_int, c_void_p]),

@@ -139,6 +139,18 @@ int nslx_fm_create(nslx_ctx ctx, int nx, int nt, const double* K, double beta, d
     });
 }
 
+int nslx_fm_create_diag(nslx_ctx ctx, int nx, int nt, const double* K, double beta, double mu, int species,
+                        nslx_fm* out) {
+    return guard([&] {
+        NSLX_REQUIRE(ctx && out && K, "fm_create_diag: null argument");
+        DeviceGuard dg(ctx->c.device);
+        auto f = std::make_unique<nslx_fm_s>();
+        nslx::fm_init(f->m, &ctx->c, nx, nt, K, beta, mu, species, nslx::DISC_DIAG);
+        *out = f.release();
+        return NSLX_OK;
+    });
+}
+
 int nslx_fm_destroy(nslx_fm fm) {
     if (fm) {
         (void)hipDeviceSynchronize();

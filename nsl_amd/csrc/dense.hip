@@ -113,6 +113,10 @@ __global__ void k_copy_r(int64_t m, const double* in, double* out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < m) out[i] = in[i];
 }
+__global__ void k_scaled_plus_identity(int n, const double* K, double a, double id, double* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < (int64_t)n * n) out[i] = a * K[i] + ((i / n == i % n) ? id : 0.0);
+}
 
 // ---- LU with partial pivoting, one column per pair of launches ----
 __global__ void k_lu_pivot(int n, int k, double2* A, double* acc) {
@@ -385,6 +389,10 @@ void real_to_complex(int64_t m, const double* in, void* out, hipStream_t s) {
 }
 void add_identity_c(int n, void* A, hipStream_t s) {
     hipLaunchKernelGGL(k_add_identity_c, dim3(g1(n)), dim3(256), 0, s, n, (double2*)A);
+    NSLX_LAUNCH_CHECK();
+}
+void scaled_plus_identity(int n, const double* K, double a, double id, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_scaled_plus_identity, dim3(g1((int64_t)n * n)), dim3(256), 0, s, n, K, a, id, out);
     NSLX_LAUNCH_CHECK();
 }
 

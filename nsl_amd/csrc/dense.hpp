@@ -40,6 +40,8 @@ void build_cyclic(int n, int nblk, const void* C, void* R, hipStream_t s);
 void set_identity_c(int n, void* A, hipStream_t s);
 void real_to_complex(int64_t count, const double* in, void* out, hipStream_t s);
 void add_identity_c(int n, void* A, hipStream_t s);
+// out = a K + id I (real n x n).
+void scaled_plus_identity(int n, const double* K, double a, double id, double* out, hipStream_t s);
 // out[i][j] = dl[i] * E[i][j] (E real, out complex).
 void dscale_rows(int n, const void* dl, const double* E, void* out, hipStream_t s);
 

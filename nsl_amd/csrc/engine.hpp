@@ -58,6 +58,7 @@ struct SolverWork {
 
 struct FermionMatrix {
     Ctx* ctx = nullptr;
+    int disc = DISC_EXP;             // HubbardExp or HubbardDiag (hop.hpp)
     int nx = 0, nt = 0;
     double beta = 1.0, mu = 0.0;
     int species = NSLX_PARTICLE;
@@ -81,7 +82,10 @@ struct FermionMatrix {
     int64_t len() const { return (int64_t)nt * nx; }
 
     EEntry& E(double a, hipStream_t s);      // memoised exp(a K)
-    const void* Eptr(int prec, hipStream_t s);  // E(s*delta) for the hop
+    EEntry& Elin(double a, double id, hipStream_t s);   // memoised id I + a K (HubbardDiag)
+    std::map<std::pair<double, double>, std::unique_ptr<EEntry>> memo_lin;
+    // the hop's dense matrix: exp(s delta K) (EXP) or s delta K (DIAG)
+    const void* Eptr(int prec, hipStream_t s);
     const void* Pptr(int prec) const { return prec == NSLX_F64 ? (const void*)P64.p : (const void*)P32.p; }
 
     void populate(const void* phi_dev, int64_t nf, int sp, hipStream_t s);
@@ -90,11 +94,16 @@ struct FermionMatrix {
     void hop(int prec, int op, const void* in, void* out, int64_t nb, hipStream_t s);
     void apply(int op, const void* in, void* out, int64_t nb, int prec, hipStream_t s);
     HopArgs hop_args(int prec, int64_t nb, hipStream_t s);
-    int tile(int prec, int64_t nb) { return ctx->tiles.get(prec, nx, nt, nb); }
+    // tuned or default tile; HubbardDiag runs on the classic tiles 0-3 only
+    int tile(int prec, int64_t nb) {
+        const int t = ctx->tiles.get(prec, nx, nt, nb);
+        return (disc == DISC_DIAG && t >= 4) ? default_tile(prec, nx, nt, nb) : t;
+    }
 };
 
 // HubbardExp(species, lat, Nt, beta, mu) on ctx's device (fm.cpp)
-void fm_init(FermionMatrix& m, Ctx* ctx, int nx, int nt, const double* K, double beta, double mu, int species);
+void fm_init(FermionMatrix& m, Ctx* ctx, int nx, int nt, const double* K, double beta, double mu, int species,
+             int disc = DISC_EXP);
 
 // Solvers (solver.cpp)
 int solve(FermionMatrix& fm, int op, int method, const void* b, void* x, const void* x0, int64_t nb, double tol,

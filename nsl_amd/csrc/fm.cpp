@@ -1,5 +1,6 @@
-// HubbardExp fermion matrix on the device (reference:
-// src/NSL/FermionMatrix/Impl/hubbardExp.hpp:48-144, hubbardExp.tpp:12-125).
+// HubbardExp and HubbardDiag fermion matrices on the device (reference:
+// src/NSL/FermionMatrix/Impl/hubbardExp.hpp:48-144, hubbardExp.tpp:12-125;
+// hubbardDiag.hpp, hubbardDiag.tpp:10-165).
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -38,12 +39,15 @@ void TileTable::set(int prec, int nx, int nt, int64_t nb, int tile) {
     else best[{prec, nx, nt, nb_class(nb)}] = tile;
 }
 
-void fm_init(FermionMatrix& m, Ctx* ctx, int nx, int nt, const double* K, double beta, double mu, int species) {
+void fm_init(FermionMatrix& m, Ctx* ctx, int nx, int nt, const double* K, double beta, double mu, int species,
+             int disc) {
     NSLX_REQUIRE(K, "fm_create: null K");
+    NSLX_REQUIRE(disc == DISC_EXP || disc == DISC_DIAG, "fm_create: bad discretisation");
     NSLX_REQUIRE(nx >= 1 && nt >= 2, "fm_create: need nx >= 1 and nt >= 2");
     NSLX_REQUIRE(species == NSLX_PARTICLE || species == NSLX_HOLE, "fm_create: bad species");
     NSLX_REQUIRE(std::isfinite(beta) && std::isfinite(mu), "fm_create: beta and mu must be finite");
     m.ctx = ctx;
+    m.disc = disc;
     m.nx = nx;
     m.nt = nt;
     m.beta = beta;
@@ -91,8 +95,26 @@ EEntry& FermionMatrix::E(double a, hipStream_t s) {
     return ref;
 }
 
+EEntry& FermionMatrix::Elin(double a, double id, hipStream_t s) {
+    // Lat.hopping_matrix(a) = a K (src/NSL/Lattice/lattice.tpp:46-49), plus id I
+    auto it = memo_lin.find({a, id});
+    if (it != memo_lin.end()) return *it->second;
+    auto e = std::make_unique<EEntry>();
+    const int n = nx;
+    e->dense.alloc((size_t)n * n);
+    scaled_plus_identity(n, K.p, a, id, e->dense.p, s);
+    const size_t padded = (size_t)e_rows(n) * e_ld(n);
+    e->e64.alloc(padded);
+    e->e32.alloc(padded);
+    e->ebf.alloc(2 * padded);
+    pack_E(n, e->dense.p, e->e64.p, e->e32.p, e->ebf.p, s);
+    auto& ref = *e;
+    memo_lin.emplace(std::make_pair(a, id), std::move(e));
+    return ref;
+}
+
 const void* FermionMatrix::Eptr(int prec, hipStream_t s) {
-    EEntry& e = E(sgn() * delta(), s);
+    EEntry& e = disc == DISC_DIAG ? Elin(sgn() * delta(), 0.0, s) : E(sgn() * delta(), s);
     if (prec == NSLX_F64) return e.e64.p;
     if (prec == NSLX_F32) return e.e32.p;
     return e.ebf.p;
@@ -141,6 +163,7 @@ HopArgs FermionMatrix::hop_args(int prec, int64_t nb, hipStream_t s) {
     a.rpf = (int)(nb / nfield);
     a.E = Eptr(prec, s);
     a.P = Pptr(prec);
+    a.disc = disc;
     return a;
 }
 

@@ -154,7 +154,7 @@ struct Smem {
 // One BM x BN output tile of rhs b (rows t0 = rt*BM.., sites y0 = ct*BN..).
 // Called by the whole workgroup; returns this thread's share of the tile's
 // reduction (|u|^2 or |r|^2 in the CG modes).
-template <class TL, class PR, int OP, int MODE>
+template <class TL, class PR, int OP, int MODE, int DISC>
 __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, size_t planeE, int b, int rt, int ct,
                            double alpha, double beta, Smem<TL, PR>& sm) {
     using S = typename PR::S;
@@ -211,7 +211,9 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
                 const int kk = (tid % TPR) * VEC;
                 const int t = t0 + i;
                 const bool vrow = t < nt;
-                const int tin = (OP == HOP_M) ? (t == 0 ? nt - 1 : t - 1) : (t + 1 >= nt ? 0 : t + 1);
+                // EXP: the GEMM reads the neighbouring slice; DIAG: the same slice
+                const int tin = (DISC == DISC_DIAG) ? t
+                              : (OP == HOP_M) ? (t == 0 ? nt - 1 : t - 1) : (t + 1 >= nt ? 0 : t + 1);
                 const size_t rowoff = bbase + (size_t)tin * nx;
                 #pragma unroll
                 for (int e = 0; e < VEC; ++e) {
@@ -219,7 +221,7 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
                     const bool ok = vrow && x < nx;
                     g.x[ps][e] = ldc_or0<S>(J.X, rowoff + x, ok);
                     if (MODE == MODE_CGA) g.po[ps][e] = ldc_or0<S>(J.Pold, rowoff + x, ok);
-                    if (OP == HOP_M) g.ph[ps][e] = ldc_or0<S>(Pf, (size_t)tin * nx + x, ok);
+                    if (OP == HOP_M && DISC == DISC_EXP) g.ph[ps][e] = ldc_or0<S>(Pf, (size_t)tin * nx + x, ok);
                 }
             }
             // rows y of E, zero padded to ldE columns / e_rows rows: always in bounds
@@ -245,7 +247,7 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
                         z.re = z.re + S(beta) * g.po[ps][e].re;
                         z.im = z.im + S(beta) * g.po[ps][e].im;
                     }
-                    if (OP == HOP_M) {
+                    if (OP == HOP_M && DISC == DISC_EXP) {
                         const CS<S> ph = g.ph[ps][e];
                         const S re = ph.re * z.re - ph.im * z.im;
                         const S im = ph.re * z.im + ph.im * z.re;
@@ -372,7 +374,30 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
                     if (y >= nx) continue;
                     const size_t idx = rowoff + y;
                     S fr = S(accR[i][j][v]), fi = S(accI[i][j][v]);
-                    if (OP == HOP_M) {
+                    if (DISC == DISC_DIAG) {
+                        // + b_t P_{t-1} psi_{t-1} (M) or + b'_t conj(P_t) psi_{t+1} (M^dag),
+                        // psi = the hop input (r + beta p in CGA)
+                        const int tn = (OP == HOP_M) ? (t == 0 ? nt - 1 : t - 1) : (t + 1 >= nt ? 0 : t + 1);
+                        const size_t nidx = bbase + (size_t)tn * nx + y;
+                        CS<S> xn = ldc<S>(J.X, nidx);
+                        if (MODE == MODE_CGA) {
+                            const CS<S> pn = ldc<S>(J.Pold, nidx);
+                            xn.re = xn.re + S(beta) * pn.re;
+                            xn.im = xn.im + S(beta) * pn.im;
+                        }
+                        const CS<S> ph = ldc<S>(Pf, (size_t)(OP == HOP_M ? tn : t) * nx + y);
+                        S dr, di;
+                        if (OP == HOP_M) {
+                            dr = ph.re * xn.re - ph.im * xn.im;
+                            di = ph.re * xn.im + ph.im * xn.re;
+                        } else {
+                            dr = ph.re * xn.re + ph.im * xn.im;
+                            di = ph.re * xn.im - ph.im * xn.re;
+                        }
+                        if (OP == HOP_M ? t == 0 : t == nt - 1) { dr = -dr; di = -di; }
+                        fr = fr + dr;
+                        fi = fi + di;
+                    } else if (OP == HOP_M) {
                         if (t == 0) { fr = -fr; fi = -fi; }
                     } else {
                         const CS<S> ph = ldc<S>(Pf, (size_t)t * nx + y);   // conj(P_t) * acc
@@ -456,7 +481,7 @@ __device__ inline bool cg_decide(const CgDev& c, int slot, int iter, bool design
     return false;
 }
 
-template <class TL, class PR, int OP, int MODE>
+template <class TL, class PR, int OP, int MODE, int DISC>
 __global__ void __launch_bounds__(TL::NT) hop_kernel(HopArgs a, int nrt, int nct, int swz, int ldE, size_t planeE) {
     constexpr int NT = TL::NT;
     __shared__ __attribute__((aligned(16))) Smem<TL, PR> sm;
@@ -503,7 +528,7 @@ __global__ void __launch_bounds__(TL::NT) hop_kernel(HopArgs a, int nrt, int nct
     double part = 0.0;
     if (!skip) {
         const JobPtrs J{a.E, a.P, a.X, a.Y, a.Pold, a.Pnew, a.Xs, a.R, a.Bv};
-        part = hop_tile<TL, PR, OP, MODE>(J, a.nx, a.nt, a.rpf, ldE, planeE, b, rt, ct, alpha, beta, sm);
+        part = hop_tile<TL, PR, OP, MODE, DISC>(J, a.nx, a.nt, a.rpf, ldE, planeE, b, rt, ct, alpha, beta, sm);
     }
     if (MODE == MODE_PLAIN || skip) return;
     // ---- this tile's partial sum, reduced by the next launch ----
@@ -545,7 +570,7 @@ __global__ void __launch_bounds__(TL::NT) hop_grouped(GroupArgs g) {
             auto v = [&](int r) -> void* { return r >= 0 ? P.v[r] : nullptr; };
             const JobPtrs J{P.E, P.P, v(g.role[0]), v(g.role[1]), v(g.role[2]), v(g.role[3]),
                             v(g.role[4]), v(g.role[5]), v(g.role[6])};
-            part = hop_tile<TL, PR, OP, MODE>(J, P.nx, P.nt, P.rpf, P.ldE, (size_t)P.planeE, b, rt, ct, alpha,
+            part = hop_tile<TL, PR, OP, MODE, DISC_EXP>(J, P.nx, P.nt, P.rpf, P.ldE, (size_t)P.planeE, b, rt, ct, alpha,
                                               beta, sm);
         }
         if (MODE != MODE_PLAIN && !skip) {
@@ -1015,10 +1040,13 @@ void launch_t(int op, int mode, const HopArgs& a, hipStream_t s) {
     const int ldE = e_ld(a.nx);
     const size_t planeE = (size_t)e_rows(a.nx) * ldE;
     dim3 gr(grid), bl(TL::NT);
-#define NSLX_HOP_CASE(OPV, MV)                                                                  \
-    if (op == OPV && mode == MV) {                                                              \
-        launch_timed(hop_kernel<TL, PR, OPV, MV>, gr, bl, 0, s, a, nrt, nct, swz, ldE, planeE);       \
-        return;                                                                                 \
+#define NSLX_HOP_CASE(OPV, MV)                                                                          \
+    if (op == OPV && mode == MV) {                                                                      \
+        if (a.disc == DISC_DIAG)                                                                        \
+            launch_timed(hop_kernel<TL, PR, OPV, MV, DISC_DIAG>, gr, bl, 0, s, a, nrt, nct, swz, ldE, planeE); \
+        else                                                                                            \
+            launch_timed(hop_kernel<TL, PR, OPV, MV, DISC_EXP>, gr, bl, 0, s, a, nrt, nct, swz, ldE, planeE);  \
+        return;                                                                                         \
     }
     NSLX_HOP_CASE(HOP_M, MODE_PLAIN)
     NSLX_HOP_CASE(HOP_MD, MODE_PLAIN)
@@ -1185,6 +1213,7 @@ void launch_hop(int prec, int op, int mode, const HopArgs& a, int tile, hipStrea
     NSLX_REQUIRE(a.nx > 0 && a.nt > 0 && a.nb > 0, "hop: empty shape");
     NSLX_REQUIRE(tile >= 0 && tile < SLICE_BASE + SLICE_N, "hop: bad tile id");
     ProfScope prof(op == HOP_M ? "hop_M" : "hop_Mdag", prec, mode, s);
+    NSLX_REQUIRE(a.disc == DISC_EXP || tile < SLICE_BASE, "hop: HubbardDiag runs on the classic tiles 0-3");
     if (tile >= SLICE_BASE) {
         const int id = tile - SLICE_BASE;
         if (prec == NSLX_F64) dispatch_slice<PF64, 1, 2, 3>(id, op, mode, a, s, SLICE_MENU64);

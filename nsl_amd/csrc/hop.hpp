@@ -48,6 +48,12 @@ struct CgDev {
     int* want = nullptr;
 };
 
+// Discretisation of the fermion matrix behind a hop launch.
+//   DISC_EXP  (HubbardExp):  (M psi)_t = psi_t - b_t E (P_{t-1} . psi_{t-1}),      E = exp(s delta K)
+//   DISC_DIAG (HubbardDiag): (M psi)_t = psi_t - E psi_t - b_t P_{t-1} . psi_{t-1}, E = s delta K
+// (b_0 = -1, else +1; M^dagger correspondingly with conj(P_t) and psi_{t+1}).
+enum HopDisc { DISC_EXP = 0, DISC_DIAG = 1 };
+
 struct HopArgs {
     int nx = 0, nt = 0, nb = 0, rpf = 1;   // rpf: right-hand sides per field
     const void* E = nullptr;   // E[y][x] in compute precision (BF16X3: hi plane then lo plane)
@@ -61,6 +67,7 @@ struct HopArgs {
     const void* Bv = nullptr;
     CgDev cg;
     int iter = 0;
+    int disc = 0;              // DISC_EXP or DISC_DIAG (classic tiles only)
 };
 
 // One problem of a grouped launch: a field block with its own lattice size,

@@ -14,6 +14,12 @@
 // diag(Pinv_{t-1}) E(-s delta), suffix products Q_t = Fk_t Q_{t+1},
 // G = (1 + Q_0)^{-1}, pi_{Nt-1} = i diag(Q_0 G), then G <- G Fk_t and
 // pi_t = i diag(Q_{t+1} G).
+//
+// HubbardDiag (hubbardDiag.tpp:166-190, 193-252) uses the same two
+// algorithms with D = 1 - s delta K in place of E: logDetM = i s sum(phi) +
+// log det(1 + fk_0 fk_1 ... fk_{Nt-1}), fk_t = D diag(Pinv_{t+1}) (the
+// product built left to right from t = 0, no mu term, as the reference);
+// gradLogDetM with Fk_t = diag(Pinv_{t-1}) D.
 #include <cmath>
 
 #include "engine.hpp"
@@ -45,21 +51,34 @@ void fm_logdet(FermionMatrix& fm, int64_t field, int nchunks, double* out2, hipS
     const int nx = fm.nx, nt = fm.nt;
     if (nchunks <= 0) nchunks = auto_chunks(fm, field, s);
     nchunks = std::max(1, std::min(nchunks, nt));
-    EEntry& e = fm.E(fm.sgn() * fm.delta(), s);
-    const double2* P = fm.P64.p + field * fm.len();
+    const bool diag = fm.disc == DISC_DIAG;
+    EEntry& e = diag ? fm.Elin(-fm.sgn() * fm.delta(), 1.0, s) : fm.E(fm.sgn() * fm.delta(), s);
+    const double2* P = (diag ? fm.Pinv64.p : fm.P64.p) + field * fm.len();
     const size_t m = (size_t)nx * nx;
     DevBuf<double2> C(m * nchunks), tmp(m), cur(m);
     for (int j = 0; j < nchunks; ++j) {
         const int t_lo = (int)std::llround((double)j * nt / nchunks);
         const int t_hi = (int)std::llround((double)(j + 1) * nt / nchunks);
         set_identity_c(nx, cur.p, s);
-        for (int t = t_hi - 1; t >= t_lo; --t) {
-            // cur = cur * E * diag(P_{t+1})   (sausage.mat_mul(prod[t]))
-            const double2* Pn = P + (size_t)((t + 1) % nt) * nx;
-            gemm_sq(nx, cur.p, 1, e.dense.p, 0, tmp.p, nullptr, nullptr, Pn, 1.0, 0.0, s);
-            std::swap(cur.p, tmp.p);
+        if (diag) {
+            // cur = cur * D * diag(Pinv_{t+1}) for t ascending (hubbardDiag.tpp:176-182)
+            for (int t = t_lo; t < t_hi; ++t) {
+                const double2* Pn = P + (size_t)((t + 1) % nt) * nx;
+                gemm_sq(nx, cur.p, 1, e.dense.p, 0, tmp.p, nullptr, nullptr, Pn, 1.0, 0.0, s);
+                std::swap(cur.p, tmp.p);
+            }
+        } else {
+            for (int t = t_hi - 1; t >= t_lo; --t) {
+                // cur = cur * E * diag(P_{t+1})   (sausage.mat_mul(prod[t]))
+                const double2* Pn = P + (size_t)((t + 1) % nt) * nx;
+                gemm_sq(nx, cur.p, 1, e.dense.p, 0, tmp.p, nullptr, nullptr, Pn, 1.0, 0.0, s);
+                std::swap(cur.p, tmp.p);
+            }
         }
-        NSLX_HIP(hipMemcpyAsync(C.p + (size_t)j * m, cur.p, m * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        // build_cyclic forms det(1 + C_{n-1} ... C_0): the ascending Diag
+        // product C_0 ... C_{n-1} is stored in reverse chunk order
+        const int slot = diag ? nchunks - 1 - j : j;
+        NSLX_HIP(hipMemcpyAsync(C.p + (size_t)slot * m, cur.p, m * sizeof(double2), hipMemcpyDeviceToDevice, s));
     }
     const int N = nx * nchunks;
     DevBuf<double2> R((size_t)N * N);
@@ -74,6 +93,17 @@ void fm_logdet(FermionMatrix& fm, int64_t field, int nchunks, double* out2, hipS
     double a = std::remainder(h[1], 2.0 * M_PI);   // (-pi, pi]
     if (a <= -M_PI) a += 2.0 * M_PI;
     out2[1] = a;
+    if (diag) {
+        // + i s sum(phi) (hubbardDiag.tpp:186), outside the principal branch
+        std::vector<double2> ph((size_t)fm.len());
+        NSLX_HIP(hipMemcpyAsync(ph.data(), fm.phi.p + field * fm.len(), ph.size() * sizeof(double2),
+                                hipMemcpyDeviceToHost, s));
+        NSLX_HIP(hipStreamSynchronize(s));
+        double sr = 0.0, si = 0.0;
+        for (const double2& z : ph) { sr += z.x; si += z.y; }
+        out2[0] -= fm.sgn() * si;
+        out2[1] += fm.sgn() * sr;
+    }
 }
 
 void fm_grad_logdet(FermionMatrix& fm, int64_t field, void* out, hipStream_t s) {
@@ -81,7 +111,7 @@ void fm_grad_logdet(FermionMatrix& fm, int64_t field, void* out, hipStream_t s) 
     NSLX_REQUIRE(field >= 0 && field < fm.nfield, "grad_logdet: field index out of range");
     NSLX_REQUIRE(out, "grad_logdet: null output");
     const int nx = fm.nx, nt = fm.nt;
-    EEntry& en = fm.E(-fm.sgn() * fm.delta(), s);
+    EEntry& en = fm.disc == DISC_DIAG ? fm.Elin(-fm.sgn() * fm.delta(), 1.0, s) : fm.E(-fm.sgn() * fm.delta(), s);
     const double2* Pi = fm.Pinv64.p + field * fm.len();
     auto Pprev = [&](int t) { return Pi + (size_t)((t - 1 + nt) % nt) * nx; };   // Pinv_{t-1}
     const size_t m = (size_t)nx * nx;

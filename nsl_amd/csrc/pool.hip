@@ -425,6 +425,7 @@ int batch_solve(Ctx& ctx, int64_t nitems, const nslx_batch_item* items, double t
     for (int64_t i = 0; i < nitems; ++i) {
         NSLX_REQUIRE(items[i].fm && items[i].b && items[i].x && items[i].nbatch >= 1, "batch_solve: bad item");
         NSLX_REQUIRE(&items[i].fm->m.ctx->tiles == &ctx.tiles, "batch_solve: every fm must belong to ctx");
+        NSLX_REQUIRE(items[i].fm->m.disc == DISC_EXP, "batch_solve: the grouped kernels take HubbardExp matrices");
         cap += items[i].nbatch;
     }
     const auto t0 = std::chrono::steady_clock::now();

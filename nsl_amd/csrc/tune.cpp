@@ -26,6 +26,7 @@ int fm_tune(FermionMatrix& fm, int prec, int64_t nb, int reps, int* best_tile, d
         const int n = (int)tile_menu(prec).size();
         for (int t = 0; t < n; ++t) {
             if (!tile_fits(prec, t, fm.nx)) continue;
+            if (fm.disc == DISC_DIAG && t >= 4) continue;   // HubbardDiag: classic tiles only
             launch_hop(prec, HOP_M, MODE_PLAIN, a, t, s);   // warm-up
             launch_hop(prec, HOP_MD, MODE_PLAIN, a, t, s);
             NSLX_HIP(hipEventRecord(e0, s));

@@ -1,4 +1,5 @@
-"""HubbardExp fermion matrix, op for op on libtorch CPU complex128 (oracle).
+"""HubbardExp and HubbardDiag fermion matrices, op for op on libtorch CPU
+complex128 (oracle).
 
 TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
 
@@ -17,7 +18,11 @@ Mdagger(M(.)), Hermiticity, the phi = 0 Matsubara spectrum; logDetM against
 LU of the dense matrix, the phi = 0 and uniform-slice closed forms, time-shift
 and 2 pi invariance; gradLogDetM against central finite differences of
 logDetM; logdet_stable against the same closed forms at large beta where the
-reference form overflows.
+reference form overflows.  HubbardDiag (hubbardDiag.tpp): M against the
+brute-force index formula, M^dagger = (M_dense)^H, the reference's
+term-by-term MMdagger / MdaggerM against M(Mdagger(.)) / Mdagger(M(.)),
+logDetM against the LU of the dense matrix at mu = 0 (mod 2 pi i),
+logdet_stable against logDetM, gradLogDetM against central differences.
 """
 
 import math
@@ -216,4 +221,114 @@ def index_formula_dense(lat, Nt, beta, phi, mu=0.0, species=PARTICLE):
         i = (t - 1) % Nt
         B = -1.0 if t == 0 else 1.0
         D[t, :, i, :] -= B * E * P[i].unsqueeze(0)
+    return D
+
+
+class HubbardDiag(HubbardExp):
+    """Diagonal discretisation (src/NSL/FermionMatrix/Impl/hubbardDiag.hpp,
+    hubbardDiag.tpp:10-252): M_{tx,iy} = d_ti d_xy - [s delta K]_xy d_ti
+    - B_t exp(i s phi_iy + s mu~) d_xy d_{t,i+1} (the formula of the comment
+    at hubbardDiag.tpp:13-18; the header's comment repeats the exponential
+    form).  Constructor and populate are HubbardExp's (hubbardDiag.hpp).
+    The reference rolls dimension 0 (Tensor::shift(n)); here the time axis
+    is dimension -2, the same for the (Nt, Nx) fields the reference takes."""
+
+    def _K(self, a):
+        return self.Lat.hopping_matrix(a)
+
+    def _Kmul(self, a, psi):
+        return torch.matmul(self._K(a), psi.transpose(-1, -2)).transpose(-1, -2)
+
+    def M(self, psi):
+        # hubbardDiag.tpp:10-36
+        psi = torch.as_tensor(psi, dtype=torch.complex128)
+        out = _shift(self.phiExp * psi, 1, -2, -1.0)
+        out = out + self._Kmul(self.sgn * self.delta, psi)
+        return psi - out
+
+    def Mdagger(self, psi):
+        # hubbardDiag.tpp:39-72: copy, slice 0 times -1, shift(-1), conj(P) *
+        psi = torch.as_tensor(psi, dtype=torch.complex128)
+        B = torch.conj(self.phiExp) * _shift(psi, -1, -2, -1.0)
+        return psi - self._Kmul(self.sgn * self.delta.conjugate(), psi) - B
+
+    def MMdagger(self, psi):
+        # hubbardDiag.tpp:75-125, term by term
+        psi = torch.as_tensor(psi, dtype=torch.complex128)
+        d, dc = self.sgn * self.delta, self.sgn * self.delta.conjugate()
+        out = self.M(psi) + self.Mdagger(psi) - psi + self._Kmul(d, self._Kmul(dc, psi))
+        out = out + self._Kmul(d, torch.conj(self.phiExp) * _shift(psi, -1, -2, -1.0))
+        B = _shift(self.phiExp * self._Kmul(dc, psi), 1, -2, -1.0)
+        return out + B + _shift(self.phiExp * torch.conj(self.phiExp), 1, -2) * psi
+
+    def MdaggerM(self, psi):
+        # hubbardDiag.tpp:128-164, term by term
+        psi = torch.as_tensor(psi, dtype=torch.complex128)
+        d, dc = self.sgn * self.delta, self.sgn * self.delta.conjugate()
+        out = self.M(psi) + self.Mdagger(psi) - psi + self._Kmul(d, self._Kmul(dc, psi))
+        out = out + torch.conj(self.phiExp) * self._Kmul(d, _shift(psi, -1, -2, -1.0))
+        B = _shift(self._Kmul(dc, self.phiExp * psi), 1, -2, -1.0)
+        return out + B + self.phiExp * torch.conj(self.phiExp) * psi
+
+    def _D(self):
+        Nx = self.phi.shape[-1]
+        return torch.eye(Nx, dtype=torch.complex128) - self._K(self.sgn * self.delta)
+
+    def logDetM(self):
+        # hubbardDiag.tpp:166-190: fk_t = D diag(Pinv_{t+1}), D = 1 - s delta K,
+        # sausage = fk_0 fk_1 ... fk_{Nt-1}; i s sum(phi) + log det(1 + sausage)
+        Nt, Nx = self.phi.shape
+        Pn = torch.roll(self.phiExpInv, -1, 0)          # LinAlg::shift(Pinv,-1)
+        fk = self._D().unsqueeze(0) * Pn.unsqueeze(1)     # [t][k][x] = D_kx Pinv_{t+1,x}
+        sausage = torch.eye(Nx, dtype=torch.complex128)
+        for t in range(Nt):
+            sausage = torch.matmul(sausage, fk[t])
+        logdet = complex(torch.log(torch.linalg.det(torch.eye(Nx, dtype=torch.complex128) + sausage)))
+        return 1j * self.sgn * complex(self.phi.sum()) + logdet
+
+    def slice_matrices(self):
+        """fk_t in product order for logdet_stable: det(1 + B_{Nt-1}..B_0)
+        with B_t = fk_{Nt-1-t} is det(1 + fk_0 ... fk_{Nt-1})."""
+        Pn = torch.roll(self.phiExpInv, -1, 0)
+        fk = self._D().unsqueeze(0) * Pn.unsqueeze(1)
+        return torch.flip(fk, dims=[0])
+
+    def logdet_stable(self, nchunks=None):
+        """logDetM through the block-cyclic LU (see HubbardExp.logdet_stable):
+        log det(1 + sausage) reduced to (-pi, pi], then + i s sum(phi)."""
+        ld = super().logdet_stable(nchunks)
+        return ld + 1j * self.sgn * complex(self.phi.sum())
+
+    def gradLogDetM(self):
+        # hubbardDiag.tpp:193-252: Fk_t = diag(Pinv_{t-1}) D, then as HubbardExp
+        Nt, Nx = self.phi.shape
+        Pi = torch.roll(self.phiExpInv, 1, 0)             # LinAlg::shift(Pinv,+1)
+        Fk = Pi.unsqueeze(2) * self._D().unsqueeze(0)
+        FkFkFk = torch.empty((Nt, Nx, Nx), dtype=torch.complex128)
+        FkFkFk[Nt - 1] = Fk[Nt - 1]
+        for t in range(Nt - 2, -1, -1):
+            FkFkFk[t] = torch.matmul(Fk[t], FkFkFk[t + 1])
+        invAp1F = torch.linalg.inv(torch.eye(Nx, dtype=torch.complex128) + FkFkFk[0])
+        pi_dot = torch.empty((Nt, Nx), dtype=torch.complex128)
+        pi_dot[Nt - 1] = 1j * torch.diagonal(torch.matmul(FkFkFk[0], invAp1F))
+        for t in range(Nt - 1):
+            invAp1F = torch.matmul(invAp1F, Fk[t])
+            pi_dot[t] = 1j * torch.diagonal(torch.matmul(FkFkFk[t + 1], invAp1F))
+        return pi_dot
+
+
+def diag_index_formula_dense(lat, Nt, beta, phi, mu=0.0, species=PARTICLE):
+    """Dense HubbardDiag M from the index formula (hubbardDiag.tpp:13-18),
+    brute force."""
+    Nx = lat.sites()
+    s = +1 if species == PARTICLE else -1
+    delta = complex(beta) / Nt
+    A = lat.hopping_matrix(s * delta)
+    P = torch.exp(1j * s * torch.as_tensor(phi, dtype=torch.complex128) + s * delta * mu)
+    D = torch.zeros((Nt, Nx, Nt, Nx), dtype=torch.complex128)
+    for t in range(Nt):
+        D[t, :, t, :] += torch.eye(Nx, dtype=torch.complex128) - A
+        i = (t - 1) % Nt
+        B = -1.0 if t == 0 else 1.0
+        D[t, :, i, :] -= B * torch.diag(P[i])
     return D

@@ -26,13 +26,14 @@ from . import rng
 
 
 class HubbardAction:
-    def __init__(self, lat, Nt, beta, U, mu=0.0, stable_logdet=False):
+    def __init__(self, lat, Nt, beta, U, mu=0.0, stable_logdet=False, fermion_matrix=None):
         self.lat = lat
         self.Nt = Nt
         self.beta = beta
         self.U = U
         self.Utilde = (beta / Nt) * U                      # hubbard.tpp:10-19
-        self.hfm = fm.HubbardExp(lat, Nt, beta, mu)
+        # HubbardFermionAction<..., FermionMatrixType> (hubbardFermiAction.tpp:23-60)
+        self.hfm = (fermion_matrix or fm.HubbardExp)(lat, Nt, beta, mu)
         self.stable = stable_logdet
 
     def _logdet(self):

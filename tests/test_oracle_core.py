@@ -203,6 +203,46 @@ def test_grad_logdet_relation():
             assert abs(g[t, x] - (1j - s * fd)) < 1e-7
 
 
+@pytest.mark.parametrize("species", [F.PARTICLE, F.HOLE])
+def test_hubbard_diag_pins(species):
+    """HubbardDiag (hubbardDiag.tpp): M against the brute-force index formula,
+    M^dagger = (M_dense)^H, the reference's term-by-term MMdagger/MdaggerM
+    against M(Mdagger .)/Mdagger(M .), logDetM against the LU of the dense
+    matrix (mu = 0), logdet_stable against logDetM, and gradLogDetM =
+    i - s d(logDetM)/d(phi) by central differences (the HubbardExp relation)."""
+    lat = lattice.ring(4)
+    Nt, beta, s = 6, 2.0, (1 if species == F.PARTICLE else -1)
+    phi = torch.as_tensor(rng.real_field(3, 0, (Nt, 4), 0.8))
+    h = F.HubbardDiag(lat, Nt, beta, species=species)
+    h.populate(phi)
+    Md = h.M_dense().reshape(Nt * 4, Nt * 4)
+    Mi = F.diag_index_formula_dense(lat, Nt, beta, phi, species=species).reshape(Nt * 4, Nt * 4)
+    assert (Md - Mi).abs().max() < 1e-14
+    psi = torch.as_tensor(rng.complex_normal(8, 0, (2, Nt, 4)))
+    ref = (Md.conj().T @ psi.reshape(2, -1).T).T.reshape(2, Nt, 4)
+    assert (h.Mdagger(psi) - ref).abs().max() < 1e-13
+    assert (h.MMdagger(psi) - h.M(h.Mdagger(psi))).abs().max() < 1e-13
+    assert (h.MdaggerM(psi) - h.Mdagger(h.M(psi))).abs().max() < 1e-13
+    sign, logabs = torch.linalg.slogdet(Md)
+    ld = h.logDetM()
+    assert abs(ld.real - float(logabs)) < 1e-12
+    dang = ld.imag - math.atan2(float(sign.imag), float(sign.real))
+    assert abs(dang - 2 * math.pi * round(dang / (2 * math.pi))) < 1e-12
+    st = h.logdet_stable(3)
+    assert abs(st.real - ld.real) < 1e-12
+    assert abs(math.remainder(st.imag - ld.imag, 2 * math.pi)) < 1e-12
+    g = h.gradLogDetM()
+    for (t, x) in ((0, 0), (2, 3), (5, 1)):
+        e = 1e-6
+        pp, pm = phi.clone(), phi.clone()
+        pp[t, x] += e
+        pm[t, x] -= e
+        hp = F.HubbardDiag(lat, Nt, beta, species=species); hp.populate(pp)
+        hm = F.HubbardDiag(lat, Nt, beta, species=species); hm.populate(pm)
+        fd = (hp.logDetM() - hm.logDetM()) / (2 * e)
+        assert abs(g[t, x] - (1j - s * fd)) < 1e-7
+
+
 def test_action_force_finite_differences():
     """Tests/Action/test_force.cpp: Nt=8, ring(2), beta=2, U=3, real phi."""
     lat = lattice.ring(2)
