@@ -261,6 +261,45 @@ int nslx_correlator_two_point(nslx_fm fm, int64_t n_tsrc, double eps, int64_t ma
                               int precision, int64_t max_rhs, void* corr, nslx_solve_report* report,
                               void* stream);
 
+/* TwoPointCorrelator::measureK(k, N) (twoPointCorrelationFunction.tpp:169-233)
+ * for the k-th set of wall sources: wall = wallSources[k], device c128
+ * (nw, nx).  For tsrc = 0, Nt/N, ... < Nt: S[s][tsrc][x] = wall[s][x],
+ * X = CG on MMdagger, G = Mdagger X shifted by -tsrc (antiperiodic),
+ * corrK += G; corrK /= N; corrKblock[t][s1][s2] = sum_x conj(wall[s1][x])
+ * corrK[s2][t][x].  corrK: device c128 (nw, nt, nx) and block: device c128
+ * (nt, nw, nw); either may be NULL.  tol_kind / precision as
+ * nslx_correlator_two_point (NSLX_TOL_REL_PER_RHS batches all time sources
+ * in one solve).  Synchronous. */
+int nslx_correlator_two_point_wall(nslx_fm fm, const void* wall, int64_t nw, int64_t n_tsrc, double eps,
+                                   int64_t maxit, int tol_kind, int precision, void* corrK, void* block,
+                                   nslx_solve_report* report, void* stream);
+
+/* Momentum quadruples of TwoBodyCorrelator::measure(N)
+ * (twoBodyCorrelationFunction.tpp:139-151, eq_modBZ_ :121-135): every
+ * (w, x, z, y) in [0, kdim)^4, looped w, x, y, z, with k_w + k_x = k_z + k_y
+ * modulo the honeycomb reciprocal vectors (4 pi/3, 0), (-2 pi/3, 2 pi/sqrt 3)
+ * (to 2e-12).  momenta: host (kdim, 2).  *nquad = count; quads (host int,
+ * 4 per quadruple, in key order w, x, z, y) is written when not NULL and
+ * *nquad on entry is at least the count. */
+int nslx_two_body_quads(const double* momenta, int kdim, int64_t* nquad, int* quads);
+
+/* TwoBodyCorrelator::measure(N) (twoBodyCorrelationFunction.tpp:244-300,
+ * channels :481-1000) on fm's populated field: wall = wallSources, device c128
+ * (kdim, bdim, nx).  For tsrc = 0, Nt/N, ... < Nt and species particle, hole:
+ * CG on MMdagger of the wall sources and on MdaggerM of their conjugates,
+ * back-multiplied by Mdagger / M, re-centred on tsrc, projected on the walls
+ * (corrPool_, corrPoolDag_), then the twelve channels I1S1Iz1Sz1,
+ * I1S1Iz1Sz0, I1S1Iz1Szn1, I1S1Iz0Sz1, I1S1Iz0Szn1, I1S1Izn1Sz1,
+ * I1S1Izn1Sz0, I1S1Izn1Szn1, I0S1Iz0Sz1, I0S1Iz0Szn1, I1S0Iz1Sz0,
+ * I1S0Izn1Sz0 accumulate into out: device c128 (12, nquad, nt, bdim^2,
+ * bdim^2), element [ch][q][t][i*bdim+j][l*bdim+k] for quadruple q =
+ * quads[4q..4q+3] = (w, x, z, y) (the key of the reference's maps).  The
+ * field's species is restored on return.  tol_kind / precision as
+ * nslx_correlator_two_point.  Synchronous. */
+int nslx_correlator_two_body(nslx_fm fm, const void* wall, int kdim, int bdim, int64_t nquad, const int* quads,
+                             int64_t n_tsrc, double eps, int64_t maxit, int tol_kind, int precision, void* out,
+                             nslx_solve_report* report, void* stream);
+
 /* ---- correlator server (continuous batching) ----------------------------------
  * No counterpart in the reference (its measurements run one configuration
  * at a time, twoPointCorrelationFunction.tpp:116-162).  A server owns a

@@ -464,6 +464,86 @@ class TwoPointCorrelator:
             log.error("TwoPointCorrelator: CG did not converge within %d iterations", self.maxIter)
         return corr
 
+    def measureK(self, wall, NumberTimeSources=None):
+        """measureK(k, N) (twoPointCorrelationFunction.tpp:169-233) for wall =
+        wallSources[k], a complex (uDim, Nx) tensor: returns (corrK, corrKblock)
+        with shapes (uDim, Nt, Nx) and (Nt, uDim, uDim)."""
+        torch = _torch()
+        wall = torch.as_tensor(wall).to(device="cuda", dtype=torch.complex128).contiguous()
+        nw = wall.shape[0]
+        n = int(NumberTimeSources or self.fm.Nt)
+        corrK = torch.empty((nw, self.fm.Nt, self.fm.Nx), dtype=torch.complex128, device="cuda")
+        block = torch.empty((self.fm.Nt, nw, nw), dtype=torch.complex128, device="cuda")
+        rep = SolveReport()
+        st = _lib.lib().nslx_correlator_two_point_wall(self.fm._h, _ptr(wall), nw, n, self.eps, self.maxIter,
+                                                       self.tol_kind, self.precision, _ptr(corrK), _ptr(block),
+                                                       ctypes.byref(rep), _stream())
+        _lib.check(st, allow=(_lib.OK, _lib.NOT_CONVERGED))
+        self.report = rep.as_dict()
+        self.converged = st == _lib.OK
+        return corrK, block
+
+
+TWO_BODY_CHANNELS = ("I1S1Iz1Sz1", "I1S1Iz1Sz0", "I1S1Iz1Szn1", "I1S1Iz0Sz1", "I1S1Iz0Szn1", "I1S1Izn1Sz1",
+                     "I1S1Izn1Sz0", "I1S1Izn1Szn1", "I0S1Iz0Sz1", "I0S1Iz0Szn1", "I1S0Iz1Sz0", "I1S0Izn1Sz0")
+
+
+def two_body_quads(momenta):
+    """Momentum quadruples (w, x, z, y) with k_w + k_x = k_z + k_y modulo the
+    honeycomb reciprocal lattice, in the reference's loop order
+    (twoBodyCorrelationFunction.tpp:139-151)."""
+    m = np.ascontiguousarray(np.asarray(momenta, dtype=np.float64))
+    kdim = m.shape[0]
+    n = ctypes.c_int64(0)
+    L = _lib.lib()
+    _lib.check(L.nslx_two_body_quads(m.ctypes.data_as(_lib.P_double), kdim, ctypes.byref(n), None))
+    q = np.zeros((max(n.value, 1), 4), dtype=np.int32)
+    _lib.check(L.nslx_two_body_quads(m.ctypes.data_as(_lib.P_double), kdim, ctypes.byref(n),
+                                     q.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+    return q[:n.value]
+
+
+class TwoBodyCorrelator:
+    """NSL::Measure::Hubbard::TwoBodyCorrelator::measure(N)
+    (twoBodyCorrelationFunction.tpp:244-300, channels :481-1000) on a
+    populated fermion matrix: wallSources (kDim, bDim, Nx) and momenta
+    (kDim, 2).  measure(N) returns {channel name: {(w, x, z, y): tensor
+    (Nt, bDim^2, bDim^2)}} like the reference's maps, from one device array
+    (12, nquad, Nt, bDim^2, bDim^2) kept as `.out`.  CG to eps with the
+    reference's batch-global criterion by default (cg_, cgDag_ built with
+    eps = 1e-12)."""
+
+    def __init__(self, fm, wallSources, momenta, eps=1e-12, maxIter=10000, tol_kind=TOL_ABS_BATCH, precision=F64):
+        torch = _torch()
+        self.fm = fm
+        self.wall = torch.as_tensor(wallSources).to(device="cuda", dtype=torch.complex128).contiguous()
+        self.kdim, self.bdim = int(self.wall.shape[0]), int(self.wall.shape[1])
+        self.quads = two_body_quads(momenta)
+        self.eps, self.maxIter = float(eps), int(maxIter)
+        self.tol_kind, self.precision = int(tol_kind), int(precision)
+        self.report = None
+        self.converged = None
+        self.out = None
+
+    def measure(self, NumberTimeSources=None):
+        torch = _torch()
+        n = int(NumberTimeSources or self.fm.Nt)
+        nq = len(self.quads)
+        b2 = self.bdim * self.bdim
+        out = torch.empty((12, nq, self.fm.Nt, b2, b2), dtype=torch.complex128, device="cuda")
+        q = np.ascontiguousarray(self.quads, dtype=np.int32)
+        rep = SolveReport()
+        st = _lib.lib().nslx_correlator_two_body(self.fm._h, _ptr(self.wall), self.kdim, self.bdim, nq,
+                                                 q.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n, self.eps,
+                                                 self.maxIter, self.tol_kind, self.precision, _ptr(out),
+                                                 ctypes.byref(rep), _stream())
+        _lib.check(st, allow=(_lib.OK, _lib.NOT_CONVERGED))
+        self.report = rep.as_dict()
+        self.converged = st == _lib.OK
+        self.out = out
+        return {name: {tuple(int(v) for v in self.quads[k]): out[c, k] for k in range(nq)}
+                for c, name in enumerate(TWO_BODY_CHANNELS)}
+
 
 # ------------------------------------------------------------------ HMC
 class HubbardAction:

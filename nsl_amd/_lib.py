@@ -115,6 +115,12 @@ PROTOS = {
     "nslx_server_stats_get": (c_int, [c_void_p, ctypes.POINTER(ServerStats)]),
     "nslx_correlator_two_point": (c_int, [c_void_p, c_int64, c_double, c_int64, c_int, c_int, c_int64, c_void_p,
                                           ctypes.POINTER(SolveReport), c_void_p]),
+    "nslx_correlator_two_point_wall": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_double, c_int64, c_int, c_int,
+                                               c_void_p, c_void_p, ctypes.POINTER(SolveReport), c_void_p]),
+    "nslx_two_body_quads": (c_int, [P_double, c_int, ctypes.POINTER(c_int64), ctypes.POINTER(ctypes.c_int)]),
+    "nslx_correlator_two_body": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int64, ctypes.POINTER(ctypes.c_int),
+                                         c_int64, c_double, c_int64, c_int, c_int, c_void_p,
+                                         ctypes.POINTER(SolveReport), c_void_p]),
     "nslx_hmc_action": (c_int, [c_void_p, c_void_p, c_double, c_int, P_double, c_void_p]),
     "nslx_hmc_force": (c_int, [c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
     "nslx_hmc_trajectory": (c_int, [c_void_p, c_void_p, c_void_p, P_double, c_double, c_double, c_int,

@@ -283,6 +283,47 @@ int nslx_correlator_two_point(nslx_fm fm, int64_t n_tsrc, double eps, int64_t ma
     });
 }
 
+int nslx_correlator_two_point_wall(nslx_fm fm, const void* wall, int64_t nw, int64_t n_tsrc, double eps,
+                                   int64_t maxit, int tol_kind, int precision, void* corrK, void* block,
+                                   nslx_solve_report* report, void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        DeviceGuard dg(m.ctx->device);
+        return nslx::fm_two_point_wall(m, wall, nw, n_tsrc, eps, maxit, tol_kind, precision, corrK, block, report,
+                                       nslx::as_stream(stream));
+    });
+}
+
+int nslx_two_body_quads(const double* momenta, int kdim, int64_t* nquad, int* quads) {
+    return guard([&] {
+        NSLX_REQUIRE(momenta && nquad && kdim >= 1, "two_body_quads: bad argument");
+        std::vector<int4> q;
+        nslx::two_body_quads(momenta, kdim, q);
+        if (quads) {
+            NSLX_REQUIRE(*nquad >= (int64_t)q.size(), "two_body_quads: quads holds fewer than the count");
+            for (size_t i = 0; i < q.size(); ++i) {
+                quads[4 * i] = q[i].x;
+                quads[4 * i + 1] = q[i].y;
+                quads[4 * i + 2] = q[i].z;
+                quads[4 * i + 3] = q[i].w;
+            }
+        }
+        *nquad = (int64_t)q.size();
+        return NSLX_OK;
+    });
+}
+
+int nslx_correlator_two_body(nslx_fm fm, const void* wall, int kdim, int bdim, int64_t nquad, const int* quads,
+                             int64_t n_tsrc, double eps, int64_t maxit, int tol_kind, int precision, void* out,
+                             nslx_solve_report* report, void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        DeviceGuard dg(m.ctx->device);
+        return nslx::fm_two_body(m, wall, kdim, bdim, nquad, quads, n_tsrc, eps, maxit, tol_kind, precision, out,
+                                 report, nslx::as_stream(stream));
+    });
+}
+
 int nslx_hmc_action(nslx_fm fm, const void* phi, double U, int nchunks, double* out_host2, void* stream) {
     return guard([&] {
         auto& m = FM(fm);

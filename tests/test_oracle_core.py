@@ -333,6 +333,64 @@ def test_cg_preconditioned_pins():
     assert ((h.MMdagger(x) - bf).norm() / bf.norm()) < 1e-9
 
 
+def honeycomb_walls(L, ks):
+    """Band-projected plane waves w[k, s, x] = exp(i k.r_x) / L on sublattice
+    s, for momenta ks = (m C + n D) / L of the L x L honeycomb."""
+    lat = lattice.honeycomb(L, L)
+    pos = torch.as_tensor(lat.sites_xyz[:, :2], dtype=torch.float64)
+    C = torch.tensor([4 * math.pi / 3, 0.0], dtype=torch.float64)
+    D = torch.tensor([-2 * math.pi / 3, 2 * math.pi / math.sqrt(3.0)], dtype=torch.float64)
+    mom = torch.stack([(m * C + n * D) / L for (m, n) in ks])
+    Nx = lat.sites()
+    w = torch.zeros((len(ks), 2, Nx), dtype=torch.complex128)
+    for s in range(2):
+        sel = torch.arange(Nx) % 2 == s
+        ph = torch.exp(1j * (mom @ pos[sel].T))
+        w[:, s, sel] = ph / L
+    return lat, w, mom
+
+
+def test_wall_correlators_linearity():
+    """measureK against the point-source correlator contracted with the
+    walls; the two-body pools against a dense inverse of M (both species,
+    both the MM^dagger and the M^dagger M pools); the quadruple count of a
+    full 3 x 3 momentum grid (momentum addition is a group: 9^3 of 9^4)."""
+    L, Nt, beta = 3, 8, 2.0
+    lat, w, mom = honeycomb_walls(L, [(0, 0), (1, 0), (2, 0)])
+    Nx = lat.sites()
+    phi = torch.as_tensor(rng.real_field(41, 0, (Nt, Nx), 0.5))
+    h = F.HubbardExp(lat, Nt, beta)
+    h.populate(phi)
+    wall = w[1]                                              # two walls (bands) of momentum 1
+    corrK, block = correlator.two_point_wall(h, Nt, 2, wall)
+    corr, _ = correlator.two_point(h, Nt, Nx, 2)
+    lin = torch.einsum("sx,tyx->sty", wall, corr)
+    assert (corrK - lin).abs().max() < 1e-9 * lin.abs().max()
+    assert (block - torch.einsum("ax,btx->tab", torch.conj(wall), lin)).abs().max() < 1e-9 * block.abs().max()
+    # pools of tsrc = 3 against inv(M_dense)
+    tsrc = 3
+
+    def make(species):
+        o = F.HubbardExp(lat, Nt, beta, species=species)
+        o.populate(phi)
+        return o
+
+    pools = correlator.two_body_pools(make, phi, w, tsrc)
+    for name, species in (("P", F.PARTICLE), ("H", F.HOLE)):
+        Mi = torch.linalg.inv(make(species).M_dense().reshape(Nt * Nx, Nt * Nx)).reshape(Nt, Nx, Nt, Nx)
+        Mdi = torch.linalg.inv(make(species).M_dense().reshape(Nt * Nx, Nt * Nx).conj().T).reshape(Nt, Nx, Nt, Nx)
+        ts = (torch.arange(Nt) + tsrc) % Nt
+        sg = torch.where(torch.arange(Nt) + tsrc >= Nt, -1.0, 1.0).to(torch.complex128)
+        G = Mi[ts][:, :, tsrc, :] * sg[:, None, None]           # [t, x, y] = sign M^-1[(t+tsrc, x), (tsrc, y)]
+        Gd = Mdi[ts][:, :, tsrc, :] * sg[:, None, None]
+        C = torch.einsum("aix,txy,bjy->abtij", torch.conj(w), G, w)
+        Dg = torch.conj(torch.einsum("aix,txy,bjy->abtij", w, Gd, torch.conj(w)))
+        assert (pools["C" + name] - C).abs().max() < 1e-9 * C.abs().max()
+        assert (pools["D" + name] - Dg).abs().max() < 1e-9 * Dg.abs().max()
+    _, _, mom9 = honeycomb_walls(3, [(m, n) for m in range(3) for n in range(3)])
+    assert len(correlator.two_body_quads(mom9)) == 9 ** 3
+
+
 def test_correlator_free_propagator():
     lat = lattice.ring(4)
     Nt, beta = 8, 2.0
