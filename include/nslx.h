@@ -395,6 +395,33 @@ int nslx_hmc_chain(nslx_fm fm, void* phi, double U, double traj_length, int nmd,
                    int64_t ntraj, uint64_t seed, int nchunks, nslx_hmc_report* reports,
                    void* stream);
 
+/* HMC::generate<Chain>(state, Nconf, saveFrequency) (src/NSL/MarkovChain/HMC.tpp:57-155)
+ * without its HDF5 read/write, with the MarkovState record of
+ * src/NSL/MarkovChain/markovState.tpp:13-96.  The start state is phi (device
+ * c128 (nt, nx), in; on return the last state's configuration) at markov time
+ * markov_time0, with its action computed here.  NSLX_CHAIN_ALL: states[0] =
+ * the start state (accepted = 1, acc_prob = 1) and states[n], n < nconf, the
+ * state after saveFrequency more trajectories (HMC.tpp:98-137);
+ * NSLX_CHAIN_LAST: Nconf * saveFrequency - 1 trajectories and states[0] =
+ * the last state (:144-152).  save_frequency <= 0 counts as 1 (:61-63).
+ * The trajectory that makes markov time m + 1 takes its momenta from
+ * nslx_rng_normal(seed, 2m, nt*nx) and its uniform from
+ * nslx_rng_uniform(seed, 2m+1, 1), so a chain resumed from any saved state
+ * (its configuration and markov time) continues bit for bit.  states: host
+ * array (nconf for ALL, 1 for LAST) or NULL; configs: device c128 (nconf, nt,
+ * nx) with every saved configuration, or NULL. */
+typedef enum { NSLX_CHAIN_ALL = 0, NSLX_CHAIN_LAST = 1 } nslx_chain;
+typedef struct {
+    double  S_re, S_im;        /* actionValue                                 */
+    double  acc_prob;          /* acceptanceProbability                       */
+    int64_t markov_time;       /* markovTime                                  */
+    int     accepted;
+    double  dH_re, dH_im;      /* H1 - H0 of the trajectory that made the state */
+} nslx_markov_state;
+int nslx_hmc_generate(nslx_fm fm, void* phi, int64_t markov_time0, double U, double traj_length,
+                      int nmd, int64_t nconf, int64_t save_frequency, int chain, uint64_t seed,
+                      int nchunks, void* configs, nslx_markov_state* states, void* stream);
+
 /* ---- hop tiles and autotuning ------------------------------------------------
  * No counterpart in the reference (its GEMMs are whatever torch::matmul
  * dispatches, Tensor/Impl/matmul.tpp:15-17).  The hop kernels come in tile

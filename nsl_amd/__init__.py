@@ -589,6 +589,27 @@ class HubbardAction:
                                              int(ntraj), int(seed), self.nchunks, reps, _stream()))
         return [r.as_dict() for r in reps]
 
+    def generate(self, phi, nconf, traj_length, nmd, seed, save_frequency=1, markov_time=0, all_states=True,
+                 keep_configs=True):
+        """HMC::generate<AllStates | LastState>(state, Nconf, saveFrequency)
+        (HMC.tpp:57-155, MarkovState markovState.tpp) from the state (phi,
+        markov_time), without the file I/O.  phi (device, in/out) ends as the
+        last state's configuration.  Returns (states, configs): a list of
+        MarkovState dicts and, if keep_configs, a device tensor (n, Nt, Nx)
+        of the saved configurations.  Resuming from any saved state (its
+        configuration and markov_time) continues the chain exactly."""
+        torch = _torch()
+        nstates = int(nconf) if all_states else 1
+        reps = (_lib.MarkovState * nstates)()
+        cfg = (torch.empty((nstates, self.fm.Nt, self.fm.Nx), dtype=torch.complex128, device="cuda")
+               if keep_configs else None)
+        _lib.check(_lib.lib().nslx_hmc_generate(self.fm._h, _ptr(phi), int(markov_time), self.U, float(traj_length),
+                                                int(nmd), int(nconf), int(save_frequency),
+                                                _lib.CHAIN_ALL if all_states else _lib.CHAIN_LAST, int(seed),
+                                                self.nchunks, _ptr(cfg) if cfg is not None else None, reps,
+                                                _stream()))
+        return [r.as_dict() for r in reps], cfg
+
 
 # ------------------------------------------------------------------ RNG
 def rng_normal(seed, offset, n):

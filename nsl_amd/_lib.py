@@ -80,6 +80,16 @@ class HmcReport(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class MarkovState(ctypes.Structure):
+    _fields_ = [("S_re", c_double), ("S_im", c_double), ("acc_prob", c_double), ("markov_time", c_int64),
+                ("accepted", c_int), ("dH_re", c_double), ("dH_im", c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+CHAIN_ALL, CHAIN_LAST = 0, 1
+
 # name -> (restype, argtypes)
 PROTOS = {
     "nslx_version": (c_int, []),
@@ -127,6 +137,8 @@ PROTOS = {
                                     c_double, c_int, ctypes.POINTER(HmcReport), c_void_p]),
     "nslx_hmc_chain": (c_int, [c_void_p, c_void_p, c_double, c_double, c_int, c_int64, c_uint64, c_int,
                                ctypes.POINTER(HmcReport), c_void_p]),
+    "nslx_hmc_generate": (c_int, [c_void_p, c_void_p, c_int64, c_double, c_double, c_int, c_int64, c_int64, c_int,
+                                  c_uint64, c_int, c_void_p, ctypes.POINTER(MarkovState), c_void_p]),
     "nslx_tile_count": (c_int, [c_int, ctypes.POINTER(c_int)]),
     "nslx_ctx_set_tile": (c_int, [c_void_p, c_int, c_int, c_int, c_int64, c_int]),
     "nslx_ctx_get_tile": (c_int, [c_void_p, c_int, c_int, c_int, c_int64, ctypes.POINTER(c_int)]),

@@ -367,6 +367,18 @@ int nslx_hmc_chain(nslx_fm fm, void* phi, double U, double traj_length, int nmd,
     });
 }
 
+int nslx_hmc_generate(nslx_fm fm, void* phi, int64_t markov_time0, double U, double traj_length, int nmd,
+                      int64_t nconf, int64_t save_frequency, int chain, uint64_t seed, int nchunks, void* configs,
+                      nslx_markov_state* states, void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        DeviceGuard dg(m.ctx->device);
+        nslx::hmc_generate(m, phi, markov_time0, U, traj_length, nmd, nconf, save_frequency, chain, seed, nchunks,
+                           configs, states, nslx::as_stream(stream));
+        return NSLX_OK;
+    });
+}
+
 int nslx_tile_count(int precision, int* count) {
     return guard([&] {
         NSLX_REQUIRE(count, "tile_count: null output");

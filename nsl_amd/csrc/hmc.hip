@@ -182,3 +182,56 @@ void hmc_chain(FermionMatrix& fm, void* phi, double U, double T, int nmd, int64_
 }
 
 }  // namespace nslx
+
+namespace nslx {
+
+void hmc_generate(FermionMatrix& fm, void* phi, int64_t mt0, double U, double T, int nmd, int64_t nconf,
+                  int64_t save, int chain, uint64_t seed, int nchunks, void* configs, nslx_markov_state* states,
+                  hipStream_t s) {
+    NSLX_REQUIRE(phi && nconf >= 1 && mt0 >= 0, "hmc_generate: bad argument");
+    NSLX_REQUIRE(chain == NSLX_CHAIN_ALL || chain == NSLX_CHAIN_LAST, "hmc_generate: bad chain kind");
+    if (save <= 0) save = 1;   // HMC.tpp:61-63
+    const int64_t n = fm.len();
+    double S[2];
+    hmc_action(fm, phi, U, nchunks, S, s);
+    std::vector<double> z(n);
+    DevBuf<double> zd(n);
+    DevBuf<double2> mom(n);
+    int64_t mt = mt0;
+    nslx_markov_state cur{S[0], S[1], 1.0, mt, 1, 0.0, 0.0};   // the start state
+    // the trajectory that makes markov time m + 1 draws its momenta from
+    // Philox counter 2m and its Metropolis uniform from 2m + 1
+    auto step = [&]() {
+        host_normal(seed, 2 * (uint64_t)mt, n, z.data());
+        double u;
+        host_uniform(seed, 2 * (uint64_t)mt + 1, 1, &u);
+        NSLX_HIP(hipMemcpyAsync(zd.p, z.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_real_to_c, dim3(g1(n)), dim3(256), 0, s, (const double*)zd.p, mom.p, n);
+        NSLX_LAUNCH_CHECK();
+        nslx_hmc_report r{};
+        hmc_trajectory(fm, phi, mom.p, S, U, T, nmd, u, nchunks, &r, s);
+        ++mt;
+        cur = nslx_markov_state{S[0], S[1], r.acc_prob, mt, r.accepted, r.dH_re, r.dH_im};
+    };
+    auto record = [&](int64_t i) {
+        if (states) states[i] = cur;
+        if (configs)
+            NSLX_HIP(hipMemcpyAsync(static_cast<double2*>(configs) + i * n, phi, n * sizeof(double2),
+                                    hipMemcpyDeviceToDevice, s));
+    };
+    if (chain == NSLX_CHAIN_ALL) {
+        // MC[0] = state; MC[n] = generate_ applied save times to MC[n-1] (HMC.tpp:98-137)
+        record(0);
+        for (int64_t i = 1; i < nconf; ++i) {
+            for (int64_t m = 0; m < save; ++m) step();
+            record(i);
+        }
+    } else {
+        // LastState: Nconf * saveFrequency - 1 steps from state (HMC.tpp:144-152)
+        for (int64_t k = 1; k < nconf * save; ++k) step();
+        record(0);
+    }
+    NSLX_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace nslx

@@ -12,6 +12,12 @@ void hmc_trajectory(FermionMatrix& fm, void* phi, const void* mom, double* S0, d
                     int nchunks, nslx_hmc_report* rep, hipStream_t s);
 void hmc_chain(FermionMatrix& fm, void* phi, double U, double T, int nmd, int64_t ntraj, uint64_t seed, int nchunks,
                nslx_hmc_report* reps, hipStream_t s);
+// HMC::generate<AllStates | LastState>(state, Nconf, saveFrequency) without
+// the file I/O: states (host, nconf or 1) and configs (device, nconf fields)
+// may be null.
+void hmc_generate(FermionMatrix& fm, void* phi, int64_t mt0, double U, double T, int nmd, int64_t nconf,
+                  int64_t save, int chain, uint64_t seed, int nchunks, void* configs, nslx_markov_state* states,
+                  hipStream_t s);
 
 // Philox-4x32-10 (Salmon et al. SC'11); stream layout documented in nslx.h.
 void host_philox(uint64_t seed, uint64_t offset, int64_t nblocks, uint32_t* out4);

@@ -125,3 +125,37 @@ def hmc_chain(action, phi0, ntraj, traj_length, nmd, seed):
         phi, S = res["phi"], res["S"]
         out.append(res)
     return out
+
+
+def hmc_generate(action, phi0, nconf, traj_length, nmd, seed, save_frequency=1, markov_time=0, all_states=True):
+    """HMC::generate<AllStates | LastState>(state, Nconf, saveFrequency)
+    (src/NSL/MarkovChain/HMC.tpp:57-155) without the HDF5 part, with
+    MarkovState records (markovState.tpp:13-96) as dicts.  The trajectory
+    that makes markov time m + 1 draws momentum_draw(seed, m) and
+    metropolis_uniform(seed, m).  AllStates: states[0] = the start state,
+    states[n] = saveFrequency trajectories after states[n-1] (:98-137);
+    LastState: Nconf * saveFrequency - 1 trajectories (:144-152)."""
+    save = save_frequency if save_frequency > 0 else 1          # :61-63
+    phi = torch.as_tensor(phi0, dtype=torch.complex128)
+    state = {"phi": phi, "S": action.eval(phi), "acc_prob": 1.0, "markov_time": markov_time,
+             "accepted": True, "dH": 0j}
+
+    def step(st):
+        m = st["markov_time"]
+        p = momentum_draw(seed, m, tuple(st["phi"].shape))
+        u = metropolis_uniform(seed, m)
+        res = hmc_trajectory(action, st["phi"], st["S"], p, u, traj_length, nmd)
+        return {"phi": res["phi"], "S": res["S"], "acc_prob": res["acc_prob"], "markov_time": m + 1,
+                "accepted": res["accepted"], "dH": res["dH"]}
+
+    if all_states:
+        MC = [state]
+        for _ in range(1, nconf):
+            tmp = MC[-1]
+            for _ in range(save):
+                tmp = step(tmp)
+            MC.append(tmp)
+        return MC
+    for _ in range(1, nconf * save):
+        state = step(state)
+    return [state]

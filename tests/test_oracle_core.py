@@ -259,6 +259,30 @@ def test_action_force_finite_differences():
             assert abs(fd - g[t, x]) < 1e-6
 
 
+def test_hmc_generate_states():
+    """HMC::generate restated: with saveFrequency 1 the saved states are the
+    trajectory-by-trajectory chain (pinned against Delta H elsewhere); with
+    saveFrequency 2 every other one; LastState after Nconf * save - 1
+    trajectories is that chain's state; a chain resumed from a saved state
+    and its markov time continues it exactly."""
+    lat = lattice.ring(4)
+    A = hmc.HubbardAction(lat, 6, 2.0, 2.0)
+    phi0 = torch.as_tensor(rng.real_field(9, 0, (6, 4), 0.8))
+    chain = hmc.hmc_chain(A, phi0, 6, 1.0, 4, 77)
+    MC = hmc.hmc_generate(A, phi0, 7, 1.0, 4, 77)
+    assert [m["markov_time"] for m in MC] == list(range(7))
+    for k in range(6):
+        assert torch.equal(MC[k + 1]["phi"], chain[k]["phi"]) and MC[k + 1]["accepted"] == chain[k]["accepted"]
+    MC2 = hmc.hmc_generate(A, phi0, 4, 1.0, 4, 77, save_frequency=2)
+    for n in range(4):
+        assert torch.equal(MC2[n]["phi"], MC[2 * n]["phi"]) and MC2[n]["markov_time"] == 2 * n
+    last = hmc.hmc_generate(A, phi0, 3, 1.0, 4, 77, save_frequency=2, all_states=False)[0]
+    assert torch.equal(last["phi"], MC[5]["phi"]) and last["markov_time"] == 5
+    res = hmc.hmc_generate(A, MC[3]["phi"], 4, 1.0, 4, 77, markov_time=3)
+    for n in range(4):
+        assert torch.equal(res[n]["phi"], MC[3 + n]["phi"])
+
+
 def test_leapfrog_reversible_and_energy():
     lat = lattice.ring(3)
     A = hmc.HubbardAction(lat, 6, 2.0, 2.0)
