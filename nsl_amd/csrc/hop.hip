@@ -40,6 +40,7 @@ struct PF64 {
     using Acc = d4;
     static constexpr int KSTEP = 4, PAD = 2, NPA = 2, NPB = 1, VEC = 2, NBUF = 2;
     static constexpr bool ROW4V = true;
+    static constexpr bool DLDS = true;    // E chunk loaded straight into LDS
 };
 struct PF32 {
     using S = float;
@@ -47,6 +48,7 @@ struct PF32 {
     using Acc = f4;
     static constexpr int KSTEP = 4, PAD = 4, NPA = 2, NPB = 1, VEC = 4, NBUF = 2;
     static constexpr bool ROW4V = false;
+    static constexpr bool DLDS = true;
 };
 struct PB3 {
     using S = float;
@@ -54,6 +56,8 @@ struct PB3 {
     using Acc = f4;
     static constexpr int KSTEP = 32, PAD = 8, NPA = 4, NPB = 2, VEC = 4, NBUF = 1;
     static constexpr bool ROW4V = false;
+    // one LDS buffer: direct E loads could not overlap the MFMAs
+    static constexpr bool DLDS = false;
 };
 
 __device__ inline uint16_t f2bf(float x) {   // round to nearest even
@@ -144,12 +148,25 @@ __device__ inline CS<S> ldc_or0(const void* base, size_t idx, bool ok) {
 
 // LDS of a classic tile: NBUF copies of the A (field, split planes) and B (E)
 // chunks; with two the next chunk is stored while the current one feeds the
-// MFMAs (one barrier per chunk).
+// MFMAs (one barrier per chunk).  With DLDS the B rows (BK * sizeof(L) =
+// 128 bytes) are unpadded and their 16-byte groups XOR-swizzled by row & 7
+// (b_col below), the layout global_load_lds writes 8 rows at a time.
 template <class TL, class PR>
 struct Smem {
     typename PR::L A[PR::NBUF][PR::NPA][TL::BM][TL::BK + PR::PAD];
-    typename PR::L B[PR::NBUF][PR::NPB][TL::BN][TL::BK + PR::PAD];
+    typename PR::L B[PR::NBUF][PR::NPB][TL::BN][TL::BK + (PR::DLDS ? 0 : PR::PAD)];
 };
+
+// Column of element k of B row `row` in the LDS layout.
+template <class PR>
+__device__ inline int b_col(int row, int k) {
+    if constexpr (PR::DLDS) {
+        constexpr int VB = 16 / sizeof(typename PR::L);
+        return (((k / VB) ^ (row & 7)) * VB) + (k % VB);
+    } else {
+        return k;
+    }
+}
 
 // One BM x BN output tile of rhs b (rows t0 = rt*BM.., sites y0 = ct*BN..).
 // Called by the whole workgroup; returns this thread's share of the tile's
@@ -195,6 +212,28 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
         constexpr int BPASS = (BN + RPPB - 1) / RPPB;
 
         static_assert(APASS * RPP == BM && BPASS * RPPB == BN, "loader passes must tile BM and BN exactly");
+        // direct B loads: one wave instruction fills 8 rows of 128 bytes
+        constexpr int NWAVE = NT / 64;
+        constexpr int BSEG = BN / 8;
+        constexpr int BPW = BSEG / NWAVE;
+        static_assert(!PR::DLDS || (BK * sizeof(L) == 128 && BSEG % NWAVE == 0), "direct B load geometry");
+        // lane's 16-byte group of E for B row seg*8 + lane/8 (swizzle of b_col)
+        auto load_b_direct = [&](int k0, int buf) {
+            #pragma unroll
+            for (int p = 0; p < BPW; ++p) {
+                const int seg = p * NWAVE + wave;
+                const int row = seg * 8 + (lane >> 3);
+                const int kg = (lane & 7) ^ (lane >> 3);
+                #pragma unroll
+                for (int pl = 0; pl < PR::NPB; ++pl) {
+                    const L* src = reinterpret_cast<const L*>(J.E) + (size_t)pl * planeE + (size_t)(y0 + row) * ldE + k0 +
+                                   kg * VB;
+                    __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1)))*)src,
+                                                     (void __attribute__((address_space(3)))*)&sB[buf][pl][seg * 8][0],
+                                                     16, 0, 0);
+                }
+            }
+        };
 
         // ---- staging of one K chunk: global loads into registers (branch-
         // free: out-of-field elements read g_zero_run), then LDS stores ----
@@ -202,9 +241,11 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
             CS<S> x[APASS][VEC];
             CS<S> po[APASS][VEC];   // CGA: previous search direction
             CS<S> ph[APASS][VEC];   // M: phase
-            uint4 e[BPASS][PR::NPB];
+            uint4 e[PR::DLDS ? 1 : BPASS][PR::NPB];
         };
-        auto load = [&](Stage& g, int k0) {
+        static_assert(!PR::DLDS || PR::NBUF == 2, "direct E loads need the second LDS buffer");
+        // global loads of chunk k0 (with DLDS the E rows go straight to LDS buffer buf)
+        auto load = [&](Stage& g, int k0, int buf) {
             #pragma unroll
             for (int ps = 0; ps < APASS; ++ps) {
                 const int i = ps * RPP + tid / TPR;
@@ -225,14 +266,19 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
                 }
             }
             // rows y of E, zero padded to ldE columns / e_rows rows: always in bounds
-            #pragma unroll
-            for (int ps = 0; ps < BPASS; ++ps) {
-                const int j = ps * RPPB + tid / TPRB;
-                const int kk = (tid % TPRB) * VB;
-                const size_t eoff = (size_t)(y0 + j) * ldE + k0 + kk;
+            if constexpr (PR::DLDS) {
+                load_b_direct(k0, buf);
+            } else {
                 #pragma unroll
-                for (int pl = 0; pl < PR::NPB; ++pl)
-                    g.e[ps][pl] = *reinterpret_cast<const uint4*>(reinterpret_cast<const L*>(J.E) + (size_t)pl * planeE + eoff);
+                for (int ps = 0; ps < BPASS; ++ps) {
+                    const int j = ps * RPPB + tid / TPRB;
+                    const int kk = (tid % TPRB) * VB;
+                    const size_t eoff = (size_t)(y0 + j) * ldE + k0 + kk;
+                    #pragma unroll
+                    for (int pl = 0; pl < PR::NPB; ++pl)
+                        g.e[ps][pl] =
+                            *reinterpret_cast<const uint4*>(reinterpret_cast<const L*>(J.E) + (size_t)pl * planeE + eoff);
+                }
             }
         };
         auto store = [&](const Stage& g, int buf) {
@@ -265,12 +311,14 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
                     }
                 }
             }
-            #pragma unroll
-            for (int ps = 0; ps < BPASS; ++ps) {
-                const int j = ps * RPPB + tid / TPRB;
-                const int kk = (tid % TPRB) * VB;
+            if constexpr (!PR::DLDS) {
                 #pragma unroll
-                for (int pl = 0; pl < PR::NPB; ++pl) *reinterpret_cast<uint4*>(&sB[buf][pl][j][kk]) = g.e[ps][pl];
+                for (int ps = 0; ps < BPASS; ++ps) {
+                    const int j = ps * RPPB + tid / TPRB;
+                    const int kk = (tid % TPRB) * VB;
+                    #pragma unroll
+                    for (int pl = 0; pl < PR::NPB; ++pl) *reinterpret_cast<uint4*>(&sB[buf][pl][j][kk]) = g.e[ps][pl];
+                }
             }
         };
 
@@ -288,7 +336,10 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
                         ai[i] = sA[cur][1][row][k];
                     }
                     #pragma unroll
-                    for (int j = 0; j < FN; ++j) bv[j] = sB[cur][0][(wn * FN + j) * 16 + r16][k];
+                    for (int j = 0; j < FN; ++j) {
+                        const int col = (wn * FN + j) * 16 + r16;
+                        bv[j] = sB[cur][0][col][b_col<PR>(col, k)];
+                    }
                     #pragma unroll
                     for (int i = 0; i < FM; ++i)
                         #pragma unroll
@@ -318,8 +369,8 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
                     #pragma unroll
                     for (int j = 0; j < FN; ++j) {
                         const int col = (wn * FN + j) * 16 + r16;
-                        bh[j] = *reinterpret_cast<const bf8*>(&sB[cur][0][col][k]);
-                        bl[j] = *reinterpret_cast<const bf8*>(&sB[cur][1][col][k]);
+                        bh[j] = *reinterpret_cast<const bf8*>(&sB[cur][0][col][b_col<PR>(col, k)]);
+                        bl[j] = *reinterpret_cast<const bf8*>(&sB[cur][1][col][b_col<PR>(col, k)]);
                     }
                     #pragma unroll
                     for (int i = 0; i < FM; ++i)
@@ -338,20 +389,25 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
 
         const int nch = (nx + BK - 1) / BK;
         Stage stg;
-        load(stg, 0);
+        load(stg, 0, 0);
         store(stg, 0);
+        if constexpr (PR::DLDS) __builtin_amdgcn_s_waitcnt(0);   // the direct E loads have landed
         __syncthreads();
         for (int c = 0; c + 1 < nch; ++c) {
             // the next chunk's global loads are in flight during this chunk's
             // MFMAs: the scheduling barriers keep global loads above and LDS
             // writes and MFMAs out of the other phases (VALU, SALU and LDS
-            // reads may cross; mask 0x146)
-            load(stg, (c + 1) * BK);
+            // reads may cross; mask 0x146).  With DLDS the E rows of the next
+            // chunk go to the other buffer, which every wave has finished
+            // reading (barrier at the end of the previous iteration).
+            const int nxt = PR::NBUF == 2 ? ((c + 1) & 1) : 0;
+            load(stg, (c + 1) * BK, nxt);
             __builtin_amdgcn_sched_barrier(0x146);
             mfma(PR::NBUF == 2 ? (c & 1) : 0);
             __builtin_amdgcn_sched_barrier(0x146);
             if (PR::NBUF == 1) __syncthreads();   // all waves done reading the only buffer
-            store(stg, PR::NBUF == 2 ? ((c + 1) & 1) : 0);
+            store(stg, nxt);
+            if constexpr (PR::DLDS) __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
         }
         mfma(PR::NBUF == 2 ? ((nch - 1) & 1) : 0);
