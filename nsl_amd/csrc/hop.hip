@@ -40,7 +40,11 @@ struct PF64 {
     using Acc = d4;
     static constexpr int KSTEP = 4, PAD = 2, NPA = 2, NPB = 1, VEC = 2, NBUF = 2;
     static constexpr bool ROW4V = true;
-    static constexpr bool DLDS = true;    // E chunk loaded straight into LDS
+    // E chunk straight into LDS (global_load_lds_dwordx4, see load_b_direct):
+    // off.  It removes the E register stage, but the unpadded 128-byte rows it
+    // needs raise the LDS bank conflicts of the B reads from 2.7 to 3.8-4.4
+    // cycles per LDS instruction and the fused CG kernels run ~7 % slower.
+    static constexpr bool DLDS = false;
 };
 struct PF32 {
     using S = float;
@@ -48,7 +52,7 @@ struct PF32 {
     using Acc = f4;
     static constexpr int KSTEP = 4, PAD = 4, NPA = 2, NPB = 1, VEC = 4, NBUF = 2;
     static constexpr bool ROW4V = false;
-    static constexpr bool DLDS = true;
+    static constexpr bool DLDS = false;   // as PF64
 };
 struct PB3 {
     using S = float;
