@@ -47,7 +47,7 @@ struct SolverWork {
     DevBuf<int> done, status, want;
     DevBuf<double> upart, rpart0, rpart1;           // fused-CG tile partials
     std::vector<double> last_rr;                    // |r|^2 per slot after the last iteration
-    DevBuf<double> rr_ref, rr_prev, xn2, best;
+    DevBuf<double> rr_ref, rr_prev, xn2, best, rstart;
     DevBuf<int> stall;
     DevBuf<unsigned> counter;
     PinBuf<int> status_h;

@@ -262,6 +262,10 @@ struct RuPhase {
     // done < 0 or done == RU_ESCALATE (a fresh recurrence from the current x).
     void run(int prec, bool x_is_zero, int stall_max, int64_t maxit) {
         SolverWork& w = fm.sw;
+        // a low-precision phase escalates RHS whose projected iterations
+        // exceed its budget (ru_scalars); FP64 phases run to maxit
+        const int budget = prec == NSLX_F64 ? 0 : (int)std::min<int64_t>(maxit, 1 << 30);
+        const int it0 = (int)it;
         const int64_t len = fm.len();
         const int low64 = prec == NSLX_F64;
         const size_t blo = (size_t)nb * len * (low64 ? 16 : 8);
@@ -279,7 +283,8 @@ struct RuPhase {
 
         true_residual(x_is_zero);
         ru_residual_to_low(low64, w.r64.p, w.r.p, w.p0.p, w.x32.p, 1, nb, len, s);
-        ru_scalars(w.rn2.p, w.xn2.p, floor2, cg, nb, nparts, 0, (int)it, 1, w.best.p, w.stall.p, stall_max, s);
+        ru_scalars(w.rn2.p, w.xn2.p, floor2, cg, nb, nparts, 0, (int)it, 1, w.best.p, w.stall.p, stall_max, w.rstart.p,
+                   it0, budget, s);
         read_flags();
         bool finished = active == 0;
         HopArgs a = fm.hop_args(prec, nb, s);
@@ -312,7 +317,7 @@ struct RuPhase {
                     ru_accumulate(low64, x, w.x32.p, nb, len, s);
                     true_residual(false);
                     ru_scalars(w.rn2.p, w.xn2.p, floor2, cg, nb, nparts, (int)(k & 1), (int)it, 0, w.best.p,
-                               w.stall.p, stall_max, s);
+                               w.stall.p, stall_max, w.rstart.p, it0, budget, s);
                     ru_residual_to_low(low64, w.r64.p, w.r.p, nullptr, nullptr, 0, nb, len, s);
                     ++nupd;
                     last_upd = k;
@@ -333,7 +338,7 @@ int solve_cg_ru(FermionMatrix& fm, int op, int prec, const void* b, void* x, con
     w.r64.ensure(b64); w.t64.ensure(b64);
     w.ensure_scalars(nb, 1, s);
     w.rr_ref.ensure(nb); w.rr_prev.ensure(nb); w.want.ensure(nb); w.xn2.ensure(nb);
-    w.best.ensure(nb); w.stall.ensure(nb);
+    w.best.ensure(nb); w.stall.ensure(nb); w.rstart.ensure(nb);
     // ||MM^dag|| = ||M||^2 <= (1 + ||E|| max|P|)^2, ||E|| <= e^{|delta| ||K||},
     // |P| <= e^{|mu~| + max|Im phi|}
     const double nM = 1.0 + std::exp(std::fabs(fm.delta()) * fm.normK + std::fabs(fm.delta() * fm.mu) + fm.max_im_phi);

@@ -249,7 +249,8 @@ __global__ void k_ru_accum(double2* x64, void* xs, int64_t n) {
 }
 
 __global__ void k_ru_scalars(const double* rn2, const double* xn2, double floor2, CgDev cg, int64_t nb, int nparts,
-                             int cur, int iter, int init, double* best, int* stall, int stall_max) {
+                             int cur, int iter, int init, double* best, int* stall, int stall_max, double* rstart,
+                             int it0, int budget) {
     for (int64_t b = threadIdx.x; b < nb; b += VNT) {
         const int d = cg.done[b];
         if (init ? (d >= 0 && d != RU_ESCALATE) : d >= 0) continue;
@@ -263,6 +264,7 @@ __global__ void k_ru_scalars(const double* rn2, const double* xn2, double floor2
         if (init) {
             best[b] = s;
             stall[b] = 0;
+            if (rstart) rstart[b] = s;
         } else if (stall_max > 0) {
             // the low-precision recurrence has lost track of the true residual
             // (true |r| 10x above the recursive one): its rounding errors
@@ -273,6 +275,16 @@ __global__ void k_ru_scalars(const double* rn2, const double* xn2, double floor2
             // stagnation of the true residual over stall_max replacements
             if (s < 0.25 * best[b]) { best[b] = s; stall[b] = 0; }
             else if (++stall[b] >= stall_max) { cg.done[b] = RU_ESCALATE; continue; }
+            // projected budget: at the average rate so far the RHS would need
+            // more low-precision iterations than the phase allows
+            const int done_its = iter - it0;
+            if (budget > 0 && rstart && done_its >= 100) {
+                const double r0 = rstart[b];
+                if (s >= r0) { cg.done[b] = RU_ESCALATE; continue; }
+                const double rate = log(r0 / s) / done_its;
+                const double need = log(s / cg.tol2[b]) / rate;
+                if (done_its + need > budget) { cg.done[b] = RU_ESCALATE; continue; }
+            }
         }
         cg.done[b] = -1;
         cg.rr_ref[b] = s;
@@ -352,9 +364,10 @@ void ru_accumulate(int low_f64, void* x64, void* xs, int64_t nb, int64_t len, hi
 }
 
 void ru_scalars(const double* rn2, const double* xn2, double floor2, const CgDev& cg, int64_t nb, int nparts, int cur,
-                int iter, int init, double* best, int* stall, int stall_max, hipStream_t s) {
+                int iter, int init, double* best, int* stall, int stall_max, double* rstart, int it0, int budget,
+                hipStream_t s) {
     hipLaunchKernelGGL(k_ru_scalars, dim3(1), dim3(VNT), 0, s, rn2, xn2, floor2, cg, nb, nparts, cur, iter, init,
-                       best, stall, stall_max);
+                       best, stall, stall_max, rstart, it0, budget);
     NSLX_LAUNCH_CHECK();
 }
 

@@ -55,12 +55,16 @@ void ru_accumulate(int low_is_f64, void* x64, void* xs, int64_t nb, int64_t len,
 // (the kernels skip it; the solver finishes it in FP64) when its true |r|^2
 // exceeds 100x the recursive |r|^2 held in rpart[cur] (the low-precision
 // recurrence has lost track of the residual), or when its true |r|^2 has not
-// fallen below a quarter of its best value for stall_max replacements.
-// init re-initialises the
-// RHS with done < 0 or done == RU_ESCALATE.  best/stall: nb entries.
+// fallen below a quarter of its best value for stall_max replacements, or
+// when budget > 0 and, 100 or more iterations after the phase started at
+// iteration it0, its average rate so far (from rstart, the |r|^2 at the
+// phase start) projects more than budget iterations to |r|^2 <= tol^2.
+// init re-initialises the RHS with done < 0 or done == RU_ESCALATE (and
+// records rstart).  best/stall/rstart: nb entries.
 constexpr int RU_ESCALATE = 1 << 30;
 void ru_scalars(const double* rn2, const double* xn2, double floor2, const CgDev& cg, int64_t nb, int nparts,
-                int cur, int iter, int init, double* best, int* stall, int stall_max, hipStream_t s);
+                int cur, int iter, int init, double* best, int* stall, int stall_max, double* rstart, int it0,
+                int budget, hipStream_t s);
 
 // Element type conversions.
 void c128_to_c64(const void* in, void* out, int64_t n, hipStream_t s);
