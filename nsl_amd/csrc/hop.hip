@@ -58,10 +58,11 @@ struct PB3 {
     using S = float;
     using L = uint16_t;    // bf16 bits
     using Acc = f4;
-    static constexpr int KSTEP = 32, PAD = 8, NPA = 4, NPB = 2, VEC = 4, NBUF = 1;
+    // BK = 32 keeps two LDS buffers of the four A and two B bf16 planes at
+    // 51 KB per workgroup (3 per CU); one k-step of the 16x16x32 MFMA per chunk
+    static constexpr int KSTEP = 32, PAD = 8, NPA = 4, NPB = 2, VEC = 4, NBUF = 2;
     static constexpr bool ROW4V = false;
-    // one LDS buffer: direct E loads could not overlap the MFMAs
-    static constexpr bool DLDS = false;
+    static constexpr bool DLDS = false;   // 64-byte B rows; direct loads need 128
 };
 
 __device__ inline uint16_t f2bf(float x) {   // round to nearest even
@@ -213,9 +214,10 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
         constexpr int VB = 16 / sizeof(L);
         constexpr int TPRB = BK / VB;
         constexpr int RPPB = NT / TPRB;
-        constexpr int BPASS = (BN + RPPB - 1) / RPPB;
+        // passes over the (plane, row) pairs of the B chunk
+        constexpr int BPASS = (PR::NPB * BN + RPPB - 1) / RPPB;
 
-        static_assert(APASS * RPP == BM && BPASS * RPPB == BN, "loader passes must tile BM and BN exactly");
+        static_assert(APASS * RPP == BM && BPASS * RPPB == PR::NPB * BN, "loader passes must tile BM and BN exactly");
         // direct B loads: one wave instruction fills 8 rows of 128 bytes
         constexpr int NWAVE = NT / 64;
         constexpr int BSEG = BN / 8;
@@ -245,7 +247,7 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
             CS<S> x[APASS][VEC];
             CS<S> po[APASS][VEC];   // CGA: previous search direction
             CS<S> ph[APASS][VEC];   // M: phase
-            uint4 e[PR::DLDS ? 1 : BPASS][PR::NPB];
+            uint4 e[PR::DLDS ? 1 : BPASS];
         };
         static_assert(!PR::DLDS || PR::NBUF == 2, "direct E loads need the second LDS buffer");
         // global loads of chunk k0 (with DLDS the E rows go straight to LDS buffer buf)
@@ -275,13 +277,11 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
             } else {
                 #pragma unroll
                 for (int ps = 0; ps < BPASS; ++ps) {
-                    const int j = ps * RPPB + tid / TPRB;
+                    const int jp = ps * RPPB + tid / TPRB;     // plane * BN + row
+                    const int pl = jp / BN, j = jp % BN;
                     const int kk = (tid % TPRB) * VB;
-                    const size_t eoff = (size_t)(y0 + j) * ldE + k0 + kk;
-                    #pragma unroll
-                    for (int pl = 0; pl < PR::NPB; ++pl)
-                        g.e[ps][pl] =
-                            *reinterpret_cast<const uint4*>(reinterpret_cast<const L*>(J.E) + (size_t)pl * planeE + eoff);
+                    const size_t eoff = (size_t)pl * planeE + (size_t)(y0 + j) * ldE + k0 + kk;
+                    g.e[ps] = *reinterpret_cast<const uint4*>(reinterpret_cast<const L*>(J.E) + eoff);
                 }
             }
         };
@@ -318,10 +318,10 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
             if constexpr (!PR::DLDS) {
                 #pragma unroll
                 for (int ps = 0; ps < BPASS; ++ps) {
-                    const int j = ps * RPPB + tid / TPRB;
+                    const int jp = ps * RPPB + tid / TPRB;
+                    const int pl = jp / BN, j = jp % BN;
                     const int kk = (tid % TPRB) * VB;
-                    #pragma unroll
-                    for (int pl = 0; pl < PR::NPB; ++pl) *reinterpret_cast<uint4*>(&sB[buf][pl][j][kk]) = g.e[ps][pl];
+                    *reinterpret_cast<uint4*>(&sB[buf][pl][j][kk]) = g.e[ps];
                 }
             }
         };
@@ -1082,10 +1082,10 @@ using T32_a = Tile<2, 2, 2, 3, 32>;
 using T32_b = Tile<2, 2, 1, 3, 32>;
 using T32_c = Tile<1, 2, 1, 1, 32>;
 using T32_d = Tile<2, 2, 2, 2, 32>;
-using TB3_a = Tile<2, 2, 2, 3, 64>;
-using TB3_b = Tile<2, 2, 1, 3, 64>;
-using TB3_c = Tile<1, 2, 1, 1, 64>;
-using TB3_d = Tile<2, 2, 2, 2, 64>;
+using TB3_a = Tile<2, 2, 2, 3, 32>;
+using TB3_b = Tile<2, 2, 1, 3, 32>;
+using TB3_c = Tile<1, 2, 1, 1, 32>;
+using TB3_d = Tile<2, 2, 2, 2, 32>;
 
 template <class TL>
 TileInfo info(int id) { return {id, TL::BM, TL::BN, TL::BK, TL::NT}; }
