@@ -163,6 +163,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with `python -m nsl_amd.build`")
+        # torch ships its own HIP/HSA runtime.  Loaded first, it is the one
+        # libnslx.so binds to (same soname), so torch tensors, torch streams and
+        # the library's kernels share one runtime and one default stream;
+        # loaded after libnslx.so, a second runtime would come up beside the
+        # system one and the two would neither order their work nor share
+        # stream handles.
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in PROTOS.items():
             fn = getattr(L, name)

@@ -84,3 +84,20 @@ def test_no_cpu_fallback_without_library(tmp_path, monkeypatch):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(ImportError):
         _lib.lib()
+
+
+def test_one_hip_runtime_in_process(libpath):
+    """The binding loads torch before libnslx.so, so the library binds to
+    torch's HIP/HSA runtime (same soname) instead of bringing up the system
+    one beside it: exactly one of each is mapped, in a fresh interpreter that
+    loads the library first."""
+    import subprocess
+    import sys
+    code = ("import nsl_amd._lib as L; L.lib(); import torch; "
+            "m = open('/proc/self/maps').read().splitlines(); "
+            "print(len({l.split()[-1] for l in m if 'libamdhip64' in l}), "
+            "len({l.split()[-1] for l in m if 'libhsa-runtime64' in l}))")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.split() == ["1", "1"], out.stdout
