@@ -216,11 +216,13 @@ int nslx_solve_precond(nslx_fm fm, nslx_fm fm_prec, int op, const void* b, void*
  * beta, fields) with its own b and x (device c128, nbatch x nt x nx); all fms
  * must belong to ctx.  Every Krylov iteration is one grouped persistent hop
  * launch per phase over the tiles of all items (per precision / tile class);
- * with use_graph the iteration loop is captured once as a HIP graph and
+ * with use_graph the iteration loop is built once as a HIP graph and
  * replayed.  Per-RHS stop: |b - A x| <= tol |b| in FP64 or the FP64 attainable
- * accuracy (c eps ||A|| |x|).  precision F32 / BF16X3 runs restarted defect
- * correction (low-precision inner CG to 1e-5 relative, FP64 residual),
- * switching an item to FP64 when a low-precision step gains less than 10x.
+ * accuracy (c eps ||A|| |x|).  precision F32 / BF16X3 runs the low-precision
+ * recurrence with FP64 reliable updates (x accumulated in FP64, the true
+ * residual replacing the recursive one at replay boundaries); RHS whose
+ * low-precision recurrence loses track, stalls or projects past its budget
+ * are finished from x in FP64.
  * An item's iterates do not depend on the other items.  Synchronous. */
 typedef struct {
     nslx_fm     fm;

@@ -1151,6 +1151,33 @@ void launch_g(int op, int mode, const GroupArgs& g, int grid, hipStream_t s) {
     throw Error(NSLX_E_ARG, "hop: bad op/mode");
 }
 
+template <class TL, class PR>
+GroupedKernel kernel_g(int op, int mode) {
+#define NSLX_HOPG_FN(OPV, MV)                                                                  \
+    if (op == OPV && mode == MV) return {reinterpret_cast<const void*>(&hop_grouped<TL, PR, OPV, MV>), TL::NT};
+    NSLX_HOPG_FN(HOP_M, MODE_PLAIN)
+    NSLX_HOPG_FN(HOP_MD, MODE_PLAIN)
+    NSLX_HOPG_FN(HOP_M, MODE_CGA)
+    NSLX_HOPG_FN(HOP_MD, MODE_CGA)
+    NSLX_HOPG_FN(HOP_M, MODE_CGB)
+    NSLX_HOPG_FN(HOP_MD, MODE_CGB)
+    NSLX_HOPG_FN(HOP_M, MODE_CGI)
+    NSLX_HOPG_FN(HOP_MD, MODE_CGI)
+#undef NSLX_HOPG_FN
+    throw Error(NSLX_E_ARG, "hop: bad op/mode");
+}
+
+template <class PR, class T0, class T1, class T2, class T3>
+GroupedKernel kernel4g(int tile, int op, int mode) {
+    switch (tile) {
+        case 0: return kernel_g<T0, PR>(op, mode);
+        case 1: return kernel_g<T1, PR>(op, mode);
+        case 2: return kernel_g<T2, PR>(op, mode);
+        case 3: return kernel_g<T3, PR>(op, mode);
+        default: throw Error(NSLX_E_ARG, "hop: bad tile id");
+    }
+}
+
 template <class PR, class T0, class T1, class T2, class T3>
 void dispatch4g(int tile, int op, int mode, const GroupArgs& g, int grid, hipStream_t s) {
     switch (tile) {
@@ -1173,6 +1200,14 @@ void launch_hop_grouped(int prec, int op, int mode, int tile, const GroupArgs& g
     else if (prec == NSLX_BF16X3) dispatch4g<PB3, TB3_a, TB3_b, TB3_c, TB3_d>(tile, op, mode, g, grid, s);
     else throw Error(NSLX_E_ARG, "hop: bad precision");
     NSLX_LAUNCH_CHECK();
+}
+
+GroupedKernel hop_grouped_kernel(int prec, int op, int mode, int tile) {
+    NSLX_REQUIRE(tile >= 0 && tile < SLICE_BASE, "hop: grouped launches take tiles 0-3");
+    if (prec == NSLX_F64) return kernel4g<PF64, T64_a, T64_b, T64_c, T64_d>(tile, op, mode);
+    if (prec == NSLX_F32) return kernel4g<PF32, T32_a, T32_b, T32_c, T32_d>(tile, op, mode);
+    if (prec == NSLX_BF16X3) return kernel4g<PB3, TB3_a, TB3_b, TB3_c, TB3_d>(tile, op, mode);
+    throw Error(NSLX_E_ARG, "hop: bad precision");
 }
 
 std::vector<int4> build_tile_list(std::vector<HopProblem>& probs, int prec, int tile) {

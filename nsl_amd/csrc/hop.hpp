@@ -119,6 +119,14 @@ struct TileInfo {
 void launch_hop(int prec, int op, int mode, const HopArgs& a, int tile, hipStream_t s);
 // Grouped persistent launch over g.ntiles tiles with `grid` workgroups.
 void launch_hop_grouped(int prec, int op, int mode, int tile, const GroupArgs& g, int grid, hipStream_t s);
+// The kernel launch_hop_grouped would launch (function and workgroup size),
+// for building graph kernel nodes; it takes one GroupArgs by value and no
+// dynamic LDS.
+struct GroupedKernel {
+    const void* fn;
+    int nthreads;
+};
+GroupedKernel hop_grouped_kernel(int prec, int op, int mode, int tile);
 // Host-side tile list for problems (nrt/nct filled in from the tile shape):
 // problems with larger nx first (longer K loops), and the column tiles of a
 // row block 8 entries apart so that a persistent grid that is a multiple of 8

@@ -6,21 +6,27 @@
 // p, |u|^2 partials) and one grouped CGB launch (M hop with x += alpha p,
 // r -= alpha t, |r|^2 partials) per (precision, tile) class, followed by a
 // one-thread kernel that advances k.  The kernels read k and the tile count
-// from device memory, so the launch sequence of P iterations is captured once
+// from device memory, so the launch sequence of P iterations is built once
 // as a HIP graph and replayed; jobs join and leave between replays by
 // rewriting the device tile lists.  A job joining before iteration k puts its
 // first search direction in p[(k-1)&1], its |r|^2 in rpart[(k-1)&1] and zeros
 // in rpart[k&1], so that its first beta is 0; P is even, so every replay
-// starts at the same parity and the captured roles stay valid.
+// starts at the same parity and the graph's roles stay valid.
 //
-// When all RHS of a job have met their recursive tolerance the job is
-// finalised in FP64: x (+= the low-precision increment), the true residual
-// b - A x, and the per-RHS test |r| <= tol |b| or |r| <= c eps ||A|| |x| (the
-// FP64 attainable accuracy).  Otherwise the job restarts from x: FP64 jobs with
-// a fresh recurrence, low-precision jobs as the next defect-correction step,
-// or in FP64 when the last low-precision step gained less than 10x.
+// FP64 jobs: when all RHS have met their recursive tolerance the job is
+// finalised: the true residual b - A x and the per-RHS test |r| <= tol |b| or
+// |r| <= c eps ||A|| |x| (the FP64 attainable accuracy); otherwise it restarts
+// from x with a fresh recurrence.
+// Low-precision jobs run CG with FP64 reliable updates, as the single-problem
+// solver: the kernels never stop a RHS on its recursive residual but flag it
+// (want) once it has fallen by delta since the last replacement; at a replay
+// boundary where half the job's active RHS want it (or 200 iterations passed)
+// the low-precision increment is folded into x, r = b - A x is recomputed in
+// FP64 and replaces the recursive residual, and the search direction is kept
+// (ru_scalars: converged RHS stop, the others continue; a RHS that lost track,
+// stalled or projects past the low-precision budget escalates the job, which
+// then restarts from x in FP64).
 #include "pool.hpp"
-#include "prof.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -34,9 +40,10 @@ __global__ void k_inc(int* it) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *it += 1;
 }
 
-// Slot initialisation of a job joining before iteration k.
+// Slot initialisation of a job joining before iteration k.  With keep, RHS
+// that reliable updates found converged (done >= 0, not RU_ESCALATE) stay done.
 __global__ void k_join(int64_t nb, int64_t rhs0, int64_t part0, int nparts, const double* __restrict__ rn2,
-                       const double* __restrict__ bn2, double tol, double delta, double* tol2, int* done, int* want,
+                       const double* __restrict__ bn2, double tol, int keep, double* tol2, int* done, int* want,
                        double* rp_cur, double* rp_prev) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nb * nparts) return;
@@ -45,13 +52,21 @@ __global__ void k_join(int64_t nb, int64_t rhs0, int64_t part0, int nparts, cons
     rp_cur[part0 + i] = k == 0 ? rn2[b] : 0.0;
     rp_prev[part0 + i] = 0.0;
     if (k == 0) {
-        tol2[rhs0 + b] = std::max(tol * tol * bn2[b], delta * delta * rn2[b]);
-        done[rhs0 + b] = -1;
+        tol2[rhs0 + b] = tol * tol * bn2[b];
         want[rhs0 + b] = 0;
+        const int d = done[rhs0 + b];
+        if (!(keep && d >= 0 && d != RU_ESCALATE)) done[rhs0 + b] = -1;
     }
 }
 
 inline unsigned g1(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+// c eps ||MM^dag||: the attainable accuracy of an FP64 residual per |x|, with
+// ||MM^dag|| = ||M||^2 <= (1 + e^{|delta| ||K||} e^{|mu~| + max|Im phi|})^2.
+double floor_c(const FermionMatrix& fm) {
+    const double nM = 1.0 + std::exp(std::fabs(fm.delta()) * fm.normK + std::fabs(fm.delta() * fm.mu) + fm.max_im_phi);
+    return env_double("NSLX_FLOOR_C", 1.0) * 1.1102230246251565e-16 * nM * nM;
+}
 
 }  // namespace
 
@@ -79,6 +94,7 @@ Pool::Pool(Ctx* ctx, hipStream_t s, int64_t cap_rhs, int use_graph, int iters_pe
     rhs_used_.assign(cap_rhs, 0);
     part_used_.assign(cap_parts_, 0);
     done_h_.assign(cap_rhs, 0);
+    want_h_.assign(cap_rhs, 0);
     const int ncu = ctx->num_cu > 0 ? ctx->num_cu : 256;
     for (int c = 0; c < NCLS; ++c) {
         const int tile = c % 4;
@@ -137,9 +153,22 @@ CgDev Pool::cgdev() const {
     return c;
 }
 
+CgDev Pool::slice(const PoolJob& j) const {
+    CgDev c = cgdev();
+    c.upart += j.part0;
+    c.rpart[0] += j.part0;
+    c.rpart[1] += j.part0;
+    c.tol2 += j.rhs0;
+    c.done += j.rhs0;
+    c.rr_ref += j.rhs0;
+    c.want += j.rhs0;
+    return c;
+}
+
 // (Re)start the recurrence of job j before global iteration iter_: fresh
-// (x = 0, r = b) or from the current x (r = b - A x in FP64).
-void Pool::start_recurrence(PoolJob& j, bool fresh) {
+// (x = 0, r = b) or from the current x (r = b - A x in FP64); keep_done:
+// RHS converged in the previous (reliable-update) recurrence stay done.
+void Pool::start_recurrence(PoolJob& j, bool fresh, bool keep_done) {
     FermionMatrix& fm = *j.fm;
     const int64_t len = fm.len();
     const size_t n = (size_t)j.nb * len;
@@ -174,19 +203,20 @@ void Pool::start_recurrence(PoolJob& j, bool fresh) {
     NSLX_REQUIRE(j.part0 >= 0, "pool: out of partial-sum slots");
     double* cur = ((iter_ - 1) & 1) ? rpart1_.p : rpart0_.p;
     double* prev = (iter_ & 1) ? rpart1_.p : rpart0_.p;
-    const double delta = j.prec == NSLX_F64 ? 0.0 : env_double("NSLX_POOL_INNER", 1e-5);
     const int64_t m = j.nb * j.nparts;
     hipLaunchKernelGGL(k_join, dim3(g1(m)), dim3(256), 0, s_, j.nb, j.rhs0, j.part0, j.nparts, j.rn2.p, j.bn2.p, j.tol,
-                       delta, tol2_.p, done_.p, want_.p, cur, prev);
+                       keep_done ? 1 : 0, tol2_.p, done_.p, want_.p, cur, prev);
     NSLX_LAUNCH_CHECK();
-    // relative residual at the start of this recurrence (host)
-    std::vector<double> rn(j.nb), bn(j.nb);
-    NSLX_HIP(hipMemcpyAsync(rn.data(), j.rn2.p, j.nb * sizeof(double), hipMemcpyDeviceToHost, s_));
-    NSLX_HIP(hipMemcpyAsync(bn.data(), j.bn2.p, j.nb * sizeof(double), hipMemcpyDeviceToHost, s_));
-    NSLX_HIP(hipStreamSynchronize(s_));
-    double rel = 0.0;
-    for (int64_t i = 0; i < j.nb; ++i) rel = std::max(rel, bn[i] > 0 ? std::sqrt(rn[i] / bn[i]) : 0.0);
-    j.rel_start = rel;
+    if (j.prec != NSLX_F64) {
+        // reliable-update state of the job's RHS (rr_ref, best, stall, rstart)
+        j.best.ensure(j.nb); j.stall.ensure(j.nb); j.rstart.ensure(j.nb);
+        norm2_perb(1, j.x, j.nb, len, j.xn2.p, fm.vsc, s_);
+        const double fc = floor_c(fm);
+        ru_scalars(j.rn2.p, j.xn2.p, fc * fc, slice(j), j.nb, j.nparts, (int)((iter_ - 1) & 1), (int)iter_, 1,
+                   j.best.p, j.stall.p, 0, j.rstart.p, (int)iter_, 0, s_);
+        j.ru_it0 = iter_;
+        j.last_upd = iter_;
+    }
     j.join_iter = iter_;
 }
 
@@ -202,7 +232,7 @@ void Pool::admit(std::unique_ptr<PoolJob> jp) {
     try {
         j.bn2.ensure(j.nb);
         norm2_perb(1, j.b, j.nb, j.fm->len(), j.bn2.p, j.fm->vsc, s_);
-        start_recurrence(j, true);
+        start_recurrence(j, true, false);
     } catch (...) {
         free_range(rhs_used_, j.rhs0, j.nb);
         if (j.part0 >= 0) free_range(part_used_, j.part0, j.nb * j.nparts);
@@ -265,14 +295,19 @@ void Pool::rebuild_class(int c) {
     NSLX_HIP(hipStreamSynchronize(s_));   // host vectors are freed on return
 }
 
-// P iterations over the active classes (captured or launched directly).
+// P iterations over the active classes, launched directly or replayed from a
+// graph.  The graph is built node by node (kernel nodes in launch order), not
+// by stream capture, so that work other threads put on the legacy stream
+// meanwhile never meets a capturing stream.
 void Pool::run_iterations() {
     unsigned mask = 0;
     for (int c = 0; c < NCLS; ++c)
         if (cls_[c].njobs > 0) mask |= 1u << c;
     if (!mask) return;
-    auto launch_all = [&]() {
+    // the launch sequence: hop(c, pass, args) per grouped launch, inc() per iteration
+    auto sequence = [&](auto&& hop, auto&& inc) {
         const CgDev cg = cgdev();
+        const double delta = env_double("NSLX_RU_DELTA", 0.1);
         for (int k = 0; k < P_; ++k) {
             const int64_t K = iter_ + k;   // parity only (the kernels read the counter)
             const int pold = ((K - 1) & 1) ? VS_P1 : VS_P0;
@@ -288,42 +323,62 @@ void Pool::run_iterations() {
                     g.ntiles_dev = cls_[c].ntiles.p;
                     g.iter_dev = it_dev_.p;
                     g.cg = cg;
+                    if (c / 4 != NSLX_F64) {   // low precision: reliable-update flags
+                        g.cg.ru = 1;
+                        g.cg.delta2 = delta * delta;
+                    }
                     if (pass == 0) {   // CGA: Y = M^dag(p_new), p_new = r + beta p_old
                         g.role[0] = VS_R; g.role[1] = VS_U; g.role[2] = (signed char)pold; g.role[3] = (signed char)pnew;
-                        launch_hop_grouped(c / 4, HOP_MD, MODE_CGA, c % 4, g, cls_[c].grid, s_);
                     } else {           // CGB: t = M u, x += alpha p_new, r -= alpha t
                         g.role[0] = VS_U; g.role[2] = (signed char)pnew; g.role[4] = VS_X; g.role[5] = VS_R;
-                        launch_hop_grouped(c / 4, HOP_M, MODE_CGB, c % 4, g, cls_[c].grid, s_);
                     }
+                    hop(c, pass, g);
                 }
             }
-            hipLaunchKernelGGL(k_inc, dim3(1), dim3(64), 0, s_, it_dev_.p);
-            NSLX_LAUNCH_CHECK();
+            inc();
         }
     };
+    auto op_of = [](int pass) { return pass == 0 ? HOP_MD : HOP_M; };
+    auto mode_of = [](int pass) { return pass == 0 ? MODE_CGA : MODE_CGB; };
     if (use_graph_) {
         if (!exec_ || mask != cls_mask_) {
             drop_graph();
-            const int was = prof_every();
-            prof_enable(0);
-            NSLX_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
-            try {
-                launch_all();
-            } catch (...) {
-                hipGraph_t g = nullptr;
-                (void)hipStreamEndCapture(s_, &g);
-                if (g) (void)hipGraphDestroy(g);
-                prof_enable(was);
-                throw;
-            }
-            NSLX_HIP(hipStreamEndCapture(s_, &graph_));
-            prof_enable(was);
+            NSLX_HIP(hipGraphCreate(&graph_, 0));
+            hipGraphNode_t last = nullptr;
+            auto add = [&](const void* fn, unsigned grid, unsigned block, void* arg) {
+                void* params[1] = {arg};   // argument values are copied into the node
+                hipKernelNodeParams kp{};
+                kp.func = const_cast<void*>(fn);
+                kp.gridDim = dim3(grid);
+                kp.blockDim = dim3(block);
+                kp.sharedMemBytes = 0;
+                kp.kernelParams = params;
+                kp.extra = nullptr;
+                hipGraphNode_t n = nullptr;
+                NSLX_HIP(hipGraphAddKernelNode(&n, graph_, last ? &last : nullptr, last ? 1 : 0, &kp));
+                last = n;
+            };
+            int* itp = it_dev_.p;
+            sequence(
+                [&](int c, int pass, GroupArgs& g) {
+                    NSLX_REQUIRE(g.nprob > 0 && cls_[c].grid > 0, "pool: empty grouped launch");
+                    const GroupedKernel gk = hop_grouped_kernel(c / 4, op_of(pass), mode_of(pass), c % 4);
+                    add(gk.fn, (unsigned)cls_[c].grid, (unsigned)gk.nthreads, &g);
+                },
+                [&]() { add(reinterpret_cast<const void*>(&k_inc), 1, 64, &itp); });
             NSLX_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
             cls_mask_ = mask;
         }
         NSLX_HIP(hipGraphLaunch(exec_, s_));
     } else {
-        launch_all();
+        sequence(
+            [&](int c, int pass, GroupArgs& g) {
+                launch_hop_grouped(c / 4, op_of(pass), mode_of(pass), c % 4, g, cls_[c].grid, s_);
+            },
+            [&]() {
+                hipLaunchKernelGGL(k_inc, dim3(1), dim3(64), 0, s_, it_dev_.p);
+                NSLX_LAUNCH_CHECK();
+            });
     }
     iter_ += P_;
     st_.iterations += P_;
@@ -343,10 +398,9 @@ bool Pool::finalize(PoolJob& j) {
     NSLX_HIP(hipMemcpyAsync(bn.data(), j.bn2.p, j.nb * sizeof(double), hipMemcpyDeviceToHost, s_));
     NSLX_HIP(hipMemcpyAsync(xn.data(), j.xn2.p, j.nb * sizeof(double), hipMemcpyDeviceToHost, s_));
     NSLX_HIP(hipStreamSynchronize(s_));
-    const double nM = 1.0 + std::exp(std::fabs(fm.delta()) * fm.normK + std::fabs(fm.delta() * fm.mu) + fm.max_im_phi);
-    const double fc = env_double("NSLX_FLOOR_C", 1.0) * 1.1102230246251565e-16 * nM * nM;
+    const double fc = floor_c(fm);
     bool all = true;
-    double rel = 0.0, maxrel = 0.0, maxfloor = 0.0;
+    double maxrel = 0.0, maxfloor = 0.0;
     int64_t fstops = 0;
     for (int64_t i = 0; i < j.nb; ++i) {
         const double fl2 = fc * fc * xn[i];
@@ -355,17 +409,19 @@ bool Pool::finalize(PoolJob& j) {
         const double r = bn[i] > 0 ? std::sqrt(rn[i] / bn[i]) : std::sqrt(rn[i]);
         maxrel = std::max(maxrel, r);
         if (bn[i] > 0) maxfloor = std::max(maxfloor, std::sqrt(fl2 / bn[i]));
-        if (!ok) { all = false; rel = std::max(rel, r); }
+        if (!ok) all = false;
     }
     j.max_rel = maxrel;
     j.max_floor_rel = maxfloor;
     j.floor_stops = fstops;
     if (all) { j.status = NSLX_OK; return true; }
     if (j.iters >= j.maxit || j.restarts >= 200) { j.status = NSLX_NOT_CONVERGED; return true; }
-    // restart: the next defect-correction step, or FP64 when low precision stalls
+    // restart from x: FP64 jobs with a fresh recurrence; a low-precision job
+    // ends here with RHS its reliable updates escalated (or out of its
+    // low-precision budget) and continues in FP64
     ++j.restarts;
     ++st_.restarts;
-    if (j.prec != NSLX_F64 && !(rel < 0.1 * j.rel_start)) {
+    if (j.prec != NSLX_F64) {
         j.prec = NSLX_F64;
         j.escalated = true;
         ++st_.escalations;
@@ -374,35 +430,77 @@ bool Pool::finalize(PoolJob& j) {
     return false;
 }
 
+// Reliable update of a low-precision job (in place, the search direction
+// kept): x += xs, r = b - A x in FP64, ru_scalars on the job's slots, and the
+// recurrence residual replaced by low(r).
+void Pool::replace_residual(PoolJob& j) {
+    FermionMatrix& fm = *j.fm;
+    const int64_t len = fm.len();
+    const double fc = floor_c(fm);
+    const int64_t lowmax = (int64_t)env_double("NSLX_RU_LOWMAX", 2000.0);
+    const int budget = (int)std::min<int64_t>(lowmax, std::max<int64_t>(1, j.maxit - j.iters));
+    ru_accumulate(0, j.x, j.xs.p, j.nb, len, s_);
+    fm.apply(j.op, j.x, j.t64.p, j.nb, NSLX_F64, s_);
+    resid64(j.b, j.t64.p, j.r64.p, j.nb, len, j.rn2.p, fm.vsc, s_);
+    norm2_perb(1, j.x, j.nb, len, j.xn2.p, fm.vsc, s_);
+    ru_scalars(j.rn2.p, j.xn2.p, fc * fc, slice(j), j.nb, j.nparts, (int)((iter_ - 1) & 1), (int)iter_, 0, j.best.p,
+               j.stall.p, (int)env_double("NSLX_RU_STALL", 16.0), j.rstart.p, (int)j.ru_it0, budget, s_);
+    ru_residual_to_low(0, j.r64.p, j.r.p, nullptr, nullptr, 0, j.nb, len, s_);
+    j.last_upd = iter_;
+}
+
 std::vector<std::unique_ptr<PoolJob>> Pool::step() {
     std::vector<std::unique_ptr<PoolJob>> out;
     if (jobs_.empty()) return out;
     run_iterations();
     int64_t hi = 0;
     for (auto& jp : jobs_) hi = std::max<int64_t>(hi, jp->rhs0 + jp->nb);
-    NSLX_HIP(hipMemcpyAsync(done_h_.data(), done_.p, hi * sizeof(int), hipMemcpyDeviceToHost, s_));
-    NSLX_HIP(hipStreamSynchronize(s_));
+    auto read_flags = [&]() {
+        NSLX_HIP(hipMemcpyAsync(done_h_.data(), done_.p, hi * sizeof(int), hipMemcpyDeviceToHost, s_));
+        NSLX_HIP(hipMemcpyAsync(want_h_.data(), want_.p, hi * sizeof(int), hipMemcpyDeviceToHost, s_));
+        NSLX_HIP(hipStreamSynchronize(s_));
+    };
+    read_flags();
     const int64_t lowmax = (int64_t)env_double("NSLX_RU_LOWMAX", 2000.0);
+    const double frac = env_double("NSLX_RU_FRAC", 0.5);
+    const int64_t maxgap = (int64_t)env_double("NSLX_RU_MAXGAP", 200.0);
+    auto low_over = [&](const PoolJob& j) {
+        return iter_ - j.ru_it0 >= std::min(lowmax, j.maxit - j.iters);
+    };
+    // reliable updates of the low-precision jobs that ask for one
+    bool replaced = false;
+    for (auto& jp : jobs_) {
+        PoolJob& j = *jp;
+        if (j.prec == NSLX_F64) continue;
+        int64_t active = 0, waiting = 0;
+        for (int64_t i = 0; i < j.nb; ++i)
+            if (done_h_[j.rhs0 + i] < 0) { ++active; if (!want_h_[j.rhs0 + i]) ++waiting; }
+        if (active > 0 && (waiting <= (1.0 - frac) * active || iter_ - j.last_upd >= maxgap || low_over(j))) {
+            replace_residual(j);
+            replaced = true;
+        }
+    }
+    if (replaced) read_flags();
     unsigned dirty = 0;
     std::vector<std::unique_ptr<PoolJob>> keep;
     for (auto& jp : jobs_) {
         PoolJob& j = *jp;
         bool all_done = true;
         for (int64_t i = 0; i < j.nb && all_done; ++i) all_done = done_h_[j.rhs0 + i] >= 0;
-        const int64_t ran = iter_ - j.join_iter;
-        const bool budget = ran >= (j.prec == NSLX_F64 ? j.maxit - j.iters : std::min(lowmax, j.maxit - j.iters));
+        const bool budget = j.prec == NSLX_F64 ? iter_ - j.join_iter >= j.maxit - j.iters : low_over(j);
         if (!all_done && !budget) { keep.push_back(std::move(jp)); continue; }
         const int old_cls = j.cls;
         dirty |= 1u << old_cls;
         free_range(part_used_, j.part0, j.nb * j.nparts);
         j.part0 = -1;
+        const bool was_low = j.prec != NSLX_F64;
         if (finalize(j)) {
             j.finished = true;
             free_range(rhs_used_, j.rhs0, j.nb);
             j.rhs0 = -1;
             out.push_back(std::move(jp));
         } else {
-            start_recurrence(j, false);
+            start_recurrence(j, false, was_low);
             dirty |= 1u << j.cls;
             keep.push_back(std::move(jp));
         }

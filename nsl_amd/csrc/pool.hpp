@@ -1,6 +1,6 @@
 // Ragged batched CG over many fermion matrices (different Nx, Nt, fields) in
 // one grouped hop launch per phase per Krylov iteration; the iteration loop
-// is captured once as a HIP graph and replayed.  Jobs join and leave between
+// is built once as a HIP graph and replayed.  Jobs join and leave between
 // replays (continuous batching).  Used by nslx_batch_solve and by the
 // correlator server.
 #pragma once
@@ -30,13 +30,16 @@ struct PoolJob {
     bool escalated = false;
     bool finished = false;
     int status = NSLX_OK;
-    double rel_start = 0.0;       // max_b |r_b| / |b_b| when the current recurrence started
     double max_rel = 0.0;         // final max_b |b - A x|_b / |b_b| (FP64)
     double max_floor_rel = 0.0;
     int64_t floor_stops = 0;
     DevBuf<unsigned char> r, p0, p1, u, xs;   // recurrence vectors (job precision)
     DevBuf<double2> r64, t64;                 // FP64 residual work
     DevBuf<double> bn2, rn2, xn2;             // per-RHS norms
+    // reliable updates of a low-precision recurrence (ru_scalars state)
+    DevBuf<double> best, rstart;
+    DevBuf<int> stall;
+    int64_t ru_it0 = 0, last_upd = 0;
     size_t bytes() const;
 };
 
@@ -84,9 +87,9 @@ class Pool {
     int64_t cap_parts_ = 0;
     Cls cls_[NCLS];
     std::vector<std::unique_ptr<PoolJob>> jobs_;
-    std::vector<int> done_h_;
+    std::vector<int> done_h_, want_h_;
     int64_t iter_ = 1;                 // next global iteration (device counter mirror)
-    unsigned cls_mask_ = 0;            // classes captured in the graph
+    unsigned cls_mask_ = 0;            // classes in the graph
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
     PoolStats st_;
@@ -94,12 +97,14 @@ class Pool {
     static int cls_of(int prec, int tile) { return (prec % 3) * 4 + (tile & 3); }
     int64_t alloc_range(std::vector<char>& used, int64_t n);
     void free_range(std::vector<char>& used, int64_t start, int64_t n);
-    void start_recurrence(PoolJob& j, bool fresh);
+    void start_recurrence(PoolJob& j, bool fresh, bool keep_done);
     void rebuild_class(int c);
     void run_iterations();
     void drop_graph();
     bool finalize(PoolJob& j);   // true if the job is complete
+    void replace_residual(PoolJob& j);
     CgDev cgdev() const;
+    CgDev slice(const PoolJob& j) const;   // cgdev() offset to the job's RHS and partial slots
 };
 
 }  // namespace nslx

@@ -255,6 +255,7 @@ __global__ void k_ru_scalars(const double* rn2, const double* xn2, double floor2
         const int d = cg.done[b];
         if (init ? (d >= 0 && d != RU_ESCALATE) : d >= 0) continue;
         const double s = rn2[b];
+        const int wanted = cg.want[b];
         cg.want[b] = 0;
         if (s <= cg.tol2[b] || (xn2 && s <= floor2 * xn2[b])) {
             cg.done[b] = iter;
@@ -272,9 +273,12 @@ __global__ void k_ru_scalars(const double* rn2, const double* xn2, double floor2
             double rec = 0.0;
             for (int k = 0; k < nparts; ++k) rec += rp[k];
             if (s > 100.0 * rec) { cg.done[b] = RU_ESCALATE; continue; }
-            // stagnation of the true residual over stall_max replacements
+            // stagnation: stall_max replacements this RHS asked for (its
+            // recursive |r| fell by delta) without its true |r|^2 falling
+            // below a quarter of its best value; replacements other RHS
+            // asked for do not count
             if (s < 0.25 * best[b]) { best[b] = s; stall[b] = 0; }
-            else if (++stall[b] >= stall_max) { cg.done[b] = RU_ESCALATE; continue; }
+            else if (wanted && ++stall[b] >= stall_max) { cg.done[b] = RU_ESCALATE; continue; }
             // projected budget: at the average rate so far the RHS would need
             // more low-precision iterations than the phase allows
             const int done_its = iter - it0;

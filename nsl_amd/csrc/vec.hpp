@@ -55,7 +55,8 @@ void ru_accumulate(int low_is_f64, void* x64, void* xs, int64_t nb, int64_t len,
 // (the kernels skip it; the solver finishes it in FP64) when its true |r|^2
 // exceeds 100x the recursive |r|^2 held in rpart[cur] (the low-precision
 // recurrence has lost track of the residual), or when its true |r|^2 has not
-// fallen below a quarter of its best value for stall_max replacements, or
+// fallen below a quarter of its best value over stall_max replacements that
+// it asked for (want[b] set: its recursive |r| fell by delta), or
 // when budget > 0 and, 100 or more iterations after the phase started at
 // iteration it0, its average rate so far (from rstart, the |r|^2 at the
 // phase start) projects more than budget iterations to |r|^2 <= tol^2.
