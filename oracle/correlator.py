@@ -18,8 +18,13 @@ contracted with the walls, corrK[s, t, y] = sum_x wall[s, x] corr[t, y, x],
 and the two-body pools are its projections (for both species and, for the
 M^dagger M pools, the transposed propagator).  The twelve contraction
 channels (TWO_BODY_TABLE) are restated from the reference and read only
-those pools; beyond that they are parity unpinned (no independent value of
-a channel exists without running the reference).
+those pools; their exchange structure is pinned by Fermi antisymmetry
+(tests/test_oracle_core.py::test_two_body_pauli: for any field the eight
+I = 1, S = 1 channels vanish where the two sink or the two source labels
+coincide, the others do not), which a wrong exchange sign, a transposed
+exchange index or a swapped pool pair fails.  The prefactors alpha remain
+parity unpinned (no independent value of a channel exists without running
+the reference).
 """
 
 import math

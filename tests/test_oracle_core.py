@@ -431,3 +431,47 @@ def test_correlator_free_propagator():
             ref[t] += sgn * Minv[tt, :, tsrc, :]
     ref /= 2
     assert (corr - ref).abs().max() < 1e-10
+
+
+def test_two_body_pauli():
+    """Antisymmetry of the two-body channels.  The I = 1, S = 1 channels pair
+    two fermions that the exchange term antisymmetrises (exchange sign -1),
+    so for any field their correlator vanishes where the two sink labels
+    coincide (y = z and band k = l) or the two source labels coincide (w = x
+    and band i = j); the I0S1 / I1S0 channels (exchange sign +1) do not.
+    A wrong exchange sign or a transposed exchange index fails this."""
+    L, Nt, beta = 3, 8, 2.0
+    lat, w, mom = honeycomb_walls(L, [(0, 0), (1, 0), (2, 0)])
+    Nx = lat.sites()
+    phi = torch.as_tensor(rng.real_field(43, 0, (Nt, Nx), 0.6))
+
+    def make(species):
+        o = F.HubbardExp(lat, Nt, beta, species=species)
+        o.populate(phi)
+        return o
+
+    res = correlator.two_body(make, phi, w, mom, 2)
+    b = w.shape[1]
+    anti = [name for name, _, terms in correlator.TWO_BODY_TABLE if all(s == -1 for (_, _, e, s) in terms if e)]
+    assert len(anti) == 8 and all(n.startswith("I1S1") for n in anti)
+    scale = max(float(c.abs().max()) for ch in res.values() for c in ch.values())
+    n_sink = n_src = 0
+    for name, ch in res.items():
+        for (qw, qx, qz, qy), c in ch.items():
+            # c[t, i * b + j, l * b + k]
+            if qz == qy:
+                n_sink += 1
+                sink = torch.stack([c[:, :, k * b + k] for k in range(b)])
+                if name in anti:
+                    assert float(sink.abs().max()) < 1e-12 * scale, (name, (qw, qx, qz, qy))
+                else:
+                    assert float(sink.abs().max()) > 1e-6 * scale, (name, (qw, qx, qz, qy))
+            if qw == qx:
+                n_src += 1
+                src = torch.stack([c[:, i * b + i, :] for i in range(b)])
+                if name in anti:
+                    assert float(src.abs().max()) < 1e-12 * scale, (name, (qw, qx, qz, qy))
+    assert n_sink > 0 and n_src > 0
+    # away from the coincident labels no channel vanishes
+    for name in anti:
+        assert max(float(c.abs().max()) for c in res[name].values()) > 1e-6 * scale, name
