@@ -49,7 +49,6 @@ struct SolverWork {
     std::vector<double> last_rr;                    // |r|^2 per slot after the last iteration
     DevBuf<double> rr_ref, rr_prev, xn2, best, rstart;
     DevBuf<int> stall;
-    DevBuf<unsigned> counter;
     PinBuf<int> status_h;
     PinBuf<double> dbl_h;
     PinBuf<int> int_h, status_h2;

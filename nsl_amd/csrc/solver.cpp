@@ -22,6 +22,7 @@
 namespace nslx {
 
 void SolverWork::ensure_scalars(int64_t nb, int64_t nparts, hipStream_t s) {
+    (void)s;
     // complex scalars of the generic solvers need 2 doubles per slot
     const size_t n = (size_t)std::max<int64_t>(nb, 1);
     rr.ensure(2 * n); alpha.ensure(2 * n); beta.ensure(2 * n); tol2.ensure(n); bn2.ensure(n); rn2.ensure(n);
@@ -31,10 +32,6 @@ void SolverWork::ensure_scalars(int64_t nb, int64_t nparts, hipStream_t s) {
     rpart0.ensure(n * std::max<int64_t>(nparts, 1) + 16);
     rpart1.ensure(n * std::max<int64_t>(nparts, 1) + 16);
     done.ensure(n); status.ensure(4); want.ensure(n);
-    if (!counter.p) {
-        counter.alloc(4);
-        NSLX_HIP(hipMemsetAsync(counter.p, 0, 4 * sizeof(unsigned), s));
-    }
     status_h.ensure(4);
     dbl_h.ensure(8 * n + 16);
     int_h.ensure(n + 4);

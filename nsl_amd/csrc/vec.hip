@@ -42,18 +42,35 @@ __device__ inline void bsum2(double& v0, double& v1, double* red) {
     v0 = a; v1 = b;
 }
 
-// Publish this block's partial (2 doubles) and return true in the last block.
-__device__ inline bool publish(double* part, unsigned* counter, size_t slot, double v0, double v1, int* flag) {
-    if (threadIdx.x == 0) { part[2 * slot] = v0; part[2 * slot + 1] = v1; }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned prev = atomicAdd(counter, 1u);
-        *flag = (prev == gridDim.x - 1) ? 1 : 0;
+// The per-RHS sums are two launches: the chunk kernel writes one partial
+// (2 doubles) per block, k_finish sums them in index order.  (A last-block
+// pass in the same launch needs a device-scope release fence per block, and
+// with one L2 per XCD that fence writes the L2 back: it made these kernels
+// 6-12x slower than their memory traffic.)
+// mode 0: out[b] = sum_k part[b][k].re; 1: out[2b], out[2b+1] = sum_k part[b][k];
+// 2: out[0..1] = sum over b and k (thread-strided over b, then the block sum).
+__global__ void k_finish(const double* __restrict__ part, int nchunk, int nb, double* out, int mode) {
+    __shared__ double red[2 * VNT / 64];
+    if (mode == 2) {
+        double r = 0.0, im = 0.0;
+        for (int bb = threadIdx.x; bb < nb; bb += VNT)
+            for (int k = 0; k < nchunk; ++k) {
+                r += part[2 * ((size_t)bb * nchunk + k)];
+                im += part[2 * ((size_t)bb * nchunk + k) + 1];
+            }
+        bsum2(r, im, red);
+        if (threadIdx.x == 0) { out[0] = r; out[1] = im; }
+        return;
     }
-    __syncthreads();
-    if (*flag) { __threadfence(); return true; }
-    return false;
+    for (int bb = blockIdx.x * VNT + threadIdx.x; bb < nb; bb += gridDim.x * VNT) {
+        double r = 0.0, im = 0.0;
+        for (int k = 0; k < nchunk; ++k) {
+            r += part[2 * ((size_t)bb * nchunk + k)];
+            if (mode == 1) im += part[2 * ((size_t)bb * nchunk + k) + 1];
+        }
+        if (mode == 1) { out[2 * bb] = r; out[2 * bb + 1] = im; }
+        else out[bb] = r;
+    }
 }
 
 __global__ void k_phase(const double2* phi, int64_t n, int sgn, double mut, double2* P, double2* Pinv, float2* P32) {
@@ -73,9 +90,8 @@ __global__ void k_phase(const double2* phi, int64_t n, int sgn, double mut, doub
 }
 
 template <class S>
-__global__ void k_norm2(const void* v, int64_t len, int nchunk, double* part, unsigned* counter, double* out, int nb) {
+__global__ void k_norm2(const void* v, int64_t len, int nchunk, double* part) {
     __shared__ double red[2 * VNT / 64];
-    __shared__ int flag;
     const int b = blockIdx.x / nchunk, c = blockIdx.x % nchunk;
     const size_t base = (size_t)b * len;
     double acc = 0.0;
@@ -85,14 +101,7 @@ __global__ void k_norm2(const void* v, int64_t len, int nchunk, double* part, un
     }
     double dummy = 0.0;
     bsum2(acc, dummy, red);
-    if (publish(part, counter, blockIdx.x, acc, 0.0, &flag)) {
-        for (int bb = threadIdx.x; bb < nb; bb += VNT) {
-            double s = 0.0;
-            for (int k = 0; k < nchunk; ++k) s += part[2 * ((size_t)bb * nchunk + k)];
-            out[bb] = s;
-        }
-        if (threadIdx.x == 0) *counter = 0u;
-    }
+    if (threadIdx.x == 0) { part[2 * (size_t)blockIdx.x] = acc; part[2 * (size_t)blockIdx.x + 1] = 0.0; }
 }
 
 __global__ void k_set_tol2(const double* bn2, double* tol2, int64_t nb, double tol, int relative, int global) {
@@ -104,10 +113,8 @@ __global__ void k_set_tol2(const double* bn2, double* tol2, int64_t nb, double t
     if (b < nb) tol2[b] = tol * tol * (relative ? bn2[b] : 1.0);
 }
 
-__global__ void k_resid64(const double2* bv, const double2* t, double2* r, int64_t len, int nchunk, int nb,
-                          double* part, unsigned* counter, double* rn2) {
+__global__ void k_resid64(const double2* bv, const double2* t, double2* r, int64_t len, int nchunk, double* part) {
     __shared__ double red[2 * VNT / 64];
-    __shared__ int flag;
     const int b = blockIdx.x / nchunk, c = blockIdx.x % nchunk;
     const size_t base = (size_t)b * len;
     double acc = 0.0;
@@ -122,14 +129,7 @@ __global__ void k_resid64(const double2* bv, const double2* t, double2* r, int64
     }
     double dummy = 0.0;
     bsum2(acc, dummy, red);
-    if (publish(part, counter, blockIdx.x, acc, 0.0, &flag)) {
-        for (int bb = threadIdx.x; bb < nb; bb += VNT) {
-            double s = 0.0;
-            for (int k = 0; k < nchunk; ++k) s += part[2 * ((size_t)bb * nchunk + k)];
-            rn2[bb] = s;
-        }
-        if (threadIdx.x == 0) *counter = 0u;
-    }
+    if (threadIdx.x == 0) { part[2 * (size_t)blockIdx.x] = acc; part[2 * (size_t)blockIdx.x + 1] = 0.0; }
 }
 
 __global__ void k_scale_to32(const double2* r64, float2* r32, const double* rn2, const double* otol2, int64_t len, int64_t n) {
@@ -166,10 +166,8 @@ __global__ void k_c64_to_c128(const float2* in, double2* out, int64_t n) {
 }
 
 template <class S>
-__global__ void k_dot(const void* a, const void* bv, int64_t len, int nchunk, int nb, double* part, unsigned* counter,
-                      double* out, int global) {
+__global__ void k_dot(const void* a, const void* bv, int64_t len, int nchunk, double* part) {
     __shared__ double red[2 * VNT / 64];
-    __shared__ int flag;
     const int b = blockIdx.x / nchunk, c = blockIdx.x % nchunk;
     const size_t base = (size_t)b * len;
     double sr = 0.0, si = 0.0;
@@ -182,29 +180,7 @@ __global__ void k_dot(const void* a, const void* bv, int64_t len, int nchunk, in
         }
     }
     bsum2(sr, si, red);
-    if (publish(part, counter, blockIdx.x, sr, si, &flag)) {
-        if (!global) {
-            for (int bb = threadIdx.x; bb < nb; bb += VNT) {
-                double r = 0.0, im = 0.0;
-                for (int k = 0; k < nchunk; ++k) {
-                    r += part[2 * ((size_t)bb * nchunk + k)];
-                    im += part[2 * ((size_t)bb * nchunk + k) + 1];
-                }
-                out[2 * bb] = r;
-                out[2 * bb + 1] = im;
-            }
-        } else {
-            double r = 0.0, im = 0.0;
-            for (int bb = threadIdx.x; bb < nb; bb += VNT)
-                for (int k = 0; k < nchunk; ++k) {
-                    r += part[2 * ((size_t)bb * nchunk + k)];
-                    im += part[2 * ((size_t)bb * nchunk + k) + 1];
-                }
-            bsum2(r, im, red);
-            if (threadIdx.x == 0) { out[0] = r; out[1] = im; }
-        }
-        if (threadIdx.x == 0) *counter = 0u;
-    }
+    if (threadIdx.x == 0) { part[2 * (size_t)blockIdx.x] = sr; part[2 * (size_t)blockIdx.x + 1] = si; }
 }
 
 template <class S>
@@ -304,12 +280,15 @@ inline unsigned grid1(int64_t n, int nt = 256) { return (unsigned)((n + nt - 1) 
 }  // namespace
 
 void VecScratch::ensure(int64_t nb, int64_t len, hipStream_t s) {
+    (void)s;
     const size_t need = (size_t)nb * nchunks(len) * 2 + 16;
     part.ensure(need);
-    if (!counter.p) {
-        counter.alloc(4);
-        NSLX_HIP(hipMemsetAsync(counter.p, 0, 4 * sizeof(unsigned), s));
-    }
+}
+
+static void finish(const double* part, int nchunk, int64_t nb, double* out, int mode, hipStream_t s) {
+    const unsigned g = mode == 2 ? 1u : grid1(nb, VNT);
+    hipLaunchKernelGGL(k_finish, dim3(g), dim3(VNT), 0, s, part, nchunk, (int)nb, out, mode);
+    NSLX_LAUNCH_CHECK();
 }
 
 void phase_tables(const void* phi, int64_t n, int sgn, double mut, void* P64, void* Pinv64, void* P32, hipStream_t s) {
@@ -322,9 +301,10 @@ void phase_tables(const void* phi, int64_t n, int sgn, double mut, void* P64, vo
 void norm2_perb(int f64, const void* v, int64_t nb, int64_t len, double* out, VecScratch& sc, hipStream_t s) {
     sc.ensure(nb, len, s);
     const int nc = nchunks(len);
-    if (f64) hipLaunchKernelGGL(k_norm2<double>, dim3(nb * nc), dim3(VNT), 0, s, v, len, nc, sc.part.p, sc.counter.p, out, (int)nb);
-    else hipLaunchKernelGGL(k_norm2<float>, dim3(nb * nc), dim3(VNT), 0, s, v, len, nc, sc.part.p, sc.counter.p, out, (int)nb);
+    if (f64) hipLaunchKernelGGL(k_norm2<double>, dim3(nb * nc), dim3(VNT), 0, s, v, len, nc, sc.part.p);
+    else hipLaunchKernelGGL(k_norm2<float>, dim3(nb * nc), dim3(VNT), 0, s, v, len, nc, sc.part.p);
     NSLX_LAUNCH_CHECK();
+    finish(sc.part.p, nc, nb, out, 0, s);
 }
 
 void set_tol2(const double* bn2, double* tol2, int64_t nb, double tol, int relative, int global, hipStream_t s) {
@@ -336,8 +316,9 @@ void resid64(const void* b, const void* t, void* r, int64_t nb, int64_t len, dou
     sc.ensure(nb, len, s);
     const int nc = nchunks(len);
     hipLaunchKernelGGL(k_resid64, dim3(nb * nc), dim3(VNT), 0, s, (const double2*)b, (const double2*)t, (double2*)r, len, nc,
-                       (int)nb, sc.part.p, sc.counter.p, rn2);
+                       sc.part.p);
     NSLX_LAUNCH_CHECK();
+    finish(sc.part.p, nc, nb, rn2, 0, s);
 }
 
 void scale_to32(const void* r64, void* r32, const double* rn2, const double* otol2, int64_t nb, int64_t len, hipStream_t s) {
@@ -390,9 +371,10 @@ void dot_perb(int f64, const void* a, const void* bv, int64_t nb, int64_t len, d
               hipStream_t s) {
     sc.ensure(nb, len, s);
     const int nc = nchunks(len);
-    if (f64) hipLaunchKernelGGL(k_dot<double>, dim3(nb * nc), dim3(VNT), 0, s, a, bv, len, nc, (int)nb, sc.part.p, sc.counter.p, out, global);
-    else hipLaunchKernelGGL(k_dot<float>, dim3(nb * nc), dim3(VNT), 0, s, a, bv, len, nc, (int)nb, sc.part.p, sc.counter.p, out, global);
+    if (f64) hipLaunchKernelGGL(k_dot<double>, dim3(nb * nc), dim3(VNT), 0, s, a, bv, len, nc, sc.part.p);
+    else hipLaunchKernelGGL(k_dot<float>, dim3(nb * nc), dim3(VNT), 0, s, a, bv, len, nc, sc.part.p);
     NSLX_LAUNCH_CHECK();
+    finish(sc.part.p, nc, nb, out, global ? 2 : 1, s);
 }
 
 void axpy_perb(int f64, const void* x, const void* y, void* z, const double* cs, double sign, int64_t nb, int64_t len,

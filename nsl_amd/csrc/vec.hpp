@@ -1,7 +1,7 @@
 // Vector (BLAS-1 style) kernels with per-right-hand-side deterministic
 // reductions.  A field batch is nb vectors of len = nt*nx complex elements.
-// Reductions use per-chunk partials and a last-block pass that sums them in
-// index order, so results do not depend on scheduling or on the rest of the
+// Reductions write per-chunk partials and a second launch sums them in index
+// order, so results do not depend on scheduling or on the rest of the
 // batch.
 #pragma once
 
@@ -10,10 +10,9 @@
 
 namespace nslx {
 
-// Scratch for vec reductions: partial sums and a zeroed counter.
+// Scratch for vec reductions: one partial (2 doubles) per chunk of a RHS.
 struct VecScratch {
     DevBuf<double> part;       // nb * nchunk * 2
-    DevBuf<unsigned> counter;  // 1, zero between launches
     void ensure(int64_t nb, int64_t len, hipStream_t s);
 };
 
