@@ -1,0 +1,90 @@
+"""The benchmark workload at full size: honeycomb 12x12 (Nx = 288), Nt = 64,
+beta = 8, U = 3, 16 fields x 8 right-hand sides (128 RHS), on the same seeded
+inputs as bench.py.  FP64 applications against the oracle on every RHS
+(1e-13), the adjoint identity <a, M b> = <M^dag a, b> per RHS (1e-12, no
+oracle involved), and CG on MMdagger to 1e-12 per RHS in FP64 and with the
+BF16x3 recurrence, each solution's true residual recomputed with the oracle's
+own operator."""
+
+import math
+
+import pytest
+import torch
+
+import nsl_amd as nsl
+from oracle import fermion as F, lattice as olat, rng
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+L, NT, BETA, U = 12, 64, 8.0, 3.0
+NF, NR = 16, 8
+OPS = ["M", "Mdagger", "MMdagger", "MdaggerM"]
+
+
+@pytest.fixture(scope="module")
+def workload():
+    lat_d, lat_o = nsl.Lattice.honeycomb(L, L), olat.honeycomb(L, L)
+    Nx = lat_d.sites()
+    std = math.sqrt(BETA / NT * U)
+    phis = [torch.as_tensor(rng.real_field(1000 + f, 0, (NT, Nx), std)) for f in range(NF)]
+    b = torch.stack([torch.as_tensor(rng.complex_normal(2000 + 16 * f + k, 0, (NT, Nx)))
+                     for f in range(NF) for k in range(NR)])
+    fm = nsl.HubbardExp(lat_d, NT, BETA)
+    fm.populate(torch.stack(phis).to(DEV))
+    ofs = []
+    for f in range(NF):
+        o = F.HubbardExp(lat_o, NT, BETA)
+        o.populate(phis[f])
+        ofs.append(o)
+    return fm, ofs, b
+
+
+def oracle_apply(ofs, name, psi):
+    return torch.stack([getattr(ofs[k // NR], name)(psi[k]) for k in range(psi.shape[0])])
+
+
+def test_inputs_are_the_bench_inputs(workload):
+    """bench.py draws the same Philox streams through the library (its
+    Box-Muller differs from numpy's in the last bits only)."""
+    import bench
+    _, _, b = workload
+    phi_b, b_b = bench.fields()
+    assert phi_b.shape == (NF, NT, 2 * L * L) and b_b.shape == tuple(b.shape)
+    assert float((torch.as_tensor(b_b) - b).abs().max()) < 1e-14
+
+
+def test_fullsize_apply_parity(workload):
+    fm, ofs, b = workload
+    for name in OPS:
+        got = getattr(fm, name)(b.to(DEV)).cpu()
+        ref = oracle_apply(ofs, name, b)
+        err = ((got - ref).abs().amax(dim=(1, 2)) / ref.abs().amax(dim=(1, 2))).max()
+        assert float(err) < 1e-13, name
+
+
+def test_fullsize_adjoint_identity(workload):
+    fm, _, b = workload
+    a = torch.as_tensor(rng.complex_normal(3000, 0, tuple(b.shape))).to(DEV)
+    bd = b.to(DEV)
+    lhs = (a.conj() * fm.M(bd)).sum(dim=(1, 2))
+    rhs = (fm.Mdagger(a).conj() * bd).sum(dim=(1, 2))
+    scale = a.abs().pow(2).sum(dim=(1, 2)).sqrt() * bd.abs().pow(2).sum(dim=(1, 2)).sqrt()
+    assert float(((lhs - rhs).abs() / scale).max()) < 1e-12
+    # MMdagger is M after Mdagger
+    mm = fm.MMdagger(bd)
+    comp = fm.M(fm.Mdagger(bd))
+    assert float((mm - comp).abs().max() / comp.abs().max()) < 1e-13
+
+
+@pytest.mark.parametrize("prec", [nsl.F64, nsl.BF16X3])
+def test_fullsize_cg_true_residual(workload, prec):
+    fm, ofs, b = workload
+    cg = nsl.CG(fm, nsl.MMdagger, 1e-12, 200000, precision=prec, tol_kind=nsl.TOL_REL_PER_RHS)
+    x = cg(b.to(DEV))
+    assert cg.converged, cg.report
+    assert cg.report["converged_rhs"] == NF * NR
+    target = max(1e-12, cg.report["max_floor_rel"])     # attainable FP64 accuracy
+    assert cg.report["max_rel_residual"] <= target
+    r = b - oracle_apply(ofs, "MMdagger", x.cpu())
+    rel = r.flatten(1).norm(dim=1) / b.flatten(1).norm(dim=1)
+    assert float(rel.max()) < 2 * target
