@@ -607,7 +607,7 @@ __global__ void __launch_bounds__(TL::NT) hop_grouped(GroupArgs g) {
     constexpr int NT = TL::NT;
     __shared__ __attribute__((aligned(16))) Smem<TL, PR> sm;
     __shared__ double red[NT / 64];
-    const int it = g.iter_dev ? *g.iter_dev : g.iter;
+    const int it = g.iter_dev ? *g.iter_dev + g.iter : g.iter;
     const int ntiles = g.ntiles_dev ? *g.ntiles_dev : g.ntiles;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int4 T = g.tiles[t];

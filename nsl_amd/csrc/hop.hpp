@@ -96,8 +96,9 @@ struct GroupArgs {
     signed char role[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
     CgDev cg;
     int iter = 0;
-    // when set, the iteration number and the tile count are read from device
-    // memory (so that one captured launch sequence can be replayed)
+    // when set, the iteration number is *iter_dev + iter and the tile count is
+    // read from device memory (so that one launch sequence built as a graph
+    // can be replayed, iter then being the iteration's offset in the sequence)
     const int* iter_dev = nullptr;
     const int* ntiles_dev = nullptr;
 };

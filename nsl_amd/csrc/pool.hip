@@ -4,9 +4,10 @@
 // and of tile-partial slots.  A global iteration counter k lives on the
 // device: iteration k is one grouped CGA launch (M^dag hop with p = r + beta
 // p, |u|^2 partials) and one grouped CGB launch (M hop with x += alpha p,
-// r -= alpha t, |r|^2 partials) per (precision, tile) class, followed by a
-// one-thread kernel that advances k.  The kernels read k and the tile count
-// from device memory, so the launch sequence of P iterations is built once
+// r -= alpha t, |r|^2 partials) per (precision, tile) class.  The kernels read
+// the base iteration and the tile count from device memory (iteration = base
+// + the launch's offset in the sequence; one one-thread kernel advances the
+// base by P after the P iterations), so the launch sequence of P iterations is built once
 // as a HIP graph and replayed; jobs join and leave between replays by
 // rewriting the device tile lists.  A job joining before iteration k puts its
 // first search direction in p[(k-1)&1], its |r|^2 in rpart[(k-1)&1] and zeros
@@ -36,8 +37,8 @@
 namespace nslx {
 namespace {
 
-__global__ void k_inc(int* it) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *it += 1;
+__global__ void k_advance(int* it, int n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *it += n;
 }
 
 // Slot initialisation of a job joining before iteration k.  With keep, RHS
@@ -304,8 +305,8 @@ void Pool::run_iterations() {
     for (int c = 0; c < NCLS; ++c)
         if (cls_[c].njobs > 0) mask |= 1u << c;
     if (!mask) return;
-    // the launch sequence: hop(c, pass, args) per grouped launch, inc() per iteration
-    auto sequence = [&](auto&& hop, auto&& inc) {
+    // the launch sequence: hop(c, pass, args) per grouped launch, then advance()
+    auto sequence = [&](auto&& hop, auto&& advance) {
         const CgDev cg = cgdev();
         const double delta = env_double("NSLX_RU_DELTA", 0.1);
         for (int k = 0; k < P_; ++k) {
@@ -322,6 +323,7 @@ void Pool::run_iterations() {
                     g.ntiles = (int)std::max<size_t>(1, cls_[c].cap_tiles);
                     g.ntiles_dev = cls_[c].ntiles.p;
                     g.iter_dev = it_dev_.p;
+                    g.iter = k;
                     g.cg = cg;
                     if (c / 4 != NSLX_F64) {   // low precision: reliable-update flags
                         g.cg.ru = 1;
@@ -335,8 +337,8 @@ void Pool::run_iterations() {
                     hop(c, pass, g);
                 }
             }
-            inc();
         }
+        advance();
     };
     auto op_of = [](int pass) { return pass == 0 ? HOP_MD : HOP_M; };
     auto mode_of = [](int pass) { return pass == 0 ? MODE_CGA : MODE_CGB; };
@@ -345,8 +347,8 @@ void Pool::run_iterations() {
             drop_graph();
             NSLX_HIP(hipGraphCreate(&graph_, 0));
             hipGraphNode_t last = nullptr;
-            auto add = [&](const void* fn, unsigned grid, unsigned block, void* arg) {
-                void* params[1] = {arg};   // argument values are copied into the node
+            // params: pointers to the argument values, which are copied into the node
+            auto add = [&](const void* fn, unsigned grid, unsigned block, void** params) {
                 hipKernelNodeParams kp{};
                 kp.func = const_cast<void*>(fn);
                 kp.gridDim = dim3(grid);
@@ -359,13 +361,18 @@ void Pool::run_iterations() {
                 last = n;
             };
             int* itp = it_dev_.p;
+            int pn = P_;
             sequence(
                 [&](int c, int pass, GroupArgs& g) {
                     NSLX_REQUIRE(g.nprob > 0 && cls_[c].grid > 0, "pool: empty grouped launch");
                     const GroupedKernel gk = hop_grouped_kernel(c / 4, op_of(pass), mode_of(pass), c % 4);
-                    add(gk.fn, (unsigned)cls_[c].grid, (unsigned)gk.nthreads, &g);
+                    void* params[1] = {&g};
+                    add(gk.fn, (unsigned)cls_[c].grid, (unsigned)gk.nthreads, params);
                 },
-                [&]() { add(reinterpret_cast<const void*>(&k_inc), 1, 64, &itp); });
+                [&]() {
+                    void* params[2] = {&itp, &pn};
+                    add(reinterpret_cast<const void*>(&k_advance), 1, 64, params);
+                });
             NSLX_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
             cls_mask_ = mask;
         }
@@ -376,7 +383,7 @@ void Pool::run_iterations() {
                 launch_hop_grouped(c / 4, op_of(pass), mode_of(pass), c % 4, g, cls_[c].grid, s_);
             },
             [&]() {
-                hipLaunchKernelGGL(k_inc, dim3(1), dim3(64), 0, s_, it_dev_.p);
+                hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s_, it_dev_.p, P_);
                 NSLX_LAUNCH_CHECK();
             });
     }
