@@ -126,8 +126,9 @@ int nslx_fm_create(nslx_ctx ctx, int nx, int nt, const double* K, double beta, d
  * nslx_correlator_two_point and the HMC entries take the handle; logdet
  * follows hubbardDiag.tpp:166-190 (i s sum(phi) + log det(1 + prod_t
  * (1 - s delta K) diag(Pinv_{t+1})), stable chunked form, no mu term like the
- * reference) and grad_logdet :193-252.  The batching engine
- * (nslx_batch_solve, nslx_server_*) takes HubbardExp matrices only. */
+ * reference) and grad_logdet :193-252.  nslx_batch_solve takes HubbardDiag
+ * handles too (mixed with HubbardExp ones); the server builds HubbardExp
+ * matrices from its requests. */
 int nslx_fm_create_diag(nslx_ctx ctx, int nx, int nt, const double* K, double beta, double mu,
                         int species, nslx_fm* out);
 int nslx_fm_destroy(nslx_fm fm);
@@ -213,9 +214,10 @@ int nslx_solve_precond(nslx_fm fm, nslx_fm fm_prec, int op, const void* b, void*
  * No single counterpart in the reference (it solves one HubbardExp at a
  * time, CG.tpp:14-113); this is the reference's CG on MMdagger for many
  * fermion matrices at once.  Each item is a populated fm (own lattice, Nt,
- * beta, fields) with its own b and x (device c128, nbatch x nt x nx); all fms
- * must belong to ctx.  Every Krylov iteration is one grouped persistent hop
- * launch per phase over the tiles of all items (per precision / tile class);
+ * beta, fields; HubbardExp or HubbardDiag) with its own b and x (device c128,
+ * nbatch x nt x nx); all fms must belong to ctx.  Every Krylov iteration is one
+ * grouped persistent hop launch per phase over the tiles of all items (per
+ * discretisation / precision / tile class);
  * with use_graph the iteration loop is built once as a HIP graph and
  * replayed.  Per-RHS stop: |b - A x| <= tol |b| in FP64 or the FP64 attainable
  * accuracy (c eps ||A|| |x|).  precision F32 / BF16X3 runs the low-precision

@@ -602,7 +602,7 @@ __global__ void __launch_bounds__(TL::NT) hop_kernel(HopArgs a, int nrt, int nct
 // Persistent grouped launch: workgroups stride over a device tile list that
 // spans many problems (each with its own nx, nt, E, phases and vectors).
 // Per-RHS decisions only (no batch-global mode).
-template <class TL, class PR, int OP, int MODE>
+template <class TL, class PR, int OP, int MODE, int DISC>
 __global__ void __launch_bounds__(TL::NT) hop_grouped(GroupArgs g) {
     constexpr int NT = TL::NT;
     __shared__ __attribute__((aligned(16))) Smem<TL, PR> sm;
@@ -630,8 +630,8 @@ __global__ void __launch_bounds__(TL::NT) hop_grouped(GroupArgs g) {
             auto v = [&](int r) -> void* { return r >= 0 ? P.v[r] : nullptr; };
             const JobPtrs J{P.E, P.P, v(g.role[0]), v(g.role[1]), v(g.role[2]), v(g.role[3]),
                             v(g.role[4]), v(g.role[5]), v(g.role[6])};
-            part = hop_tile<TL, PR, OP, MODE, DISC_EXP>(J, P.nx, P.nt, P.rpf, P.ldE, (size_t)P.planeE, b, rt, ct, alpha,
-                                              beta, sm);
+            part = hop_tile<TL, PR, OP, MODE, DISC>(J, P.nx, P.nt, P.rpf, P.ldE, (size_t)P.planeE, b, rt, ct, alpha,
+                                          beta, sm);
         }
         if (MODE != MODE_PLAIN && !skip) {
             const double tot = block_sum<NT>(part, red);
@@ -1131,12 +1131,19 @@ void dispatch4(int tile, int op, int mode, const HopArgs& a, hipStream_t s) {
     }
 }
 
+// Grouped HubbardDiag launches exist for the CG pair the batching engine
+// runs (M^dagger CGA, M CGB); HubbardExp for every op and mode.
 template <class TL, class PR>
-void launch_g(int op, int mode, const GroupArgs& g, int grid, hipStream_t s) {
+void launch_g(int op, int mode, int disc, const GroupArgs& g, int grid, hipStream_t s) {
     dim3 gr(grid), bl(TL::NT);
+    if (disc == DISC_DIAG) {
+        if (op == HOP_MD && mode == MODE_CGA) { launch_timed(hop_grouped<TL, PR, HOP_MD, MODE_CGA, DISC_DIAG>, gr, bl, 0, s, g); return; }
+        if (op == HOP_M && mode == MODE_CGB) { launch_timed(hop_grouped<TL, PR, HOP_M, MODE_CGB, DISC_DIAG>, gr, bl, 0, s, g); return; }
+        throw Error(NSLX_E_ARG, "hop: grouped HubbardDiag launches are M^dagger CGA and M CGB");
+    }
 #define NSLX_HOPG_CASE(OPV, MV)                                                    \
     if (op == OPV && mode == MV) {                                                 \
-        launch_timed(hop_grouped<TL, PR, OPV, MV>, gr, bl, 0, s, g);                \
+        launch_timed(hop_grouped<TL, PR, OPV, MV, DISC_EXP>, gr, bl, 0, s, g);      \
         return;                                                                    \
     }
     NSLX_HOPG_CASE(HOP_M, MODE_PLAIN)
@@ -1152,9 +1159,16 @@ void launch_g(int op, int mode, const GroupArgs& g, int grid, hipStream_t s) {
 }
 
 template <class TL, class PR>
-GroupedKernel kernel_g(int op, int mode) {
+GroupedKernel kernel_g(int op, int mode, int disc) {
+    if (disc == DISC_DIAG) {
+        if (op == HOP_MD && mode == MODE_CGA)
+            return {reinterpret_cast<const void*>(&hop_grouped<TL, PR, HOP_MD, MODE_CGA, DISC_DIAG>), TL::NT};
+        if (op == HOP_M && mode == MODE_CGB)
+            return {reinterpret_cast<const void*>(&hop_grouped<TL, PR, HOP_M, MODE_CGB, DISC_DIAG>), TL::NT};
+        throw Error(NSLX_E_ARG, "hop: grouped HubbardDiag launches are M^dagger CGA and M CGB");
+    }
 #define NSLX_HOPG_FN(OPV, MV)                                                                  \
-    if (op == OPV && mode == MV) return {reinterpret_cast<const void*>(&hop_grouped<TL, PR, OPV, MV>), TL::NT};
+    if (op == OPV && mode == MV) return {reinterpret_cast<const void*>(&hop_grouped<TL, PR, OPV, MV, DISC_EXP>), TL::NT};
     NSLX_HOPG_FN(HOP_M, MODE_PLAIN)
     NSLX_HOPG_FN(HOP_MD, MODE_PLAIN)
     NSLX_HOPG_FN(HOP_M, MODE_CGA)
@@ -1168,45 +1182,45 @@ GroupedKernel kernel_g(int op, int mode) {
 }
 
 template <class PR, class T0, class T1, class T2, class T3>
-GroupedKernel kernel4g(int tile, int op, int mode) {
+GroupedKernel kernel4g(int tile, int op, int mode, int disc) {
     switch (tile) {
-        case 0: return kernel_g<T0, PR>(op, mode);
-        case 1: return kernel_g<T1, PR>(op, mode);
-        case 2: return kernel_g<T2, PR>(op, mode);
-        case 3: return kernel_g<T3, PR>(op, mode);
+        case 0: return kernel_g<T0, PR>(op, mode, disc);
+        case 1: return kernel_g<T1, PR>(op, mode, disc);
+        case 2: return kernel_g<T2, PR>(op, mode, disc);
+        case 3: return kernel_g<T3, PR>(op, mode, disc);
         default: throw Error(NSLX_E_ARG, "hop: bad tile id");
     }
 }
 
 template <class PR, class T0, class T1, class T2, class T3>
-void dispatch4g(int tile, int op, int mode, const GroupArgs& g, int grid, hipStream_t s) {
+void dispatch4g(int tile, int op, int mode, int disc, const GroupArgs& g, int grid, hipStream_t s) {
     switch (tile) {
-        case 0: launch_g<T0, PR>(op, mode, g, grid, s); break;
-        case 1: launch_g<T1, PR>(op, mode, g, grid, s); break;
-        case 2: launch_g<T2, PR>(op, mode, g, grid, s); break;
-        case 3: launch_g<T3, PR>(op, mode, g, grid, s); break;
+        case 0: launch_g<T0, PR>(op, mode, disc, g, grid, s); break;
+        case 1: launch_g<T1, PR>(op, mode, disc, g, grid, s); break;
+        case 2: launch_g<T2, PR>(op, mode, disc, g, grid, s); break;
+        case 3: launch_g<T3, PR>(op, mode, disc, g, grid, s); break;
         default: throw Error(NSLX_E_ARG, "hop: bad tile id");
     }
 }
 
 }  // namespace
 
-void launch_hop_grouped(int prec, int op, int mode, int tile, const GroupArgs& g, int grid, hipStream_t s) {
+void launch_hop_grouped(int prec, int op, int mode, int tile, int disc, const GroupArgs& g, int grid, hipStream_t s) {
     NSLX_REQUIRE(g.nprob > 0 && g.ntiles > 0 && grid > 0, "hop: empty grouped launch");
     NSLX_REQUIRE(tile >= 0 && tile < SLICE_BASE, "hop: grouped launches take tiles 0-3");
     ProfScope prof(op == HOP_M ? "grp_M" : "grp_Mdag", prec, mode, s);
-    if (prec == NSLX_F64) dispatch4g<PF64, T64_a, T64_b, T64_c, T64_d>(tile, op, mode, g, grid, s);
-    else if (prec == NSLX_F32) dispatch4g<PF32, T32_a, T32_b, T32_c, T32_d>(tile, op, mode, g, grid, s);
-    else if (prec == NSLX_BF16X3) dispatch4g<PB3, TB3_a, TB3_b, TB3_c, TB3_d>(tile, op, mode, g, grid, s);
+    if (prec == NSLX_F64) dispatch4g<PF64, T64_a, T64_b, T64_c, T64_d>(tile, op, mode, disc, g, grid, s);
+    else if (prec == NSLX_F32) dispatch4g<PF32, T32_a, T32_b, T32_c, T32_d>(tile, op, mode, disc, g, grid, s);
+    else if (prec == NSLX_BF16X3) dispatch4g<PB3, TB3_a, TB3_b, TB3_c, TB3_d>(tile, op, mode, disc, g, grid, s);
     else throw Error(NSLX_E_ARG, "hop: bad precision");
     NSLX_LAUNCH_CHECK();
 }
 
-GroupedKernel hop_grouped_kernel(int prec, int op, int mode, int tile) {
+GroupedKernel hop_grouped_kernel(int prec, int op, int mode, int tile, int disc) {
     NSLX_REQUIRE(tile >= 0 && tile < SLICE_BASE, "hop: grouped launches take tiles 0-3");
-    if (prec == NSLX_F64) return kernel4g<PF64, T64_a, T64_b, T64_c, T64_d>(tile, op, mode);
-    if (prec == NSLX_F32) return kernel4g<PF32, T32_a, T32_b, T32_c, T32_d>(tile, op, mode);
-    if (prec == NSLX_BF16X3) return kernel4g<PB3, TB3_a, TB3_b, TB3_c, TB3_d>(tile, op, mode);
+    if (prec == NSLX_F64) return kernel4g<PF64, T64_a, T64_b, T64_c, T64_d>(tile, op, mode, disc);
+    if (prec == NSLX_F32) return kernel4g<PF32, T32_a, T32_b, T32_c, T32_d>(tile, op, mode, disc);
+    if (prec == NSLX_BF16X3) return kernel4g<PB3, TB3_a, TB3_b, TB3_c, TB3_d>(tile, op, mode, disc);
     throw Error(NSLX_E_ARG, "hop: bad precision");
 }
 

@@ -118,8 +118,10 @@ struct TileInfo {
 // Launch one hop.  prec: NSLX_F64 / NSLX_F32 / NSLX_BF16X3; op: HOP_M/HOP_MD;
 // tile: index into tile_menu(prec).  Throws on bad arguments.
 void launch_hop(int prec, int op, int mode, const HopArgs& a, int tile, hipStream_t s);
-// Grouped persistent launch over g.ntiles tiles with `grid` workgroups.
-void launch_hop_grouped(int prec, int op, int mode, int tile, const GroupArgs& g, int grid, hipStream_t s);
+// Grouped persistent launch over g.ntiles tiles with `grid` workgroups; every
+// problem of one launch has the discretisation disc (DISC_DIAG: the CG pair
+// HOP_MD/MODE_CGA and HOP_M/MODE_CGB only).
+void launch_hop_grouped(int prec, int op, int mode, int tile, int disc, const GroupArgs& g, int grid, hipStream_t s);
 // The kernel launch_hop_grouped would launch (function and workgroup size),
 // for building graph kernel nodes; it takes one GroupArgs by value and no
 // dynamic LDS.
@@ -127,7 +129,7 @@ struct GroupedKernel {
     const void* fn;
     int nthreads;
 };
-GroupedKernel hop_grouped_kernel(int prec, int op, int mode, int tile);
+GroupedKernel hop_grouped_kernel(int prec, int op, int mode, int tile, int disc);
 // Host-side tile list for problems (nrt/nct filled in from the tile shape):
 // problems with larger nx first (longer K loops), and the column tiles of a
 // row block 8 entries apart so that a persistent grid that is a multiple of 8

@@ -98,7 +98,7 @@ Pool::Pool(Ctx* ctx, hipStream_t s, int64_t cap_rhs, int use_graph, int iters_pe
     want_h_.assign(cap_rhs, 0);
     const int ncu = ctx->num_cu > 0 ? ctx->num_cu : 256;
     for (int c = 0; c < NCLS; ++c) {
-        const int tile = c % 4;
+        const int tile = tile_of(c);
         const int per_cu = tile == 2 ? 8 : 4;
         cls_[c].grid = (ncu * per_cu + 7) / 8 * 8;
         cls_[c].ntiles.alloc(1);
@@ -198,7 +198,7 @@ void Pool::start_recurrence(PoolJob& j, bool fresh, bool keep_done) {
     int tile = fm.tile(j.prec, j.nb);
     if (tile >= 4 || !tile_fits(j.prec, tile, fm.nx)) tile = default_tile(j.prec, fm.nx, fm.nt, j.nb);
     j.tile = tile;
-    j.cls = cls_of(j.prec, tile);
+    j.cls = cls_of(j.prec, tile, fm.disc);
     j.nparts = hop_nparts(j.prec, tile, fm.nx, fm.nt);
     j.part0 = alloc_range(part_used_, j.nb * j.nparts);
     NSLX_REQUIRE(j.part0 >= 0, "pool: out of partial-sum slots");
@@ -273,8 +273,8 @@ void Pool::rebuild_class(int c) {
     K.njobs = (int)probs.size();
     std::vector<int4> tiles;
     if (!probs.empty()) {
-        const int tile = c % 4;
-        const int prec = c / 4;
+        const int tile = tile_of(c);
+        const int prec = prec_of(c);
         tiles = build_tile_list(probs, prec, tile);
     }
     if (probs.size() > K.cap_probs) {
@@ -325,7 +325,7 @@ void Pool::run_iterations() {
                     g.iter_dev = it_dev_.p;
                     g.iter = k;
                     g.cg = cg;
-                    if (c / 4 != NSLX_F64) {   // low precision: reliable-update flags
+                    if (prec_of(c) != NSLX_F64) {   // low precision: reliable-update flags
                         g.cg.ru = 1;
                         g.cg.delta2 = delta * delta;
                     }
@@ -365,7 +365,7 @@ void Pool::run_iterations() {
             sequence(
                 [&](int c, int pass, GroupArgs& g) {
                     NSLX_REQUIRE(g.nprob > 0 && cls_[c].grid > 0, "pool: empty grouped launch");
-                    const GroupedKernel gk = hop_grouped_kernel(c / 4, op_of(pass), mode_of(pass), c % 4);
+                    const GroupedKernel gk = hop_grouped_kernel(prec_of(c), op_of(pass), mode_of(pass), tile_of(c), disc_of(c));
                     void* params[1] = {&g};
                     add(gk.fn, (unsigned)cls_[c].grid, (unsigned)gk.nthreads, params);
                 },
@@ -380,7 +380,7 @@ void Pool::run_iterations() {
     } else {
         sequence(
             [&](int c, int pass, GroupArgs& g) {
-                launch_hop_grouped(c / 4, op_of(pass), mode_of(pass), c % 4, g, cls_[c].grid, s_);
+                launch_hop_grouped(prec_of(c), op_of(pass), mode_of(pass), tile_of(c), disc_of(c), g, cls_[c].grid, s_);
             },
             [&]() {
                 hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s_, it_dev_.p, P_);
@@ -530,7 +530,6 @@ int batch_solve(Ctx& ctx, int64_t nitems, const nslx_batch_item* items, double t
     for (int64_t i = 0; i < nitems; ++i) {
         NSLX_REQUIRE(items[i].fm && items[i].b && items[i].x && items[i].nbatch >= 1, "batch_solve: bad item");
         NSLX_REQUIRE(&items[i].fm->m.ctx->tiles == &ctx.tiles, "batch_solve: every fm must belong to ctx");
-        NSLX_REQUIRE(items[i].fm->m.disc == DISC_EXP, "batch_solve: the grouped kernels take HubbardExp matrices");
         cap += items[i].nbatch;
     }
     const auto t0 = std::chrono::steady_clock::now();

@@ -66,7 +66,8 @@ class Pool {
     const PoolStats& stats() const { return st_; }
 
    private:
-    static constexpr int NCLS = 12;   // (F64, F32, BF16X3) x classic tiles 0-3
+    // class = (discretisation, precision, classic tile 0-3): (disc * 3 + prec) * 4 + tile
+    static constexpr int NCLS = 24;
     struct Cls {
         DevBuf<HopProblem> probs;
         DevBuf<int4> tiles;
@@ -94,7 +95,10 @@ class Pool {
     hipGraphExec_t exec_ = nullptr;
     PoolStats st_;
 
-    static int cls_of(int prec, int tile) { return (prec % 3) * 4 + (tile & 3); }
+    static int cls_of(int prec, int tile, int disc) { return ((disc & 1) * 3 + prec % 3) * 4 + (tile & 3); }
+    static int prec_of(int c) { return (c / 4) % 3; }
+    static int tile_of(int c) { return c % 4; }
+    static int disc_of(int c) { return c / 12; }
     int64_t alloc_range(std::vector<char>& used, int64_t n);
     void free_range(std::vector<char>& used, int64_t start, int64_t n);
     void start_recurrence(PoolJob& j, bool fresh, bool keep_done);

@@ -2,7 +2,7 @@
 restatement of hubbardDiag.tpp: M, M^dagger, MM^dagger, M^dagger M in every
 precision and every classic tile, the fused CG (reference semantics and
 reliable updates), preconditioned CG, log det and force (both species), the
-two-point correlator, HMC Delta H, and the batching engine's refusal.
+two-point correlator, HMC Delta H, and the batching engine.
 Tolerances as for HubbardExp: 1e-13 relative (FP64), 2e-6 (FP32), 4e-5
 (BF16x3) for applications."""
 
@@ -188,8 +188,31 @@ def test_diag_hmc_delta_h():
     assert relerr(phi, outs[-1]["phi"]) < 1e-9
 
 
-def test_diag_refused_by_batching_engine():
-    fm, _, _, _, _ = make("honeycomb", (3, 3), 16, 4.0)
-    b = torch.as_tensor(rng.complex_normal(1, 0, (2, 16, 18))).to(DEV)
-    with pytest.raises(nsl.NslxError):
-        nsl.batch_solve([(fm, b)], tol=1e-10)
+@pytest.mark.parametrize("prec", [nsl.F64, nsl.F32, nsl.BF16X3])
+def test_diag_batching_engine(prec):
+    """HubbardDiag matrices in the batching engine, mixed with a HubbardExp
+    one (separate grouped classes): every solution's true residual with the
+    oracle's own MMdagger, and a HubbardDiag item solved alone gives the same
+    bits as in the mix."""
+    ctx = nsl.Context(0)
+    d1 = make("honeycomb", (3, 3), 16, 4.0, nfield=2, ctx=ctx, seed=5100)
+    d2 = make("nanotube", (6, 2), 20, 3.0, ctx=ctx, seed=5200)
+    lat_e = nsl.Lattice.honeycomb(3, 3)
+    phi_e = torch.as_tensor(rng.real_field(5300, 0, (12, 18), 0.5))
+    fe = nsl.HubbardExp(lat_e, 12, 2.0, ctx=ctx)
+    fe.populate(phi_e.to(DEV))
+    oe = F.HubbardExp(olat.honeycomb(3, 3), 12, 2.0)
+    oe.populate(phi_e)
+    b1 = torch.as_tensor(rng.complex_normal(5101, 0, (4, 16, 18)))
+    b2 = torch.as_tensor(rng.complex_normal(5201, 0, (3, 20, d2[2].sites())))
+    be = torch.as_tensor(rng.complex_normal(5301, 0, (2, 12, 18)))
+    items = [(d1[0], b1.to(DEV)), (fe, be.to(DEV)), (d2[0], b2.to(DEV))]
+    xs, rep = nsl.batch_solve(items, tol=1e-10, precision=prec, ctx=ctx)
+    assert rep["converged"], rep
+    target = max(1e-10, rep["max_floor_rel"])
+    for (ofs, b), x in zip([(d1[1], b1), ([oe], be), (d2[1], b2)], xs):
+        r = b - oracle_apply(ofs, "MMdagger", x.cpu())
+        rel = r.flatten(1).norm(dim=1) / b.flatten(1).norm(dim=1)
+        assert float(rel.max()) < 2 * target
+    alone, _ = nsl.batch_solve([items[2]], tol=1e-10, precision=prec, ctx=ctx)
+    assert torch.equal(alone[0], xs[2])
