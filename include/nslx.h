@@ -127,8 +127,8 @@ int nslx_fm_create(nslx_ctx ctx, int nx, int nt, const double* K, double beta, d
  * follows hubbardDiag.tpp:166-190 (i s sum(phi) + log det(1 + prod_t
  * (1 - s delta K) diag(Pinv_{t+1})), stable chunked form, no mu term like the
  * reference) and grad_logdet :193-252.  nslx_batch_solve takes HubbardDiag
- * handles too (mixed with HubbardExp ones); the server builds HubbardExp
- * matrices from its requests. */
+ * handles too (mixed with HubbardExp ones), and a server request selects its
+ * discretisation (nslx_request.discretisation). */
 int nslx_fm_create_diag(nslx_ctx ctx, int nx, int nt, const double* K, double beta, double mu,
                         int species, nslx_fm* out);
 int nslx_fm_destroy(nslx_fm fm);
@@ -322,6 +322,10 @@ int nslx_correlator_two_body(nslx_fm fm, const void* wall, int kdim, int bdim, i
  * Latency = completion - submit, host clock.  Thread-safe. */
 typedef struct nslx_server_s* nslx_server;
 
+/* Fermion matrix of a request: HubbardExp (hubbardExp.hpp) or HubbardDiag
+ * (hubbardDiag.hpp), as nslx_fm_create / nslx_fm_create_diag. */
+typedef enum { NSLX_DISC_EXP = 0, NSLX_DISC_DIAG = 1 } nslx_discretisation;
+
 typedef struct {
     int64_t max_rhs;            /* RHS slots of the pool (<= 0: 65536)          */
     double  hbm_budget;         /* bytes of device memory for admitted work     */
@@ -339,6 +343,7 @@ typedef struct {
     double        tol;          /* per-RHS relative FP64 residual               */
     int           precision;    /* NSLX_F64 / NSLX_F32 / NSLX_BF16X3            */
     int64_t       maxit;
+    int           discretisation; /* NSLX_DISC_EXP (HubbardExp, 0) or NSLX_DISC_DIAG */
 } nslx_request;
 
 typedef struct {

@@ -392,13 +392,20 @@ class Server:
         self._shapes = {}
 
     def submit(self, lat, Nt, beta, phi, n_tsrc=1, mu=0.0, species=Particle, tol=1e-10, precision=F64,
-               maxIter=100000):
+               maxIter=100000, fermion_matrix=None):
+        """Queue TwoPointCorrelator::measure(n_tsrc) on the fermion matrix
+        fermion_matrix (HubbardExp by default, or HubbardDiag) of lat, Nt,
+        beta, mu, species populated with phi; returns the request id."""
         phi = np.ascontiguousarray(np.asarray(phi.cpu() if hasattr(phi, "cpu") else phi, dtype=np.complex128))
         Nx = lat.sites()
         if phi.shape != (Nt, Nx):
             raise ValueError(f"phi must be (Nt, Nx) = ({Nt}, {Nx})")
+        fmcls = fermion_matrix or HubbardExp
+        if fmcls not in (HubbardExp, HubbardDiag):
+            raise ValueError("fermion_matrix must be HubbardExp or HubbardDiag")
+        disc = _lib.DISC_DIAG if fmcls is HubbardDiag else _lib.DISC_EXP
         req = _lib.Request(Nx, int(Nt), lat.K.ctypes.data_as(_lib.P_double), float(beta), float(mu), int(species),
-                           phi.ctypes.data, int(n_tsrc), float(tol), int(precision), int(maxIter))
+                           phi.ctypes.data, int(n_tsrc), float(tol), int(precision), int(maxIter), disc)
         rid = ctypes.c_int64()
         _lib.check(_lib.lib().nslx_server_submit(self._h, ctypes.byref(req), ctypes.byref(rid)))
         self._shapes[rid.value] = (int(Nt), Nx)

@@ -52,7 +52,7 @@ class ServerConfig(ctypes.Structure):
 class Request(ctypes.Structure):
     _fields_ = [("nx", c_int), ("nt", c_int), ("K", P_double), ("beta", c_double), ("mu", c_double),
                 ("species", c_int), ("phi", c_void_p), ("n_tsrc", c_int64), ("tol", c_double),
-                ("precision", c_int), ("maxit", c_int64)]
+                ("precision", c_int), ("maxit", c_int64), ("discretisation", c_int)]
 
 
 class Result(ctypes.Structure):
@@ -89,6 +89,7 @@ class MarkovState(ctypes.Structure):
 
 
 CHAIN_ALL, CHAIN_LAST = 0, 1
+DISC_EXP, DISC_DIAG = 0, 1
 
 # name -> (restype, argtypes)
 PROTOS = {

@@ -91,6 +91,8 @@ int64_t Server::submit(const nslx_request& r) {
     NSLX_REQUIRE(r.precision == NSLX_F64 || r.precision == NSLX_F32 || r.precision == NSLX_BF16X3,
                  "server_submit: bad precision");
     NSLX_REQUIRE(r.species == NSLX_PARTICLE || r.species == NSLX_HOLE, "server_submit: bad species");
+    NSLX_REQUIRE(r.discretisation == NSLX_DISC_EXP || r.discretisation == NSLX_DISC_DIAG,
+                 "server_submit: bad discretisation");
     auto q = std::make_shared<Request>();
     q->r = r;
     q->K.assign(r.K, r.K + (size_t)r.nx * r.nx);
@@ -125,7 +127,8 @@ int64_t Server::submit(const nslx_request& r) {
 void Server::admit(Request& q) {
     const nslx_request& r = q.r;
     q.fm = std::make_unique<FermionMatrix>();
-    fm_init(*q.fm, ctx_, r.nx, r.nt, q.K.data(), r.beta, r.mu, r.species);
+    fm_init(*q.fm, ctx_, r.nx, r.nt, q.K.data(), r.beta, r.mu, r.species,
+            r.discretisation == NSLX_DISC_DIAG ? DISC_DIAG : DISC_EXP);
     DevBuf<double2> phi((size_t)r.nt * r.nx);
     NSLX_HIP(hipMemcpyAsync(phi.p, q.phi.data(), phi.n * sizeof(double2), hipMemcpyHostToDevice, s_));
     q.fm->populate(phi.p, 1, r.species, s_);   // deep copy; synchronises

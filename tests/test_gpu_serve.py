@@ -45,6 +45,32 @@ def test_server_vs_oracle_cfg0():
     assert relerr(corr, ref) < 1e-9
 
 
+def test_server_hubbard_diag_vs_oracle():
+    """A HubbardDiag request (served in the same pool as a HubbardExp one):
+    against the oracle's correlator on HubbardDiag to 1e-9, and the
+    HubbardExp request against its own oracle."""
+    lat = nsl.Lattice.honeycomb(3, 3)
+    phi = field(lat, 16, 4.0, 2.0, 1001)
+    srv = nsl.Server()
+    try:
+        rd = srv.submit(lat, 16, 4.0, phi, n_tsrc=2, tol=1e-12, fermion_matrix=nsl.HubbardDiag)
+        re_ = srv.submit(lat, 16, 4.0, phi, n_tsrc=2, tol=1e-12)
+        corr_d, res_d = srv.wait(rd)
+        corr_e, res_e = srv.wait(re_)
+    finally:
+        srv.close()
+    assert res_d["status"] == 0 and res_e["status"] == 0, (res_d, res_e)
+    od = F.HubbardDiag(olat.honeycomb(3, 3), 16, 4.0)
+    od.populate(phi)
+    ref_d, _ = ocorr.two_point(od, 16, 18, 2)
+    assert relerr(corr_d, ref_d) < 1e-9
+    oe = F.HubbardExp(olat.honeycomb(3, 3), 16, 4.0)
+    oe.populate(phi)
+    ref_e, _ = ocorr.two_point(oe, 16, 18, 2)
+    assert relerr(corr_e, ref_e) < 1e-9
+    assert relerr(corr_d, corr_e) > 1e-6          # the two discretisations differ
+
+
 def test_server_alone_equals_in_mix():
     lats = [nsl.Lattice.honeycomb(3, 3), nsl.Lattice.honeycomb(6, 6), nsl.Lattice.armchair_nanotube(3, 4),
             nsl.Lattice.ring(10)]
@@ -118,5 +144,18 @@ def test_server_errors():
             srv.submit(lat, 16, 4.0, np.zeros((16, 18)), n_tsrc=17)
         with pytest.raises(nsl.NslxError):                  # larger than the memory budget
             srv.submit(lat, 16, 4.0, np.zeros((16, 18)))
+        with pytest.raises(ValueError):                     # not a fermion matrix class
+            srv.submit(lat, 16, 4.0, np.zeros((16, 18)), fermion_matrix=dict)
+    finally:
+        srv.close()
+    import ctypes
+    from nsl_amd import _lib
+    srv = nsl.Server()
+    try:
+        phi = np.zeros((16, 18), dtype=np.complex128)
+        req = _lib.Request(18, 16, lat.K.ctypes.data_as(_lib.P_double), 4.0, 0.0, 0, phi.ctypes.data, 1, 1e-10,
+                           nsl.F64, 1000, 7)                # no such discretisation
+        rid = ctypes.c_int64()
+        assert _lib.lib().nslx_server_submit(srv._h, ctypes.byref(req), ctypes.byref(rid)) == _lib.E_ARG
     finally:
         srv.close()
