@@ -34,6 +34,12 @@ struct Tile {
     static constexpr int BM = WM * FM * 16, BN = WN * FN * 16, NT = 64 * WM * WN;
 };
 
+// 16 bytes of E in a native vector type: held in HIP's union-based uint4,
+// the E stage of the M^dagger and HubbardDiag instantiations stayed in
+// private memory (64-96 bytes of scratch per lane, one store and one load per
+// K chunk); as a clang vector it is promoted to registers in every variant.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
 struct PF64 {
     using S = double;      // field scalar
     using L = double;      // LDS / MFMA operand scalar
@@ -247,7 +253,7 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
             CS<S> x[APASS][VEC];
             CS<S> po[APASS][VEC];   // CGA: previous search direction
             CS<S> ph[APASS][VEC];   // M: phase
-            uint4 e[PR::DLDS ? 1 : BPASS];
+            u32x4 e[PR::DLDS ? 1 : BPASS];
         };
         static_assert(!PR::DLDS || PR::NBUF == 2, "direct E loads need the second LDS buffer");
         // global loads of chunk k0 (with DLDS the E rows go straight to LDS buffer buf)
@@ -281,7 +287,7 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
                     const int pl = jp / BN, j = jp % BN;
                     const int kk = (tid % TPRB) * VB;
                     const size_t eoff = (size_t)pl * planeE + (size_t)(y0 + j) * ldE + k0 + kk;
-                    g.e[ps] = *reinterpret_cast<const uint4*>(reinterpret_cast<const L*>(J.E) + eoff);
+                    g.e[ps] = *reinterpret_cast<const u32x4*>(reinterpret_cast<const L*>(J.E) + eoff);
                 }
             }
         };
@@ -321,7 +327,7 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
                     const int jp = ps * RPPB + tid / TPRB;
                     const int pl = jp / BN, j = jp % BN;
                     const int kk = (tid % TPRB) * VB;
-                    *reinterpret_cast<uint4*>(&sB[buf][pl][j][kk]) = g.e[ps];
+                    *reinterpret_cast<u32x4*>(&sB[buf][pl][j][kk]) = g.e[ps];
                 }
             }
         };
