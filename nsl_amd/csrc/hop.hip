@@ -926,9 +926,9 @@ __global__ void __launch_bounds__(64 * W) hop_slice(HopArgs a, int nrt, int nct,
         const L* src = reinterpret_cast<const L*>(a.E) + (size_t)pl * planeE;
         for (int i = threadIdx.x; i < BN * vpr; i += NT) {
             const int j = i / vpr, kv = (i - j * vpr) * VL;
-            const uint4 w = kv < ldE ? *reinterpret_cast<const uint4*>(src + (size_t)(y0 + j) * ldE + kv)
-                                     : make_uint4(0, 0, 0, 0);
-            *reinterpret_cast<uint4*>(sE + (size_t)(pl * BN + j) * ld + kv) = w;
+            const u32x4 w = kv < ldE ? *reinterpret_cast<const u32x4*>(src + (size_t)(y0 + j) * ldE + kv)
+                                     : u32x4{0u, 0u, 0u, 0u};
+            *reinterpret_cast<u32x4*>(sE + (size_t)(pl * BN + j) * ld + kv) = w;
         }
     }
     const int nparts = nrt * nct;
