@@ -330,7 +330,7 @@ typedef struct {
     int64_t max_rhs;            /* RHS slots of the pool (<= 0: 65536)          */
     double  hbm_budget;         /* bytes of device memory for admitted work     */
     int     use_graph;          /* replay the iteration loop as a HIP graph     */
-    int     iters_per_replay;   /* even; <= 0: 8                                */
+    int     iters_per_replay;   /* even; <= 0: 32                               */
 } nslx_server_config;
 
 typedef struct {

@@ -383,7 +383,7 @@ class Server:
     precision), wait for their two-point correlators; admission by device
     footprint, requests share every Krylov iteration with what is running."""
 
-    def __init__(self, ctx=None, max_rhs=0, hbm_budget=0.0, use_graph=True, iters_per_replay=8):
+    def __init__(self, ctx=None, max_rhs=0, hbm_budget=0.0, use_graph=True, iters_per_replay=32):
         self.ctx = ctx or Context.default()
         cfg = _lib.ServerConfig(int(max_rhs), float(hbm_budget), int(bool(use_graph)), int(iters_per_replay))
         h = ctypes.c_void_p()

@@ -538,7 +538,7 @@ int batch_solve(Ctx& ctx, int64_t nitems, const nslx_batch_item* items, double t
     NSLX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     nslx_batch_report agg{};
     try {
-        Pool pool(&ctx, s, cap, use_graph, (int)env_double("NSLX_POOL_P", 8.0));
+        Pool pool(&ctx, s, cap, use_graph, (int)env_double("NSLX_POOL_P", 32.0));
         for (int64_t i = 0; i < nitems; ++i) {
             auto j = std::make_unique<PoolJob>();
             j->id = i;

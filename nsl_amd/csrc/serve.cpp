@@ -64,7 +64,7 @@ double footprint(const nslx_request& r) {
 
 Server::Server(Ctx* ctx, const nslx_server_config& cfg) : ctx_(ctx), cfg_(cfg) {
     if (cfg_.max_rhs <= 0) cfg_.max_rhs = 65536;
-    if (cfg_.iters_per_replay <= 0) cfg_.iters_per_replay = 8;
+    if (cfg_.iters_per_replay <= 0) cfg_.iters_per_replay = 32;
     if (!(cfg_.hbm_budget > 0)) cfg_.hbm_budget = 64.0 * (1ull << 30);
     NSLX_HIP(hipSetDevice(ctx_->device));
     NSLX_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));

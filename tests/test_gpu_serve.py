@@ -84,7 +84,7 @@ def test_server_alone_equals_in_mix():
         corr_alone, res_alone = srv.wait(srv.submit(lat, Nt, beta, phi_mine, n_tsrc=nts, tol=1e-11))
     finally:
         srv.close()
-    srv = nsl.Server(iters_per_replay=8)
+    srv = nsl.Server()            # same replay length: reported iterations count whole replays
     try:
         ids = {}
         for k, (lt, nt, bt, u, n) in enumerate(reqs):
