@@ -40,7 +40,7 @@ void SolverWork::ensure_scalars(int64_t nb, int64_t nparts, hipStream_t s) {
 namespace {
 
 inline int poll_every() {
-    static const int v = (int)env_double("NSLX_POLL", 8.0);
+    static const int v = (int)env_double("NSLX_POLL", 16.0);
     return std::max(1, v);
 }
 
