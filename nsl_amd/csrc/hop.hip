@@ -407,23 +407,38 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
         for (int c = 0; c + 1 < nch; ++c) {
             // the next chunk's global loads are in flight during this chunk's
             // MFMAs: the scheduling barrier keeps global loads above the MFMAs
-            // (VALU, SALU and LDS reads may cross; mask 0x146); the LDS writes
-            // of the next chunk are placed among the last MFMAs (F64, BF16x3)
-            // or after them (mask 0x146 again).  With DLDS the E rows of the next
+            // (VALU, SALU and LDS reads may cross; mask 0x146; for F64 the
+            // loads go between the first MFMAs instead); the LDS writes of the
+            // next chunk are placed among the last MFMAs (F64, BF16x3) or after
+            // them (mask 0x146 again).  With DLDS the E rows of the next
             // chunk go to the other buffer, which every wave has finished
             // reading (barrier at the end of the previous iteration).
             const int nxt = PR::NBUF == 2 ? ((c + 1) & 1) : 0;
             load(stg, (c + 1) * BK, nxt);
-            __builtin_amdgcn_sched_barrier(0x146);
+            if constexpr (!std::is_same<PR, PF64>::value) __builtin_amdgcn_sched_barrier(0x146);
             mfma(PR::NBUF == 2 ? (c & 1) : 0);
             if (PR::NBUF == 1) __syncthreads();   // all waves done reading the only buffer
             store(stg, nxt);
             if constexpr (std::is_same<PR, PF64>::value && PR::NBUF == 2) {
-                // F64: the LDS writes of the next chunk (2 A and 3 E rows of
-                // 16 bytes per lane) go between the last MFMAs of this one
-                // rather than after them: 16 MFMAs, then (1 write, 2 MFMAs) x 4
-                // and the last write.  CG kernels 2-3 % faster.
-                __builtin_amdgcn_sched_group_barrier(0x008, 16, 0);
+                // F64 (24 MFMAs per chunk): the 7 global loads of the next
+                // chunk one per MFMA at the start, its 5 LDS writes (2 A and 3
+                // E rows of 16 bytes per lane) one per two MFMAs at the end
+                // rather than after them: CG kernels 4-5 % faster.
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 7, 0);
                 __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
                 __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
                 __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
@@ -433,6 +448,7 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
                 __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
                 __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
                 __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
             } else if constexpr (std::is_same<PR, PB3>::value && PR::NBUF == 2) {
                 // BF16x3 (18 MFMAs, 4 A and 3 E writes per chunk): plain M 3 %
                 // faster.  The same for F32 was 1 % slower: not used there.
