@@ -1340,6 +1340,9 @@ int hop_nparts(int prec, int tile, int nx, int nt) {
 int default_tile(int prec, int nx, int nt, int64_t nb) {
     (void)prec;
     if (nx <= 32) return 2;
+    // fewer than 512 workgroups with the 32 x 96 tile: the 16 x 32 one
+    // (6x the workgroups, and no half-empty row tiles at Nt = 16)
+    if (nb * ((nt + 31) / 32) * ((nx + 95) / 96) < 512) return 2;
     const int64_t rows = nb * nt;
     // enough workgroups to cover 256 CUs at least twice with the big tile?
     const int64_t tiles_a = ((rows + 63) / 64) * ((nx + 95) / 96);
