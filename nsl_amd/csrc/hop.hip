@@ -407,20 +407,23 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
         for (int c = 0; c + 1 < nch; ++c) {
             // the next chunk's global loads are in flight during this chunk's
             // MFMAs: the scheduling barrier keeps global loads above the MFMAs
-            // (VALU, SALU and LDS reads may cross; mask 0x146; for F64 the
-            // loads go between the first MFMAs instead); the LDS writes of the
-            // next chunk are placed among the last MFMAs (F64, BF16x3) or after
-            // them (mask 0x146 again).  With DLDS the E rows of the next
+            // (VALU, SALU and LDS reads may cross; mask 0x146; for the F64
+            // 32 x 96 tile the loads go between the first MFMAs instead); the
+            // LDS writes of the next chunk are placed among the last MFMAs (F64
+            // 32 x 96 tile, BF16x3) or after them (mask 0x146 again).  On the
+            // other F64 tiles the fixed pattern was slower (16 x 32: batching
+            // leg -3 %, serving p99 +3 %).  With DLDS the E rows of the next
             // chunk go to the other buffer, which every wave has finished
             // reading (barrier at the end of the previous iteration).
             const int nxt = PR::NBUF == 2 ? ((c + 1) & 1) : 0;
             load(stg, (c + 1) * BK, nxt);
-            if constexpr (!std::is_same<PR, PF64>::value) __builtin_amdgcn_sched_barrier(0x146);
+            constexpr bool F64B = std::is_same<PR, PF64>::value && TL::FM == 1 && TL::FN == 3;
+            if constexpr (!F64B) __builtin_amdgcn_sched_barrier(0x146);
             mfma(PR::NBUF == 2 ? (c & 1) : 0);
             if (PR::NBUF == 1) __syncthreads();   // all waves done reading the only buffer
             store(stg, nxt);
-            if constexpr (std::is_same<PR, PF64>::value && PR::NBUF == 2) {
-                // F64 (24 MFMAs per chunk): the 7 global loads of the next
+            if constexpr (F64B && PR::NBUF == 2) {
+                // F64, 32 x 96 tile (24 MFMAs per chunk): the 7 global loads of the next
                 // chunk one per MFMA at the start, its 5 LDS writes (2 A and 3
                 // E rows of 16 bytes per lane) one per two MFMAs at the end
                 // rather than after them: CG kernels 4-5 % faster.
