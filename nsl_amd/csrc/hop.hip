@@ -475,85 +475,200 @@ __device__ double hop_tile(const JobPtrs& J, int nx, int nt, int rpf, int ldE, s
         __syncthreads();   // the LDS is reused by the next tile of a grouped launch
 
         // ---- epilogue: shift sign, identity, conj(P), fused recurrences ----
-        #pragma unroll
-        for (int i = 0; i < FM; ++i) {
+        // BF16x3: per accumulator row (i, v), first every load of the row's
+        // FN elements (branch-free, out-of-range elements read g_zero_run),
+        // then the arithmetic and the stores; element by element each load
+        // waits for the previous element's stores (the compiler cannot move a
+        // load above a store to a buffer it may alias): plain BF16x3 applies
+        // 5 % faster, the phi = 0 mixed solve 4 %.  F64 and F32 keep the
+        // element-by-element form: the batched loads took the F64 32 x 96 CG
+        // kernels past the register budget of 3 waves per SIMD (any
+        // rearrangement there measured slower), and the F32 grouped kernels
+        // of the batching leg ran 2 % slower with them.
+        if constexpr (std::is_same<PR, PB3>::value) {
             #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                // accumulator register v of lane (q, r16) holds row 4q+v of the
-                // 16x16 block for the f32/bf16 MFMAs, row 4v+q for f64 (its
-                // output groups have size 1)
-                const int t = t0 + (wm * FM + i) * 16 + (PR::ROW4V ? 4 * v + q : 4 * q + v);
-                if (t >= nt) continue;
-                const size_t rowoff = bbase + (size_t)t * nx;
+            for (int i = 0; i < FM; ++i) {
                 #pragma unroll
-                for (int j = 0; j < FN; ++j) {
-                    const int y = y0 + (wn * FN + j) * 16 + r16;
-                    if (y >= nx) continue;
-                    const size_t idx = rowoff + y;
-                    S fr = S(accR[i][j][v]), fi = S(accI[i][j][v]);
-                    if (DISC == DISC_DIAG) {
-                        // + b_t P_{t-1} psi_{t-1} (M) or + b'_t conj(P_t) psi_{t+1} (M^dag),
-                        // psi = the hop input (r + beta p in CGA)
-                        const int tn = (OP == HOP_M) ? (t == 0 ? nt - 1 : t - 1) : (t + 1 >= nt ? 0 : t + 1);
-                        const size_t nidx = bbase + (size_t)tn * nx + y;
-                        CS<S> xn = ldc<S>(J.X, nidx);
-                        if (MODE == MODE_CGA) {
-                            const CS<S> pn = ldc<S>(J.Pold, nidx);
-                            xn.re = xn.re + S(beta) * pn.re;
-                            xn.im = xn.im + S(beta) * pn.im;
+                for (int v = 0; v < 4; ++v) {
+                    // accumulator register v of lane (q, r16) holds row 4q+v of the
+                    // 16x16 block for the f32/bf16 MFMAs, row 4v+q for f64 (its
+                    // output groups have size 1)
+                    const int t = t0 + (wm * FM + i) * 16 + (PR::ROW4V ? 4 * v + q : 4 * q + v);
+                    const bool tok = t < nt;
+                    const int tc = tok ? t : 0;
+                    const size_t rowoff = bbase + (size_t)tc * nx;
+                    // DIAG: the neighbouring slice of the hop input and its phase
+                    const int tn = (OP == HOP_M) ? (tc == 0 ? nt - 1 : tc - 1) : (tc + 1 >= nt ? 0 : tc + 1);
+                    CS<S> lx[FN], lp[FN], lxs[FN], lr[FN], lph[FN], ln[FN], lpn[FN];
+                    #pragma unroll
+                    for (int j = 0; j < FN; ++j) {
+                        const int y = y0 + (wn * FN + j) * 16 + r16;
+                        const bool ok = tok && y < nx;
+                        const size_t idx = rowoff + y;
+                        lx[j] = ldc_or0<S>(J.X, idx, ok);
+                        if (MODE == MODE_CGA || MODE == MODE_CGB) lp[j] = ldc_or0<S>(J.Pold, idx, ok);
+                        if (MODE == MODE_CGB) {
+                            lxs[j] = ldc_or0<S>(J.Xs, idx, ok);
+                            lr[j] = ldc_or0<S>(J.R, idx, ok);
                         }
-                        const CS<S> ph = ldc<S>(Pf, (size_t)(OP == HOP_M ? tn : t) * nx + y);
-                        S dr, di;
-                        if (OP == HOP_M) {
-                            dr = ph.re * xn.re - ph.im * xn.im;
-                            di = ph.re * xn.im + ph.im * xn.re;
+                        if (MODE == MODE_CGI) lr[j] = ldc_or0<S>(J.Bv, idx, ok);
+                        if (DISC == DISC_DIAG) {
+                            const size_t nidx = bbase + (size_t)tn * nx + y;
+                            ln[j] = ldc_or0<S>(J.X, nidx, ok);
+                            if (MODE == MODE_CGA) lpn[j] = ldc_or0<S>(J.Pold, nidx, ok);
+                            lph[j] = ldc_or0<S>(Pf, (size_t)(OP == HOP_M ? tn : tc) * nx + y, ok);
+                        } else if (OP != HOP_M) {
+                            lph[j] = ldc_or0<S>(Pf, (size_t)tc * nx + y, ok);   // conj(P_t) * acc
+                        }
+                    }
+                    if (!tok) continue;
+                    #pragma unroll
+                    for (int j = 0; j < FN; ++j) {
+                        const int y = y0 + (wn * FN + j) * 16 + r16;
+                        if (y >= nx) continue;
+                        const size_t idx = rowoff + y;
+                        S fr = S(accR[i][j][v]), fi = S(accI[i][j][v]);
+                        if (DISC == DISC_DIAG) {
+                            // + b_t P_{t-1} psi_{t-1} (M) or + b'_t conj(P_t) psi_{t+1} (M^dag),
+                            // psi = the hop input (r + beta p in CGA)
+                            CS<S> xn = ln[j];
+                            if (MODE == MODE_CGA) {
+                                xn.re = xn.re + S(beta) * lpn[j].re;
+                                xn.im = xn.im + S(beta) * lpn[j].im;
+                            }
+                            const CS<S> ph = lph[j];
+                            S dr, di;
+                            if (OP == HOP_M) {
+                                dr = ph.re * xn.re - ph.im * xn.im;
+                                di = ph.re * xn.im + ph.im * xn.re;
+                            } else {
+                                dr = ph.re * xn.re + ph.im * xn.im;
+                                di = ph.re * xn.im - ph.im * xn.re;
+                            }
+                            if (OP == HOP_M ? t == 0 : t == nt - 1) { dr = -dr; di = -di; }
+                            fr = fr + dr;
+                            fi = fi + di;
+                        } else if (OP == HOP_M) {
+                            if (t == 0) { fr = -fr; fi = -fi; }
                         } else {
-                            dr = ph.re * xn.re + ph.im * xn.im;
-                            di = ph.re * xn.im - ph.im * xn.re;
+                            const CS<S> ph = lph[j];
+                            const S re = ph.re * fr + ph.im * fi;
+                            const S im = ph.re * fi - ph.im * fr;
+                            fr = re; fi = im;
+                            if (t == nt - 1) { fr = -fr; fi = -fi; }
                         }
-                        if (OP == HOP_M ? t == 0 : t == nt - 1) { dr = -dr; di = -di; }
-                        fr = fr + dr;
-                        fi = fi + di;
-                    } else if (OP == HOP_M) {
-                        if (t == 0) { fr = -fr; fi = -fi; }
-                    } else {
-                        const CS<S> ph = ldc<S>(Pf, (size_t)t * nx + y);   // conj(P_t) * acc
-                        const S re = ph.re * fr + ph.im * fi;
-                        const S im = ph.re * fi - ph.im * fr;
-                        fr = re; fi = im;
-                        if (t == nt - 1) { fr = -fr; fi = -fi; }
+                        CS<S> id;
+                        if (MODE == MODE_CGA) {
+                            id.re = lx[j].re + S(beta) * lp[j].re;
+                            id.im = lx[j].im + S(beta) * lp[j].im;
+                        } else {
+                            id = lx[j];
+                        }
+                        const CS<S> out{id.re - fr, id.im - fi};
+                        if (MODE == MODE_PLAIN) {
+                            stc<S>(J.Y, idx, out);
+                        } else if (MODE == MODE_CGA) {
+                            stc<S>(J.Y, idx, out);
+                            stc<S>(J.Pnew, idx, id);   // p_new = r + beta p_old, each element once
+                            part += double(out.re) * out.re + double(out.im) * out.im;
+                        } else if (MODE == MODE_CGB) {
+                            CS<S> xv = lxs[j], rv = lr[j];
+                            xv.re += S(alpha) * lp[j].re; xv.im += S(alpha) * lp[j].im;
+                            rv.re -= S(alpha) * out.re; rv.im -= S(alpha) * out.im;
+                            stc<S>(J.Xs, idx, xv);
+                            stc<S>(J.R, idx, rv);
+                            part += double(rv.re) * rv.re + double(rv.im) * rv.im;
+                        } else {  // CGI
+                            const CS<S> bv = lr[j];
+                            const CS<S> rv{bv.re - out.re, bv.im - out.im};
+                            stc<S>(J.R, idx, rv);
+                            stc<S>(J.Pnew, idx, rv);
+                            part += double(rv.re) * rv.re + double(rv.im) * rv.im;
+                        }
                     }
-                    CS<S> id;
-                    if (MODE == MODE_CGA) {
-                        const CS<S> rv = ldc<S>(J.X, idx);
-                        const CS<S> po = ldc<S>(J.Pold, idx);
-                        id.re = rv.re + S(beta) * po.re;
-                        id.im = rv.im + S(beta) * po.im;
-                    } else {
-                        id = ldc<S>(J.X, idx);
-                    }
-                    const CS<S> out{id.re - fr, id.im - fi};
-                    if (MODE == MODE_PLAIN) {
-                        stc<S>(J.Y, idx, out);
-                    } else if (MODE == MODE_CGA) {
-                        stc<S>(J.Y, idx, out);
-                        stc<S>(J.Pnew, idx, id);   // p_new = r + beta p_old, each element once
-                        part += double(out.re) * out.re + double(out.im) * out.im;
-                    } else if (MODE == MODE_CGB) {
-                        const CS<S> pv = ldc<S>(J.Pold, idx);
-                        CS<S> xv = ldc<S>(J.Xs, idx);
-                        CS<S> rv = ldc<S>(J.R, idx);
-                        xv.re += S(alpha) * pv.re; xv.im += S(alpha) * pv.im;
-                        rv.re -= S(alpha) * out.re; rv.im -= S(alpha) * out.im;
-                        stc<S>(J.Xs, idx, xv);
-                        stc<S>(J.R, idx, rv);
-                        part += double(rv.re) * rv.re + double(rv.im) * rv.im;
-                    } else {  // CGI
-                        const CS<S> bv = ldc<S>(J.Bv, idx);
-                        const CS<S> rv{bv.re - out.re, bv.im - out.im};
-                        stc<S>(J.R, idx, rv);
-                        stc<S>(J.Pnew, idx, rv);
-                        part += double(rv.re) * rv.re + double(rv.im) * rv.im;
+                }
+            }
+        } else {
+            #pragma unroll
+            for (int i = 0; i < FM; ++i) {
+                #pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    // accumulator register v of lane (q, r16) holds row 4q+v of the
+                    // 16x16 block for the f32/bf16 MFMAs, row 4v+q for f64 (its
+                    // output groups have size 1)
+                    const int t = t0 + (wm * FM + i) * 16 + (PR::ROW4V ? 4 * v + q : 4 * q + v);
+                    if (t >= nt) continue;
+                    const size_t rowoff = bbase + (size_t)t * nx;
+                    #pragma unroll
+                    for (int j = 0; j < FN; ++j) {
+                        const int y = y0 + (wn * FN + j) * 16 + r16;
+                        if (y >= nx) continue;
+                        const size_t idx = rowoff + y;
+                        S fr = S(accR[i][j][v]), fi = S(accI[i][j][v]);
+                        if (DISC == DISC_DIAG) {
+                            // + b_t P_{t-1} psi_{t-1} (M) or + b'_t conj(P_t) psi_{t+1} (M^dag),
+                            // psi = the hop input (r + beta p in CGA)
+                            const int tn = (OP == HOP_M) ? (t == 0 ? nt - 1 : t - 1) : (t + 1 >= nt ? 0 : t + 1);
+                            const size_t nidx = bbase + (size_t)tn * nx + y;
+                            CS<S> xn = ldc<S>(J.X, nidx);
+                            if (MODE == MODE_CGA) {
+                                const CS<S> pn = ldc<S>(J.Pold, nidx);
+                                xn.re = xn.re + S(beta) * pn.re;
+                                xn.im = xn.im + S(beta) * pn.im;
+                            }
+                            const CS<S> ph = ldc<S>(Pf, (size_t)(OP == HOP_M ? tn : t) * nx + y);
+                            S dr, di;
+                            if (OP == HOP_M) {
+                                dr = ph.re * xn.re - ph.im * xn.im;
+                                di = ph.re * xn.im + ph.im * xn.re;
+                            } else {
+                                dr = ph.re * xn.re + ph.im * xn.im;
+                                di = ph.re * xn.im - ph.im * xn.re;
+                            }
+                            if (OP == HOP_M ? t == 0 : t == nt - 1) { dr = -dr; di = -di; }
+                            fr = fr + dr;
+                            fi = fi + di;
+                        } else if (OP == HOP_M) {
+                            if (t == 0) { fr = -fr; fi = -fi; }
+                        } else {
+                            const CS<S> ph = ldc<S>(Pf, (size_t)t * nx + y);   // conj(P_t) * acc
+                            const S re = ph.re * fr + ph.im * fi;
+                            const S im = ph.re * fi - ph.im * fr;
+                            fr = re; fi = im;
+                            if (t == nt - 1) { fr = -fr; fi = -fi; }
+                        }
+                        CS<S> id;
+                        if (MODE == MODE_CGA) {
+                            const CS<S> rv = ldc<S>(J.X, idx);
+                            const CS<S> po = ldc<S>(J.Pold, idx);
+                            id.re = rv.re + S(beta) * po.re;
+                            id.im = rv.im + S(beta) * po.im;
+                        } else {
+                            id = ldc<S>(J.X, idx);
+                        }
+                        const CS<S> out{id.re - fr, id.im - fi};
+                        if (MODE == MODE_PLAIN) {
+                            stc<S>(J.Y, idx, out);
+                        } else if (MODE == MODE_CGA) {
+                            stc<S>(J.Y, idx, out);
+                            stc<S>(J.Pnew, idx, id);   // p_new = r + beta p_old, each element once
+                            part += double(out.re) * out.re + double(out.im) * out.im;
+                        } else if (MODE == MODE_CGB) {
+                            const CS<S> pv = ldc<S>(J.Pold, idx);
+                            CS<S> xv = ldc<S>(J.Xs, idx);
+                            CS<S> rv = ldc<S>(J.R, idx);
+                            xv.re += S(alpha) * pv.re; xv.im += S(alpha) * pv.im;
+                            rv.re -= S(alpha) * out.re; rv.im -= S(alpha) * out.im;
+                            stc<S>(J.Xs, idx, xv);
+                            stc<S>(J.R, idx, rv);
+                            part += double(rv.re) * rv.re + double(rv.im) * rv.im;
+                        } else {  // CGI
+                            const CS<S> bv = ldc<S>(J.Bv, idx);
+                            const CS<S> rv{bv.re - out.re, bv.im - out.im};
+                            stc<S>(J.R, idx, rv);
+                            stc<S>(J.Pnew, idx, rv);
+                            part += double(rv.re) * rv.re + double(rv.im) * rv.im;
+                        }
                     }
                 }
             }
