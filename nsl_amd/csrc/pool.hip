@@ -123,6 +123,34 @@ int64_t Pool::free_rhs() const {
     return (int64_t)std::count(rhs_used_.begin(), rhs_used_.end(), (char)0);
 }
 
+int64_t Pool::reserve_rhs(int64_t nb) { return nb >= 1 ? alloc_range(rhs_used_, nb) : -1; }
+
+void Pool::release_rhs(int64_t start, int64_t nb) { free_range(rhs_used_, start, nb); }
+
+// Partial-sum slots per RHS depend on the tile a job gets (ceil(nt/BM) *
+// ceil(nx/BN) of them: 144 for the 16 x 32 tile at Nx = 1152, Nt = 64), so the
+// arrays grow on demand: the live partials of running jobs are copied at the
+// same offsets and the graph, which holds the old addresses, is rebuilt.
+void Pool::grow_parts(int64_t need) {
+    const int64_t used = (int64_t)std::count(part_used_.begin(), part_used_.end(), (char)1);
+    const int64_t cap = std::max<int64_t>(2 * cap_parts_, used + need + (int64_t)part_used_.size() / 4 + 1024);
+    DevBuf<double> u2(cap), r0(cap), r1(cap);
+    const size_t old = (size_t)cap_parts_ * sizeof(double);
+    NSLX_HIP(hipMemsetAsync(u2.p, 0, cap * sizeof(double), s_));
+    NSLX_HIP(hipMemsetAsync(r0.p, 0, cap * sizeof(double), s_));
+    NSLX_HIP(hipMemsetAsync(r1.p, 0, cap * sizeof(double), s_));
+    NSLX_HIP(hipMemcpyAsync(u2.p, upart_.p, old, hipMemcpyDeviceToDevice, s_));
+    NSLX_HIP(hipMemcpyAsync(r0.p, rpart0_.p, old, hipMemcpyDeviceToDevice, s_));
+    NSLX_HIP(hipMemcpyAsync(r1.p, rpart1_.p, old, hipMemcpyDeviceToDevice, s_));
+    NSLX_HIP(hipStreamSynchronize(s_));   // the old arrays are freed below
+    upart_ = std::move(u2);
+    rpart0_ = std::move(r0);
+    rpart1_ = std::move(r1);
+    part_used_.resize(cap, 0);
+    cap_parts_ = cap;
+    drop_graph();
+}
+
 int64_t Pool::alloc_range(std::vector<char>& used, int64_t n) {
     int64_t run = 0;
     for (int64_t i = 0; i < (int64_t)used.size(); ++i) {
@@ -210,6 +238,10 @@ void Pool::start_recurrence(PoolJob& j, bool fresh, bool keep_done) {
     j.cls = cls_of(j.prec, tile, fm.disc);
     j.nparts = hop_nparts(j.prec, tile, fm.nx, fm.nt);
     j.part0 = alloc_range(part_used_, j.nb * j.nparts);
+    if (j.part0 < 0) {
+        grow_parts(j.nb * j.nparts);
+        j.part0 = alloc_range(part_used_, j.nb * j.nparts);
+    }
     NSLX_REQUIRE(j.part0 >= 0, "pool: out of partial-sum slots");
     double* cur = ((iter_ - 1) & 1) ? rpart1_.p : rpart0_.p;
     double* prev = (iter_ & 1) ? rpart1_.p : rpart0_.p;
@@ -237,7 +269,7 @@ void Pool::admit(std::unique_ptr<PoolJob> jp) {
     NSLX_REQUIRE(j.nb % j.fm->nfield == 0, "pool: nbatch must be a multiple of the populated fields");
     NSLX_REQUIRE(j.op == NSLX_OP_MMDAGGER, "pool: only MMdagger systems are batched");
     NSLX_REQUIRE(j.prec == NSLX_F64 || j.prec == NSLX_F32 || j.prec == NSLX_BF16X3, "pool: bad precision");
-    j.rhs0 = alloc_range(rhs_used_, j.nb);
+    if (j.rhs0 < 0) j.rhs0 = alloc_range(rhs_used_, j.nb);   // else reserved by reserve_rhs
     NSLX_REQUIRE(j.rhs0 >= 0, "pool: out of RHS slots");
     try {
         j.bn2.ensure(j.nb);

@@ -62,6 +62,12 @@ class Pool {
     bool idle() const { return jobs_.empty(); }
     size_t active() const { return jobs_.size(); }
     int64_t free_rhs() const;
+    // reserve a contiguous run of nb RHS slots for a job about to be admitted
+    // (its PoolJob::rhs0; -1 if no run is free: fragmented slots that count
+    // enough by number do not suffice), or give such a run back unused.  The
+    // partial-sum slots grow on demand.
+    int64_t reserve_rhs(int64_t nb);
+    void release_rhs(int64_t start, int64_t nb);
     hipStream_t stream() const { return s_; }
     const PoolStats& stats() const { return st_; }
 
@@ -102,6 +108,7 @@ class Pool {
     int64_t alloc_range(std::vector<char>& used, int64_t n);
     void free_range(std::vector<char>& used, int64_t start, int64_t n);
     void start_recurrence(PoolJob& j, bool fresh, bool keep_done);
+    void grow_parts(int64_t need);
     void rebuild_class(int c);
     void run_iterations();
     void drop_graph();

@@ -35,6 +35,7 @@ struct Request {
     std::unique_ptr<FermionMatrix> fm;
     DevBuf<double2> S, X, G, corr;
     int64_t nb = 0;
+    int64_t rhs0 = -1;          // pool RHS slots reserved at admission
     int step = 1, nsrc = 1;
     double bytes = 0.0;
     nslx_result res{};
@@ -145,6 +146,8 @@ void Server::admit(Request& q) {
     j->tol = r.tol;
     j->prec = r.precision;
     j->maxit = r.maxit;
+    j->rhs0 = q.rhs0;
+    q.rhs0 = -1;   // the pool owns the slots from here (and frees them if admit fails)
     pool_->admit(std::move(j));
 }
 
@@ -207,7 +210,11 @@ void Server::run() {
             while (!stop_ && !queue_.empty()) {
                 auto& q = queue_.front();
                 if (used_bytes_ + q->bytes > cfg_.hbm_budget) break;
-                if (q->nb > pool_->free_rhs()) break;
+                // a job needs one contiguous run of RHS slots, reserved here so
+                // that the next request of this round sees it taken; a request
+                // that finds none waits at the front until jobs finish
+                q->rhs0 = pool_->reserve_rhs(q->nb);
+                if (q->rhs0 < 0) break;
                 used_bytes_ += q->bytes;
                 bytes_peak_ = std::max(bytes_peak_, used_bytes_);
                 q->t_admit = Clock::now();
@@ -223,6 +230,10 @@ void Server::run() {
                 try {
                     admit(*q);
                 } catch (const Error& e) {
+                    if (q->rhs0 >= 0) {   // failed before the pool took the slots
+                        pool_->release_rhs(q->rhs0, q->nb);
+                        q->rhs0 = -1;
+                    }
                     std::lock_guard<std::mutex> g(mu_);
                     running_.erase(q->id);
                     used_bytes_ -= q->bytes;

@@ -101,3 +101,18 @@ def test_batch_errors():
     other = nsl.Context(0)
     with pytest.raises(nsl.NslxError):                       # fm of another context
         nsl.batch_solve([(fm, b.to(DEV))], ctx=other)
+
+
+def test_batch_large_lattice_grows_partials():
+    """One Nx = 1152 (24 x 24 honeycomb), Nt = 64 item with 4 RHS: the 16 x 32
+    tile it gets needs 4 * 36 = 144 partial-sum slots per RHS, more than the
+    pool's initial 128 per RHS slot, so the partial arrays must grow.  Every
+    RHS to 1e-8 (F32 recurrence, FP64 reliable updates); one RHS checked
+    against the oracle's MMdagger."""
+    ctx = nsl.Context(0)
+    fm, ofs, b = make("honeycomb", (24, 24), 64, 2.0, 1.0, 1, 4, 500, ctx=ctx)
+    xs, rep = nsl.batch_solve([(fm, b.to(DEV))], tol=1e-8, precision=nsl.F32, ctx=ctx)
+    assert rep["converged"], rep
+    assert rep["max_rel_residual"] <= 1e-8
+    r = b[0] - ofs[0].MMdagger(xs[0][0].cpu())
+    assert float(r.norm() / b[0].norm()) < 2e-8

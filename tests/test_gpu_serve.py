@@ -159,3 +159,31 @@ def test_server_errors():
         assert _lib.lib().nslx_server_submit(srv._h, ctypes.byref(req), ctypes.byref(rid)) == _lib.E_ARG
     finally:
         srv.close()
+
+
+def test_server_admission_waits_for_contiguous_slots():
+    """A pool of 100 RHS slots and requests of 72 and 18 RHS (honeycomb
+    6x6 / 3x3, one time source each): as requests finish out of order the
+    free slots fragment; a request that fits by count but not in one run
+    must wait at the queue front, never fail.  Every result must be OK and
+    equal to the same request served alone."""
+    big, small = nsl.Lattice.honeycomb(6, 6), nsl.Lattice.honeycomb(3, 3)
+    spec = [(small, 16, 2.0, 1.0, 3000), (big, 16, 3.0, 2.0, 3001), (small, 16, 6.0, 3.0, 3002),
+            (small, 16, 2.0, 1.0, 3003), (big, 16, 2.0, 1.0, 3004), (small, 16, 5.0, 3.0, 3005),
+            (small, 16, 2.0, 1.0, 3006), (big, 16, 4.0, 2.0, 3007)]
+    srv = nsl.Server(max_rhs=100, iters_per_replay=8)
+    try:
+        ids = [srv.submit(lat, Nt, beta, field(lat, Nt, beta, U, seed), tol=1e-10) for lat, Nt, beta, U, seed in spec]
+        out = [srv.wait(i) for i in ids]
+    finally:
+        srv.close()
+    for (lat, Nt, beta, U, seed), (corr, res) in zip(spec, out):
+        assert res["status"] == 0, res
+    lat, Nt, beta, U, seed = spec[4]
+    alone = nsl.Server(max_rhs=100, iters_per_replay=8)
+    try:
+        c1, r1 = alone.wait(alone.submit(lat, Nt, beta, field(lat, Nt, beta, U, seed), tol=1e-10))
+    finally:
+        alone.close()
+    assert r1["status"] == 0
+    assert torch.equal(c1, out[4][0])
