@@ -146,6 +146,19 @@ int nslx_fm_populate(nslx_fm fm, const void* phi, int64_t nfield, int species, v
 int nslx_fm_apply(nslx_fm fm, int op, const void* in, void* out, int64_t nbatch,
                   int precision, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): nslx_fm_apply with the same arguments, results and
+ * kernels, allocating nothing.  The MMdagger / MdaggerM intermediate lives in
+ * a library-owned device array of NSLX_NOALLOC_SCRATCH_BYTES per GPU (held
+ * until the process ends); NSLX_E_ARG when nbatch*nt*nx*(16 for NSLX_F64,
+ * else 8) exceeds it.  Calls that overlap on one GPU share that array and
+ * race.  NSLX_E_STATE before populate (which builds the hop matrix) or while
+ * kernel timing (nslx_prof_enable) is on. */
+#define NSLX_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int nslx_fm_apply_noalloc(nslx_fm fm, int op, const void* in, void* out, int64_t nbatch,
+                          int precision, void* stream);
+
 /* logDetM() for field `field` (hubbardExp.tpp:128-150), FP64.  Computed
  * as log det of the block-cyclic matrix of chunked slice products with
  * partial pivoting (stable where the reference's det overflows); nchunks <= 0

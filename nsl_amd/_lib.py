@@ -111,6 +111,7 @@ PROTOS = {
     "nslx_fm_destroy": (c_int, [c_void_p]),
     "nslx_fm_populate": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_void_p]),
     "nslx_fm_apply": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int64, c_int, c_void_p]),
+    "nslx_fm_apply_noalloc": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int64, c_int, c_void_p]),
     "nslx_fm_logdet": (c_int, [c_void_p, c_int64, c_int, P_double, c_void_p]),
     "nslx_fm_grad_logdet": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
     "nslx_solve": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int64, c_double, c_int,

@@ -79,6 +79,11 @@ int nslx_ctx_create(int device, nslx_ctx* out) {
         int n = 0;
         NSLX_HIP(hipGetDeviceCount(&n));
         NSLX_REQUIRE(device >= 0 && device < n, "ctx_create: no such device");
+        {
+            // with the device current: its copy of the no-alloc scratch array
+            DeviceGuard dg(device);
+            nslx::noalloc_init(device);
+        }
         auto* c = new nslx_ctx_s();
         c->c.device = device;
         hipDeviceProp_t p;
@@ -173,6 +178,16 @@ int nslx_fm_apply(nslx_fm fm, int op, const void* in, void* out, int64_t nbatch,
         auto& m = FM(fm);
         DeviceGuard dg(m.ctx->device);
         m.apply(op, in, out, nbatch, precision, nslx::as_stream(stream));
+        return NSLX_OK;
+    });
+}
+
+int nslx_fm_apply_noalloc(nslx_fm fm, int op, const void* in, void* out, int64_t nbatch, int precision,
+                          void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        DeviceGuard dg(m.ctx->device);
+        nslx::apply_noalloc(m, op, in, out, nbatch, precision, nslx::as_stream(stream));
         return NSLX_OK;
     });
 }

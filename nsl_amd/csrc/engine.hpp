@@ -85,6 +85,10 @@ struct FermionMatrix {
     std::map<std::pair<double, double>, std::unique_ptr<EEntry>> memo_lin;
     // the hop's dense matrix: exp(s delta K) (EXP) or s delta K (DIAG)
     const void* Eptr(int prec, hipStream_t s);
+    // whether Eptr would find it memoised (no allocation)
+    bool has_hop_matrix() const {
+        return disc == DISC_DIAG ? memo_lin.count({sgn() * delta(), 0.0}) != 0 : memo.count(sgn() * delta()) != 0;
+    }
     const void* Pptr(int prec) const { return prec == NSLX_F64 ? (const void*)P64.p : (const void*)P32.p; }
 
     void populate(const void* phi_dev, int64_t nf, int sp, hipStream_t s);
@@ -114,6 +118,12 @@ int solve_precond(FermionMatrix& fm, FermionMatrix& fp, int op, const void* b, v
 // log det / force (logdet.cpp)
 void fm_logdet(FermionMatrix& fm, int64_t field, int nchunks, double* out2, hipStream_t s);
 void fm_grad_logdet(FermionMatrix& fm, int64_t field, void* out, hipStream_t s);
+
+// Allocation-free application (noalloc.hip): noalloc_init looks up the
+// current device's scratch array (called by nslx_ctx_create); apply_noalloc
+// is nslx_fm_apply with the two-hop intermediate in that array.
+void noalloc_init(int device);
+void apply_noalloc(FermionMatrix& fm, int op, const void* in, void* out, int64_t nb, int prec, hipStream_t s);
 
 // hop tile autotuner (tune.cpp)
 int fm_tune(FermionMatrix& fm, int prec, int64_t nb, int reps, int* best_tile, double* best_us, hipStream_t s);

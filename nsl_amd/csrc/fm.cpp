@@ -145,6 +145,9 @@ void FermionMatrix::populate(const void* phi_dev, int64_t nf, int sp, hipStream_
     double m = 0.0;
     for (const double2& z : h) m = std::max(m, std::fabs(z.y));
     max_im_phi = m;
+    // build (or find) the hop matrix of this species now, so that applies
+    // after populate allocate nothing (nslx_fm_apply_noalloc relies on it)
+    (void)Eptr(NSLX_F64, s);
 }
 
 void FermionMatrix::require_populated() const {
