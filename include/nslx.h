@@ -223,6 +223,53 @@ int nslx_solve_precond(nslx_fm fm, nslx_fm fm_prec, int op, const void* b, void*
                        int64_t nbatch, double tol, int tol_kind, int64_t maxit, int inner_precision,
                        nslx_solve_report* report, void* stream);
 
+/* ---- time-slice direct solve and the preconditioned solver -------------------
+ * No counterpart in the reference (it applies M, hubbardExp.tpp:12-125, and
+ * solves only by Krylov iteration, CG.tpp:14-113).  M x = y is the recurrence
+ * x_t = y_t + b_t E (P_{t-1} . x_{t-1}) around the time circle; cutting the
+ * Nt slices into chunks of c (c delta ||K|| <= 3) and eliminating the inner
+ * slices leaves a block-cyclic system of Nt/c blocks of Nx, factorised per
+ * field by Gauss-Jordan elimination with partial pivoting (HubbardExp only).
+ *
+ * nslx_fm_solve_direct: out = M^{-1} in (op NSLX_OP_M), M^{-dag} in
+ * (NSLX_OP_MDAGGER) or (M M^dag)^{-1} in (NSLX_OP_MMDAGGER) for nbatch RHS;
+ * precision NSLX_F64 (c128 in/out) or NSLX_F32 (c64 in/out).  The first call
+ * after a populate factorises (nchunks <= 0: chosen from the growth per slice). */
+int nslx_fm_solve_direct(nslx_fm fm, int op, const void* in, void* out, int64_t nbatch, int precision,
+                         int nchunks, void* stream);
+
+/* CG on MMdagger (the CG of CG.tpp:14-113 on MatrixCombination MMdagger,
+ * Solver.hpp:86-96), preconditioned by the time-slice solve in
+ * precond_precision (NSLX_F64 or NSLX_F32: the preconditioner's factorisation
+ * and applies on FP32 matrix cores, the CG recurrence in FP64), wrapped in a
+ * double-double defect correction: every RHS stops when
+ * |b - MM^dag (x + x_lo)| <= tol |b| with the residual evaluated in
+ * double-double on the FP64 operator.  This reaches targets below the FP64
+ * floor eps ||MM^dag|| |x| (where |x| / |b| is large); x receives the
+ * double-double solution rounded to FP64, x_lo (device c128, may be NULL) the
+ * remainder.  x0 NULL means x0 = 0.  report: iterations = inner CG
+ * iterations, iterations_total = preconditioner applications, outer_steps =
+ * double-double residual evaluations after the start, max_rel_residual =
+ * max_k |b_k - A (x_k + x_lo_k)| / |b_k| (double-double).  NSLX_NOT_CONVERGED
+ * (x written) when maxit inner iterations did not suffice. */
+int nslx_solve_tss(nslx_fm fm, int op, const void* b, void* x, void* x_lo, const void* x0, int64_t nbatch,
+                   double tol, int64_t maxit, int precond_precision, int nchunks,
+                   nslx_solve_report* report, void* stream);
+
+/* Double-double residual (a measurement aid; no counterpart in the
+ * reference): r = b - A (x + x_lo), A = MMdagger or MdaggerM of a HubbardExp
+ * fm, every product and sum in double-double on the FP64 numbers of
+ * exp(s delta K) and of the phase table; rel_host[k] = |r_k| / |b_k|.  x_lo
+ * and r may be NULL.  Synchronous. */
+int nslx_dd_residual(nslx_fm fm, int op, const void* b, const void* x, const void* x_lo, void* r,
+                     double* rel_host, int64_t nbatch, void* stream);
+
+/* The FP64 operator data the engine applies (for checking residuals on the
+ * host bit for bit): E_host = exp(s delta K) (nx*nx, row-major) of the
+ * current species, P_host = the phase table P = exp(i s phi + s delta mu),
+ * c128 (nfield, nt, nx).  Either may be NULL.  Synchronous. */
+int nslx_fm_export(nslx_fm fm, double* E_host, void* P_host, void* stream);
+
 /* ---- batched ragged solve ----------------------------------------------------------
  * No single counterpart in the reference (it solves one HubbardExp at a
  * time, CG.tpp:14-113); this is the reference's CG on MMdagger for many

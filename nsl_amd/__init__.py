@@ -244,13 +244,37 @@ class HubbardExp:
         _lib.check(_lib.lib().nslx_fm_grad_logdet(self._h, int(field), _ptr(out), _stream()))
         return out
 
+    # -- time-slice direct solve (no counterpart in the reference) ----------
+    def solve_direct(self, op, y, out=None, precision=F64, nchunks=0):
+        """M^{-1} y (op M), M^{-dag} y (Mdagger) or (M M^dag)^{-1} y
+        (MMdagger) by the time-slice solve (nslx_fm_solve_direct); y is
+        complex128 for F64, complex64 for F32."""
+        torch = _torch()
+        dt = torch.complex128 if precision == F64 else torch.complex64
+        if y.dtype != dt or not y.is_contiguous():
+            y = y.to(dtype=dt).contiguous()
+        nb = int(np.prod(y.shape[:-2])) if y.dim() > 2 else 1
+        if out is None:
+            out = torch.empty_like(y)
+        _lib.check(_lib.lib().nslx_fm_solve_direct(self._h, int(op), _ptr(y), _ptr(out), nb, int(precision),
+                                                   int(nchunks), _stream()))
+        return out
+
+    def export(self):
+        """(E, P): the FP64 exp(s delta K) (Nx, Nx) and phase table
+        (nfield, Nt, Nx) the engine applies (nslx_fm_export)."""
+        E = np.empty((self.Nx, self.Nx), dtype=np.float64)
+        P = np.empty((self.nfield, self.Nt, self.Nx), dtype=np.complex128)
+        _lib.check(_lib.lib().nslx_fm_export(self._h, E.ctypes.data_as(_lib.P_double), P.ctypes.data, _stream()))
+        return E, P
+
 
 class HubbardDiag(HubbardExp):
     """NSL::FermionMatrix::HubbardDiag on the device (hubbardDiag.hpp,
     hubbardDiag.tpp): (M psi)_t = psi_t - s delta K psi_t - B_t P_{t-1} psi_{t-1}.
     Same constructors, populate, M/Mdagger/MMdagger/MdaggerM, logDetM and
     gradLogDetM as HubbardExp; solvers, the correlator and the HMC entries
-    take it too.  The batching engine (batch_solve, Server) does not."""
+    take it too, and batch_solve and Server accept it (MMdagger systems)."""
 
     _create = "nslx_fm_create_diag"
 
@@ -345,6 +369,60 @@ class CGpreconditioned:
         if not self.converged:
             log.error("CGpreconditioned did not converge within %d iterations", self.maxIter)
         return x
+
+
+class CGTimeSlice:
+    """CG on MMdagger preconditioned by the time-slice direct solve, in a
+    double-double defect correction to |b - MM^dag x| <= eps |b| per RHS
+    (nslx_solve_tss; no counterpart in the reference, whose CG on MMdagger
+    is CG.tpp:14-113).  precision: the preconditioner's (F32: factorisation
+    and applies on FP32 matrix cores, the CG recurrence in FP64; F64).
+    After a call, .x_lo holds the low part of the double-double solution
+    (x + x_lo) and .report the solver report."""
+
+    def __init__(self, fm, eps=1e-12, maxIter=10000, precision=F32, nchunks=0):
+        self.fm = fm
+        self.eps = float(eps)
+        self.maxIter = int(maxIter)
+        self.precision = int(precision)
+        self.nchunks = int(nchunks)
+        self.report = None
+        self.converged = None
+        self.x_lo = None
+
+    def __call__(self, b, x0=None, out=None, x_lo=None):
+        torch = _torch()
+        b = b.to(dtype=torch.complex128).contiguous()
+        nb = int(np.prod(b.shape[:-2])) if b.dim() > 2 else 1
+        x = torch.empty_like(b) if out is None else out
+        lo = torch.empty_like(b) if x_lo is None else x_lo
+        if x0 is not None:
+            x0 = x0.to(dtype=torch.complex128).contiguous()
+        rep = SolveReport()
+        st = _lib.lib().nslx_solve_tss(self.fm._h, int(MatrixCombination.MMdagger), _ptr(b), _ptr(x), _ptr(lo),
+                                       _ptr(x0) if x0 is not None else None, nb, self.eps, self.maxIter,
+                                       self.precision, self.nchunks, ctypes.byref(rep), _stream())
+        _lib.check(st, allow=(_lib.OK, _lib.NOT_CONVERGED))
+        self.report = rep.as_dict()
+        self.converged = st == _lib.OK
+        self.x_lo = lo
+        if not self.converged:
+            log.error("CGTimeSlice did not converge within %d iterations", self.maxIter)
+        return x
+
+
+def dd_residual(fm, b, x, x_lo=None, op=MatrixCombination.MMdagger):
+    """Per-RHS |b - A (x + x_lo)| / |b| with the residual in double-double
+    (nslx_dd_residual); numpy array of nbatch values."""
+    torch = _torch()
+    b = b.to(dtype=torch.complex128).contiguous()
+    x = x.to(dtype=torch.complex128).contiguous()
+    nb = int(np.prod(b.shape[:-2])) if b.dim() > 2 else 1
+    rel = np.empty(nb, dtype=np.float64)
+    _lib.check(_lib.lib().nslx_dd_residual(fm._h, int(op), _ptr(b), _ptr(x),
+                                           _ptr(x_lo.contiguous()) if x_lo is not None else None, None,
+                                           rel.ctypes.data_as(_lib.P_double), nb, _stream()))
+    return rel
 
 
 def batch_solve(problems, tol=1e-12, precision=F64, maxIter=100000, use_graph=True, ctx=None):

@@ -118,6 +118,12 @@ PROTOS = {
                            c_int64, c_int, c_int, ctypes.POINTER(SolveReport), c_void_p]),
     "nslx_solve_precond": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int64, c_double,
                                    c_int, c_int64, c_int, ctypes.POINTER(SolveReport), c_void_p]),
+    "nslx_fm_solve_direct": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int64, c_int, c_int, c_void_p]),
+    "nslx_solve_tss": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_double, c_int64,
+                               c_int, c_int, ctypes.POINTER(SolveReport), c_void_p]),
+    "nslx_dd_residual": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, P_double, c_int64,
+                                 c_void_p]),
+    "nslx_fm_export": (c_int, [c_void_p, P_double, c_void_p, c_void_p]),
     "nslx_batch_solve": (c_int, [c_void_p, c_int64, ctypes.POINTER(BatchItem), c_double, c_int, c_int64, c_int,
                                  ctypes.POINTER(BatchReport), c_void_p]),
     "nslx_server_create": (c_int, [c_void_p, ctypes.POINTER(ServerConfig), ctypes.POINTER(c_void_p)]),

@@ -1,0 +1,324 @@
+// Batched complex NT GEMM on CDNA4 matrix cores (see BJob in tss.hpp), the
+// engine of the time-slice solver: chunk products, the forward/back sweeps
+// of the reduced block system, the Gauss-Jordan trailing updates and the
+// in-chunk recurrences are all launches of this kernel over job lists.
+//
+// A workgroup (4 waves, 2 x 2) owns one BM x BN tile of one job; the K loop
+// stages BK = 16 columns of A (re / im planes, phase applied by the loader)
+// and of B (re [/ im] planes) in LDS and feeds v_mfma_f64_16x16x4f64 (FP64) or
+// v_mfma_f32_16x16x4f32 (FP32).  Complex A x real B is two real products
+// (re, im share B); complex x complex is four (rr, ii, ri, ir), combined in
+// the epilogue.
+#include <functional>
+
+#include "tss.hpp"
+
+namespace nslx {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <class S> struct Acc;
+template <> struct Acc<double> { using T = d4; };
+template <> struct Acc<float> { using T = f4; };
+
+template <class S>
+struct C2 { S re, im; };
+
+template <class S>
+__device__ inline C2<S> ldc(const void* base, long long idx) {
+    if constexpr (sizeof(S) == 8) {
+        const double2 v = reinterpret_cast<const double2*>(base)[idx];
+        return {v.x, v.y};
+    } else {
+        const float2 v = reinterpret_cast<const float2*>(base)[idx];
+        return {v.x, v.y};
+    }
+}
+template <class S>
+__device__ inline void stc(void* base, long long idx, C2<S> v) {
+    if constexpr (sizeof(S) == 8) reinterpret_cast<double2*>(base)[idx] = make_double2(v.re, v.im);
+    else reinterpret_cast<float2*>(base)[idx] = make_float2(v.re, v.im);
+}
+
+template <class S>
+__device__ inline typename Acc<S>::T mfma(S a, S b, typename Acc<S>::T c) {
+    if constexpr (sizeof(S) == 8) return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    else return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+constexpr int BK = 16;
+constexpr int PAD = 1;
+
+template <class S, bool CB, int PH, int BM, int BN>
+__global__ void __launch_bounds__(256) k_bgemm(const BJob* __restrict__ jobs, const int4* __restrict__ tiles) {
+    constexpr int FM = BM / 32, FN = BN / 32;          // 16 x 16 blocks per wave (2 x 2 waves)
+    constexpr int NPB = CB ? 2 : 1;
+    __shared__ S sA[2][BM][BK + PAD];
+    __shared__ S sB[NPB][BN][BK + PAD];
+    using AT = typename Acc<S>::T;
+    const int4 tl = tiles[blockIdx.x];
+    const BJob& J = jobs[tl.x];
+    const int rows = J.rows, cols = J.cols, K = J.K;
+    const int r0 = tl.y * BM, c0 = tl.z * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int r16 = lane & 15, q = lane >> 4;
+
+    AT acc[CB ? 4 : 2][FM][FN];
+    #pragma unroll
+    for (int a = 0; a < (CB ? 4 : 2); ++a)
+        #pragma unroll
+        for (int i = 0; i < FM; ++i)
+            #pragma unroll
+            for (int j = 0; j < FN; ++j) acc[a][i][j] = AT{0, 0, 0, 0};
+
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        // A chunk: BM rows x BK columns (complex), phase applied (PH == 1)
+        #pragma unroll
+        for (int e = tid; e < BM * BK; e += 256) {
+            const int row = e / BK, kk = e % BK;
+            const int r = r0 + row, k = k0 + kk;
+            C2<S> z{S(0), S(0)};
+            if (r < rows && k < K) {
+                z = k < J.kseg ? ldc<S>(J.A0, (long long)r * J.lda0 + k)
+                               : ldc<S>(J.A1, (long long)r * J.lda1 + (k - J.kseg));
+                if constexpr (PH == 1) {
+                    const C2<S> p = ldc<S>(J.ph, (long long)(r / J.phdiv) * J.ldph + k);
+                    const S re = p.re * z.re - p.im * z.im;
+                    const S im = p.re * z.im + p.im * z.re;
+                    z.re = re;
+                    z.im = im;
+                }
+            }
+            sA[0][row][kk] = z.re;
+            sA[1][row][kk] = z.im;
+        }
+        // B chunk: BN output columns x BK
+        #pragma unroll
+        for (int e = tid; e < BN * BK; e += 256) {
+            const int col = e / BK, kk = e % BK;
+            const int c = c0 + col, k = k0 + kk;
+            if constexpr (CB) {
+                C2<S> z{S(0), S(0)};
+                if (c < cols && k < K) z = ldc<S>(J.B, (long long)c * J.ldb + k);
+                sB[0][col][kk] = z.re;
+                sB[1][col][kk] = z.im;
+            } else {
+                S z = S(0);
+                if (c < cols && k < K) z = reinterpret_cast<const S*>(J.B)[(long long)c * J.ldb + k];
+                sB[0][col][kk] = z;
+            }
+        }
+        __syncthreads();
+        #pragma unroll
+        for (int ks = 0; ks < BK / 4; ++ks) {
+            const int k = 4 * ks + q;
+            S ar[FM], ai[FM], br[FN], bi[FN];
+            #pragma unroll
+            for (int i = 0; i < FM; ++i) {
+                const int row = wm * (BM / 2) + 16 * i + r16;
+                ar[i] = sA[0][row][k];
+                ai[i] = sA[1][row][k];
+            }
+            #pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                const int col = wn * (BN / 2) + 16 * j + r16;
+                br[j] = sB[0][col][k];
+                if constexpr (CB) bi[j] = sB[1][col][k];
+            }
+            #pragma unroll
+            for (int i = 0; i < FM; ++i)
+                #pragma unroll
+                for (int j = 0; j < FN; ++j) {
+                    acc[0][i][j] = mfma<S>(ar[i], br[j], acc[0][i][j]);
+                    acc[1][i][j] = mfma<S>(ai[i], br[j], acc[1][i][j]);
+                    if constexpr (CB) {
+                        acc[2][i][j] = mfma<S>(ai[i], bi[j], acc[2][i][j]);
+                        acc[3][i][j] = mfma<S>(ar[i], bi[j], acc[3][i][j]);
+                    }
+                }
+        }
+        __syncthreads();
+    }
+
+    // epilogue: register v of lane (q, r16) holds row 4v + q (FP64) or
+    // 4q + v (FP32) of its 16 x 16 block, column r16
+    const S sg = S(J.sgn);
+    #pragma unroll
+    for (int i = 0; i < FM; ++i)
+        #pragma unroll
+        for (int j = 0; j < FN; ++j)
+            #pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int rr = (sizeof(S) == 8) ? 4 * v + q : 4 * q + v;
+                const int r = r0 + wm * (BM / 2) + 16 * i + rr;
+                const int c = c0 + wn * (BN / 2) + 16 * j + r16;
+                if (r >= rows || c >= cols) continue;
+                S re, im;
+                if constexpr (CB) {
+                    re = S(acc[0][i][j][v]) - S(acc[2][i][j][v]);
+                    im = S(acc[1][i][j][v]) + S(acc[3][i][j][v]);
+                } else {
+                    re = S(acc[0][i][j][v]);
+                    im = S(acc[1][i][j][v]);
+                }
+                if constexpr (PH == 2) {
+                    const C2<S> p = ldc<S>(J.ph, (long long)(r / J.phdiv) * J.ldph + c);
+                    const S tr = p.re * re + p.im * im;   // conj(p) * (re + i im)
+                    const S ti = p.re * im - p.im * re;
+                    re = tr;
+                    im = ti;
+                }
+                C2<S> out{sg * re, sg * im};
+                if (J.Add) {
+                    const C2<S> a = ldc<S>(J.Add, (long long)r * J.ldadd + c);
+                    out.re = out.re + a.re;
+                    out.im = out.im + a.im;
+                }
+                stc<S>(J.Y, (long long)r * J.ldy + c, out);
+            }
+}
+
+template <class S, bool CB, int PH>
+void launch_t(int tile, const BJob* j, const int4* t, int n, hipStream_t s) {
+    if (tile == 0) hipLaunchKernelGGL((k_bgemm<S, CB, PH, 64, 64>), dim3(n), dim3(256), 0, s, j, t);
+    else hipLaunchKernelGGL((k_bgemm<S, CB, PH, 32, 32>), dim3(n), dim3(256), 0, s, j, t);
+}
+
+template <class S>
+void launch_s(const BVariant& v, const BJob* j, const int4* t, int n, hipStream_t s) {
+    if (v.cb) {
+        if (v.ph == 0) launch_t<S, true, 0>(v.tile, j, t, n, s);
+        else if (v.ph == 1) launch_t<S, true, 1>(v.tile, j, t, n, s);
+        else launch_t<S, true, 2>(v.tile, j, t, n, s);
+    } else {
+        if (v.ph == 0) launch_t<S, false, 0>(v.tile, j, t, n, s);
+        else if (v.ph == 1) launch_t<S, false, 1>(v.tile, j, t, n, s);
+        else launch_t<S, false, 2>(v.tile, j, t, n, s);
+    }
+}
+
+}  // namespace
+
+int bgemm_tile_rows(int tile) { return tile == 0 ? 64 : 32; }
+
+void bgemm_tiles(const std::vector<BJob>& jobs, int tile, std::vector<int4>& out) {
+    const int bm = bgemm_tile_rows(tile), bn = bm;
+    for (size_t k = 0; k < jobs.size(); ++k) {
+        const BJob& j = jobs[k];
+        if (j.rows <= 0 || j.cols <= 0) continue;
+        const int nr = (j.rows + bm - 1) / bm, nc = (j.cols + bn - 1) / bn;
+        for (int r = 0; r < nr; ++r)
+            for (int c = 0; c < nc; ++c) out.push_back(make_int4((int)k, r, c, 0));
+    }
+}
+
+void launch_bgemm(const BVariant& v, const BJob* d_jobs, const int4* d_tiles, int ntiles, hipStream_t s) {
+    if (ntiles <= 0) return;
+    NSLX_REQUIRE(v.tile == 0 || v.tile == 1, "bgemm: bad tile");
+    if (v.prec == NSLX_F64) launch_s<double>(v, d_jobs, d_tiles, ntiles, s);
+    else if (v.prec == NSLX_F32) launch_s<float>(v, d_jobs, d_tiles, ntiles, s);
+    else throw Error(NSLX_E_ARG, "bgemm: precision must be F64 or F32");
+    NSLX_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------- BPlan
+void BPlan::gemm(const std::vector<BJob>& jobs, BVariant v, int tile) {
+    if (tile < 0) {
+        // 64 x 64 tiles when they alone give two workgroups per CU of MI355X
+        std::vector<int4> t0;
+        bgemm_tiles(jobs, 0, t0);
+        tile = t0.size() >= 512 ? 0 : 1;
+    }
+    v.tile = tile;
+    Op op;
+    op.is_gemm = true;
+    op.v = v;
+    op.job0 = jobs_.size();
+    op.tile0 = tiles_.size();
+    // tile job indices are local to this launch: run() passes the launch's
+    // own slice of the job array
+    std::vector<int4> t;
+    bgemm_tiles(jobs, tile, t);
+    for (const auto& x : t) NSLX_REQUIRE(x.x >= 0 && x.x < (int)jobs.size(), "bgemm plan: tile job index out of range");
+    jobs_.insert(jobs_.end(), jobs.begin(), jobs.end());
+    tiles_.insert(tiles_.end(), t.begin(), t.end());
+    op.ntiles = (int)t.size();
+    ops_.push_back(std::move(op));
+    uploaded_ = false;
+}
+
+void BPlan::custom(std::function<void(hipStream_t)> fn) {
+    Op op;
+    op.fn = std::move(fn);
+    ops_.push_back(std::move(op));
+}
+
+namespace {
+// Extents (in bytes) of every buffer a job touches, checked against the
+// library's own device allocations when NSLX_CHECK_PLANS is set: a plan whose
+// indexing is wrong fails on the host before anything is launched.
+void check_job(const BJob& j, const BVariant& v, size_t idx) {
+    const size_t ce = v.prec == NSLX_F64 ? 16 : 8;         // complex element
+    const size_t be = v.cb ? ce : ce / 2;                   // B element
+    auto need = [&](const void* p, long long last_elem, size_t es, const char* what) {
+        if (!p) throw Error(NSLX_E_INTERNAL, std::string("bgemm plan: null ") + what);
+        if (last_elem < 0) return;
+        if (!alloc_covers(p, (size_t)(last_elem + 1) * es))
+            throw Error(NSLX_E_INTERNAL, "bgemm plan: job " + std::to_string(idx) + " reads/writes " + what +
+                                             " outside any device allocation");
+    };
+    if (j.rows <= 0 || j.cols <= 0 || j.K <= 0) return;
+    const int k0 = std::min(j.K, j.kseg);
+    need(j.A0, (long long)(j.rows - 1) * j.lda0 + k0 - 1, ce, "A0");
+    if (j.K > j.kseg) need(j.A1, (long long)(j.rows - 1) * j.lda1 + (j.K - j.kseg) - 1, ce, "A1");
+    need(j.B, (long long)(j.cols - 1) * j.ldb + j.K - 1, be, "B");
+    need(j.Y, (long long)(j.rows - 1) * j.ldy + j.cols - 1, ce, "Y");
+    if (j.Add) need(j.Add, (long long)(j.rows - 1) * j.ldadd + j.cols - 1, ce, "Add");
+    if (v.ph) {
+        const long long prow = (long long)((j.rows - 1) / j.phdiv) * j.ldph;
+        need(j.ph, prow + (v.ph == 1 ? j.K : j.cols) - 1, ce, "phase");
+    }
+}
+}  // namespace
+
+void BPlan::upload(hipStream_t s) {
+    if (alloc_checks_on()) {
+        for (const Op& op : ops_)
+            if (op.is_gemm) {
+                const size_t nj = (op.job0 < jobs_.size()) ? jobs_.size() - op.job0 : 0;
+                for (int t = 0; t < op.ntiles; ++t) {
+                    const int4 tl = tiles_[op.tile0 + t];
+                    NSLX_REQUIRE(tl.x >= 0 && (size_t)tl.x < nj, "bgemm plan: tile job index out of range");
+                }
+            }
+        for (size_t o = 0; o < ops_.size(); ++o) {
+            const Op& op = ops_[o];
+            if (!op.is_gemm) continue;
+            size_t end = jobs_.size();
+            for (size_t q = o + 1; q < ops_.size(); ++q)
+                if (ops_[q].is_gemm) { end = ops_[q].job0; break; }
+            for (size_t k = op.job0; k < end; ++k) check_job(jobs_[k], op.v, k);
+        }
+    }
+    djobs_.alloc(std::max<size_t>(jobs_.size(), 1));
+    dtiles_.alloc(std::max<size_t>(tiles_.size(), 1));
+    if (!jobs_.empty())
+        NSLX_HIP(hipMemcpyAsync(djobs_.p, jobs_.data(), jobs_.size() * sizeof(BJob), hipMemcpyHostToDevice, s));
+    if (!tiles_.empty())
+        NSLX_HIP(hipMemcpyAsync(dtiles_.p, tiles_.data(), tiles_.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+    NSLX_HIP(hipStreamSynchronize(s));
+    uploaded_ = true;
+}
+
+void BPlan::run(hipStream_t s) const {
+    NSLX_REQUIRE(uploaded_, "bgemm plan run before upload");
+    for (const Op& op : ops_) {
+        if (op.is_gemm) launch_bgemm(op.v, djobs_.p + op.job0, dtiles_.p + op.tile0, op.ntiles, s);
+        else op.fn(s);
+    }
+}
+
+}  // namespace nslx

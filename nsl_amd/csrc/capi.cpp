@@ -233,6 +233,76 @@ int nslx_solve_precond(nslx_fm fm, nslx_fm fm_prec, int op, const void* b, void*
     });
 }
 
+int nslx_fm_solve_direct(nslx_fm fm, int op, const void* in, void* out, int64_t nbatch, int precision, int nchunks,
+                         void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        DeviceGuard dg(m.ctx->device);
+        NSLX_REQUIRE(in && out && in != out, "solve_direct: null or aliased buffers");
+        NSLX_REQUIRE(precision == NSLX_F64 || precision == NSLX_F32, "solve_direct: precision must be F64 or F32");
+        m.require_populated();
+        auto s = nslx::as_stream(stream);
+        m.time_slice_solver(precision, nchunks, s).apply(op, in, out, nbatch, false, s);
+        return NSLX_OK;
+    });
+}
+
+int nslx_solve_tss(nslx_fm fm, int op, const void* b, void* x, void* x_lo, const void* x0, int64_t nbatch, double tol,
+                   int64_t maxit, int precond_precision, int nchunks, nslx_solve_report* report, void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        DeviceGuard dg(m.ctx->device);
+        return nslx::solve_tss(m, op, b, x, x_lo, x0, nbatch, tol, maxit, precond_precision, nchunks, report,
+                               nslx::as_stream(stream));
+    });
+}
+
+int nslx_dd_residual(nslx_fm fm, int op, const void* b, const void* x, const void* x_lo, void* r, double* rel_host,
+                     int64_t nbatch, void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        DeviceGuard dg(m.ctx->device);
+        NSLX_REQUIRE(b && x, "dd_residual: null buffer");
+        auto s = nslx::as_stream(stream);
+        const size_t bytes = (size_t)nbatch * m.len() * 16;
+        nslx::DevBuf<unsigned char> rr;
+        void* rp = r;
+        if (!rp) {
+            rr.alloc(bytes);
+            rp = rr.p;
+        }
+        nslx::DevBuf<double> rn2(nbatch > 0 ? nbatch : 1), bn2(nbatch > 0 ? nbatch : 1);
+        nslx::dd_residual(m, op, b, x, x_lo, rp, rn2.p, nbatch, m.ddw, s);
+        nslx::norm2_perb(1, b, nbatch, m.len(), bn2.p, m.vsc, s);
+        std::vector<double> hr(nbatch), hb(nbatch);
+        NSLX_HIP(hipMemcpyAsync(hr.data(), rn2.p, nbatch * sizeof(double), hipMemcpyDeviceToHost, s));
+        NSLX_HIP(hipMemcpyAsync(hb.data(), bn2.p, nbatch * sizeof(double), hipMemcpyDeviceToHost, s));
+        NSLX_HIP(hipStreamSynchronize(s));
+        if (rel_host)
+            for (int64_t k = 0; k < nbatch; ++k) rel_host[k] = hb[k] > 0 ? std::sqrt(hr[k] / hb[k]) : std::sqrt(hr[k]);
+        return NSLX_OK;
+    });
+}
+
+int nslx_fm_export(nslx_fm fm, double* E_host, void* P_host, void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        DeviceGuard dg(m.ctx->device);
+        m.require_populated();
+        auto s = nslx::as_stream(stream);
+        if (E_host) {
+            const double a = m.disc == nslx::DISC_DIAG ? 0.0 : m.sgn() * m.delta();
+            NSLX_REQUIRE(m.disc == nslx::DISC_EXP, "fm_export: HubbardExp only");
+            const double* e = m.E(a, s).dense.p;
+            NSLX_HIP(hipMemcpyAsync(E_host, e, (size_t)m.nx * m.nx * sizeof(double), hipMemcpyDeviceToHost, s));
+        }
+        if (P_host)
+            NSLX_HIP(hipMemcpyAsync(P_host, m.P64.p, (size_t)m.nfield * m.len() * sizeof(double2), hipMemcpyDeviceToHost, s));
+        NSLX_HIP(hipStreamSynchronize(s));
+        return NSLX_OK;
+    });
+}
+
 int nslx_batch_solve(nslx_ctx ctx, int64_t nitems, const nslx_batch_item* items, double tol, int precision,
                      int64_t maxit, int use_graph, nslx_batch_report* report, void* stream) {
     return guard([&] {

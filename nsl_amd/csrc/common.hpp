@@ -5,6 +5,9 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -36,6 +39,49 @@ struct Error : std::runtime_error {
 
 #define NSLX_LAUNCH_CHECK() NSLX_HIP(hipGetLastError())
 
+// Registry of the library's device allocations, kept only when the
+// environment sets NSLX_CHECK_PLANS (launch plans are then checked against it
+// on the host before they run).
+struct AllocRegistry {
+    std::mutex mu;
+    std::map<uintptr_t, size_t> m;
+    bool on = false;
+    AllocRegistry() {
+        const char* v = std::getenv("NSLX_CHECK_PLANS");
+        on = v && *v && *v != '0';
+    }
+};
+inline AllocRegistry& alloc_registry() {
+    static AllocRegistry r;
+    return r;
+}
+inline bool alloc_checks_on() { return alloc_registry().on; }
+inline void alloc_note(const void* p, size_t bytes, bool add) {
+    AllocRegistry& r = alloc_registry();
+    if (!r.on || !p) return;
+    std::lock_guard<std::mutex> g(r.mu);
+    if (add) r.m[(uintptr_t)p] = bytes;
+    else r.m.erase((uintptr_t)p);
+}
+// whether [p, p + bytes) lies inside one registered allocation
+inline bool alloc_covers(const void* p, size_t bytes) {
+    AllocRegistry& r = alloc_registry();
+    std::lock_guard<std::mutex> g(r.mu);
+    const uintptr_t a = (uintptr_t)p;
+    auto it = r.m.upper_bound(a);
+    if (it == r.m.begin()) return false;
+    --it;
+    return a >= it->first && a + bytes <= it->first + it->second;
+}
+
+// With the registry on: throws unless elements [0, last_elem] of size es
+// starting at p lie in one registered allocation.
+inline void check_extent(const void* p, long long last_elem, size_t es, const char* what) {
+    if (!alloc_checks_on() || last_elem < 0) return;
+    if (!p || !alloc_covers(p, (size_t)(last_elem + 1) * es))
+        throw Error(NSLX_E_INTERNAL, std::string("launch plan: ") + what + " outside any device allocation");
+}
+
 // Owning device allocation (hipMalloc / hipFree).
 template <class T>
 struct DevBuf {
@@ -56,10 +102,11 @@ struct DevBuf {
         if (count == 0) return;
         NSLX_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
         n = count;
+        alloc_note(p, count * sizeof(T), true);
     }
     void ensure(size_t count) { if (count > n) alloc(count); }
     void reset() {
-        if (p) { (void)hipFree(p); p = nullptr; n = 0; }
+        if (p) { alloc_note(p, 0, false); (void)hipFree(p); p = nullptr; n = 0; }
     }
     T* get() const { return p; }
 };

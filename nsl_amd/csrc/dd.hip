@@ -1,0 +1,283 @@
+// Double-double (about 106-bit) residuals of M M^dag and the double-double
+// solution update, for solves whose target lies below the FP64 residual
+// floor eps ||MM^dag|| |x| (random fields at large beta: |x| / |b| up to
+// 1e8, so an FP64 residual cannot show 1e-12).
+//
+// A double-double value is a pair (hi, lo) with |lo| <= ulp(hi) / 2; the
+// error-free transforms are Knuth's TwoSum and the fused-multiply-add
+// TwoProduct (p = a b, e = fma(a, b, -p)).  The operator is the one the
+// engine applies: the FP64 numbers of exp(s delta K) and of the phase table,
+// taken as exact; every product and sum of M^dag and M is carried in
+// double-double, and the residual b - M M^dag (x_hi + x_lo) is rounded to
+// FP64 only at the end (its own relative precision is then 2^-53, which is
+// all a residual needs).
+#pragma clang fp contract(off)
+
+#include "dd.hpp"
+#include "engine.hpp"
+
+namespace nslx {
+namespace {
+
+struct dd { double h, l; };
+
+__device__ inline dd two_sum(double a, double b) {
+    const double s = a + b;
+    const double bb = s - a;
+    return {s, (a - (s - bb)) + (b - bb)};
+}
+__device__ inline dd quick(double a, double b) {
+    const double s = a + b;
+    return {s, b - (s - a)};
+}
+__device__ inline dd add(dd a, dd b) {
+    dd s = two_sum(a.h, b.h);
+    const dd t = two_sum(a.l, b.l);
+    s.l = s.l + t.h;
+    s = quick(s.h, s.l);
+    s.l = s.l + t.l;
+    return quick(s.h, s.l);
+}
+__device__ inline dd neg(dd a) { return {-a.h, -a.l}; }
+__device__ inline dd mul_d(dd a, double b) {
+    const double p = a.h * b;
+    double e = __builtin_fma(a.h, b, -p);
+    e = __builtin_fma(a.l, b, e);
+    return quick(p, e);
+}
+
+struct cdd { dd re, im; };
+
+__device__ inline cdd cadd(cdd a, cdd b) { return {add(a.re, b.re), add(a.im, b.im)}; }
+__device__ inline cdd neg_c(cdd a) { return {neg(a.re), neg(a.im)}; }
+// a * p for a complex double p; conj_p uses conj(p)
+__device__ inline cdd cmul_d(cdd a, double2 p, bool conj_p) {
+    const double pr = p.x, pi = conj_p ? -p.y : p.y;
+    const cdd out{add(mul_d(a.re, pr), neg(mul_d(a.im, pi))), add(mul_d(a.re, pi), mul_d(a.im, pr))};
+    return out;
+}
+__device__ inline cdd cscale(cdd a, double s) { return {mul_d(a.re, s), mul_d(a.im, s)}; }
+
+__device__ inline cdd load(const double2* h, const double2* l, long long i) {
+    const double2 a = h[i];
+    const double2 b = l ? l[i] : make_double2(0.0, 0.0);
+    return {quick(a.x, b.x), quick(a.y, b.y)};
+}
+
+// accumulator: sum of exact products e * v.h kept as (s, c) with TwoSum, the
+// products' rounding errors and e * v.l summed in c
+struct Acc {
+    double s = 0.0, c = 0.0;
+    __device__ inline void fma(double e, dd v) {
+        const double p = e * v.h;
+        const double pe = __builtin_fma(e, v.h, -p);
+        const dd t = two_sum(s, p);
+        s = t.h;
+        c = c + t.l + pe;
+        c = __builtin_fma(e, v.l, c);
+    }
+    __device__ inline dd value() const { return quick(s, c); }
+};
+
+constexpr int DCOLS = 128;   // threads per block: one output column each
+constexpr int DROWS = 8;     // output rows (rhs, t) per block
+constexpr int DKB = 32;      // K chunk staged in LDS
+
+// One hop of M^dag (OP 1) or M (OP 0) on a double-double field, rows
+// g = b*nt + t of a batch:
+//   M^dag: out_t = in_t - b'_t conj(P_t) (E in_{t+1})      b'_{nt-1} = -1
+//   M:     out_t = in_t - b_t E (P_{t-1} in_{t-1})          b_0 = -1
+// FINAL (M only): r = bvec - out, rounded; part[g * ncol + blockIdx.x] = |r_g|^2
+// over this block's columns.  ET: the transpose of the hop's E (ET[x][y] =
+// E[y][x], so that a block's threads read consecutive columns).  E is not
+// used as its own transpose: the FP64 exp(s delta K) is symmetric only to an
+// ulp, and at |x| / |b| ~ 1e8 that ulp is a 1e-9 change of the residual.
+template <int OP, bool FINAL>
+__global__ void __launch_bounds__(DCOLS) k_dd_hop(int n, int nt, long long nrows, int rpf, const double* __restrict__ ET,
+                                                  const double2* __restrict__ P, const double2* inh, const double2* inl,
+                                                  double2* outh, double2* outl, const double2* bvec, double2* r,
+                                                  double* part) {
+    __shared__ double sh[4][DROWS][DKB + 1];   // re.h, re.l, im.h, im.l of the staged input rows
+    const int tid = threadIdx.x;
+    const int y = blockIdx.x * DCOLS + tid;
+    const long long g0 = (long long)blockIdx.y * DROWS;
+    Acc ar[DROWS], ai[DROWS];
+    for (int x0 = 0; x0 < n; x0 += DKB) {
+        for (int e = tid; e < DROWS * DKB; e += DCOLS) {
+            const int rr = e / DKB, kk = e % DKB;
+            const long long g = g0 + rr;
+            const int x = x0 + kk;
+            cdd v{{0.0, 0.0}, {0.0, 0.0}};
+            if (g < nrows && x < n) {
+                const long long b = g / nt;
+                const int t = (int)(g % nt);
+                const int tin = OP == 0 ? (t == 0 ? nt - 1 : t - 1) : (t + 1 == nt ? 0 : t + 1);
+                const long long idx = (b * nt + tin) * n + x;
+                v = load(inh, inl, idx);
+                if (OP == 0) {
+                    const long long f = b / rpf;
+                    v = cmul_d(v, P[(f * nt + tin) * n + x], false);
+                }
+            }
+            sh[0][rr][kk] = v.re.h;
+            sh[1][rr][kk] = v.re.l;
+            sh[2][rr][kk] = v.im.h;
+            sh[3][rr][kk] = v.im.l;
+        }
+        __syncthreads();
+        if (y < n) {
+            const int kmax = min(DKB, n - x0);
+            for (int kk = 0; kk < kmax; ++kk) {
+                const double e = ET[(long long)(x0 + kk) * n + y];
+                #pragma unroll
+                for (int rr = 0; rr < DROWS; ++rr) {
+                    ar[rr].fma(e, dd{sh[0][rr][kk], sh[1][rr][kk]});
+                    ai[rr].fma(e, dd{sh[2][rr][kk], sh[3][rr][kk]});
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double rsq[DROWS];
+    #pragma unroll
+    for (int rr = 0; rr < DROWS; ++rr) {
+        rsq[rr] = 0.0;
+        const long long g = g0 + rr;
+        if (g >= nrows || y >= n) continue;
+        const long long b = g / nt;
+        const int t = (int)(g % nt);
+        const long long idx = g * n + y;
+        cdd res{ar[rr].value(), ai[rr].value()};
+        double sg;
+        if (OP == 1) {
+            const long long f = b / rpf;
+            res = cmul_d(res, P[(f * nt + t) * n + y], true);
+            sg = t == nt - 1 ? -1.0 : 1.0;
+        } else {
+            sg = t == 0 ? -1.0 : 1.0;
+        }
+        const cdd in = load(inh, inl, idx);
+        const cdd out = cadd(in, neg_c(cscale(res, sg)));
+        if (FINAL) {
+            const double2 bv = bvec[idx];
+            const cdd bb{{bv.x, 0.0}, {bv.y, 0.0}};
+            const cdd rd = cadd(bb, neg_c(out));
+            const double rr_ = rd.re.h + rd.re.l, ri = rd.im.h + rd.im.l;
+            r[idx] = make_double2(rr_, ri);
+            rsq[rr] = rr_ * rr_ + ri * ri;
+        } else {
+            outh[idx] = make_double2(out.re.h, out.im.h);
+            outl[idx] = make_double2(out.re.l, out.im.l);
+        }
+    }
+    if (FINAL) {
+        __shared__ double red[DROWS][DCOLS];
+        #pragma unroll
+        for (int rr = 0; rr < DROWS; ++rr) red[rr][tid] = rsq[rr];
+        __syncthreads();
+        if (tid < DROWS) {
+            const long long g = g0 + tid;
+            double sum = 0.0;
+            for (int k = 0; k < DCOLS; ++k) sum += red[tid][k];
+            if (g < nrows) part[g * gridDim.x + blockIdx.x] = sum;
+        }
+    }
+}
+
+// rn2[b] = sum over its nt rows and ncol column blocks, in index order
+__global__ void k_dd_rsum(const double* part, int nt, int ncol, long long nb, double* rn2) {
+    const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    double s = 0.0;
+    const long long m = (long long)nt * ncol;
+    for (long long k = 0; k < m; ++k) s += part[b * m + k];
+    rn2[b] = s;
+}
+
+// (xh, xl) += d for RHS b with done[b] < 0 (done may be null)
+__global__ void k_dd_accumulate(double2* xh, double2* xl, const double2* d, long long len, long long n,
+                                const int* done) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (done && done[i / len] >= 0) return;
+    const double2 h = xh[i], l = xl[i], v = d[i];
+    dd re = two_sum(h.x, v.x), im = two_sum(h.y, v.y);
+    re.l = re.l + l.x;
+    im.l = im.l + l.y;
+    re = quick(re.h, re.l);
+    im = quick(im.h, im.l);
+    xh[i] = make_double2(re.h, im.h);
+    xl[i] = make_double2(re.l, im.l);
+}
+
+__global__ void k_transpose(int n, const double* E, double* ET) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)n * n) return;
+    const int y = (int)(e / n), x = (int)(e % n);
+    ET[(long long)x * n + y] = E[e];
+}
+
+inline unsigned blocks1(long long n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+void DdWork::ensure(int64_t nb, int64_t len, int n, int nt) {
+    const size_t v = (size_t)nb * len;
+    uh.ensure(v);
+    ul.ensure(v);
+    const int ncol = (n + DCOLS - 1) / DCOLS;
+    part.ensure((size_t)nb * nt * ncol);
+}
+
+void dd_residual(FermionMatrix& fm, int op, const void* b, const void* xh, const void* xl, void* r, double* rn2,
+                 int64_t nb, DdWork& w, hipStream_t s) {
+    NSLX_REQUIRE(op == NSLX_OP_MMDAGGER || op == NSLX_OP_MDAGGERM, "dd residual: MMdagger or MdaggerM");
+    NSLX_REQUIRE(fm.disc == DISC_EXP, "dd residual: HubbardExp only");
+    fm.require_populated();
+    NSLX_REQUIRE(nb >= 1 && nb % fm.nfield == 0, "dd residual: nbatch must be a positive multiple of the fields");
+    const int n = fm.nx, nt = fm.nt;
+    const int rpf = (int)(nb / fm.nfield);
+    const int64_t len = fm.len();
+    w.ensure(nb, len, n, nt);
+    const double* E = fm.E(fm.sgn() * fm.delta(), s).dense.p;
+    if (w.et_src != E) {
+        w.et.ensure((size_t)n * n);
+        hipLaunchKernelGGL(k_transpose, dim3(blocks1((long long)n * n)), dim3(256), 0, s, n, E, w.et.p);
+        NSLX_LAUNCH_CHECK();
+        w.et_src = E;
+    }
+    const double* ET = w.et.p;
+    const double2* P = fm.P64.p;
+    const long long nrows = (long long)nb * nt;
+    const int ncol = (n + DCOLS - 1) / DCOLS;
+    dim3 g(ncol, (unsigned)((nrows + DROWS - 1) / DROWS));
+    const double2* ih = static_cast<const double2*>(xh);
+    const double2* il = static_cast<const double2*>(xl);
+    // first hop into (uh, ul), second into the residual
+    if (op == NSLX_OP_MMDAGGER) {
+        hipLaunchKernelGGL((k_dd_hop<1, false>), g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, ih, il, w.uh.p, w.ul.p,
+                           nullptr, nullptr, nullptr);
+        NSLX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_dd_hop<0, true>), g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, w.uh.p, w.ul.p, nullptr,
+                           nullptr, static_cast<const double2*>(b), static_cast<double2*>(r), w.part.p);
+    } else {
+        hipLaunchKernelGGL((k_dd_hop<0, false>), g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, ih, il, w.uh.p, w.ul.p,
+                           nullptr, nullptr, nullptr);
+        NSLX_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_dd_hop<1, true>), g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, w.uh.p, w.ul.p, nullptr,
+                           nullptr, static_cast<const double2*>(b), static_cast<double2*>(r), w.part.p);
+    }
+    NSLX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_dd_rsum, dim3(blocks1(nb)), dim3(256), 0, s, w.part.p, nt, ncol, (long long)nb, rn2);
+    NSLX_LAUNCH_CHECK();
+}
+
+void dd_accumulate(void* xh, void* xl, const void* d, int64_t nb, int64_t len, const int* done, hipStream_t s) {
+    const long long n = (long long)nb * len;
+    if (!n) return;
+    hipLaunchKernelGGL(k_dd_accumulate, dim3(blocks1(n)), dim3(256), 0, s, static_cast<double2*>(xh),
+                       static_cast<double2*>(xl), static_cast<const double2*>(d), (long long)len, n, done);
+    NSLX_LAUNCH_CHECK();
+}
+
+}  // namespace nslx

@@ -7,8 +7,10 @@
 #include <mutex>
 
 #include "common.hpp"
+#include "dd.hpp"
 #include "dense.hpp"
 #include "hop.hpp"
+#include "tss.hpp"
 #include "vec.hpp"
 
 namespace nslx {
@@ -75,6 +77,12 @@ struct FermionMatrix {
     VecScratch vsc;
     DevBuf<unsigned char> work;     // apply temporaries
     SolverWork sw;
+    uint64_t pop_gen = 0;           // incremented by every populate()
+    // time-slice direct solvers (FP64, FP32) of the populated fields, built on
+    // first use and refactorised after a populate (tss.hip)
+    std::unique_ptr<TimeSliceSolver> tss[2];
+    TimeSliceSolver& time_slice_solver(int prec, int nc, hipStream_t s);
+    DdWork ddw;
 
     int sgn() const { return species == NSLX_PARTICLE ? +1 : -1; }
     double delta() const { return beta / nt; }
@@ -114,6 +122,12 @@ int solve(FermionMatrix& fm, int op, int method, const void* b, void* x, const v
 // Outer CG on fm preconditioned by an inner CG on fp (CGpreconditioned).
 int solve_precond(FermionMatrix& fm, FermionMatrix& fp, int op, const void* b, void* x, const void* x0, int64_t nb,
                   double tol, int tol_kind, int64_t maxit, int inner_prec, nslx_solve_report* rep, hipStream_t s);
+
+// CG on MMdagger preconditioned by the time-slice solve in precision pprec
+// (FP64 / FP32), double-double defect correction to |b - A x| <= tol |b| per
+// RHS; x = high part, xlo (device c128 or null) = low part (tss_pcg.hip)
+int solve_tss(FermionMatrix& fm, int op, const void* b, void* x, void* xlo, const void* x0, int64_t nb, double tol,
+              int64_t maxit, int pprec, int nchunks, nslx_solve_report* rep, hipStream_t s);
 
 // log det / force (logdet.cpp)
 void fm_logdet(FermionMatrix& fm, int64_t field, int nchunks, double* out2, hipStream_t s);

@@ -135,6 +135,7 @@ void FermionMatrix::populate(const void* phi_dev, int64_t nf, int sp, hipStream_
         P32.alloc(n);
         nfield = nf;
     }
+    ++pop_gen;
     // deep copy of phi (the reference's populate copies into its own buffer)
     NSLX_HIP(hipMemcpyAsync(phi.p, phi_dev, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
     phase_tables(phi.p, n, sgn(), delta() * mu, P64.p, Pinv64.p, P32.p, s);
@@ -148,6 +149,21 @@ void FermionMatrix::populate(const void* phi_dev, int64_t nf, int sp, hipStream_
     // build (or find) the hop matrix of this species now, so that applies
     // after populate allocate nothing (nslx_fm_apply_noalloc relies on it)
     (void)Eptr(NSLX_F64, s);
+}
+
+TimeSliceSolver& FermionMatrix::time_slice_solver(int prec, int nc, hipStream_t s) {
+    NSLX_REQUIRE(prec == NSLX_F64 || prec == NSLX_F32, "time-slice solve: precision must be F64 or F32");
+    auto& t = tss[prec == NSLX_F64 ? 0 : 1];
+    // a new factorisation when the fields changed (count, chunking or values)
+    const double g = std::fabs(delta()) * normK + std::fabs(delta() * mu) + max_im_phi;
+    const int want = nc > 0 ? nc : nt / tss_chunk(nt, g);
+    if (!t || t->chunks() != want || t->fields() != nfield) {
+        t.reset();
+        t = std::make_unique<TimeSliceSolver>(*this, prec, want, s);
+    } else if (t->generation() != pop_gen) {
+        t->refactor(s);
+    }
+    return *t;
 }
 
 void FermionMatrix::require_populated() const {

@@ -1,0 +1,880 @@
+// Time-slice direct solve of HubbardExp's M and M^dag (algebra in tss.hpp).
+//
+// Setup per populated field set, in the solver's precision:
+//   1. chunk products T_j (c launches of the batched GEMM, every field and
+//      chunk at once): row i of Tt_j is T_j e_i, built as A_t applied to the
+//      unit vectors slice by slice;
+//   2. for k = 0 .. nc-2 the 2Nx-row matrix Q_k = [block rows k, k+1 of R |
+//      identity] is formed and its first Nx columns are eliminated by blocked
+//      Gauss-Jordan with partial pivoting (per 16-column block: pivot search
+//      and LU of the sub-panel in one workgroup per field, row swaps + pivot
+//      block inverse per column, rank-16 update of all other rows on MFMA);
+//   3. the last diagonal block is inverted the same way, and conjugate
+//      transposes of the stored blocks are taken for the R^H solve.
+// Apply (M^{-1}): c-1 in-chunk recurrence launches, the reduced forward and
+// back sweeps (one launch per block step, every field at once), c launches of
+// the in-chunk recurrence from the reduced solution; M^{-dag} mirrors it
+// with the backward recurrence, the chunks shifted by one slice, and R^H.
+#include <algorithm>
+#include <cmath>
+
+#include "engine.hpp"
+#include "tss.hpp"
+#include "vec.hpp"
+
+namespace nslx {
+namespace {
+
+template <class S>
+struct Z { S re, im; };
+
+template <class S>
+__device__ inline Z<S> ldz(const void* p, long long i) {
+    if constexpr (sizeof(S) == 8) { const double2 v = reinterpret_cast<const double2*>(p)[i]; return {v.x, v.y}; }
+    else { const float2 v = reinterpret_cast<const float2*>(p)[i]; return {v.x, v.y}; }
+}
+template <class S>
+__device__ inline void st(void* p, long long i, Z<S> v) {
+    if constexpr (sizeof(S) == 8) reinterpret_cast<double2*>(p)[i] = make_double2(v.re, v.im);
+    else reinterpret_cast<float2*>(p)[i] = make_float2(v.re, v.im);
+}
+template <class S>
+__device__ inline Z<S> zmul(Z<S> a, Z<S> b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+template <class S>
+__device__ inline Z<S> zsub(Z<S> a, Z<S> b) { return {a.re - b.re, a.im - b.im}; }
+template <class S>
+__device__ inline Z<S> zinv(Z<S> a) {
+    // 1/a with the larger component scaled out (no overflow for |a| near the range ends)
+    if (fabs((double)a.re) >= fabs((double)a.im)) {
+        const S r = a.im / a.re, d = a.re + a.im * r;
+        return {S(1) / d, -r / d};
+    }
+    const S r = a.re / a.im, d = a.re * r + a.im;
+    return {r / d, S(-1) / d};
+}
+
+// out[i][j] = (i == j) for an n x n complex matrix (ld n)
+template <class S>
+__global__ void k_ident(int n, void* out) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)n * n) return;
+    const int i = (int)(e / n), j = (int)(e % n);
+    st<S>(out, e, Z<S>{S(i == j ? 1 : 0), S(0)});
+}
+
+// Q_k (2n rows, ld W, width Wk = last ? 4n : 5n) of field blockIdx.z:
+//   rows [0, n):  [D | 0 | S | I_2n top]   or, last:  [D | S | I_2n top]
+//   rows [n, 2n): [-T_{k+1} | I | 0 | I_2n bottom]     [-T_{k+1} | I | I_2n bottom]
+// D, S: from the previous step's bottom rows (prevQ + n rows, cols n.. and
+// 2n..), or, for k = 0, D = I and S = -T_0.  Tt: row i of T_j is T_j e_i.
+template <class S>
+__global__ void k_build_q(int n, int k, int last, long long W, const void* Tt, long long strideT, const void* prevQ,
+                          long long strideQ, void* Q) {
+    const int f = blockIdx.z;
+    const int Wk = last ? 4 * n : 5 * n;
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 2LL * n * Wk) return;
+    const int row = (int)(e / Wk), col = (int)(e % Wk);
+    const char* tb = reinterpret_cast<const char*>(Tt) + (size_t)f * strideT * sizeof(Z<S>);
+    const long long nn = (long long)n * n;
+    auto T = [&](int j, int x, int i) {   // T_j[x][i] = Tt_j[i][x]
+        return ldz<S>(tb, (long long)j * nn + (long long)i * n + x);
+    };
+    const char* pq = prevQ ? reinterpret_cast<const char*>(prevQ) + (size_t)f * strideQ * sizeof(Z<S>) : nullptr;
+    Z<S> v{S(0), S(0)};
+    const int aug = last ? 2 * n : 3 * n;
+    if (col >= aug) {
+        v.re = S(col - aug == row ? 1 : 0);
+    } else if (row < n) {
+        const int blk = col / n, x = col % n;
+        const bool dcol = blk == 0, scol = last ? blk == 1 : blk == 2;
+        if (dcol) {
+            if (k == 0) v.re = S(x == row ? 1 : 0);
+            else v = ldz<S>(pq, (long long)(n + row) * W + n + x);
+        } else if (scol) {
+            if (k == 0) { const Z<S> t = T(0, row, x); v = {-t.re, -t.im}; }
+            else v = ldz<S>(pq, (long long)(n + row) * W + 2 * n + x);
+        }
+    } else {
+        const int r = row - n, blk = col / n, x = col % n;
+        if (blk == 0) { const Z<S> t = T(k + 1, r, x); v = {-t.re, -t.im}; }
+        else if (blk == 1) v.re = S(x == r ? 1 : 0);
+    }
+    st<S>(reinterpret_cast<char*>(Q) + (size_t)f * strideQ * sizeof(Z<S>), (long long)row * W + col, v);
+}
+
+// Qfin (n x 2n, ld 2n) = [D | I]: D from the last step's bottom rows (cols n..2n
+// of prevQ) or, with one chunk, I - T_0.
+template <class S>
+__global__ void k_build_fin(int n, long long W, const void* prevQ, long long strideQ, const void* Tt, long long strideT,
+                            void* Qf) {
+    const int f = blockIdx.z;
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 2LL * n * n) return;
+    const int row = (int)(e / (2 * n)), col = (int)(e % (2 * n));
+    Z<S> v{S(0), S(0)};
+    if (col >= n) {
+        v.re = S(col - n == row ? 1 : 0);
+    } else if (prevQ) {
+        v = ldz<S>(reinterpret_cast<const char*>(prevQ) + (size_t)f * strideQ * sizeof(Z<S>), (long long)(n + row) * W + n + col);
+    } else {
+        const Z<S> t = ldz<S>(reinterpret_cast<const char*>(Tt) + (size_t)f * strideT * sizeof(Z<S>), (long long)col * n + row);
+        v = {S(row == col ? 1 : 0) - t.re, -t.im};
+    }
+    st<S>(reinterpret_cast<char*>(Qf) + (size_t)f * 2LL * n * n * sizeof(Z<S>), e, v);
+}
+
+constexpr int GJB = 16;   // column block of the Gauss-Jordan elimination
+
+// One column block [jb, jb + w) of the Gauss-Jordan elimination of field
+// blockIdx.x's matrix (rows x ld, row-major): LU with partial pivoting of the
+// sub-panel rows [jb, rows) (the pivot search covers the rows not yet pivotal),
+// then Binv = (L_B U_B)^{-1} of the w x w pivot block and C = the sub-panel's
+// values before this block at every non-pivot row (rows above jb untouched,
+// rows below = L21 U_B), zero at the pivot rows.  piv[jj] = row swapped with
+// jb + jj.  Only the sub-panel columns of Q are modified.
+template <class S>
+__global__ void __launch_bounds__(256) k_gj_panel(void* Qb, long long ld, long long strideQ, int rows, int jb, int w,
+                                                  int* pivb, void* Binvb, void* Cb, int* singular) {
+    const int f = blockIdx.x;
+    char* Q = reinterpret_cast<char*>(Qb) + (size_t)f * strideQ * sizeof(Z<S>);
+    int* piv = pivb + f * GJB;
+    char* Binv = reinterpret_cast<char*>(Binvb) + (size_t)f * GJB * GJB * sizeof(Z<S>);
+    char* C = reinterpret_cast<char*>(Cb) + (size_t)f * rows * GJB * sizeof(Z<S>);
+    const int tid = threadIdx.x;
+    __shared__ double rv[256];
+    __shared__ int ri[256];
+    __shared__ int s_p;
+    auto at = [&](int i, int j) { return (long long)i * ld + j; };
+    for (int jj = 0; jj < w; ++jj) {
+        const int j = jb + jj;
+        // pivot: largest |Q[i][j]|^2 over i >= j, lowest index on ties
+        double best = -1.0;
+        int bi = j;
+        for (int i = j + tid; i < rows; i += 256) {
+            const Z<S> z = ldz<S>(Q, at(i, j));
+            const double a = (double)z.re * z.re + (double)z.im * z.im;
+            if (a > best) { best = a; bi = i; }
+        }
+        rv[tid] = best;
+        ri[tid] = bi;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (tid < o) {
+                const double a = rv[tid + o];
+                const int ia = ri[tid + o];
+                if (a > rv[tid] || (a == rv[tid] && ia < ri[tid])) { rv[tid] = a; ri[tid] = ia; }
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            s_p = ri[0];
+            piv[jj] = ri[0];
+            if (!(rv[0] > 0.0)) atomicExch(singular, 1);
+        }
+        __syncthreads();
+        const int p = s_p;
+        if (p != j && tid < w) {
+            const Z<S> a = ldz<S>(Q, at(j, jb + tid)), b = ldz<S>(Q, at(p, jb + tid));
+            st<S>(Q, at(j, jb + tid), b);
+            st<S>(Q, at(p, jb + tid), a);
+        }
+        __syncthreads();
+        const Z<S> d = ldz<S>(Q, at(j, j));
+        const Z<S> dinv = zinv<S>(d);
+        for (int i = j + 1 + tid; i < rows; i += 256) {
+            const Z<S> l = zmul<S>(ldz<S>(Q, at(i, j)), dinv);
+            st<S>(Q, at(i, j), l);
+            for (int cc = jj + 1; cc < w; ++cc) {
+                const Z<S> u = ldz<S>(Q, at(j, jb + cc));
+                st<S>(Q, at(i, jb + cc), zsub<S>(ldz<S>(Q, at(i, jb + cc)), zmul<S>(l, u)));
+            }
+        }
+        __syncthreads();
+    }
+    // Binv column c (thread c < w): solve L_B y = e_c, then U_B x = y
+    if (tid < w) {
+        Z<S> y[GJB];
+        for (int i = 0; i < w; ++i) {
+            Z<S> s{S(i == tid ? 1 : 0), S(0)};
+            for (int m = 0; m < i; ++m) s = zsub<S>(s, zmul<S>(ldz<S>(Q, at(jb + i, jb + m)), y[m]));
+            y[i] = s;
+        }
+        for (int i = w - 1; i >= 0; --i) {
+            Z<S> s = y[i];
+            for (int m = i + 1; m < w; ++m) s = zsub<S>(s, zmul<S>(ldz<S>(Q, at(jb + i, jb + m)), y[m]));
+            y[i] = zmul<S>(s, zinv<S>(ldz<S>(Q, at(jb + i, jb + i))));
+        }
+        for (int i = 0; i < GJB; ++i) st<S>(Binv, (long long)i * GJB + tid, i < w ? y[i] : Z<S>{S(0), S(0)});
+    } else if (tid < GJB) {
+        for (int i = 0; i < GJB; ++i) st<S>(Binv, (long long)i * GJB + tid, Z<S>{S(0), S(0)});
+    }
+    // C rows: above jb the current values, pivot rows 0, below L21 U_B
+    for (int i = tid; i < rows; i += 256) {
+        for (int cc = 0; cc < GJB; ++cc) {
+            Z<S> v{S(0), S(0)};
+            if (cc < w) {
+                if (i < jb) {
+                    v = ldz<S>(Q, at(i, jb + cc));
+                } else if (i >= jb + w) {
+                    for (int m = 0; m <= cc; ++m)
+                        v = {v.re + (ldz<S>(Q, at(i, jb + m)).re * ldz<S>(Q, at(jb + m, jb + cc)).re -
+                                     ldz<S>(Q, at(i, jb + m)).im * ldz<S>(Q, at(jb + m, jb + cc)).im),
+                             v.im + (ldz<S>(Q, at(i, jb + m)).re * ldz<S>(Q, at(jb + m, jb + cc)).im +
+                                     ldz<S>(Q, at(i, jb + m)).im * ldz<S>(Q, at(jb + m, jb + cc)).re)};
+                }
+            }
+            st<S>(C, (long long)i * GJB + cc, v);
+        }
+    }
+}
+
+// Columns [c0, width) of field blockIdx.y: the block's row swaps, then the
+// pivot rows become Binv x (also written transposed to YT, ld 16, for the
+// rank-w update of the other rows).
+template <class S>
+__global__ void __launch_bounds__(256) k_gj_swap(void* Qb, long long ld, long long strideQ, int jb, int w, int c0,
+                                                 int width, const int* pivb, const void* Binvb, void* YTb) {
+    const int f = blockIdx.y;
+    const int c = c0 + blockIdx.x * 256 + threadIdx.x;
+    if (c >= width) return;
+    char* Q = reinterpret_cast<char*>(Qb) + (size_t)f * strideQ * sizeof(Z<S>);
+    const int* piv = pivb + f * GJB;
+    const char* Binv = reinterpret_cast<const char*>(Binvb) + (size_t)f * GJB * GJB * sizeof(Z<S>);
+    char* YT = reinterpret_cast<char*>(YTb) + (size_t)f * ld * GJB * sizeof(Z<S>);
+    for (int jj = 0; jj < w; ++jj) {
+        const int p = piv[jj], j = jb + jj;
+        if (p != j) {
+            const Z<S> a = ldz<S>(Q, (long long)j * ld + c), b = ldz<S>(Q, (long long)p * ld + c);
+            st<S>(Q, (long long)j * ld + c, b);
+            st<S>(Q, (long long)p * ld + c, a);
+        }
+    }
+    Z<S> x[GJB];
+    #pragma unroll
+    for (int m = 0; m < GJB; ++m) x[m] = m < w ? ldz<S>(Q, (long long)(jb + m) * ld + c) : Z<S>{S(0), S(0)};
+    // Binv is zero outside its w x w block and x beyond w, so the full 16 x 16
+    // product is exact; rows m >= w of YT are written as zeros
+    #pragma unroll
+    for (int m = 0; m < GJB; ++m) {
+        Z<S> s{S(0), S(0)};
+        #pragma unroll
+        for (int l = 0; l < GJB; ++l) {
+            const Z<S> b = ldz<S>(Binv, (long long)m * GJB + l);
+            s = {s.re + (b.re * x[l].re - b.im * x[l].im), s.im + (b.re * x[l].im + b.im * x[l].re)};
+        }
+        if (m < w) st<S>(Q, (long long)(jb + m) * ld + c, s);
+        st<S>(YT, (long long)(c - c0) * GJB + m, s);
+    }
+}
+
+// out[i][j] = conj(in[j][i]), i < rows_out, j < cols_out; field blockIdx.z
+template <class S>
+__global__ void k_conj_t(const void* in, long long ldin, long long sin, void* out, long long ldout, long long sout,
+                         int rows_out, int cols_out) {
+    __shared__ Z<S> tile[32][33];
+    const int f = blockIdx.z;
+    const char* ib = reinterpret_cast<const char*>(in) + (size_t)f * sin * sizeof(Z<S>);
+    char* ob = reinterpret_cast<char*>(out) + (size_t)f * sout * sizeof(Z<S>);
+    const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 256 threads: 32 x 8
+    for (int r = ty; r < 32; r += 8) {
+        const int j = j0 + r, i = i0 + tx;                     // read in[j][i] along i
+        if (j < cols_out && i < rows_out) tile[r][tx] = ldz<S>(ib, (long long)j * ldin + i);
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + r, j = j0 + tx;
+        if (i < rows_out && j < cols_out) {
+            const Z<S> z = tile[tx][r];
+            st<S>(ob, (long long)i * ldout + j, Z<S>{z.re, -z.im});
+        }
+    }
+}
+
+inline unsigned blocks(long long n) { return (unsigned)((n + 255) / 256); }
+
+template <class S>
+void conj_t(const void* in, long long ldin, long long sin, void* out, long long ldout, long long sout, int rows_out,
+            int cols_out, int nf, hipStream_t s) {
+    dim3 g((cols_out + 31) / 32, (rows_out + 31) / 32, nf);
+    hipLaunchKernelGGL(k_conj_t<S>, g, dim3(256), 0, s, in, ldin, sin, out, ldout, sout, rows_out, cols_out);
+    NSLX_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+int tss_chunk(int nt, double g) {
+    int best = 1;
+    for (int c = 1; c <= std::min(nt, 16); ++c)
+        if (nt % c == 0 && (c == 1 || c * g <= 3.0)) best = c;
+    return best;
+}
+
+const void* TimeSliceSolver::E() const { return fm_.Eptr(prec_, nullptr); }
+const void* TimeSliceSolver::P() const { return fm_.Pptr(prec_); }
+
+TimeSliceSolver::TimeSliceSolver(FermionMatrix& fm, int prec, int nc, hipStream_t s)
+    : fm_(fm), prec_(prec), n_(fm.nx), nt_(fm.nt), nf_((int)fm.nfield) {
+    NSLX_REQUIRE(prec == NSLX_F64 || prec == NSLX_F32, "time-slice solve: precision must be F64 or F32");
+    NSLX_REQUIRE(fm.disc == DISC_EXP, "time-slice solve: HubbardExp only");
+    fm.require_populated();
+    esz_ = prec == NSLX_F64 ? 16 : 8;
+    if (nc > 0) {
+        NSLX_REQUIRE(nt_ % nc == 0, "time-slice solve: nchunks must divide Nt");
+        nc_ = nc;
+    } else {
+        const double g = std::fabs(fm.delta()) * fm.normK + std::fabs(fm.delta() * fm.mu) + fm.max_im_phi;
+        nc_ = nt_ / tss_chunk(nt_, g);
+    }
+    c_ = nt_ / nc_;
+    const size_t n = n_, nn = n * n;
+    Tt_.alloc((size_t)nf_ * nc_ * nn * esz_ * 2);   // chunk products + ping-pong half
+    if (nc_ >= 2) Q_.alloc((size_t)nf_ * (nc_ - 1) * (2 * n) * (5 * n) * esz_);
+    Qfin_.alloc((size_t)nf_ * n * 2 * n * esz_);
+    DinvH_.alloc((size_t)nf_ * nn * esz_);
+    if (nc_ >= 2) {
+        PhiH_.alloc((size_t)nf_ * (nc_ - 1) * 4 * nn * esz_);
+        WHcat_.alloc((size_t)nf_ * n * (nc_ - 1) * n * esz_);
+    }
+    if (nc_ >= 3) VH_.alloc((size_t)nf_ * (nc_ - 2) * nn * esz_);
+    ident_.alloc(nn * esz_);
+    if (prec_ == NSLX_F64) hipLaunchKernelGGL(k_ident<double>, dim3(blocks(nn)), dim3(256), 0, s, n_, ident_.p);
+    else hipLaunchKernelGGL(k_ident<float>, dim3(blocks(nn)), dim3(256), 0, s, n_, ident_.p);
+    NSLX_LAUNCH_CHECK();
+    setup(s);
+}
+
+size_t TimeSliceSolver::bytes() const {
+    size_t b = Tt_.n + Q_.n + Qfin_.n + DinvH_.n + PhiH_.n + WHcat_.n + VH_.n + ident_.n;
+    for (auto& kv : plans_) {
+        const ApplyBufs& B = *kv.second;
+        b += B.mid.n + B.atil.n + B.v[0].n + B.v[1].n + B.z.n + B.q.n + B.cp[0].n + B.cp[1].n + B.sb.n + B.in.n + B.out.n;
+    }
+    return b;
+}
+
+void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long strideQ, int rows, int width) {
+    const bool f64 = prec_ == NSLX_F64;
+    int* piv = gj_piv_.p;
+    int* sing = gj_sing_.p;
+    unsigned char* binv = gj_binv_.p;
+    unsigned char* C = gj_C_.p;
+    unsigned char* YT = gj_YT_.p;
+    const int nf = nf_;
+    const size_t e = esz_;
+    check_extent(Q, (long long)(nf - 1) * strideQ + (long long)(rows - 1) * ldq + width - 1, e, "Gauss-Jordan matrix");
+    NSLX_REQUIRE(width <= ldq, "Gauss-Jordan: width exceeds the row stride");
+    check_extent(C, (long long)nf * rows * GJB - 1, e, "Gauss-Jordan C");
+    check_extent(YT, (long long)nf * ldq * GJB - 1, e, "Gauss-Jordan YT");
+    check_extent(binv, (long long)nf * GJB * GJB - 1, e, "Gauss-Jordan Binv");
+    check_extent(piv, (long long)nf * GJB - 1, sizeof(int), "Gauss-Jordan pivots");
+    for (int jb = 0; jb < n_; jb += GJB) {
+        const int w = std::min(GJB, n_ - jb);
+        p.custom([=](hipStream_t s) {
+            if (f64)
+                hipLaunchKernelGGL(k_gj_panel<double>, dim3(nf), dim3(256), 0, s, Q, ldq, strideQ, rows, jb, w, piv, binv,
+                                   C, sing);
+            else
+                hipLaunchKernelGGL(k_gj_panel<float>, dim3(nf), dim3(256), 0, s, Q, ldq, strideQ, rows, jb, w, piv, binv, C,
+                                   sing);
+            NSLX_LAUNCH_CHECK();
+        });
+        const int c0 = jb + w;
+        const int ncol = width - c0;
+        p.custom([=](hipStream_t s) {
+            dim3 g(blocks(ncol), nf);
+            if (f64)
+                hipLaunchKernelGGL(k_gj_swap<double>, g, dim3(256), 0, s, Q, ldq, strideQ, jb, w, c0, width, piv, binv, YT);
+            else
+                hipLaunchKernelGGL(k_gj_swap<float>, g, dim3(256), 0, s, Q, ldq, strideQ, jb, w, c0, width, piv, binv, YT);
+            NSLX_LAUNCH_CHECK();
+        });
+        std::vector<BJob> jobs(nf);
+        for (int f = 0; f < nf; ++f) {
+            BJob& j = jobs[f];
+            j.A0 = C + (size_t)f * rows * GJB * e;
+            j.lda0 = GJB;
+            j.rows = rows;
+            j.cols = ncol;
+            j.K = w;
+            j.B = YT + (size_t)f * ldq * GJB * e;
+            j.ldb = GJB;
+            j.Y = Q + ((size_t)f * strideQ + c0) * e;
+            j.ldy = ldq;
+            j.Add = j.Y;
+            j.ldadd = ldq;
+            j.sgn = -1.0;
+        }
+        p.gemm(jobs, BVariant{prec_, true, 0, 0});
+    }
+}
+
+void TimeSliceSolver::refactor(hipStream_t s) {
+    fm_.require_populated();
+    NSLX_REQUIRE(fm_.nfield == nf_, "time-slice solve: the field count changed");
+    // the plans hold E and phase-table addresses: rebuilt if either moved
+    if (E() != e_used_ || P() != p_used_) {
+        plans_.clear();
+        setup_plan_ = BPlan();
+        setup_built_ = false;
+    }
+    setup(s);
+}
+
+void TimeSliceSolver::setup(hipStream_t s) {
+    if (!setup_built_) build_setup(s);
+    NSLX_HIP(hipMemsetAsync(gj_sing_.p, 0, sizeof(int), s));
+    setup_plan_.run(s);
+    int singular = 0;
+    NSLX_HIP(hipMemcpyAsync(&singular, gj_sing_.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    NSLX_HIP(hipStreamSynchronize(s));
+    if (singular) throw Error(NSLX_E_STATE, "time-slice solve: the reduced block system is singular");
+    gen_ = fm_.pop_gen;
+}
+
+void TimeSliceSolver::build_setup(hipStream_t s) {
+    const int n = n_, nf = nf_, nc = nc_, c = c_, nt = nt_;
+    const long long nn = (long long)n * n;
+    const bool f64 = prec_ == NSLX_F64;
+    const void* Ep = fm_.Eptr(prec_, s);
+    const long long ldE = e_ld(n);
+    const void* Pp = P();
+    e_used_ = Ep;
+    p_used_ = Pp;
+    BPlan& p = setup_plan_;
+    p = BPlan();
+    const long long W = 5LL * n;
+    gj_piv_.alloc((size_t)nf * GJB);
+    gj_sing_.alloc(1);
+    gj_binv_.alloc((size_t)nf * GJB * GJB * esz_);
+    gj_C_.alloc((size_t)nf * 2 * n * GJB * esz_);
+    gj_YT_.alloc((size_t)nf * W * GJB * esz_);
+    // 1. chunk products: step q writes buffer (c-1-q) & 1 (0 = Tt, 1 = ping-pong half)
+    {
+        unsigned char* buf[2] = {Tt_.p, Tt_.p + (size_t)nf * nc * nn * esz_};
+        for (int q = 0; q < c; ++q) {
+            unsigned char* dst = buf[(c - 1 - q) & 1];
+            unsigned char* src = buf[(c - q) & 1];
+            std::vector<BJob> jobs;
+            for (int f = 0; f < nf; ++f)
+                for (int j = 0; j < nc; ++j) {
+                    const int t = j * c + q;
+                    BJob b;
+                    const size_t off = ((size_t)f * nc + j) * nn * esz_;
+                    b.A0 = q == 0 ? (const void*)ident_.p : (const void*)(src + off);
+                    b.lda0 = n;
+                    b.rows = n;
+                    b.cols = n;
+                    b.K = n;
+                    b.B = Ep;
+                    b.ldb = ldE;
+                    b.Y = dst + off;
+                    b.ldy = n;
+                    b.ph = static_cast<const char*>(Pp) + ((size_t)f * nt + (t - 1 + nt) % nt) * n * esz_;
+                    b.ldph = 0;
+                    b.sgn = t == 0 ? -1.0 : 1.0;
+                    jobs.push_back(b);
+                }
+            p.gemm(jobs, BVariant{prec_, false, 1, 0});
+        }
+    }
+    const long long strideQ = 2LL * n * W;                 // per (field, step) in Q_
+    const long long strideT = (long long)nc * nn;          // per field in Tt
+    const long long fstride = (long long)(nc - 1) * strideQ;   // between fields in Q_
+    unsigned char* Tt = Tt_.p;
+    // 2. block steps (Q_ holds [field][step][2n][5n])
+    for (int k = 0; k + 1 < nc; ++k) {
+        const bool last = k == nc - 2;
+        unsigned char* Qk = Q_.p + (size_t)k * strideQ * esz_;
+        const unsigned char* prev = k == 0 ? nullptr : Q_.p + (size_t)(k - 1) * strideQ * esz_;
+        const int Wk = last ? 4 * n : 5 * n;
+        check_extent(Qk, (long long)(nf - 1) * fstride + (2LL * n - 1) * W + Wk - 1, esz_, "Q_k");
+        if (prev) check_extent(prev, (long long)(nf - 1) * fstride + (2LL * n - 1) * W + 3LL * n - 1, esz_, "Q_{k-1}");
+        check_extent(Tt, (long long)(nf - 1) * strideT + (long long)nc * nn - 1, esz_, "chunk products");
+        p.custom([=](hipStream_t st) {
+            dim3 g(blocks(2LL * n * Wk), 1, nf);
+            if (f64)
+                hipLaunchKernelGGL(k_build_q<double>, g, dim3(256), 0, st, n, k, last ? 1 : 0, W, Tt, strideT, prev,
+                                   fstride, Qk);
+            else
+                hipLaunchKernelGGL(k_build_q<float>, g, dim3(256), 0, st, n, k, last ? 1 : 0, W, Tt, strideT, prev, fstride,
+                                   Qk);
+            NSLX_LAUNCH_CHECK();
+        });
+        gj(p, Qk, W, fstride, 2 * n, Wk);
+    }
+    {
+        const unsigned char* prev = nc >= 2 ? Q_.p + (size_t)(nc - 2) * strideQ * esz_ : nullptr;
+        unsigned char* Qf = Qfin_.p;
+        check_extent(Qf, (long long)nf * 2 * nn - 1, esz_, "Qfin");
+        if (prev) check_extent(prev, (long long)(nf - 1) * fstride + (2LL * n - 1) * W + 2LL * n - 1, esz_, "Q_{nc-2}");
+        p.custom([=](hipStream_t st) {
+            dim3 g(blocks(2LL * nn), 1, nf);
+            if (f64)
+                hipLaunchKernelGGL(k_build_fin<double>, g, dim3(256), 0, st, n, W, prev, fstride, Tt, strideT, Qf);
+            else
+                hipLaunchKernelGGL(k_build_fin<float>, g, dim3(256), 0, st, n, W, prev, fstride, Tt, strideT, Qf);
+            NSLX_LAUNCH_CHECK();
+        });
+        gj(p, Qfin_.p, 2 * n, 2LL * nn, n, 2 * n);
+    }
+    // 3. conjugate transposes for the R^H solve
+    auto ct = [&](const void* in, long long ldin, long long sin, void* out, long long ldout, long long sout, int ro, int co) {
+        check_extent(in, (long long)(nf - 1) * sin + (long long)(co - 1) * ldin + ro - 1, esz_, "conj-transpose input");
+        check_extent(out, (long long)(nf - 1) * sout + (long long)(ro - 1) * ldout + co - 1, esz_, "conj-transpose output");
+        p.custom([=](hipStream_t st) {
+            if (f64) conj_t<double>(in, ldin, sin, out, ldout, sout, ro, co, nf, st);
+            else conj_t<float>(in, ldin, sin, out, ldout, sout, ro, co, nf, st);
+        });
+    };
+    ct(Qfin_.p + (size_t)n * esz_, 2 * n, 2LL * nn, DinvH_.p, n, nn, n, n);
+    if (nc >= 2) {
+        const long long wcat = (long long)(nc - 1) * n;
+        for (int k = 0; k + 1 < nc; ++k) {
+            const bool last = k == nc - 2;
+            const unsigned char* Qk = Q_.p + (size_t)k * strideQ * esz_;
+            const long long aug = last ? 2LL * n : 3LL * n;
+            ct(Qk + (size_t)aug * esz_, W, fstride, PhiH_.p + (size_t)k * 4 * nn * esz_, 2 * n, (long long)(nc - 1) * 4 * nn,
+               2 * n, 2 * n);
+            if (last) {
+                // WHcat block 0 = V_{nc-2}^H
+                ct(Qk + (size_t)n * esz_, W, fstride, WHcat_.p, wcat, (long long)n * wcat, n, n);
+            } else {
+                // VH_k = V_k^H;  WHcat block k + 1 = W_k^H
+                ct(Qk + (size_t)n * esz_, W, fstride, VH_.p + (size_t)k * nn * esz_, n, (long long)(nc - 2) * nn, n, n);
+                ct(Qk + (size_t)2 * n * esz_, W, fstride, WHcat_.p + (size_t)(k + 1) * n * esz_, wcat, (long long)n * wcat,
+                   n, n);
+            }
+        }
+    }
+    p.upload(s);
+    setup_launches_ = p.launches();
+    setup_built_ = true;
+}
+
+TimeSliceSolver::ApplyBufs& TimeSliceSolver::bufs(int64_t nb, hipStream_t s) {
+    auto it = plans_.find(nb);
+    if (it != plans_.end()) return *it->second;
+    NSLX_REQUIRE(nb >= 1 && nb % nf_ == 0, "time-slice solve: nbatch must be a positive multiple of the fields");
+    auto B = std::make_unique<ApplyBufs>();
+    const size_t vec = (size_t)nb * nt_ * n_ * esz_, red = (size_t)nb * nc_ * n_ * esz_, one = (size_t)nb * n_ * esz_;
+    B->in.alloc(vec); B->mid.alloc(vec); B->out.alloc(vec);
+    B->atil.alloc(red); B->z.alloc(red); B->q.alloc(red);
+    B->v[0].alloc(one); B->v[1].alloc(one); B->cp[0].alloc(one); B->cp[1].alloc(one); B->sb.alloc(one);
+    build_minv(B->Minv, B->in.p, B->mid.p, *B, nb);
+    build_mdinv(B->Mdinv, B->mid.p, B->out.p, *B, nb);
+    B->Minv.upload(s);
+    B->Mdinv.upload(s);
+    auto& ref = *B;
+    plans_.emplace(nb, std::move(B));
+    return ref;
+}
+
+void TimeSliceSolver::build_minv(BPlan& p, const void* y, void* x, ApplyBufs& B, int64_t nb) {
+    const int n = n_, nt = nt_, nc = nc_, c = c_;
+    const int rpf = (int)(nb / nf_);
+    const long long lv = (long long)nt * n;      // row stride of a field vector
+    const long long lr = (long long)nc * n;      // row stride of a reduced vector
+    const size_t e = esz_;
+    const char* Y = static_cast<const char*>(y);
+    char* X = static_cast<char*>(x);
+    const void* Ep = E();
+    const long long ldE = e_ld(n);
+    const char* Pp = static_cast<const char*>(P());
+    auto pass_job = [&](int t, const void* a, long long lda) {
+        BJob b;
+        b.A0 = a;
+        b.lda0 = lda;
+        b.rows = (int)nb;
+        b.cols = n;
+        b.K = n;
+        b.B = Ep;
+        b.ldb = ldE;
+        b.Y = X + (size_t)t * n * e;
+        b.ldy = lv;
+        b.Add = Y + (size_t)t * n * e;
+        b.ldadd = lv;
+        b.ph = Pp + (size_t)((t - 1 + nt) % nt) * n * e;
+        b.ldph = lv;
+        b.phdiv = rpf;
+        b.sgn = t == 0 ? -1.0 : 1.0;
+        return b;
+    };
+    // pass 1: in-chunk recurrence from zero inflow
+    for (int q = 1; q < c; ++q) {
+        std::vector<BJob> jobs;
+        for (int j = 0; j < nc; ++j) {
+            const int t = j * c + q;
+            jobs.push_back(pass_job(t, (q == 1 ? Y : X) + (size_t)(t - 1) * n * e, lv));
+        }
+        p.gemm(jobs, BVariant{prec_, false, 1, 0});
+    }
+    const char* wsrc = c == 1 ? Y : X;
+    auto w_at = [&](int j, int f) { return wsrc + ((size_t)f * rpf * lv + (size_t)(j * c + c - 1) * n) * e; };
+    char* Z_ = reinterpret_cast<char*>(B.z.p);
+    char* At = reinterpret_cast<char*>(B.atil.p);
+    auto zrow = [&](char* base, int f, int slot) { return base + ((size_t)f * rpf * lr + (size_t)slot * n) * e; };
+    auto vrow = [&](DevBuf<unsigned char>& b, int f) { return reinterpret_cast<char*>(b.p) + (size_t)f * rpf * n * e; };
+    const long long W = 5LL * n, strideQ = 2LL * n * W, fstride = (long long)(nc - 1) * strideQ;
+    auto Qk = [&](int f, int k) { return reinterpret_cast<const char*>(Q_.p) + ((size_t)f * fstride + (size_t)k * strideQ) * e; };
+    auto Qf = [&](int f) { return reinterpret_cast<const char*>(Qfin_.p) + (size_t)f * 2 * n * n * e; };
+    auto red_job = [&](int f) {
+        BJob b;
+        b.rows = rpf;
+        b.cols = n;
+        return b;
+    };
+    if (nc == 1) {
+        std::vector<BJob> jobs;
+        for (int f = 0; f < nf_; ++f) {
+            BJob b = red_job(f);
+            b.A0 = w_at(0, f); b.lda0 = lv; b.K = n;
+            b.B = Qf(f) + (size_t)n * e; b.ldb = 2 * n;
+            b.Y = zrow(Z_, f, 0); b.ldy = lr;
+            jobs.push_back(b);
+        }
+        p.gemm(jobs, BVariant{prec_, true, 0, 1});
+    } else {
+        for (int k = 0; k + 1 < nc; ++k) {
+            const bool last = k == nc - 2;
+            const long long aug = last ? 2LL * n : 3LL * n;
+            std::vector<BJob> jobs;
+            for (int f = 0; f < nf_; ++f)
+                for (int part = 0; part < 2; ++part) {
+                    BJob b = red_job(f);
+                    if (k == 0) { b.A0 = w_at(0, f); b.lda0 = lv; }
+                    else { b.A0 = vrow(B.v[k & 1], f); b.lda0 = n; }
+                    b.A1 = w_at(k + 1, f); b.lda1 = lv; b.kseg = n; b.K = 2 * n;
+                    b.B = Qk(f, k) + ((size_t)part * n * W + aug) * e; b.ldb = W;
+                    if (part == 0) { b.Y = zrow(At, f, k); b.ldy = lr; }
+                    else { b.Y = vrow(B.v[(k + 1) & 1], f); b.ldy = n; }
+                    jobs.push_back(b);
+                }
+            p.gemm(jobs, BVariant{prec_, true, 0, 1});
+        }
+        {
+            std::vector<BJob> jobs;
+            for (int f = 0; f < nf_; ++f) {
+                BJob b = red_job(f);
+                b.A0 = vrow(B.v[(nc - 1) & 1], f); b.lda0 = n; b.K = n;
+                b.B = Qf(f) + (size_t)n * e; b.ldb = 2 * n;
+                b.Y = zrow(Z_, f, nc - 1); b.ldy = lr;
+                jobs.push_back(b);
+            }
+            p.gemm(jobs, BVariant{prec_, true, 0, 1});
+        }
+        for (int k = nc - 2; k >= 0; --k) {
+            std::vector<BJob> jobs;
+            for (int f = 0; f < nf_; ++f) {
+                BJob b = red_job(f);
+                b.A0 = zrow(Z_, f, k + 1); b.lda0 = lr;
+                b.A1 = zrow(Z_, f, nc - 1); b.lda1 = lr; b.kseg = n;
+                b.K = k == nc - 2 ? n : 2 * n;
+                b.B = Qk(f, k) + (size_t)n * e; b.ldb = W;
+                b.Add = zrow(At, f, k); b.ldadd = lr;
+                b.Y = zrow(Z_, f, k); b.ldy = lr;
+                b.sgn = -1.0;
+                jobs.push_back(b);
+            }
+            p.gemm(jobs, BVariant{prec_, true, 0, 1});
+        }
+    }
+    // pass 2: in-chunk recurrence from the reduced solution
+    for (int q = 0; q < c; ++q) {
+        std::vector<BJob> jobs;
+        for (int j = 0; j < nc; ++j) {
+            const int t = j * c + q;
+            if (q == 0) jobs.push_back(pass_job(t, Z_ + (size_t)((j - 1 + nc) % nc) * n * e, lr));
+            else jobs.push_back(pass_job(t, X + (size_t)(t - 1) * n * e, lv));
+        }
+        p.gemm(jobs, BVariant{prec_, false, 1, 0});
+    }
+}
+
+void TimeSliceSolver::build_mdinv(BPlan& p, const void* y, void* x, ApplyBufs& B, int64_t nb) {
+    const int n = n_, nt = nt_, nc = nc_, c = c_;
+    const int rpf = (int)(nb / nf_);
+    const long long lv = (long long)nt * n, lr = (long long)nc * n;
+    const size_t e = esz_;
+    const char* Y = static_cast<const char*>(y);
+    char* X = static_cast<char*>(x);
+    const void* Ep = E();
+    const long long ldE = e_ld(n);
+    const char* Pp = static_cast<const char*>(P());
+    auto first = [&](int j) { return (j * c - 1 + nt) % nt; };     // f_j
+    auto lastix = [&](int j) { return (j * c + c - 2 + nt) % nt; };  // e_j
+    auto pass_job = [&](int s, const void* a, long long lda) {
+        BJob b;
+        b.A0 = a;
+        b.lda0 = lda;
+        b.rows = (int)nb;
+        b.cols = n;
+        b.K = n;
+        b.B = Ep;
+        b.ldb = ldE;
+        b.Y = X + (size_t)s * n * e;
+        b.ldy = lv;
+        b.Add = Y + (size_t)s * n * e;
+        b.ldadd = lv;
+        b.ph = Pp + (size_t)s * n * e;
+        b.ldph = lv;
+        b.phdiv = rpf;
+        b.sgn = s == nt - 1 ? -1.0 : 1.0;
+        return b;
+    };
+    for (int m = 1; m < c; ++m) {
+        std::vector<BJob> jobs;
+        for (int j = 0; j < nc; ++j) {
+            const int s = (lastix(j) - m + nt) % nt;
+            jobs.push_back(pass_job(s, (m == 1 ? Y : X) + (size_t)((s + 1) % nt) * n * e, lv));
+        }
+        p.gemm(jobs, BVariant{prec_, false, 2, 0});
+    }
+    const char* wsrc = c == 1 ? Y : X;
+    // what_k = w'_{k+1}, at slot f_{(k+1) mod nc}
+    auto wh_at = [&](int k, int f) { return wsrc + ((size_t)f * rpf * lv + (size_t)first((k + 1) % nc) * n) * e; };
+    char* Z_ = reinterpret_cast<char*>(B.z.p);
+    char* Qb = reinterpret_cast<char*>(B.q.p);
+    auto zrow = [&](char* base, int f, int slot) { return base + ((size_t)f * rpf * lr + (size_t)slot * n) * e; };
+    auto vrow = [&](DevBuf<unsigned char>& b, int f) { return reinterpret_cast<char*>(b.p) + (size_t)f * rpf * n * e; };
+    const long long nn = (long long)n * n;
+    auto DH = [&](int f) { return reinterpret_cast<const char*>(DinvH_.p) + (size_t)f * nn * e; };
+    auto red_job = [&]() {
+        BJob b;
+        b.rows = rpf;
+        b.cols = n;
+        return b;
+    };
+    if (nc == 1) {
+        std::vector<BJob> jobs;
+        for (int f = 0; f < nf_; ++f) {
+            BJob b = red_job();
+            b.A0 = wh_at(0, f); b.lda0 = lv; b.K = n;
+            b.B = DH(f); b.ldb = n;
+            b.Y = zrow(Z_, f, 0); b.ldy = lr;
+            jobs.push_back(b);
+        }
+        p.gemm(jobs, BVariant{prec_, true, 0, 1});
+    } else {
+        // q_0 = what_0 (a strided copy into the q array)
+        {
+            const char* src = wh_at(0, 0);
+            char* dst = Qb;
+            const size_t spitch = (size_t)lv * e, dpitch = (size_t)lr * e, width = (size_t)n * e;
+            const size_t height = (size_t)nb;
+            check_extent(src, (long long)((height - 1) * spitch + width) - 1, 1, "q_0 copy source");
+            check_extent(dst, (long long)((height - 1) * dpitch + width) - 1, 1, "q_0 copy destination");
+            p.custom([=](hipStream_t st) {
+                NSLX_HIP(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, hipMemcpyDeviceToDevice, st));
+            });
+        }
+        for (int k = 1; k + 1 < nc; ++k) {
+            std::vector<BJob> jobs;
+            for (int f = 0; f < nf_; ++f) {
+                BJob b = red_job();
+                b.A0 = zrow(Qb, f, k - 1); b.lda0 = lr; b.K = n;
+                b.B = reinterpret_cast<const char*>(VH_.p) + ((size_t)f * (nc - 2) + (k - 1)) * nn * e; b.ldb = n;
+                b.Add = wh_at(k, f); b.ldadd = lv;
+                b.Y = zrow(Qb, f, k); b.ldy = lr;
+                b.sgn = -1.0;
+                jobs.push_back(b);
+            }
+            p.gemm(jobs, BVariant{prec_, true, 0, 1});
+        }
+        const long long wcat = (long long)(nc - 1) * n;
+        {
+            std::vector<BJob> jobs;
+            for (int f = 0; f < nf_; ++f) {
+                BJob b = red_job();
+                b.A0 = zrow(Qb, f, nc - 2); b.lda0 = lr;
+                b.A1 = zrow(Qb, f, 0); b.lda1 = lr; b.kseg = n;
+                b.K = (int)wcat;
+                b.B = reinterpret_cast<const char*>(WHcat_.p) + (size_t)f * n * wcat * e; b.ldb = wcat;
+                b.Add = wh_at(nc - 1, f); b.ldadd = lv;
+                b.Y = vrow(B.sb, f); b.ldy = n;
+                b.sgn = -1.0;
+                jobs.push_back(b);
+            }
+            p.gemm(jobs, BVariant{prec_, true, 0, 1});
+        }
+        {
+            std::vector<BJob> jobs;
+            for (int f = 0; f < nf_; ++f) {
+                BJob b = red_job();
+                b.A0 = vrow(B.sb, f); b.lda0 = n; b.K = n;
+                b.B = DH(f); b.ldb = n;
+                b.Y = zrow(Qb, f, nc - 1); b.ldy = lr;
+                jobs.push_back(b);
+            }
+            p.gemm(jobs, BVariant{prec_, true, 0, 1});
+        }
+        for (int k = nc - 2; k >= 0; --k) {
+            std::vector<BJob> jobs;
+            for (int f = 0; f < nf_; ++f)
+                for (int part = 0; part < 2; ++part) {
+                    BJob b = red_job();
+                    b.A0 = zrow(Qb, f, k); b.lda0 = lr;
+                    if (k == nc - 2) { b.A1 = zrow(Qb, f, nc - 1); b.lda1 = lr; }
+                    else { b.A1 = vrow(B.cp[(k + 1) & 1], f); b.lda1 = n; }
+                    b.kseg = n;
+                    b.K = 2 * n;
+                    b.B = reinterpret_cast<const char*>(PhiH_.p) +
+                          (((size_t)f * (nc - 1) + k) * 4 * nn + (size_t)part * n * 2 * n) * e;
+                    b.ldb = 2 * n;
+                    if (part == 0) {
+                        if (k == 0) { b.Y = zrow(Z_, f, 0); b.ldy = lr; }
+                        else { b.Y = vrow(B.cp[k & 1], f); b.ldy = n; }
+                    } else {
+                        b.Y = zrow(Z_, f, k + 1); b.ldy = lr;
+                    }
+                    jobs.push_back(b);
+                }
+            p.gemm(jobs, BVariant{prec_, true, 0, 1});
+        }
+    }
+    // pass 2: backward recurrence from zhat_j = z'_{j+1}
+    for (int m = 0; m < c; ++m) {
+        std::vector<BJob> jobs;
+        for (int j = 0; j < nc; ++j) {
+            const int s = (lastix(j) - m + nt) % nt;
+            if (m == 0) jobs.push_back(pass_job(s, Z_ + (size_t)j * n * e, lr));
+            else jobs.push_back(pass_job(s, X + (size_t)((s + 1) % nt) * n * e, lv));
+        }
+        p.gemm(jobs, BVariant{prec_, false, 2, 0});
+    }
+}
+
+void TimeSliceSolver::apply(int op, const void* y, void* x, int64_t nb, bool io_c128, hipStream_t s) {
+    NSLX_REQUIRE(op == NSLX_OP_M || op == NSLX_OP_MDAGGER || op == NSLX_OP_MMDAGGER, "time-slice solve: bad op");
+    NSLX_REQUIRE(y && x && y != x, "time-slice solve: null or aliased buffers");
+    NSLX_REQUIRE(gen_ == fm_.pop_gen, "time-slice solve: the fermion matrix was re-populated since the factorisation");
+    ApplyBufs& B = bufs(nb, s);
+    const int64_t n = nb * (int64_t)nt_ * n_;
+    const size_t bytes = (size_t)n * esz_;
+    const bool conv = io_c128 && prec_ == NSLX_F32;
+    auto in = [&](void* dst) {
+        if (conv) c128_to_c64(y, dst, n, s);
+        else NSLX_HIP(hipMemcpyAsync(dst, y, bytes, hipMemcpyDeviceToDevice, s));
+    };
+    auto out = [&](const void* src) {
+        if (conv) c64_to_c128(src, x, n, s);
+        else NSLX_HIP(hipMemcpyAsync(x, src, bytes, hipMemcpyDeviceToDevice, s));
+    };
+    if (op == NSLX_OP_M) {
+        in(B.in.p);
+        B.Minv.run(s);
+        out(B.mid.p);
+    } else if (op == NSLX_OP_MDAGGER) {
+        in(B.mid.p);
+        B.Mdinv.run(s);
+        out(B.out.p);
+    } else {
+        in(B.in.p);
+        B.Minv.run(s);
+        B.Mdinv.run(s);
+        out(B.out.p);
+    }
+}
+
+}  // namespace nslx

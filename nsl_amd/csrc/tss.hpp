@@ -1,0 +1,167 @@
+// Time-slice direct solve of the HubbardExp fermion matrix (M^{-1}, M^{-dag})
+// and the batched complex GEMM it runs on.  No counterpart in the reference,
+// which only applies M (hubbardExp.tpp:12-125) and solves with Krylov methods
+// (CG.tpp:14-113); here the solve serves as the preconditioner of CG on
+// MM^dag (tss_pcg.cpp).
+//
+// Algebra.  (M x)_t = x_t - A_t x_{t-1} with the transfer matrices
+// A_t = b_t E diag(P_{t-1}) (b_0 = -1), so M x = y is the recurrence
+// x_t = y_t + A_t x_{t-1} around the time circle.  The Nt slices are cut into
+// nc chunks of c slices, [t_j, t_j + c); eliminating the inner slices leaves
+// the block-cyclic system R z = w of size nc*Nx for the last slice of each
+// chunk, z_j = x_{t_j + c - 1}:
+//     R = I - S,   S_{j, j-1} = T_j = A_{t_j + c - 1} ... A_{t_j}   (j-1 mod nc),
+// w_j = the chunk's forward recurrence from y with zero inflow.  Chunks are
+// short enough (growth of T_j ~ e^{c delta ||K||} <= e^3) that T_j is well
+// conditioned; R is factorised once per field by Gauss-Jordan elimination
+// with partial pivoting, block column by block column: step k eliminates
+// block column k from block rows k, k+1 (the only rows where it is nonzero;
+// fill appears only in the last block column), which leaves
+//     top rows:    [I | V_k | W_k | G_k]      bottom rows: [0 | D_{k+1} | S_{k+1} | F_k]
+// where [G_k; F_k] (2Nx x 2Nx) is the whole elimination (row swaps included)
+// applied to the identity.  The solve is then a forward sweep
+// [a_k; v_{k+1}] = [G_k; F_k] [v_k; w_{k+1}], z_{nc-1} = D^{-1} v_{nc-1}, and a
+// back sweep z_k = a_k - V_k z_{k+1} - W_k z_{nc-1}; every step is a GEMM
+// with a stored matrix.  M^dag = (A'_s = A_{s+1}^H, backward recurrence) uses
+// the same factorisation through R^H (chunks shifted by one slice).
+#pragma once
+
+#include <functional>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+
+namespace nslx {
+
+struct FermionMatrix;
+
+// One job of a batched NT GEMM launch:
+//   Y[r][c] = Add[r][c] + sgn * sum_k A[r][k] B[c][k]            (PH = 0)
+//   A[r][k] <- A[r][k] * ph_r[k]                                  (PH = 1)
+//   Y[r][c] = Add[r][c] + sgn * conj(ph_r[c]) * sum_k A[r][k] B[c][k]   (PH = 2)
+// for r < rows, c < cols, k < K.  A's row r is A0 + r*lda0 for k < kseg and
+// A1 + r*lda1 (shifted by kseg) beyond; ph_r = ph + (r / phdiv) * ldph.  All
+// strides count complex elements (B: real elements for a real B).  Add may
+// be null, and may equal Y (each element is read and written by one thread).
+struct BJob {
+    const void* A0 = nullptr;
+    const void* A1 = nullptr;
+    long long lda0 = 0, lda1 = 0;
+    int kseg = 1 << 30;
+    int rows = 0, cols = 0, K = 0;
+    const void* B = nullptr;
+    long long ldb = 0;
+    void* Y = nullptr;
+    long long ldy = 0;
+    const void* Add = nullptr;
+    long long ldadd = 0;
+    const void* ph = nullptr;
+    long long ldph = 0;
+    int phdiv = 1 << 30;
+    double sgn = 1.0;
+};
+
+// Kernel variant of a launch: element precision (NSLX_F64 / NSLX_F32),
+// complex or real B, phase mode, tile (0: 64 x 64, 1: 32 x 32).
+struct BVariant {
+    int prec = NSLX_F64;
+    bool cb = true;
+    int ph = 0;
+    int tile = 0;
+};
+
+// Tile list of a set of jobs for a variant.
+void bgemm_tiles(const std::vector<BJob>& jobs, int tile, std::vector<int4>& out);
+int bgemm_tile_rows(int tile);
+void launch_bgemm(const BVariant& v, const BJob* d_jobs, const int4* d_tiles, int ntiles, hipStream_t s);
+
+// A recorded sequence of launches whose job and tile arrays live in one
+// device buffer each (uploaded once); replayed by run().
+class BPlan {
+   public:
+    // append a batched GEMM over jobs (tile shape picked from the tile count
+    // when tile < 0)
+    void gemm(const std::vector<BJob>& jobs, BVariant v, int tile = -1);
+    // append any other launch
+    void custom(std::function<void(hipStream_t)> fn);
+    void upload(hipStream_t s);
+    void run(hipStream_t s) const;
+    size_t launches() const { return ops_.size(); }
+
+   private:
+    struct Op {
+        bool is_gemm = false;
+        BVariant v;
+        size_t job0 = 0, tile0 = 0;
+        int ntiles = 0;
+        std::function<void(hipStream_t)> fn;
+    };
+    std::vector<Op> ops_;
+    std::vector<BJob> jobs_;
+    std::vector<int4> tiles_;
+    DevBuf<BJob> djobs_;
+    DevBuf<int4> dtiles_;
+    bool uploaded_ = false;
+};
+
+// Chunk length of the time-slice solve for nt slices whose one-slice growth
+// is g (delta ||K|| + |delta mu| + max |Im phi|): the largest divisor c of nt
+// with c g <= 3 (at least 1, at most 16).
+int tss_chunk(int nt, double g);
+
+// Factorisation of R per field and the apply plans, in one precision.
+class TimeSliceSolver {
+   public:
+    // Factorises R for every populated field of fm (chunk products, blocked
+    // Gauss-Jordan) in precision prec (NSLX_F64 or NSLX_F32).  nc <= 0 picks
+    // the chunk count from tss_chunk.
+    TimeSliceSolver(FermionMatrix& fm, int prec, int nc, hipStream_t s);
+    // x = M^{-1} y (op NSLX_OP_M), M^{-dag} y (NSLX_OP_MDAGGER) or
+    // (M M^dag)^{-1} y (NSLX_OP_MMDAGGER) for nb RHS (nb a multiple of the
+    // fields).  y and x: distinct device buffers, c128 when io_c128 (converted
+    // to and from an FP32 solver's precision), else in the solver's precision.
+    void apply(int op, const void* y, void* x, int64_t nb, bool io_c128, hipStream_t s);
+    int prec() const { return prec_; }
+    int chunks() const { return nc_; }
+    int chunk_len() const { return c_; }
+    int64_t fields() const { return nf_; }
+    // setup launches and device bytes held
+    size_t setup_launches() const { return setup_launches_; }
+    size_t bytes() const;
+    // generation of the populate() the factorisation belongs to
+    uint64_t generation() const { return gen_; }
+    // factorise again for the fields now populated (same count and chunking)
+    void refactor(hipStream_t s);
+
+   private:
+    struct ApplyBufs {
+        DevBuf<unsigned char> mid, atil, v[2], z, q, cp[2], sb;
+        BPlan Minv, Mdinv;     // M^{-1}: y -> x; M^{-dag}: y -> x (both via bound buffers)
+        DevBuf<unsigned char> in, out;
+    };
+    FermionMatrix& fm_;
+    int prec_, n_, nt_, nc_, c_, nf_;
+    size_t esz_;
+    uint64_t gen_ = 0;
+    size_t setup_launches_ = 0;
+    DevBuf<unsigned char> Tt_, Q_, Qfin_, PhiH_, VH_, WHcat_, DinvH_, ident_;
+    std::map<int64_t, std::unique_ptr<ApplyBufs>> plans_;
+    BPlan setup_plan_;
+    bool setup_built_ = false;
+    const void* e_used_ = nullptr;
+    const void* p_used_ = nullptr;
+    DevBuf<int> gj_piv_, gj_sing_;
+    DevBuf<unsigned char> gj_binv_, gj_C_, gj_YT_;
+    void setup(hipStream_t s);
+    void build_setup(hipStream_t s);
+    void gj(BPlan& p, unsigned char* Q, long long ldq, long long strideQ, int rows, int width);
+    ApplyBufs& bufs(int64_t nb, hipStream_t s);
+    void build_minv(BPlan& p, const void* y, void* x, ApplyBufs& B, int64_t nb);
+    void build_mdinv(BPlan& p, const void* y, void* x, ApplyBufs& B, int64_t nb);
+    const void* E() const;
+    const void* P() const;
+};
+
+}  // namespace nslx

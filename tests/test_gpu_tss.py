@@ -1,0 +1,178 @@
+"""The time-slice direct solve (nslx_fm_solve_direct) and CG on MM^dag
+preconditioned by it inside a double-double defect correction
+(nslx_solve_tss), through the C ABI.
+
+Parity: the direct solves are checked by their residual against the
+oracle's own M / M^dag (hubbardExp.tpp:12-75 restated): |M x - y| / |y| <=
+1e-10 in FP64 (the lattices here have cond(M) < 1e4) and <= 1e-3 in FP32.
+The preconditioned solver must bring every right-hand side to
+|b - MM^dag (x + x_lo)| <= 1e-12 |b|; that residual is recomputed on the CPU
+in double-double (tests/ddarith.py, pinned against exact rational
+arithmetic) from the FP64 E and phase table the engine exported, so the
+check does not trust the device's own residual kernel."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import nsl_amd as nsl
+from oracle import fermion as F, lattice as olat, rng
+from tests import ddarith as dd
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def make(kind, size, Nt, beta, nfield=1, seed=1000, std=0.7, mu=0.0, species=nsl.Particle, im_std=0.0):
+    if kind == "honeycomb":
+        lat_d, lat_o = nsl.Lattice.honeycomb(*size), olat.honeycomb(*size)
+    elif kind == "ring":
+        lat_d, lat_o = nsl.Lattice.ring(size), olat.ring(size)
+    else:
+        lat_d, lat_o = nsl.Lattice.armchair_nanotube(*size), olat.armchair_nanotube(*size)
+    Nx = lat_d.sites()
+    phis = []
+    for f in range(nfield):
+        p = torch.as_tensor(rng.real_field(seed + f, 0, (Nt, Nx), std))
+        if im_std:
+            p = p + 1j * torch.as_tensor(rng.real_field(seed + 500 + f, 0, (Nt, Nx), im_std))
+        phis.append(p.to(torch.complex128))
+    fm = nsl.HubbardExp(lat_d, Nt, beta, mu, species)
+    fm.populate(torch.stack(phis).to(DEV))
+    ofs = []
+    for p in phis:
+        o = F.HubbardExp(lat_o, Nt, beta, mu, species)
+        o.populate(p)
+        ofs.append(o)
+    return fm, ofs
+
+
+def oracle_apply(ofs, name, psi):
+    rpf = psi.shape[0] // len(ofs)
+    return torch.stack([getattr(ofs[k // rpf], name)(psi[k]) for k in range(psi.shape[0])])
+
+
+def rel(a, b):
+    return float(((a - b).abs() ** 2).sum().sqrt() / (b.abs() ** 2).sum().sqrt())
+
+
+# (kind, size, Nt, beta, extra): the chunk count comes out as 4, 8, 2, 4, 11
+# (Nt prime: one slice per chunk) and, forced, 1 (the whole product)
+CASES = [
+    ("honeycomb", (3, 3), 16, 4.0, {}),
+    ("honeycomb", (6, 6), 32, 6.0, {}),
+    ("ring", 10, 12, 2.0, {}),
+    ("nanotube", (3, 4), 20, 3.0, {}),
+    ("honeycomb", (3, 3), 11, 2.0, {}),
+    ("ring", 8, 8, 1.0, {"nchunks": 1}),
+    # complex phi, hole species, nonzero mu (the reference's complex-phi tests,
+    # test_fermionMatrixHubbardExp.cpp:150-151)
+    ("honeycomb", (3, 3), 16, 4.0, {"im_std": 0.3, "species": nsl.Hole, "mu": 0.3}),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("prec", [nsl.F64, nsl.F32])
+def test_direct_solve_residual(case, prec):
+    kind, size, Nt, beta, extra = CASES[case]
+    nch = extra.get("nchunks", 0)
+    kw = {k: v for k, v in extra.items() if k != "nchunks"}
+    fm, ofs = make(kind, size, Nt, beta, nfield=2, seed=1100 + case, **kw)
+    Nx = fm.Nx
+    y = torch.as_tensor(rng.complex_normal(1200 + case, 0, (4, Nt, Nx)))
+    tol = 1e-10 if prec == nsl.F64 else 1e-3
+    for op, name in ((nsl.MatrixCombination.M, "M"), (nsl.MatrixCombination.Mdagger, "Mdagger")):
+        x = fm.solve_direct(op, y.to(DEV), precision=prec, nchunks=nch).to("cpu", torch.complex128)
+        r = rel(oracle_apply(ofs, name, x), y)
+        assert r < tol, (name, r)
+    # (M M^dag)^{-1} = M^{-dag} M^{-1}: residual of MM^dag, FP64 only (FP32's
+    # error is cond(M) * eps32 amplified once more)
+    if prec == nsl.F64:
+        x = fm.solve_direct(nsl.MatrixCombination.MMdagger, y.to(DEV), nchunks=nch).cpu()
+        r = rel(oracle_apply(ofs, "MMdagger", x), y)
+        assert r < 1e-8, r
+
+
+def cpu_dd_check(fm, b, x, x_lo, rpf):
+    E, P = fm.export()
+    _, relv = dd.mmdag_residual(E, P, b.cpu().numpy(), x.cpu().numpy(), x_lo.cpu().numpy(), rpf=rpf)
+    return relv
+
+
+@pytest.mark.parametrize("species", [nsl.Particle, nsl.Hole])
+def test_dd_residual_matches_cpu(species):
+    """nslx_dd_residual against the CPU double-double residual on the same
+    exported operator, at a near-solution (the FP64 direct solve) whose
+    residual is dominated by cancellation, with a nonzero low part."""
+    Nt, beta, U = 32, 6.0, 3.0
+    fm, _ = make("honeycomb", (6, 6), Nt, beta, nfield=2, seed=1300, std=math.sqrt(beta / Nt * U),
+                 species=species, im_std=0.1)
+    b = torch.as_tensor(rng.complex_normal(1301, 0, (4, Nt, 72)))
+    x = fm.solve_direct(nsl.MatrixCombination.MMdagger, b.to(DEV))
+    lo = (x * 1e-17).contiguous()
+    dev = nsl.dd_residual(fm, b.to(DEV), x, lo)
+    relv = cpu_dd_check(fm, b, x, lo, rpf=2)
+    assert np.allclose(dev, relv, rtol=1e-9, atol=0.0), (dev, relv)
+
+
+@pytest.mark.parametrize("prec", [nsl.F64, nsl.F32])
+def test_tss_pcg_reaches_1e12(prec):
+    """Honeycomb 6x6 (Nx = 72), Nt = 64, beta = 8, U = 3: random fields whose
+    MM^dag has condition numbers up to ~1e10, where an FP64 residual cannot
+    go below ~1e-7; the double-double solution must reach 1e-12."""
+    Nt, beta, U = 64, 8.0, 3.0
+    fm, ofs = make("honeycomb", (6, 6), Nt, beta, nfield=2, seed=1000, std=math.sqrt(beta / Nt * U))
+    b = torch.as_tensor(rng.complex_normal(2000, 0, (6, Nt, 72)))
+    cg = nsl.CGTimeSlice(fm, 1e-12, 2000, precision=prec)
+    x = cg(b.to(DEV))
+    rep = cg.report
+    assert cg.converged, rep
+    assert rep["max_rel_residual"] <= 1e-12, rep
+    relv = cpu_dd_check(fm, b, x, cg.x_lo, rpf=3)
+    assert relv.max() <= 1.05e-12, relv
+    # the device's double-double residual agrees with the CPU's
+    dev = nsl.dd_residual(fm, b.to(DEV), x, cg.x_lo)
+    assert np.allclose(dev, relv, rtol=1e-6, atol=1e-20), (dev, relv)
+    # x alone (rounded to FP64) is at the FP64 floor, far above 1e-12 for the
+    # ill-conditioned field: x_lo carries the digits
+    xonly = cpu_dd_check(fm, b, x, torch.zeros_like(x), rpf=3)
+    assert xonly.max() > 1e-11
+
+
+def test_tss_full_size_bench_workload():
+    """The bench workload (12x12 honeycomb, Nx = 288, Nt = 64, beta = 8, U = 3,
+    16 fields x 8 RHS) with the FP32 preconditioner: every RHS to 1e-12, and
+    one RHS of the first field checked on the CPU in double-double."""
+    L, Nt, beta, U = 12, 64, 8.0, 3.0
+    Nx = 2 * L * L
+    std = math.sqrt(beta / Nt * U)
+    phi = torch.stack([torch.as_tensor(rng.real_field(1000 + f, 0, (Nt, Nx), std)) for f in range(16)])
+    fm = nsl.HubbardExp(nsl.Lattice.honeycomb(L, L), Nt, beta)
+    fm.populate(phi.to(torch.complex128).to(DEV))
+    b = torch.stack([torch.as_tensor(rng.complex_normal(2000 + 16 * f + k, 0, (Nt, Nx)))
+                     for f in range(16) for k in range(8)])
+    cg = nsl.CGTimeSlice(fm, 1e-12, 2000, precision=nsl.F32)
+    x = cg(b.to(DEV))
+    assert cg.converged, cg.report
+    assert cg.report["max_rel_residual"] <= 1e-12
+    E, P = fm.export()
+    _, relv = dd.mmdag_residual(E, P[:1], b[:1].numpy(), x[:1].cpu().numpy(), cg.x_lo[:1].cpu().numpy(), rpf=1)
+    assert relv.max() <= 1.05e-12, relv
+
+
+def test_tss_errors():
+    lat = nsl.Lattice.honeycomb(3, 3)
+    fd = nsl.HubbardDiag(lat, 16, 4.0)
+    fd.populate(torch.zeros((16, 18), dtype=torch.complex128, device=DEV))
+    y = torch.zeros((1, 16, 18), dtype=torch.complex128, device=DEV)
+    with pytest.raises(nsl.NslxError):
+        fd.solve_direct(nsl.MatrixCombination.M, y)
+    fm = nsl.HubbardExp(lat, 16, 4.0)
+    fm.populate(torch.zeros((16, 18), dtype=torch.complex128, device=DEV))
+    with pytest.raises(nsl.NslxError):
+        fm.solve_direct(nsl.MatrixCombination.MdaggerM, y)
+    with pytest.raises(nsl.NslxError):                  # nchunks must divide Nt
+        fm.solve_direct(nsl.MatrixCombination.M, y, nchunks=3)
