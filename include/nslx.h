@@ -65,10 +65,14 @@ typedef enum {
        absolute stop sum_all |r|^2 <= tol^2 on the recursive residual
        (CG.hpp:34, CG.tpp:70, inner_product.tpp:31-33)                      */
     NSLX_TOL_ABS_BATCH = 0,
-    /* per right-hand side alpha/beta; stop each RHS when
-       |r_k| <= tol * |b_k| (recursive residual for F64; for F32/BF16X3 the
-       FP64 true residual of the defect-correction loop)                     */
-    NSLX_TOL_REL_PER_RHS = 1
+    /* per right-hand side alpha/beta; stop each RHS when its FP64 true
+       relative residual |b_k - A x_k| <= tol |b_k| (reliable updates), or
+       at the FP64 attainable accuracy c eps ||A|| |x_k| when that lies above
+       tol |b_k| -- such a floor stop is reported (report.floor_stops) and
+       the call returns NSLX_NOT_CONVERGED (nslx_solve_dd reaches tol)      */
+    NSLX_TOL_REL_PER_RHS = 1,
+    /* as NSLX_TOL_REL_PER_RHS, but a floor stop counts as converged          */
+    NSLX_TOL_REL_PER_RHS_FLOOR = 2
 } nslx_tol_kind;
 
 typedef struct nslx_ctx_s* nslx_ctx;
@@ -202,6 +206,20 @@ typedef struct {
 int nslx_solve(nslx_fm fm, int op, int method, const void* b, void* x, const void* x0,
                int64_t nbatch, double tol, int tol_kind, int64_t maxit, int precision,
                int check_true_residual, nslx_solve_report* report, void* stream);
+
+/* CG::operator()(b[, x0]) on MMdagger / MdaggerM (CG.tpp:14-113) per RHS to
+ * |b_k - A (x_k + x_lo_k)| <= tol |b_k| with the solution in double-double:
+ * the reliable-update recurrence of nslx_solve (NSLX_TOL_REL_PER_RHS) in
+ * `precision`, its increments accumulated into x + x_lo with TwoSum, and,
+ * once an FP64 true residual comes within 1e4 of the FP64 floor
+ * eps ||A|| |x|, every later residual replacement evaluated in double-double
+ * (nslx_dd_residual).  Reaches targets that an FP64 x cannot represent (the
+ * ill-conditioned MM^dag of random fields at large beta).  x: the solution
+ * rounded to FP64; x_lo (device c128 or NULL): the remainder.  x0 NULL
+ * means x0 = 0.  HubbardExp only.  report.max_rel_residual is the final
+ * true residual (double-double once switched), report.floor_stops = 0. */
+int nslx_solve_dd(nslx_fm fm, int op, const void* b, void* x, void* x_lo, const void* x0, int64_t nbatch,
+                  double tol, int64_t maxit, int precision, nslx_solve_report* report, void* stream);
 
 /* Preconditioned CG.  Replaces NSL::LinAlg::CGpreconditioned<Type>
  * (src/NSL/LinAlg/Solver/Impl/CGpreconditioned.hpp:34-41 (std::function M,

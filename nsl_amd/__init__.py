@@ -18,7 +18,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (NslxError, SolveReport, HmcReport, F64, F32, BF16X3, TOL_ABS_BATCH,  # noqa: F401
-                   TOL_REL_PER_RHS)
+                   TOL_REL_PER_RHS, TOL_REL_PER_RHS_FLOOR)
 
 log = logging.getLogger("nsl_amd")
 
@@ -284,7 +284,7 @@ class _Solver:
     method = _lib.CG_METHOD
 
     def __init__(self, fm, matrixCombination=MatrixCombination.M, eps=1e-12, maxIter=10000, precision=F64,
-                 tol_kind=None, check_residual=False):
+                 tol_kind=None, check_residual=False, double_double=False):
         self.fm = fm
         self.op = int(matrixCombination)
         self.eps = float(eps)
@@ -294,6 +294,10 @@ class _Solver:
             tol_kind = TOL_ABS_BATCH if precision == F64 else TOL_REL_PER_RHS
         self.tol_kind = int(tol_kind)
         self.check = bool(check_residual)
+        # per-RHS relative tolerance with the solution in double-double
+        # (nslx_solve_dd, CG only); .x_lo holds the low part after a call
+        self.dd = bool(double_double)
+        self.x_lo = None
         self.report = None
         self.converged = None
 
@@ -305,9 +309,18 @@ class _Solver:
         if x0 is not None:
             x0 = x0.to(dtype=torch.complex128).contiguous()
         rep = SolveReport()
-        st = _lib.lib().nslx_solve(self.fm._h, self.op, self.method, _ptr(b), _ptr(x),
-                                   _ptr(x0) if x0 is not None else None, nb, self.eps, self.tol_kind,
-                                   self.maxIter, self.precision, int(self.check), ctypes.byref(rep), _stream())
+        if self.dd:
+            if self.method != _lib.CG_METHOD:
+                raise ValueError("double_double needs CG")
+            if self.x_lo is None or self.x_lo.shape != b.shape:
+                self.x_lo = torch.empty_like(b)
+            st = _lib.lib().nslx_solve_dd(self.fm._h, self.op, _ptr(b), _ptr(x), _ptr(self.x_lo),
+                                          _ptr(x0) if x0 is not None else None, nb, self.eps, self.maxIter,
+                                          self.precision, ctypes.byref(rep), _stream())
+        else:
+            st = _lib.lib().nslx_solve(self.fm._h, self.op, self.method, _ptr(b), _ptr(x),
+                                       _ptr(x0) if x0 is not None else None, nb, self.eps, self.tol_kind,
+                                       self.maxIter, self.precision, int(self.check), ctypes.byref(rep), _stream())
         _lib.check(st, allow=(_lib.OK, _lib.NOT_CONVERGED))
         self.report = rep.as_dict()
         self.converged = st == _lib.OK

@@ -19,7 +19,7 @@ OP_M, OP_MDAGGER, OP_MMDAGGER, OP_MDAGGERM = range(4)
 PARTICLE, HOLE = 0, 1
 F64, F32, BF16X3 = 0, 1, 2
 CG_METHOD, BICGSTAB_METHOD = 0, 1
-TOL_ABS_BATCH, TOL_REL_PER_RHS = 0, 1
+TOL_ABS_BATCH, TOL_REL_PER_RHS, TOL_REL_PER_RHS_FLOOR = 0, 1, 2
 
 
 class SolveReport(ctypes.Structure):
@@ -116,6 +116,8 @@ PROTOS = {
     "nslx_fm_grad_logdet": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
     "nslx_solve": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int64, c_double, c_int,
                            c_int64, c_int, c_int, ctypes.POINTER(SolveReport), c_void_p]),
+    "nslx_solve_dd": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_double, c_int64,
+                              c_int, ctypes.POINTER(SolveReport), c_void_p]),
     "nslx_solve_precond": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int64, c_double,
                                    c_int, c_int64, c_int, ctypes.POINTER(SolveReport), c_void_p]),
     "nslx_fm_solve_direct": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int64, c_int, c_int, c_void_p]),

@@ -221,6 +221,15 @@ int nslx_solve(nslx_fm fm, int op, int method, const void* b, void* x, const voi
     });
 }
 
+int nslx_solve_dd(nslx_fm fm, int op, const void* b, void* x, void* x_lo, const void* x0, int64_t nbatch, double tol,
+                  int64_t maxit, int precision, nslx_solve_report* report, void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        DeviceGuard dg(m.ctx->device);
+        return nslx::solve_dd(m, op, b, x, x_lo, x0, nbatch, tol, maxit, precision, report, nslx::as_stream(stream));
+    });
+}
+
 int nslx_solve_precond(nslx_fm fm, nslx_fm fm_prec, int op, const void* b, void* x, const void* x0, int64_t nbatch,
                        double tol, int tol_kind, int64_t maxit, int inner_precision, nslx_solve_report* report,
                        void* stream) {

@@ -131,7 +131,9 @@ int fm_two_point(FermionMatrix& fm, int64_t ntsrc, double eps, int64_t maxit, in
         nslx_solve_report r{};
         // the reference's CG starts from x0 = b (CG.tpp:97-103); the per-RHS
         // reliable-update solver starts from zero
-        const int st = solve(fm, NSLX_OP_MMDAGGER, NSLX_CG, S.p, X.p, nullptr, nb, eps, tol_kind, maxit, prec, 0, &r, s);
+        // per-RHS solves accept a stop at the FP64 attainable accuracy (counted in floor_stops)
+        const int tk = tol_kind == NSLX_TOL_REL_PER_RHS ? NSLX_TOL_REL_PER_RHS_FLOOR : tol_kind;
+        const int st = solve(fm, NSLX_OP_MMDAGGER, NSLX_CG, S.p, X.p, nullptr, nb, eps, tk, maxit, prec, 0, &r, s);
         if (st != NSLX_OK) status = st;
         fm.apply(NSLX_OP_MDAGGER, X.p, G.p, nb, NSLX_F64, s);
         hipLaunchKernelGGL(k_accumulate, dim3(grid_for(ncorr)), dim3(256), 0, s, corr, G.p, ng, tsrc0, step, nt, nx,

@@ -197,6 +197,9 @@ void aggregate(nslx_solve_report& agg, const nslx_solve_report& r) {
     agg.rhs_iters_f64 += r.rhs_iters_f64;
 }
 
+// per-RHS solves accept a stop at the FP64 attainable accuracy (counted in floor_stops)
+int floor_kind(int tol_kind) { return tol_kind == NSLX_TOL_REL_PER_RHS ? NSLX_TOL_REL_PER_RHS_FLOOR : tol_kind; }
+
 void check_common(FermionMatrix& fm, int64_t ntsrc, int tol_kind, int prec) {
     fm.require_populated();
     NSLX_REQUIRE(fm.nfield == 1, "correlator: populate exactly one field");
@@ -236,7 +239,7 @@ int fm_two_point_wall(FermionMatrix& fm, const void* wall, int64_t nw, int64_t n
                            static_cast<const double2*>(wall), (int)nw, ng, tsrc0, step, nt, nx, 0);
         NSLX_LAUNCH_CHECK();
         nslx_solve_report r{};
-        const int st = solve(fm, NSLX_OP_MMDAGGER, NSLX_CG, S.p, X.p, nullptr, nb, eps, tol_kind, maxit, prec, 0, &r, s);
+        const int st = solve(fm, NSLX_OP_MMDAGGER, NSLX_CG, S.p, X.p, nullptr, nb, eps, floor_kind(tol_kind), maxit, prec, 0, &r, s);
         if (st != NSLX_OK) status = st;
         fm.apply(NSLX_OP_MDAGGER, X.p, G.p, nb, NSLX_F64, s);
         hipLaunchKernelGGL(k_wall_accumulate, dim3(blocks_for(nk)), dim3(256), 0, s, corrK.p, G.p, (int)nw, ng, tsrc0,
@@ -331,7 +334,7 @@ int fm_two_body(FermionMatrix& fm, const void* wall, int kdim, int bdim, int64_t
                 fm.populate(phi.p, 1, species, s);
                 nslx_solve_report r{};
                 // invMMdag = cg_(src); corrK = M^dag invMMdag (:263-266)
-                int st = solve(fm, NSLX_OP_MMDAGGER, NSLX_CG, S.p, X.p, nullptr, nw, eps, tol_kind, maxit, prec, 0, &r, s);
+                int st = solve(fm, NSLX_OP_MMDAGGER, NSLX_CG, S.p, X.p, nullptr, nw, eps, floor_kind(tol_kind), maxit, prec, 0, &r, s);
                 if (st != NSLX_OK) status = st;
                 aggregate(agg, r);
                 fm.apply(NSLX_OP_MDAGGER, X.p, G.p, nw, NSLX_F64, s);
@@ -341,7 +344,7 @@ int fm_two_body(FermionMatrix& fm, const void* wall, int kdim, int bdim, int64_t
                 NSLX_LAUNCH_CHECK();
                 // invMdagM = cgDag_(conj(src)); corrKDag = M invMdagM; pool = conj(<conj(w), .>) (:264-281)
                 r = nslx_solve_report{};
-                st = solve(fm, NSLX_OP_MDAGGERM, NSLX_CG, Sc.p, X.p, nullptr, nw, eps, tol_kind, maxit, prec, 0, &r, s);
+                st = solve(fm, NSLX_OP_MDAGGERM, NSLX_CG, Sc.p, X.p, nullptr, nw, eps, floor_kind(tol_kind), maxit, prec, 0, &r, s);
                 if (st != NSLX_OK) status = st;
                 aggregate(agg, r);
                 fm.apply(NSLX_OP_M, X.p, G.p, nw, NSLX_F64, s);

@@ -50,6 +50,7 @@ struct SolverWork {
     DevBuf<double> upart, rpart0, rpart1;           // fused-CG tile partials
     std::vector<double> last_rr;                    // |r|^2 per slot after the last iteration
     DevBuf<double> rr_ref, rr_prev, xn2, best, rstart;
+    DevBuf<unsigned char> xlo;                      // low part of a double-double x
     DevBuf<int> stall;
     PinBuf<int> status_h;
     PinBuf<double> dbl_h;
@@ -119,6 +120,11 @@ void fm_init(FermionMatrix& m, Ctx* ctx, int nx, int nt, const double* K, double
 // Solvers (solver.cpp)
 int solve(FermionMatrix& fm, int op, int method, const void* b, void* x, const void* x0, int64_t nb, double tol,
           int tol_kind, int64_t maxit, int prec, int check, nslx_solve_report* rep, hipStream_t s);
+// Reliable-update CG (recurrence in prec) with x in double-double and the
+// replacements' true residual in double-double near the FP64 floor, to
+// |b - A (x + xlo)| <= tol |b| per RHS (xlo may be null).
+int solve_dd(FermionMatrix& fm, int op, const void* b, void* x, void* xlo, const void* x0, int64_t nb, double tol,
+             int64_t maxit, int prec, nslx_solve_report* rep, hipStream_t s);
 // Outer CG on fm preconditioned by an inner CG on fp (CGpreconditioned).
 int solve_precond(FermionMatrix& fm, FermionMatrix& fp, int op, const void* b, void* x, const void* x0, int64_t nb,
                   double tol, int tol_kind, int64_t maxit, int inner_prec, nslx_solve_report* rep, hipStream_t s);

@@ -102,7 +102,7 @@ def test_diag_cg_reference_semantics_and_per_rhs():
     assert abs(cg.report["iterations"] - info["iterations"]) <= 2
     assert relerr(x, xo) < 1e-7
     for prec in (nsl.F64, nsl.F32, nsl.BF16X3):
-        cg = nsl.CG(fm, nsl.MdaggerM, 1e-11, 20000, precision=prec, tol_kind=nsl.TOL_REL_PER_RHS,
+        cg = nsl.CG(fm, nsl.MdaggerM, 1e-11, 20000, precision=prec, tol_kind=nsl.TOL_REL_PER_RHS_FLOOR,
                     check_residual=True)
         x = cg(b.to(DEV))
         assert cg.converged, prec

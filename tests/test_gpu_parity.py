@@ -139,7 +139,7 @@ def test_cg_per_rhs_relative(op):
     fm, ofs, phi, lat, _ = make("honeycomb", (3, 3), 16, 4.0, nfield=2)
     b = torch.as_tensor(rng.complex_normal(2001, 0, (4, 16, 18)))
     b[3] *= 1e-6                               # per-RHS scaling must not matter
-    cg = nsl.CG(fm, op, 1e-12, 20000, tol_kind=nsl.TOL_REL_PER_RHS, check_residual=True)
+    cg = nsl.CG(fm, op, 1e-12, 20000, tol_kind=nsl.TOL_REL_PER_RHS_FLOOR, check_residual=True)
     x = cg(b.to(DEV))
     assert cg.converged
     name = "MMdagger" if op == nsl.MMdagger else "MdaggerM"
@@ -158,7 +158,7 @@ def test_cg_reliable_update_true_residual(prec, kind, size, Nt, beta):
     with the oracle's own MMdagger to 2e-12)."""
     fm, ofs, phi, lat, _ = make(kind, size, Nt, beta, nfield=2, std=math.sqrt(beta / Nt * 3.0))
     b = torch.as_tensor(rng.complex_normal(2002, 0, (4, Nt, lat.sites())))
-    cg = nsl.CG(fm, nsl.MMdagger, 1e-12, 200000, precision=prec, tol_kind=nsl.TOL_REL_PER_RHS)
+    cg = nsl.CG(fm, nsl.MMdagger, 1e-12, 200000, precision=prec, tol_kind=nsl.TOL_REL_PER_RHS_FLOOR)
     x = cg(b.to(DEV))
     assert cg.converged, cg.report
     target = max(1e-12, cg.report["max_floor_rel"])     # attainable FP64 accuracy
@@ -176,6 +176,28 @@ def test_mixed_solution_matches_fp64_cfg0():
     for prec in (nsl.F32, nsl.BF16X3):
         x = nsl.CG(fm, nsl.MMdagger, 1e-12, 20000, precision=prec)(b.to(DEV))
         assert relerr(x, x64) < 1e-9
+
+
+def test_cg_floor_stop_is_not_converged():
+    """TOL_REL_PER_RHS on a system whose 1e-12 lies below the FP64 floor
+    (6x6 honeycomb, Nt = 64, beta = 8, U = 3, seed 1000: cond(MM^dag) ~ 6e9):
+    the RHS stop at the attainable accuracy, the report counts them, and the
+    call returns NOT_CONVERGED; TOL_REL_PER_RHS_FLOOR accepts the same stop,
+    and the double-double solver reaches 1e-12."""
+    fm, ofs, phi, lat, _ = make("honeycomb", (6, 6), 64, 8.0, nfield=1, std=math.sqrt(8.0 / 64 * 3.0))
+    b = torch.as_tensor(rng.complex_normal(2003, 0, (2, 64, 72))).to(DEV)
+    cg = nsl.CG(fm, nsl.MMdagger, 1e-12, 100000, tol_kind=nsl.TOL_REL_PER_RHS)
+    cg(b)
+    assert not cg.converged and cg.report["floor_stops"] >= 1, cg.report
+    cg = nsl.CG(fm, nsl.MMdagger, 1e-12, 100000, tol_kind=nsl.TOL_REL_PER_RHS_FLOOR)
+    cg(b)
+    assert cg.converged and cg.report["floor_stops"] >= 1
+    cg = nsl.CG(fm, nsl.MMdagger, 1e-12, 100000, double_double=True)
+    x = cg(b)
+    assert cg.converged and cg.report["floor_stops"] == 0, cg.report
+    assert cg.report["max_rel_residual"] <= 1e-12
+    rel = nsl.dd_residual(fm, b, x, cg.x_lo)
+    assert rel.max() <= 1e-12
 
 
 def test_cg_not_converged_returns_iterate():

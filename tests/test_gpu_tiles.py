@@ -96,7 +96,7 @@ def test_every_tile_cg(prec):
     b = torch.as_tensor(rng.complex_normal(92, 0, (nb, Nt, Nx)))
     for t in fitting_tiles(ctx, prec, Nx, Nt, nb):
         ctx.set_tile(prec, Nx, Nt, nb, t)
-        cg = nsl.CG(fm, nsl.MMdagger, 1e-12, 100000, precision=prec, tol_kind=nsl.TOL_REL_PER_RHS)
+        cg = nsl.CG(fm, nsl.MMdagger, 1e-12, 100000, precision=prec, tol_kind=nsl.TOL_REL_PER_RHS_FLOOR)
         x = cg(b.to(DEV))
         assert cg.converged, (t, cg.report)
         target = max(1e-12, cg.report["max_floor_rel"])
