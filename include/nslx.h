@@ -527,12 +527,15 @@ int nslx_fm_tune(nslx_fm fm, int precision, int64_t nbatch, int reps, int* best_
 
 /* ---- kernel timing ----------------------------------------------------------
  * No counterpart in the reference (its only profiling is a host stopwatch,
- * src/NSL/logger.hpp:174-191).  nslx_prof_enable(n), n > 0: every n-th hop
- * launch of each kernel class is bracketed by HIP events on its own stream
- * (n = 1: every launch; 0 turns timing off).  nslx_prof_read writes one line
- * per kernel class, "name launches total_ms timed\n" (launches counted,
- * milliseconds summed over the timed ones), into buf (NUL terminated) and
- * returns the number of classes.  Process-wide; not for concurrent use. */
+ * src/NSL/logger.hpp:174-191).  nslx_prof_enable(n), n > 0: every n-th
+ * launch of each kernel class (hop, batched GEMM, double-double hop) is
+ * bracketed by HIP events on its own stream (n = 1: every launch; 0 turns
+ * timing off).  nslx_prof_read writes one line per kernel class,
+ * "name launches total_ms timed flops bytes\n" (launches counted;
+ * milliseconds, algorithmic flops and bytes summed over the timed launches;
+ * flops and bytes are 0 for the hop, whose work the caller knows), into buf
+ * (NUL terminated) and returns the number of classes.  Process-wide; not for
+ * concurrent use. */
 int nslx_prof_enable(int on);
 int nslx_prof_reset(void);
 int nslx_prof_read(char* buf, int64_t size);

@@ -9,8 +9,10 @@
 // v_mfma_f32_16x16x4f32 (FP32).  Complex A x real B is two real products
 // (re, im share B); complex x complex is four (rr, ii, ri, ir), combined in
 // the epilogue.
+#include <cmath>
 #include <functional>
 
+#include "prof.hpp"
 #include "tss.hpp"
 
 namespace nslx {
@@ -183,8 +185,8 @@ __global__ void __launch_bounds__(256) k_bgemm(const BJob* __restrict__ jobs, co
 
 template <class S, bool CB, int PH>
 void launch_t(int tile, const BJob* j, const int4* t, int n, hipStream_t s) {
-    if (tile == 0) hipLaunchKernelGGL((k_bgemm<S, CB, PH, 64, 64>), dim3(n), dim3(256), 0, s, j, t);
-    else hipLaunchKernelGGL((k_bgemm<S, CB, PH, 32, 32>), dim3(n), dim3(256), 0, s, j, t);
+    if (tile == 0) launch_timed(k_bgemm<S, CB, PH, 64, 64>, dim3(n), dim3(256), 0, s, j, t);
+    else launch_timed(k_bgemm<S, CB, PH, 32, 32>, dim3(n), dim3(256), 0, s, j, t);
 }
 
 template <class S>
@@ -215,9 +217,30 @@ void bgemm_tiles(const std::vector<BJob>& jobs, int tile, std::vector<int4>& out
     }
 }
 
-void launch_bgemm(const BVariant& v, const BJob* d_jobs, const int4* d_tiles, int ntiles, hipStream_t s) {
+std::string bgemm_class(const BVariant& v) {
+    return std::string("bgemm_") + (v.prec == NSLX_F64 ? "f64" : "f32") + (v.cb ? "_cplx" : "_real") + "_ph" +
+           std::to_string(v.ph) + (v.tile == 0 ? "_64" : "_32");
+}
+
+void bgemm_work(const std::vector<BJob>& jobs, const BVariant& v, double* flops, double* bytes) {
+    const double ce = v.prec == NSLX_F64 ? 16.0 : 8.0, be = v.cb ? ce : ce / 2;
+    double f = 0.0, b = 0.0;
+    for (const BJob& j : jobs) {
+        const double r = j.rows, c = j.cols, k = j.K;
+        f += (v.cb ? 8.0 : 4.0) * r * c * k;
+        b += r * k * ce + c * k * be + r * c * ce * (j.Add ? 2.0 : 1.0);
+        if (v.ph) b += std::ceil(r / std::min<double>(j.phdiv, r)) * (v.ph == 1 ? k : c) * ce;
+    }
+    *flops = f;
+    *bytes = b;
+}
+
+void launch_bgemm(const BVariant& v, const BJob* d_jobs, const int4* d_tiles, int ntiles, hipStream_t s, double flops,
+                  double bytes) {
     if (ntiles <= 0) return;
     NSLX_REQUIRE(v.tile == 0 || v.tile == 1, "bgemm: bad tile");
+    ProfScope prof(prof_on() ? bgemm_class(v) : std::string(), s);
+    prof.work(flops, bytes);
     if (v.prec == NSLX_F64) launch_s<double>(v, d_jobs, d_tiles, ntiles, s);
     else if (v.prec == NSLX_F32) launch_s<float>(v, d_jobs, d_tiles, ntiles, s);
     else throw Error(NSLX_E_ARG, "bgemm: precision must be F64 or F32");
@@ -246,6 +269,7 @@ void BPlan::gemm(const std::vector<BJob>& jobs, BVariant v, int tile) {
     jobs_.insert(jobs_.end(), jobs.begin(), jobs.end());
     tiles_.insert(tiles_.end(), t.begin(), t.end());
     op.ntiles = (int)t.size();
+    bgemm_work(jobs, v, &op.flops, &op.bytes);
     ops_.push_back(std::move(op));
     uploaded_ = false;
 }
@@ -316,7 +340,7 @@ void BPlan::upload(hipStream_t s) {
 void BPlan::run(hipStream_t s) const {
     NSLX_REQUIRE(uploaded_, "bgemm plan run before upload");
     for (const Op& op : ops_) {
-        if (op.is_gemm) launch_bgemm(op.v, djobs_.p + op.job0, dtiles_.p + op.tile0, op.ntiles, s);
+        if (op.is_gemm) launch_bgemm(op.v, djobs_.p + op.job0, dtiles_.p + op.tile0, op.ntiles, s, op.flops, op.bytes);
         else op.fn(s);
     }
 }

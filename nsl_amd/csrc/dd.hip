@@ -15,6 +15,7 @@
 
 #include "dd.hpp"
 #include "engine.hpp"
+#include "prof.hpp"
 
 namespace nslx {
 namespace {
@@ -253,21 +254,38 @@ void dd_residual(FermionMatrix& fm, int op, const void* b, const void* xh, const
     dim3 g(ncol, (unsigned)((nrows + DROWS - 1) / DROWS));
     const double2* ih = static_cast<const double2*>(xh);
     const double2* il = static_cast<const double2*>(xl);
+    // one hop's algorithmic work: per output element and site two real
+    // double-double multiply-adds (TwoProduct with FMA + TwoSum + the low
+    // parts, 10 flops each); bytes: the input and output fields in
+    // double-double (32 B per complex element), E once
+    const double flops = 20.0 * (double)nrows * n * n;
+    const double bytes = 64.0 * (double)nrows * n + 8.0 * (double)n * n;
     // first hop into (uh, ul), second into the residual
-    if (op == NSLX_OP_MMDAGGER) {
-        hipLaunchKernelGGL((k_dd_hop<1, false>), g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, ih, il, w.uh.p, w.ul.p,
-                           nullptr, nullptr, nullptr);
+    {
+        ProfScope prof(prof_on() ? std::string(op == NSLX_OP_MMDAGGER ? "dd_hop_mdag" : "dd_hop_m") : std::string(), s);
+        prof.work(flops, bytes);
+        if (op == NSLX_OP_MMDAGGER)
+            launch_timed(k_dd_hop<1, false>, g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, ih, il, w.uh.p, w.ul.p,
+                         (const double2*)nullptr, (double2*)nullptr, (double*)nullptr);
+        else
+            launch_timed(k_dd_hop<0, false>, g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, ih, il, w.uh.p, w.ul.p,
+                         (const double2*)nullptr, (double2*)nullptr, (double*)nullptr);
         NSLX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_dd_hop<0, true>), g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, w.uh.p, w.ul.p, nullptr,
-                           nullptr, static_cast<const double2*>(b), static_cast<double2*>(r), w.part.p);
-    } else {
-        hipLaunchKernelGGL((k_dd_hop<0, false>), g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, ih, il, w.uh.p, w.ul.p,
-                           nullptr, nullptr, nullptr);
-        NSLX_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_dd_hop<1, true>), g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, w.uh.p, w.ul.p, nullptr,
-                           nullptr, static_cast<const double2*>(b), static_cast<double2*>(r), w.part.p);
     }
-    NSLX_LAUNCH_CHECK();
+    {
+        ProfScope prof(prof_on() ? std::string(op == NSLX_OP_MMDAGGER ? "dd_hop_m_resid" : "dd_hop_mdag_resid")
+                                 : std::string(), s);
+        prof.work(flops, bytes);
+        const double2* uh = w.uh.p;
+        const double2* ul = w.ul.p;
+        if (op == NSLX_OP_MMDAGGER)
+            launch_timed(k_dd_hop<0, true>, g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, uh, ul, (double2*)nullptr,
+                         (double2*)nullptr, static_cast<const double2*>(b), static_cast<double2*>(r), w.part.p);
+        else
+            launch_timed(k_dd_hop<1, true>, g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, uh, ul, (double2*)nullptr,
+                         (double2*)nullptr, static_cast<const double2*>(b), static_cast<double2*>(r), w.part.p);
+        NSLX_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(k_dd_rsum, dim3(blocks1(nb)), dim3(256), 0, s, w.part.p, nt, ncol, (long long)nb, rn2);
     NSLX_LAUNCH_CHECK();
 }

@@ -10,6 +10,7 @@ namespace {
 struct Rec {
     std::string name;
     hipEvent_t a, b;
+    double flops, bytes;
 };
 std::mutex g_mu;
 int g_every = 0;                       // 0: off; n: time every n-th launch of a class
@@ -47,7 +48,21 @@ ProfScope::ProfScope(const char* name, int prec, int mode, hipStream_t s) : acti
     static const char* mn[] = {"plain", "cga", "cgb", "cgi"};
     std::lock_guard<std::mutex> g(g_mu);
     name_ = std::string(name) + "_" + pn[prec % 3] + "_" + mn[mode % 4];
-    // every g_every-th launch of a class is timed
+    begin();
+}
+
+ProfScope::ProfScope(const std::string& full_name, hipStream_t s) : active_(g_every > 0) {
+    (void)s;
+    outer_ = t_scope;
+    t_scope = this;
+    if (!active_) return;
+    std::lock_guard<std::mutex> g(g_mu);
+    name_ = full_name;
+    begin();
+}
+
+// with g_mu held: count the launch; every g_every-th of a class is timed
+void ProfScope::begin() {
     const long n = g_count[name_]++;
     if (n % g_every != 0) {
         active_ = false;
@@ -70,7 +85,7 @@ ProfScope::~ProfScope() {
     if (!active_) return;
     std::lock_guard<std::mutex> g(g_mu);
     if (used_) {
-        g_recs.push_back({name_, (hipEvent_t)a_, (hipEvent_t)b_});
+        g_recs.push_back({name_, (hipEvent_t)a_, (hipEvent_t)b_, flops_, bytes_});
     } else {   // no kernel was launched in the scope (an error path)
         g_pool.push_back((hipEvent_t)a_);
         g_pool.push_back((hipEvent_t)b_);
@@ -88,20 +103,24 @@ void prof_reset() {
 
 std::string prof_report(int* nclasses) {
     std::lock_guard<std::mutex> g(g_mu);
-    std::map<std::string, std::pair<long, double>> acc;
+    struct Sum { long n = 0; double ms = 0.0, flops = 0.0, bytes = 0.0; };
+    std::map<std::string, Sum> acc;
     for (auto& r : g_recs) {
         NSLX_HIP(hipEventSynchronize(r.b));
         float ms = 0.f;
         NSLX_HIP(hipEventElapsedTime(&ms, r.a, r.b));
         auto& e = acc[r.name];
-        e.first += 1;
-        e.second += ms;
+        e.n += 1;
+        e.ms += ms;
+        e.flops += r.flops;
+        e.bytes += r.bytes;
     }
     std::ostringstream os;
+    os.precision(17);
     for (auto& kv : acc) {
         const auto c = g_count.find(kv.first);
-        os << kv.first << " " << (c == g_count.end() ? kv.second.first : c->second) << " " << kv.second.second << " "
-           << kv.second.first << "\n";
+        os << kv.first << " " << (c == g_count.end() ? kv.second.n : c->second) << " " << kv.second.ms << " "
+           << kv.second.n << " " << kv.second.flops << " " << kv.second.bytes << "\n";
     }
     if (nclasses) *nclasses = (int)acc.size();
     return os.str();

@@ -17,11 +17,16 @@ namespace nslx {
 class ProfScope {
    public:
     ProfScope(const char* name, int prec, int mode, hipStream_t s);
+    // a class named in full (kernels other than the hop)
+    ProfScope(const std::string& full_name, hipStream_t s);
     ~ProfScope();
     ProfScope(const ProfScope&) = delete;
     ProfScope& operator=(const ProfScope&) = delete;
     // the event pair for the scope's kernel, once (false when timing is off)
     bool take_events(hipEvent_t* start, hipEvent_t* stop);
+    // algorithmic flops and bytes of the scope's launch (summed per class
+    // over the timed launches, reported beside their time)
+    void work(double flops, double bytes) { flops_ = flops; bytes_ = bytes; }
 
    private:
     bool active_;
@@ -29,6 +34,8 @@ class ProfScope {
     std::string name_;
     void* a_ = nullptr;
     void* b_ = nullptr;
+    double flops_ = 0.0, bytes_ = 0.0;
+    void begin();
     ProfScope* outer_ = nullptr;
 };
 

@@ -19,6 +19,7 @@
 #include <cmath>
 
 #include "engine.hpp"
+#include "prof.hpp"
 #include "tss.hpp"
 #include "vec.hpp"
 
@@ -371,23 +372,38 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
     check_extent(piv, (long long)nf * GJB - 1, sizeof(int), "Gauss-Jordan pivots");
     for (int jb = 0; jb < n_; jb += GJB) {
         const int w = std::min(GJB, n_ - jb);
+        // work of the pivot search and sub-panel LU: the sub-panel read and
+        // written once, C written; 8 flops per complex multiply-add
+        const double pbytes = (double)nf * rows * (2.0 * w + GJB) * e;
+        double pflops = 0.0;
+        for (int jj = 0; jj < w; ++jj) pflops += 8.0 * (rows - jb - jj) * (w - jj);
+        pflops = (pflops + 4.0 * rows * w * w) * nf;
         p.custom([=](hipStream_t s) {
+            ProfScope prof(prof_on() ? std::string(f64 ? "gj_panel_f64" : "gj_panel_f32") : std::string(), s);
+            prof.work(pflops, pbytes);
             if (f64)
-                hipLaunchKernelGGL(k_gj_panel<double>, dim3(nf), dim3(256), 0, s, Q, ldq, strideQ, rows, jb, w, piv, binv,
-                                   C, sing);
+                launch_timed(k_gj_panel<double>, dim3(nf), dim3(256), 0, s, (void*)Q, ldq, strideQ, rows, jb, w, piv,
+                             (void*)binv, (void*)C, sing);
             else
-                hipLaunchKernelGGL(k_gj_panel<float>, dim3(nf), dim3(256), 0, s, Q, ldq, strideQ, rows, jb, w, piv, binv, C,
-                                   sing);
+                launch_timed(k_gj_panel<float>, dim3(nf), dim3(256), 0, s, (void*)Q, ldq, strideQ, rows, jb, w, piv,
+                             (void*)binv, (void*)C, sing);
             NSLX_LAUNCH_CHECK();
         });
         const int c0 = jb + w;
         const int ncol = width - c0;
+        // swaps (2w reads and writes), the pivot rows read and written, YT written
+        const double sbytes = (double)nf * ncol * 7.0 * w * e;
+        const double sflops = (double)nf * ncol * 8.0 * GJB * GJB;
         p.custom([=](hipStream_t s) {
+            ProfScope prof(prof_on() ? std::string(f64 ? "gj_swap_f64" : "gj_swap_f32") : std::string(), s);
+            prof.work(sflops, sbytes);
             dim3 g(blocks(ncol), nf);
             if (f64)
-                hipLaunchKernelGGL(k_gj_swap<double>, g, dim3(256), 0, s, Q, ldq, strideQ, jb, w, c0, width, piv, binv, YT);
+                launch_timed(k_gj_swap<double>, g, dim3(256), 0, s, (void*)Q, ldq, strideQ, jb, w, c0, width,
+                             (const int*)piv, (const void*)binv, (void*)YT);
             else
-                hipLaunchKernelGGL(k_gj_swap<float>, g, dim3(256), 0, s, Q, ldq, strideQ, jb, w, c0, width, piv, binv, YT);
+                launch_timed(k_gj_swap<float>, g, dim3(256), 0, s, (void*)Q, ldq, strideQ, jb, w, c0, width,
+                             (const int*)piv, (const void*)binv, (void*)YT);
             NSLX_LAUNCH_CHECK();
         });
         std::vector<BJob> jobs(nf);

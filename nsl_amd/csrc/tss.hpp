@@ -75,7 +75,13 @@ struct BVariant {
 // Tile list of a set of jobs for a variant.
 void bgemm_tiles(const std::vector<BJob>& jobs, int tile, std::vector<int4>& out);
 int bgemm_tile_rows(int tile);
-void launch_bgemm(const BVariant& v, const BJob* d_jobs, const int4* d_tiles, int ntiles, hipStream_t s);
+// flops / bytes: the launch's algorithmic work, recorded by kernel timing
+void launch_bgemm(const BVariant& v, const BJob* d_jobs, const int4* d_tiles, int ntiles, hipStream_t s,
+                  double flops = 0.0, double bytes = 0.0);
+// Algorithmic flops (8 per complex x complex, 4 per complex x real
+// multiply-add) and bytes (A, B, Y, Add and phase rows read or written once)
+// of a job list.
+void bgemm_work(const std::vector<BJob>& jobs, const BVariant& v, double* flops, double* bytes);
 
 // A recorded sequence of launches whose job and tile arrays live in one
 // device buffer each (uploaded once); replayed by run().
@@ -96,6 +102,7 @@ class BPlan {
         BVariant v;
         size_t job0 = 0, tile0 = 0;
         int ntiles = 0;
+        double flops = 0.0, bytes = 0.0;
         std::function<void(hipStream_t)> fn;
     };
     std::vector<Op> ops_;

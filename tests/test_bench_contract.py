@@ -47,6 +47,27 @@ def test_roofline_picks_largest_total_and_weights_active_rhs():
     assert rf["traffic"] == 123 and rf["timed_launches"] == 50
 
 
+def test_roofline_of_reported_work_classes():
+    """Classes other than the hop report their own flops and bytes (summed
+    over the timed launches): the batched GEMM is priced on the matrix cores
+    of its precision, the double-double hop on the FP64 vector units, the
+    Gauss-Jordan helpers on HBM."""
+    text = "\n".join([
+        "bgemm_f32_cplx_ph0_64 100 10.0 50 5.0e11 1.0e10",   # 200 us, 1e10 flops per launch
+        "dd_hop_m_resid 4 3.2 4 8.0e10 4.0e8",               # 800 us
+        "gj_panel_f64 10 1.0 5 1e6 5.0e7",                   # 200 us
+    ])
+    rf, rows = bench.roofline_from_prof(text, 128, NX, NT, {})
+    assert rf["kernel"] == "bgemm_f32_cplx_ph0_64"
+    assert rf["bound"] == "mfma" and rf["peak"] == 157.3
+    assert rf["achieved"] == pytest.approx(1e10 / 200e-6 / 1e12, rel=1e-3)
+    table = {d["kernel"]: d for d in bench.kernel_table(rows)}
+    assert table["dd_hop_m_resid"]["bound"] == "valu"
+    assert table["dd_hop_m_resid"]["achieved"] == pytest.approx(2e10 / 800e-6 / 1e12, rel=1e-3)
+    assert table["gj_panel_f64"]["bound"] == "hbm"
+    assert table["gj_panel_f64"]["achieved"] == pytest.approx(1e7 / 200e-6 / 1e9, rel=1e-3)
+
+
 def test_bf16_roofline_is_hbm_bound():
     rf, _ = bench.roofline_from_prof("hop_M_bf16x3_plain 10 0.25 10", 128, NX, NT, {})
     assert rf["bound"] == "hbm" and rf["unit"] == "GB/s"
