@@ -230,6 +230,178 @@ __global__ void __launch_bounds__(256) k_gj_panel(void* Qb, long long ld, long l
     }
 }
 
+// The same column block with the sub-panel rows [jb, rows) held in registers:
+// GJT threads, thread t owning rows jb + t + m GJT (m < RPT), all 16 columns.
+// Per pivot column: a block argmax (largest |.|^2, lowest row on ties), the
+// pivot row and the row it replaces exchanged through LDS, the pivot row
+// broadcast from LDS, the rank-1 update of the rows below in registers.
+// Results (pivots, Binv, C, the sub-panel LU) are those of k_gj_panel.
+constexpr int GJT = 512;
+
+template <class S, int RPT>
+__global__ void __launch_bounds__(GJT) k_gj_panel_reg(void* Qb, long long ld, long long strideQ, int rows, int jb,
+                                                       int w, int* pivb, void* Binvb, void* Cb, int* singular) {
+    const int f = blockIdx.x;
+    char* Q = reinterpret_cast<char*>(Qb) + (size_t)f * strideQ * sizeof(Z<S>);
+    int* piv = pivb + f * GJB;
+    char* Binv = reinterpret_cast<char*>(Binvb) + (size_t)f * GJB * GJB * sizeof(Z<S>);
+    char* C = reinterpret_cast<char*>(Cb) + (size_t)f * rows * GJB * sizeof(Z<S>);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    constexpr int NW = GJT / 64;
+    __shared__ double red_v[NW];
+    __shared__ int red_i[NW];
+    __shared__ Z<S> rowj[GJB], rowp[GJB];
+    __shared__ Z<S> ub[GJB][GJB + 1];   // the pivot block after the LU (L below, U on and above the diagonal)
+    __shared__ int s_p;
+    Z<S> a[RPT][GJB];
+    #pragma unroll
+    for (int m = 0; m < RPT; ++m) {
+        const int gi = jb + tid + m * GJT;
+        #pragma unroll
+        for (int c = 0; c < GJB; ++c)
+            a[m][c] = (gi < rows && c < w) ? ldz<S>(Q, (long long)gi * ld + jb + c) : Z<S>{S(0), S(0)};
+    }
+    #pragma unroll
+    for (int jj = 0; jj < GJB; ++jj) {
+        // fully unrolled with a constant jj (a guard, not a break: a dynamic
+        // column index would put the sub-panel in scratch memory); w is
+        // uniform over the block, so every thread takes the same path
+        if (jj < w) {
+        const int j = jb + jj;
+        // argmax over rows >= j
+        double best = -1.0;
+        int bi = rows;
+        #pragma unroll
+        for (int m = 0; m < RPT; ++m) {
+            const int gi = jb + tid + m * GJT;
+            if (gi >= j && gi < rows) {
+                const double v = (double)a[m][jj].re * a[m][jj].re + (double)a[m][jj].im * a[m][jj].im;
+                if (v > best) { best = v; bi = gi; }
+            }
+        }
+        #pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ov = __shfl_xor(best, o, 64);
+            const int oi = __shfl_xor(bi, o, 64);
+            if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+        }
+        if (lane == 0) { red_v[wv] = best; red_i[wv] = bi; }
+        __syncthreads();
+        if (tid == 0) {
+            double bv = red_v[0];
+            int bx = red_i[0];
+            for (int k = 1; k < NW; ++k)
+                if (red_v[k] > bv || (red_v[k] == bv && red_i[k] < bx)) { bv = red_v[k]; bx = red_i[k]; }
+            s_p = bx;
+            piv[jj] = bx;
+            if (!(bv > 0.0)) atomicExch(singular, 1);
+        }
+        __syncthreads();
+        const int p = s_p;
+        // exchange rows j and p through LDS
+        const int oj = j - jb, op = p - jb;
+        #pragma unroll
+        for (int m = 0; m < RPT; ++m) {
+            if (oj == tid + m * GJT) {
+                #pragma unroll
+                for (int c = 0; c < GJB; ++c) rowj[c] = a[m][c];
+            }
+            if (op == tid + m * GJT) {
+                #pragma unroll
+                for (int c = 0; c < GJB; ++c) rowp[c] = a[m][c];
+            }
+        }
+        __syncthreads();
+        #pragma unroll
+        for (int m = 0; m < RPT; ++m) {
+            if (op == tid + m * GJT && p != j) {
+                #pragma unroll
+                for (int c = 0; c < GJB; ++c) a[m][c] = rowj[c];
+            }
+            if (oj == tid + m * GJT) {
+                #pragma unroll
+                for (int c = 0; c < GJB; ++c) a[m][c] = rowp[c];
+            }
+        }
+        // rowp now holds the pivot row (row j); update the rows below
+        const Z<S> dinv = zinv<S>(rowp[jj]);
+        #pragma unroll
+        for (int m = 0; m < RPT; ++m) {
+            const int gi = jb + tid + m * GJT;
+            if (gi > j && gi < rows) {
+                const Z<S> l = zmul<S>(a[m][jj], dinv);
+                a[m][jj] = l;
+                #pragma unroll
+                for (int c = 0; c < GJB; ++c)
+                    if (c > jj && c < w) a[m][c] = zsub<S>(a[m][c], zmul<S>(l, rowp[c]));
+            }
+        }
+        __syncthreads();   // rowj / rowp are rewritten by the next column
+        }
+    }
+    // write the sub-panel back; the pivot block to LDS
+    #pragma unroll
+    for (int m = 0; m < RPT; ++m) {
+        const int gi = jb + tid + m * GJT;
+        if (gi < rows) {
+            #pragma unroll
+            for (int c = 0; c < GJB; ++c)
+                if (c < w) st<S>(Q, (long long)gi * ld + jb + c, a[m][c]);
+        }
+        const int li = tid + m * GJT;
+        if (li < w) {
+            #pragma unroll
+            for (int c = 0; c < GJB; ++c) ub[li][c] = a[m][c];
+        }
+    }
+    __syncthreads();
+    // Binv column c (thread c < w): solve L_B y = e_c, then U_B x = y
+    if (tid < GJB) {
+        Z<S> y[GJB];
+        #pragma unroll
+        for (int i = 0; i < GJB; ++i) {
+            Z<S> sm{S(i == tid ? 1 : 0), S(0)};
+            #pragma unroll
+            for (int m = 0; m < GJB; ++m)
+                if (m < i) sm = zsub<S>(sm, zmul<S>(ub[i][m], y[m]));
+            y[i] = sm;
+        }
+        #pragma unroll
+        for (int i = GJB - 1; i >= 0; --i) {
+            Z<S> sm = y[i];
+            #pragma unroll
+            for (int m = 0; m < GJB; ++m)
+                if (m > i && m < w) sm = zsub<S>(sm, zmul<S>(ub[i][m], y[m]));
+            y[i] = i < w ? zmul<S>(sm, zinv<S>(ub[i][i])) : Z<S>{S(0), S(0)};
+        }
+        #pragma unroll
+        for (int i = 0; i < GJB; ++i)
+            st<S>(Binv, (long long)i * GJB + tid, (tid < w && i < w) ? y[i] : Z<S>{S(0), S(0)});
+    }
+    // C: rows above jb their current values, pivot rows 0, below L21 U_B
+    for (int i = tid; i < jb; i += GJT) {
+        #pragma unroll
+        for (int c = 0; c < GJB; ++c)
+            st<S>(C, (long long)i * GJB + c, c < w ? ldz<S>(Q, (long long)i * ld + jb + c) : Z<S>{S(0), S(0)});
+    }
+    #pragma unroll
+    for (int m = 0; m < RPT; ++m) {
+        const int gi = jb + tid + m * GJT;
+        if (gi >= rows) continue;
+        #pragma unroll
+        for (int c = 0; c < GJB; ++c) {
+            Z<S> v{S(0), S(0)};
+            if (gi >= jb + w && c < w) {
+                #pragma unroll
+                for (int k = 0; k < GJB; ++k)
+                    if (k <= c) v = {v.re + (a[m][k].re * ub[k][c].re - a[m][k].im * ub[k][c].im),
+                                     v.im + (a[m][k].re * ub[k][c].im + a[m][k].im * ub[k][c].re)};
+            }
+            st<S>(C, (long long)gi * GJB + c, v);
+        }
+    }
+}
+
 // Columns [c0, width) of field blockIdx.y: the block's row swaps, then the
 // pivot rows become Binv x (also written transposed to YT, ld 16, for the
 // rank-w update of the other rows).
@@ -378,15 +550,27 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
         double pflops = 0.0;
         for (int jj = 0; jj < w; ++jj) pflops += 8.0 * (rows - jb - jj) * (w - jj);
         pflops = (pflops + 4.0 * rows * w * w) * nf;
+        // rows of the sub-panel per thread of the register kernel (0: too many
+        // for its register budget, the global-memory kernel runs instead)
+        const int live = rows - jb;
+        const int rpt = (live + GJT - 1) / GJT;
+        const int reg = rpt <= (f64 ? 2 : 3) ? rpt : 0;
         p.custom([=](hipStream_t s) {
             ProfScope prof(prof_on() ? std::string(f64 ? "gj_panel_f64" : "gj_panel_f32") : std::string(), s);
             prof.work(pflops, pbytes);
-            if (f64)
-                launch_timed(k_gj_panel<double>, dim3(nf), dim3(256), 0, s, (void*)Q, ldq, strideQ, rows, jb, w, piv,
-                             (void*)binv, (void*)C, sing);
-            else
-                launch_timed(k_gj_panel<float>, dim3(nf), dim3(256), 0, s, (void*)Q, ldq, strideQ, rows, jb, w, piv,
-                             (void*)binv, (void*)C, sing);
+            void* Qv = (void*)Q;
+            void* bv = (void*)binv;
+            void* Cv = (void*)C;
+            if (f64) {
+                if (reg == 1) launch_timed(k_gj_panel_reg<double, 1>, dim3(nf), dim3(GJT), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
+                else if (reg == 2) launch_timed(k_gj_panel_reg<double, 2>, dim3(nf), dim3(GJT), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
+                else launch_timed(k_gj_panel<double>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
+            } else {
+                if (reg == 1) launch_timed(k_gj_panel_reg<float, 1>, dim3(nf), dim3(GJT), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
+                else if (reg == 2) launch_timed(k_gj_panel_reg<float, 2>, dim3(nf), dim3(GJT), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
+                else if (reg == 3) launch_timed(k_gj_panel_reg<float, 3>, dim3(nf), dim3(GJT), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
+                else launch_timed(k_gj_panel<float>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
+            }
             NSLX_LAUNCH_CHECK();
         });
         const int c0 = jb + w;
