@@ -1,11 +1,11 @@
-"""Benchmark: CG solves/s to a 1e-12 residual on MM^dag, mixed precision
-against the plain FP64 GPU path, on the 12x12 periodic honeycomb (Nx = 288),
+"""Benchmark: CG solves/s to a 1e-12 residual on MM^dag, preconditioned with
+double-double correction against the plain FP64 GPU path, on the 12x12 periodic honeycomb (Nx = 288),
 Nt = 64, beta = 8, U = 3: 16 seeded phi fields x 8 Gaussian right-hand
 sides = 128 solves per step, all resident in HBM before the timed region.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--fp64-steps K64]
-                    [--no-cpu] [--no-fp64] [--no-refcrit] [--no-batch]
-                    [--no-serve] [--prof-every N]
+                    [--precond f64|f32] [--no-cpu] [--no-fp64] [--no-refcrit]
+                    [--no-batch] [--no-serve] [--prof-every N]
 
 A step solves all 128 systems to |b - MM^dag x| <= 1e-12 |b| per right-hand
 side, that residual evaluated in double-double on the FP64 operator (the
@@ -14,21 +14,24 @@ vector cannot represent a solution with a 1e-12 residual: x is returned as a
 double-double pair x + x_lo).  Every step starts with populate(phi), so the
 preconditioned forms refactorise inside the timed region.
 
-  value (mixed):  CG on MM^dag preconditioned by the time-slice direct solve
-                  of M and M^dag factorised and applied in FP32 on the matrix
-                  cores, FP64 CG recurrence, double-double defect correction
+  value:          CG on MM^dag preconditioned by the time-slice direct solve
+                  of M and M^dag (factorised and applied on the matrix cores,
+                  FP64 by default with 2 chunks of 32 slices; --precond f32:
+                  FP32 with 8 chunks of 8), FP64 CG recurrence,
+                  double-double defect correction
   fp64:           the plain FP64 path -- unpreconditioned CG (the reference's
                   algorithm, CG.tpp:14-113) per RHS with FP64 reliable updates
                   and the solution in double-double
-  fp64_tss:       the preconditioned solver with the time-slice solve in FP64
+  f32_tss:        the preconditioned solver with the time-slice solve in the
+  (or f64_tss)    other precision
   refcrit:        the reference's own criterion (x0 = b, one alpha/beta per
                   field's 8-RHS batch, sum |r|^2 <= 1e-24 on the recursive
                   residual, CG.tpp:70) on a few fields, FP64
 
 Prints one JSON line.  Profiling helpers (no JSON contract line):
     python bench.py --hops N        N plain M and M^dag applies per precision
-    python bench.py --tss N         N populate + preconditioned solves (FP32),
-                                    for rocprofv3 runs
+    python bench.py --tss N         N populate + preconditioned solves (in the
+                                    --precond precision), for rocprofv3 runs
     python bench.py --copy-calib M  three device copies of M MiB (scales the
                                     FETCH_SIZE / WRITE_SIZE counters)
 """
@@ -358,9 +361,10 @@ def hop_profile(n):
             print(f"{name} launches={k} avg_us={ms / timed * 1e3:.2f}", flush=True)
 
 
-def tss_profile(n):
-    """n populate + preconditioned (FP32 time-slice) solves of the bench
-    workload, for profiler runs; prints each step's time."""
+def tss_profile(n, precond):
+    """n populate + preconditioned (time-slice solve in precond's precision)
+    solves of the bench workload, for profiler runs; prints each step's
+    time."""
     import torch
     import nsl_amd as nsl
     phi_h, b_h = fields()
@@ -368,7 +372,7 @@ def tss_profile(n):
     phi = torch.as_tensor(phi_h).cuda()
     b = torch.as_tensor(b_h).cuda()
     x = torch.empty_like(b)
-    cg = nsl.CGTimeSlice(fm, TOL, 2000, precision=nsl.F32)
+    cg = nsl.CGTimeSlice(fm, TOL, 2000, precision=nsl.F64 if precond == "f64" else nsl.F32)
     for k in range(n):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -436,6 +440,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--hops", type=int, default=0)
     ap.add_argument("--tss", type=int, default=0)
+    ap.add_argument("--precond", choices=["f64", "f32"], default="f64",
+                    help="precision of the time-slice preconditioner of the headline (and of --tss)")
     ap.add_argument("--copy-calib", type=int, default=0)
     ap.add_argument("--prof-every", type=int, default=PROF_EVERY,
                     help="time every n-th launch of a kernel class with HIP events (0: none)")
@@ -445,7 +451,7 @@ def main():
     if args.hops:
         return hop_profile(args.hops)
     if args.tss:
-        return tss_profile(args.tss)
+        return tss_profile(args.tss, args.precond)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the path shards by whole requests/fields with no communication, see DESIGN.md",
@@ -490,24 +496,30 @@ def main():
         rel = nsl.dd_residual(fm, b, x, getattr(cg, "x_lo", None))
         return t, dict(cg.report), bool(cg.converged), float(rel.max()), text
 
-    # ---- headline: mixed precision (FP32 time-slice preconditioner)
-    cg_mix = nsl.CGTimeSlice(fm, TOL, 5000, precision=nsl.F32)
+    # ---- headline: CG preconditioned by the time-slice solve (FP64 by
+    # default: its wider chunks give a near-exact preconditioner with a
+    # quarter of the factorisation's block steps), double-double correction
+    pprec = nsl.F64 if args.precond == "f64" else nsl.F32
+    cg_mix = nsl.CGTimeSlice(fm, TOL, 5000, precision=pprec)
     t_mix, rep_mix, conv_mix, rel_mix, prof_text = leg(cg_mix, args.steps, args.warmup, args.prof_every > 0)
-    print(f"# mixed: {t_mix / args.steps * 1e3:.1f} ms/step, report {rep_mix}, dd residual {rel_mix:.3e}",
-          file=sys.stderr)
+    print(f"# headline ({args.precond} preconditioner): {t_mix / args.steps * 1e3:.1f} ms/step, report {rep_mix}, "
+          f"dd residual {rel_mix:.3e}", file=sys.stderr)
+    chunk_rule = ("chunks of the largest divisor c of Nt with c g <= 12 (c = 32 here: 2 chunks)" if pprec == nsl.F64
+                  else "chunks of the largest divisor c of Nt with c g <= 3 (c = 8 here: 8 chunks)")
     print("# kernel timing (name launches total_ms timed flops bytes):\n# " +
           prof_text.strip().replace("\n", "\n# "), file=sys.stderr)
     out = {"metric": METRIC, "value": round(nb * args.steps / t_mix, 3), "unit": "solves/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": round(t_mix / args.steps * 1e3, 3), "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64+dd",
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64+dd" if pprec == nsl.F64 else "f32+f64+dd",
            "data": "synthetic (Philox-seeded phi ~ N(0, sqrt(U~)), Gaussian RHS)",
            "config": {"workload": "precision form: honeycomb 12x12 (Nx=288), Nt=64, beta=8, U=3, "
                                   "16 fields x 8 RHS, CG on MMdagger to 1e-12 true residual per RHS "
                                   "(double-double), populate + solve per step",
                       "Nx": Nx, "Nt": NT, "beta": BETA, "U": U, "fields": NFIELD,
                       "rhs_per_field": NRHS, "tol": TOL,
-                      "method": "CG, time-slice preconditioner FP32 (factorised per step), FP64 recurrence, "
+                      "method": f"CG, time-slice preconditioner {args.precond.upper()} (factorised per step; "
+                                f"{chunk_rule}, g = delta ||K|| = 0.375), FP64 recurrence, "
                                 "double-double defect correction"},
            "converged": conv_mix, "max_rel_residual": rel_mix, "reached_tol_all": rel_mix <= TOL,
            "iterations": rep_mix["iterations"], "outer_steps": rep_mix["outer_steps"],
@@ -530,13 +542,15 @@ def main():
         out["fp64_reached_tol_all"] = rel64 <= TOL and conv64
         out["speedup_vs_fp64"] = round(out["value"] / v64, 3)
         print(f"# fp64: {t64 / args.fp64_steps * 1e3:.1f} ms/step, report {rep64}", file=sys.stderr)
-        # the same preconditioned solver with the time-slice solve in FP64
-        cg_t64 = nsl.CGTimeSlice(fm, TOL, 5000, precision=nsl.F64)
-        tt, rept, convt, relt, _ = leg(cg_t64, args.steps, 1, False)
-        out["fp64_tss"] = {"solves_per_s": round(nb * args.steps / tt, 3),
-                           "ms_per_step": round(tt / args.steps * 1e3, 3), "iterations": rept["iterations"],
-                           "max_rel_residual": relt, "reached_tol_all": relt <= TOL and convt,
-                           "mixed_over_this": round(out["value"] / (nb * args.steps / tt), 3)}
+        # the same preconditioned solver with the time-slice solve in the
+        # other precision
+        alt = "f32" if pprec == nsl.F64 else "f64"
+        cg_alt = nsl.CGTimeSlice(fm, TOL, 5000, precision=nsl.F32 if alt == "f32" else nsl.F64)
+        tt, rept, convt, relt, _ = leg(cg_alt, args.steps, 1, False)
+        out[f"{alt}_tss"] = {"solves_per_s": round(nb * args.steps / tt, 3),
+                             "ms_per_step": round(tt / args.steps * 1e3, 3), "iterations": rept["iterations"],
+                             "max_rel_residual": relt, "reached_tol_all": relt <= TOL and convt,
+                             "headline_over_this": round(out["value"] / (nb * args.steps / tt), 3)}
     if not args.no_refcrit:
         out["refcrit"] = refcrit_leg(phi_h, b_h, args.refcrit_fields)
     if not args.no_batch:

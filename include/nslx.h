@@ -269,7 +269,12 @@ int nslx_fm_solve_direct(nslx_fm fm, int op, const void* in, void* out, int64_t 
  * iterations, iterations_total = preconditioner applications, outer_steps =
  * double-double residual evaluations after the start, max_rel_residual =
  * max_k |b_k - A (x_k + x_lo_k)| / |b_k| (double-double).  NSLX_NOT_CONVERGED
- * (x written) when maxit inner iterations did not suffice. */
+ * (x written) when maxit inner iterations did not suffice.  nchunks <= 0
+ * picks the chunk length c from the growth per slice g = delta ||K|| +
+ * |delta mu| + max |Im phi|: the largest divisor of Nt with c g <= 3 (at most
+ * 16) for an FP32 preconditioner, c g <= 12 (at most 64) for an FP64 one,
+ * whose wider chunks keep a near-exact preconditioner with fewer block
+ * steps. */
 int nslx_solve_tss(nslx_fm fm, int op, const void* b, void* x, void* x_lo, const void* x0, int64_t nbatch,
                    double tol, int64_t maxit, int precond_precision, int nchunks,
                    nslx_solve_report* report, void* stream);

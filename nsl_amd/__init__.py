@@ -391,7 +391,8 @@ class CGTimeSlice:
     is CG.tpp:14-113).  precision: the preconditioner's (F32: factorisation
     and applies on FP32 matrix cores, the CG recurrence in FP64; F64).
     After a call, .x_lo holds the low part of the double-double solution
-    (x + x_lo) and .report the solver report."""
+    (x + x_lo) and .report the solver report.  nchunks 0: the chunk count
+    nslx_solve_tss picks for the precision (wider chunks in F64)."""
 
     def __init__(self, fm, eps=1e-12, maxIter=10000, precision=F32, nchunks=0):
         self.fm = fm

@@ -50,13 +50,18 @@ __device__ inline typename Acc<S>::T mfma(S a, S b, typename Acc<S>::T c) {
     else return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-constexpr int BK = 16;
 constexpr int PAD = 1;
+
+// K chunk staged per step: 32 for the 32 x 32 tile (the skinny reduced-sweep
+// and recurrence launches, where the K loop is long), 16 for 64 x 64 (LDS)
+template <int BM> struct KChunk { static constexpr int BK = BM == 32 ? 32 : 16; };
 
 template <class S, bool CB, int PH, int BM, int BN>
 __global__ void __launch_bounds__(256) k_bgemm(const BJob* __restrict__ jobs, const int4* __restrict__ tiles) {
+    constexpr int BK = KChunk<BM>::BK;
     constexpr int FM = BM / 32, FN = BN / 32;          // 16 x 16 blocks per wave (2 x 2 waves)
     constexpr int NPB = CB ? 2 : 1;
+    constexpr int NA = BM * BK / 256, NB = BN * BK / 256;   // staged elements per thread
     __shared__ S sA[2][BM][BK + PAD];
     __shared__ S sB[NPB][BN][BK + PAD];
     using AT = typename Acc<S>::T;
@@ -76,11 +81,15 @@ __global__ void __launch_bounds__(256) k_bgemm(const BJob* __restrict__ jobs, co
             #pragma unroll
             for (int j = 0; j < FN; ++j) acc[a][i][j] = AT{0, 0, 0, 0};
 
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        // A chunk: BM rows x BK columns (complex), phase applied (PH == 1)
+    // register stage of one K chunk: the next chunk's global loads are issued
+    // before the current chunk's MFMAs and stored to LDS after them
+    C2<S> ga[NA];
+    C2<S> gb[NB];
+    auto load = [&](int k0) {
         #pragma unroll
-        for (int e = tid; e < BM * BK; e += 256) {
-            const int row = e / BK, kk = e % BK;
+        for (int e = 0; e < NA; ++e) {
+            const int idx = tid + e * 256;
+            const int row = idx / BK, kk = idx % BK;
             const int r = r0 + row, k = k0 + kk;
             C2<S> z{S(0), S(0)};
             if (r < rows && k < K) {
@@ -94,26 +103,40 @@ __global__ void __launch_bounds__(256) k_bgemm(const BJob* __restrict__ jobs, co
                     z.im = im;
                 }
             }
-            sA[0][row][kk] = z.re;
-            sA[1][row][kk] = z.im;
+            ga[e] = z;
         }
-        // B chunk: BN output columns x BK
         #pragma unroll
-        for (int e = tid; e < BN * BK; e += 256) {
-            const int col = e / BK, kk = e % BK;
+        for (int e = 0; e < NB; ++e) {
+            const int idx = tid + e * 256;
+            const int col = idx / BK, kk = idx % BK;
             const int c = c0 + col, k = k0 + kk;
-            if constexpr (CB) {
-                C2<S> z{S(0), S(0)};
-                if (c < cols && k < K) z = ldc<S>(J.B, (long long)c * J.ldb + k);
-                sB[0][col][kk] = z.re;
-                sB[1][col][kk] = z.im;
-            } else {
-                S z = S(0);
-                if (c < cols && k < K) z = reinterpret_cast<const S*>(J.B)[(long long)c * J.ldb + k];
-                sB[0][col][kk] = z;
+            C2<S> z{S(0), S(0)};
+            if (c < cols && k < K) {
+                if constexpr (CB) z = ldc<S>(J.B, (long long)c * J.ldb + k);
+                else z.re = reinterpret_cast<const S*>(J.B)[(long long)c * J.ldb + k];
             }
+            gb[e] = z;
         }
+    };
+    auto store = [&]() {
+        #pragma unroll
+        for (int e = 0; e < NA; ++e) {
+            const int idx = tid + e * 256;
+            sA[0][idx / BK][idx % BK] = ga[e].re;
+            sA[1][idx / BK][idx % BK] = ga[e].im;
+        }
+        #pragma unroll
+        for (int e = 0; e < NB; ++e) {
+            const int idx = tid + e * 256;
+            sB[0][idx / BK][idx % BK] = gb[e].re;
+            if constexpr (CB) sB[1][idx / BK][idx % BK] = gb[e].im;
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        store();
         __syncthreads();
+        if (k0 + BK < K) load(k0 + BK);
         #pragma unroll
         for (int ks = 0; ks < BK / 4; ++ks) {
             const int k = 4 * ks + q;

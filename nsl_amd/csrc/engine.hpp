@@ -80,9 +80,13 @@ struct FermionMatrix {
     SolverWork sw;
     uint64_t pop_gen = 0;           // incremented by every populate()
     // time-slice direct solvers (FP64, FP32) of the populated fields, built on
-    // first use and refactorised after a populate (tss.hip)
+    // first use and refactorised after a populate (tss.hip).  nc <= 0 picks
+    // the chunk count by tss_chunk: the direct solve's bound, or, for a
+    // preconditioner (precond) in FP64, the wider one.
     std::unique_ptr<TimeSliceSolver> tss[2];
-    TimeSliceSolver& time_slice_solver(int prec, int nc, hipStream_t s);
+    TimeSliceSolver& time_slice_solver(int prec, int nc, bool precond, hipStream_t s);
+    // delta ||K|| + |delta mu| + max |Im phi|: the log of a slice's growth bound
+    double slice_growth() const;
     DdWork ddw;
 
     int sgn() const { return species == NSLX_PARTICLE ? +1 : -1; }

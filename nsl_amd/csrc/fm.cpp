@@ -151,12 +151,18 @@ void FermionMatrix::populate(const void* phi_dev, int64_t nf, int sp, hipStream_
     (void)Eptr(NSLX_F64, s);
 }
 
-TimeSliceSolver& FermionMatrix::time_slice_solver(int prec, int nc, hipStream_t s) {
+double FermionMatrix::slice_growth() const {
+    return std::fabs(delta()) * normK + std::fabs(delta() * mu) + max_im_phi;
+}
+
+TimeSliceSolver& FermionMatrix::time_slice_solver(int prec, int nc, bool precond, hipStream_t s) {
     NSLX_REQUIRE(prec == NSLX_F64 || prec == NSLX_F32, "time-slice solve: precision must be F64 or F32");
     auto& t = tss[prec == NSLX_F64 ? 0 : 1];
     // a new factorisation when the fields changed (count, chunking or values)
-    const double g = std::fabs(delta()) * normK + std::fabs(delta() * mu) + max_im_phi;
-    const int want = nc > 0 ? nc : nt / tss_chunk(nt, g);
+    const bool wide = precond && prec == NSLX_F64;
+    const int want = nc > 0 ? nc
+                            : nt / tss_chunk(nt, slice_growth(), wide ? TSS_BOUND_PRECOND_F64 : TSS_BOUND_DIRECT,
+                                             wide ? TSS_CAP_PRECOND : TSS_CAP_DIRECT);
     if (!t || t->chunks() != want || t->fields() != nfield) {
         t.reset();
         t = std::make_unique<TimeSliceSolver>(*this, prec, want, s);

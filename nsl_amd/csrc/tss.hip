@@ -477,10 +477,10 @@ void conj_t(const void* in, long long ldin, long long sin, void* out, long long 
 
 }  // namespace
 
-int tss_chunk(int nt, double g) {
+int tss_chunk(int nt, double g, double bound, int cap) {
     int best = 1;
-    for (int c = 1; c <= std::min(nt, 16); ++c)
-        if (nt % c == 0 && (c == 1 || c * g <= 3.0)) best = c;
+    for (int c = 1; c <= std::min(nt, cap); ++c)
+        if (nt % c == 0 && (c == 1 || c * g <= bound)) best = c;
     return best;
 }
 
@@ -497,8 +497,7 @@ TimeSliceSolver::TimeSliceSolver(FermionMatrix& fm, int prec, int nc, hipStream_
         NSLX_REQUIRE(nt_ % nc == 0, "time-slice solve: nchunks must divide Nt");
         nc_ = nc;
     } else {
-        const double g = std::fabs(fm.delta()) * fm.normK + std::fabs(fm.delta() * fm.mu) + fm.max_im_phi;
-        nc_ = nt_ / tss_chunk(nt_, g);
+        nc_ = nt_ / tss_chunk(nt_, fm.slice_growth());
     }
     c_ = nt_ / nc_;
     const size_t n = n_, nn = n * n;

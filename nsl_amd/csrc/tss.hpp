@@ -115,8 +115,14 @@ class BPlan {
 
 // Chunk length of the time-slice solve for nt slices whose one-slice growth
 // is g (delta ||K|| + |delta mu| + max |Im phi|): the largest divisor c of nt
-// with c g <= 3 (at least 1, at most 16).
-int tss_chunk(int nt, double g);
+// with c g <= bound (at least 1, at most cap).  A chunk product's condition
+// number is at most e^{2 c g}: bound 3 keeps it below ~400, where an FP32
+// product is accurate to ~3e-5 (the direct solve's rule in both precisions);
+// bound 12 keeps it below ~2.6e10, ~3e-6 relative in FP64 (enough for a
+// preconditioner, with a quarter of the block steps).
+constexpr double TSS_BOUND_DIRECT = 3.0, TSS_BOUND_PRECOND_F64 = 12.0;
+constexpr int TSS_CAP_DIRECT = 16, TSS_CAP_PRECOND = 64;
+int tss_chunk(int nt, double g, double bound = TSS_BOUND_DIRECT, int cap = TSS_CAP_DIRECT);
 
 // Factorisation of R per field and the apply plans, in one precision.
 class TimeSliceSolver {

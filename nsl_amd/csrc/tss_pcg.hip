@@ -129,7 +129,7 @@ int solve_tss(FermionMatrix& fm, int op, const void* b, void* x, void* xlo, cons
     int* nact = w.status.p;
     int* nact_h = w.status_h.p;
 
-    TimeSliceSolver& ts = fm.time_slice_solver(pprec, nchunks, s);
+    TimeSliceSolver& ts = fm.time_slice_solver(pprec, nchunks, true, s);
     norm2_perb(1, b, nb, len, bn2, fm.vsc, s);
     if (x0) {
         NSLX_HIP(hipMemcpyAsync(x, x0, bytes, hipMemcpyDeviceToDevice, s));
