@@ -252,7 +252,10 @@ int nslx_solve_precond(nslx_fm fm, nslx_fm fm_prec, int op, const void* b, void*
  * nslx_fm_solve_direct: out = M^{-1} in (op NSLX_OP_M), M^{-dag} in
  * (NSLX_OP_MDAGGER) or (M M^dag)^{-1} in (NSLX_OP_MMDAGGER) for nbatch RHS;
  * precision NSLX_F64 (c128 in/out) or NSLX_F32 (c64 in/out).  The first call
- * after a populate factorises (nchunks <= 0: chosen from the growth per slice). */
+ * after a populate factorises (nchunks <= 0: chosen from the growth per slice).
+ * Synchronous.  The in-chunk recurrences run as one persistent launch per
+ * pass where the device holds its grid (NSLX_CHAINS=0: one launch per slice);
+ * NSLX_E_INTERNAL if such a launch stopped waiting for its workgroups. */
 int nslx_fm_solve_direct(nslx_fm fm, int op, const void* in, void* out, int64_t nbatch, int precision,
                          int nchunks, void* stream);
 

@@ -10,6 +10,7 @@
 // (re, im share B); complex x complex is four (rr, ii, ri, ir), combined in
 // the epilogue.
 #include <cmath>
+#include <cstdlib>
 #include <functional>
 
 #include "prof.hpp"
@@ -283,6 +284,7 @@ void BPlan::gemm(const std::vector<BJob>& jobs, BVariant v, int tile) {
     op.is_gemm = true;
     op.v = v;
     op.job0 = jobs_.size();
+    op.njobs = jobs.size();
     op.tile0 = tiles_.size();
     // tile job indices are local to this launch: run() passes the launch's
     // own slice of the job array
@@ -295,6 +297,65 @@ void BPlan::gemm(const std::vector<BJob>& jobs, BVariant v, int tile) {
     bgemm_work(jobs, v, &op.flops, &op.bytes);
     ops_.push_back(std::move(op));
     uploaded_ = false;
+}
+
+void BPlan::chain(const std::vector<std::vector<BJob>>& steps, BVariant v) {
+    if (steps.empty()) return;
+    const size_t nch = steps[0].size();
+    NSLX_REQUIRE(nch >= 1, "chain plan: no jobs in a step");
+    const BJob& j0 = steps[0][0];
+    for (size_t s = 0; s < steps.size(); ++s) {
+        NSLX_REQUIRE(steps[s].size() == nch, "chain plan: steps differ in their job count");
+        for (size_t i = 0; i < nch; ++i) {
+            const BJob& j = steps[s][i];
+            NSLX_REQUIRE(j.B == j0.B && j.ldb == j0.ldb && j.rows == j0.rows && j.cols == j0.cols && j.K == j0.K,
+                         "chain plan: steps differ in B or shape");
+            NSLX_REQUIRE(j.K <= j.kseg, "chain plan: split A rows");
+            if (s > 0) {
+                const BJob& p = steps[s - 1][i];
+                NSLX_REQUIRE(j.A0 == p.Y && j.lda0 == p.ldy && j.K == j.cols,
+                             "chain plan: a step does not read the previous step's rows");
+            }
+        }
+    }
+    const char* env = std::getenv("NSLX_CHAINS");   // read per plan: tests compare both paths
+    const bool enabled = !(env && env[0] == '0');
+    v.tile = 1;
+    if (!enabled || v.cb || !chain_supported(v, (int)steps.size(), (int)nch, j0.rows, j0.cols, j0.K)) {
+        for (const auto& st : steps) gemm(st, v);
+        return;
+    }
+    Op op;
+    op.is_chain = true;
+    op.v = v;
+    op.job0 = jobs_.size();
+    op.njobs = steps.size() * nch;
+    op.steps = (int)steps.size();
+    op.nchain = (int)nch;
+    op.rows = j0.rows;
+    op.cols = j0.cols;
+    for (const auto& st : steps) {
+        jobs_.insert(jobs_.end(), st.begin(), st.end());
+        double f = 0.0, b = 0.0;
+        bgemm_work(st, v, &f, &b);
+        op.flops += f;
+        op.bytes += b;
+    }
+    sync_words_ = std::max(sync_words_, chain_sync_words(op.nchain, op.rows));
+    ops_.push_back(std::move(op));
+    uploaded_ = false;
+}
+
+bool BPlan::chain_failed(hipStream_t s) {
+    if (!err_.p) return false;
+    int e = 0;
+    NSLX_HIP(hipMemcpyAsync(&e, err_.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    NSLX_HIP(hipStreamSynchronize(s));
+    if (!e) return false;
+    NSLX_HIP(hipMemsetAsync(err_.p, 0, sizeof(int), s));
+    NSLX_HIP(hipMemsetAsync(sync_.p, 0, sync_words_ * sizeof(unsigned), s));
+    NSLX_HIP(hipStreamSynchronize(s));
+    return true;
 }
 
 void BPlan::custom(std::function<void(hipStream_t)> fn) {
@@ -341,17 +402,18 @@ void BPlan::upload(hipStream_t s) {
                     NSLX_REQUIRE(tl.x >= 0 && (size_t)tl.x < nj, "bgemm plan: tile job index out of range");
                 }
             }
-        for (size_t o = 0; o < ops_.size(); ++o) {
-            const Op& op = ops_[o];
-            if (!op.is_gemm) continue;
-            size_t end = jobs_.size();
-            for (size_t q = o + 1; q < ops_.size(); ++q)
-                if (ops_[q].is_gemm) { end = ops_[q].job0; break; }
-            for (size_t k = op.job0; k < end; ++k) check_job(jobs_[k], op.v, k);
-        }
+        for (const Op& op : ops_)
+            if (op.is_gemm || op.is_chain)
+                for (size_t k = op.job0; k < op.job0 + op.njobs; ++k) check_job(jobs_[k], op.v, k);
     }
     djobs_.alloc(std::max<size_t>(jobs_.size(), 1));
     dtiles_.alloc(std::max<size_t>(tiles_.size(), 1));
+    if (sync_words_) {
+        sync_.alloc(sync_words_);
+        err_.alloc(1);
+        NSLX_HIP(hipMemsetAsync(sync_.p, 0, sync_words_ * sizeof(unsigned), s));
+        NSLX_HIP(hipMemsetAsync(err_.p, 0, sizeof(int), s));
+    }
     if (!jobs_.empty())
         NSLX_HIP(hipMemcpyAsync(djobs_.p, jobs_.data(), jobs_.size() * sizeof(BJob), hipMemcpyHostToDevice, s));
     if (!tiles_.empty())
@@ -364,6 +426,9 @@ void BPlan::run(hipStream_t s) const {
     NSLX_REQUIRE(uploaded_, "bgemm plan run before upload");
     for (const Op& op : ops_) {
         if (op.is_gemm) launch_bgemm(op.v, djobs_.p + op.job0, dtiles_.p + op.tile0, op.ntiles, s, op.flops, op.bytes);
+        else if (op.is_chain)
+            launch_chain(op.v, djobs_.p + op.job0, op.steps, op.nchain, op.rows, op.cols, sync_.p, err_.p, s, op.flops,
+                         op.bytes);
         else op.fn(s);
     }
 }

@@ -251,7 +251,9 @@ int nslx_fm_solve_direct(nslx_fm fm, int op, const void* in, void* out, int64_t 
         NSLX_REQUIRE(precision == NSLX_F64 || precision == NSLX_F32, "solve_direct: precision must be F64 or F32");
         m.require_populated();
         auto s = nslx::as_stream(stream);
-        m.time_slice_solver(precision, nchunks, false, s).apply(op, in, out, nbatch, false, s);
+        auto& ts = m.time_slice_solver(precision, nchunks, false, s);
+        ts.apply(op, in, out, nbatch, false, s);
+        ts.check_passes(s);
         return NSLX_OK;
     });
 }

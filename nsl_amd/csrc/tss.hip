@@ -801,13 +801,17 @@ void TimeSliceSolver::build_minv(BPlan& p, const void* y, void* x, ApplyBufs& B,
         return b;
     };
     // pass 1: in-chunk recurrence from zero inflow
-    for (int q = 1; q < c; ++q) {
-        std::vector<BJob> jobs;
-        for (int j = 0; j < nc; ++j) {
-            const int t = j * c + q;
-            jobs.push_back(pass_job(t, (q == 1 ? Y : X) + (size_t)(t - 1) * n * e, lv));
+    {
+        std::vector<std::vector<BJob>> steps;
+        for (int q = 1; q < c; ++q) {
+            std::vector<BJob> jobs;
+            for (int j = 0; j < nc; ++j) {
+                const int t = j * c + q;
+                jobs.push_back(pass_job(t, (q == 1 ? Y : X) + (size_t)(t - 1) * n * e, lv));
+            }
+            steps.push_back(std::move(jobs));
         }
-        p.gemm(jobs, BVariant{prec_, false, 1, 0});
+        p.chain(steps, BVariant{prec_, false, 1, 0});
     }
     const char* wsrc = c == 1 ? Y : X;
     auto w_at = [&](int j, int f) { return wsrc + ((size_t)f * rpf * lv + (size_t)(j * c + c - 1) * n) * e; };
@@ -880,14 +884,18 @@ void TimeSliceSolver::build_minv(BPlan& p, const void* y, void* x, ApplyBufs& B,
         }
     }
     // pass 2: in-chunk recurrence from the reduced solution
-    for (int q = 0; q < c; ++q) {
-        std::vector<BJob> jobs;
-        for (int j = 0; j < nc; ++j) {
-            const int t = j * c + q;
-            if (q == 0) jobs.push_back(pass_job(t, Z_ + (size_t)((j - 1 + nc) % nc) * n * e, lr));
-            else jobs.push_back(pass_job(t, X + (size_t)(t - 1) * n * e, lv));
+    {
+        std::vector<std::vector<BJob>> steps;
+        for (int q = 0; q < c; ++q) {
+            std::vector<BJob> jobs;
+            for (int j = 0; j < nc; ++j) {
+                const int t = j * c + q;
+                if (q == 0) jobs.push_back(pass_job(t, Z_ + (size_t)((j - 1 + nc) % nc) * n * e, lr));
+                else jobs.push_back(pass_job(t, X + (size_t)(t - 1) * n * e, lv));
+            }
+            steps.push_back(std::move(jobs));
         }
-        p.gemm(jobs, BVariant{prec_, false, 1, 0});
+        p.chain(steps, BVariant{prec_, false, 1, 0});
     }
 }
 
@@ -922,13 +930,17 @@ void TimeSliceSolver::build_mdinv(BPlan& p, const void* y, void* x, ApplyBufs& B
         b.sgn = s == nt - 1 ? -1.0 : 1.0;
         return b;
     };
-    for (int m = 1; m < c; ++m) {
-        std::vector<BJob> jobs;
-        for (int j = 0; j < nc; ++j) {
-            const int s = (lastix(j) - m + nt) % nt;
-            jobs.push_back(pass_job(s, (m == 1 ? Y : X) + (size_t)((s + 1) % nt) * n * e, lv));
+    {
+        std::vector<std::vector<BJob>> steps;
+        for (int m = 1; m < c; ++m) {
+            std::vector<BJob> jobs;
+            for (int j = 0; j < nc; ++j) {
+                const int s = (lastix(j) - m + nt) % nt;
+                jobs.push_back(pass_job(s, (m == 1 ? Y : X) + (size_t)((s + 1) % nt) * n * e, lv));
+            }
+            steps.push_back(std::move(jobs));
         }
-        p.gemm(jobs, BVariant{prec_, false, 2, 0});
+        p.chain(steps, BVariant{prec_, false, 2, 0});
     }
     const char* wsrc = c == 1 ? Y : X;
     // what_k = w'_{k+1}, at slot f_{(k+1) mod nc}
@@ -1033,15 +1045,31 @@ void TimeSliceSolver::build_mdinv(BPlan& p, const void* y, void* x, ApplyBufs& B
         }
     }
     // pass 2: backward recurrence from zhat_j = z'_{j+1}
-    for (int m = 0; m < c; ++m) {
-        std::vector<BJob> jobs;
-        for (int j = 0; j < nc; ++j) {
-            const int s = (lastix(j) - m + nt) % nt;
-            if (m == 0) jobs.push_back(pass_job(s, Z_ + (size_t)j * n * e, lr));
-            else jobs.push_back(pass_job(s, X + (size_t)((s + 1) % nt) * n * e, lv));
+    {
+        std::vector<std::vector<BJob>> steps;
+        for (int m = 0; m < c; ++m) {
+            std::vector<BJob> jobs;
+            for (int j = 0; j < nc; ++j) {
+                const int s = (lastix(j) - m + nt) % nt;
+                if (m == 0) jobs.push_back(pass_job(s, Z_ + (size_t)j * n * e, lr));
+                else jobs.push_back(pass_job(s, X + (size_t)((s + 1) % nt) * n * e, lv));
+            }
+            steps.push_back(std::move(jobs));
         }
-        p.gemm(jobs, BVariant{prec_, false, 2, 0});
+        p.chain(steps, BVariant{prec_, false, 2, 0});
     }
+}
+
+void TimeSliceSolver::check_passes(hipStream_t s) {
+    bool failed = false;
+    for (auto& kv : plans_) {
+        failed = kv.second->Minv.chain_failed(s) || failed;
+        failed = kv.second->Mdinv.chain_failed(s) || failed;
+    }
+    if (failed)
+        throw Error(NSLX_E_INTERNAL,
+                    "time-slice solve: a recurrence launch stopped waiting for its workgroups (results invalid; "
+                    "set NSLX_CHAINS=0 for one launch per slice)");
 }
 
 void TimeSliceSolver::apply(int op, const void* y, void* x, int64_t nb, bool io_c128, hipStream_t s) {

@@ -83,6 +83,19 @@ void launch_bgemm(const BVariant& v, const BJob* d_jobs, const int4* d_tiles, in
 // of a job list.
 void bgemm_work(const std::vector<BJob>& jobs, const BVariant& v, double* flops, double* bytes);
 
+// Persistent recurrence launch (chain.hip) of steps x nchain jobs, job
+// s * nchain + i = step s of chain i, all with a real B (the same B, rows,
+// cols and K), step s + 1 of a chain reading as A the rows step s wrote.
+// chain_supported: whether the launch fits the current device (K within the
+// LDS budget, one workgroup per CU for the whole grid); sync: the plan's
+// chain_sync_words(nchain, rows) counters (zero between launches); err: set
+// to 1 when a workgroup gave up waiting for its group (results invalid).
+bool chain_supported(const BVariant& v, int steps, int nchain, int rows, int cols, int K);
+size_t chain_sync_words(int nchain, int rows);
+std::string chain_class(const BVariant& v);
+void launch_chain(const BVariant& v, const BJob* d_jobs, int steps, int nchain, int rows, int cols, unsigned* sync,
+                  int* err, hipStream_t s, double flops = 0.0, double bytes = 0.0);
+
 // A recorded sequence of launches whose job and tile arrays live in one
 // device buffer each (uploaded once); replayed by run().
 class BPlan {
@@ -90,18 +103,27 @@ class BPlan {
     // append a batched GEMM over jobs (tile shape picked from the tile count
     // when tile < 0)
     void gemm(const std::vector<BJob>& jobs, BVariant v, int tile = -1);
+    // append a recurrence: steps[s][i] = step s of chain i, where step s + 1
+    // reads as A the rows step s wrote as Y (A0 == the previous Y, same
+    // stride).  One persistent launch when chain_supported() (and
+    // NSLX_CHAINS is not 0), else one batched GEMM launch per step.
+    void chain(const std::vector<std::vector<BJob>>& steps, BVariant v);
     // append any other launch
     void custom(std::function<void(hipStream_t)> fn);
     void upload(hipStream_t s);
     void run(hipStream_t s) const;
     size_t launches() const { return ops_.size(); }
+    // whether a chain launch of the plan gave up waiting (clears the flag
+    // and the counters; synchronises s)
+    bool chain_failed(hipStream_t s);
 
    private:
     struct Op {
-        bool is_gemm = false;
+        bool is_gemm = false, is_chain = false;
         BVariant v;
-        size_t job0 = 0, tile0 = 0;
+        size_t job0 = 0, njobs = 0, tile0 = 0;
         int ntiles = 0;
+        int steps = 0, nchain = 0, rows = 0, cols = 0;
         double flops = 0.0, bytes = 0.0;
         std::function<void(hipStream_t)> fn;
     };
@@ -110,6 +132,9 @@ class BPlan {
     std::vector<int4> tiles_;
     DevBuf<BJob> djobs_;
     DevBuf<int4> dtiles_;
+    DevBuf<unsigned> sync_;
+    DevBuf<int> err_;
+    size_t sync_words_ = 0;
     bool uploaded_ = false;
 };
 
@@ -147,6 +172,9 @@ class TimeSliceSolver {
     uint64_t generation() const { return gen_; }
     // factorise again for the fields now populated (same count and chunking)
     void refactor(hipStream_t s);
+    // throws when a persistent recurrence launch of an apply since the last
+    // check gave up waiting (synchronises s)
+    void check_passes(hipStream_t s);
 
    private:
     struct ApplyBufs {

@@ -205,6 +205,7 @@ int solve_tss(FermionMatrix& fm, int op, const void* b, void* x, void* xlo, cons
     NSLX_HIP(hipMemcpyAsync(hb.data(), bn2, nb * sizeof(double), hipMemcpyDeviceToHost, s));
     NSLX_HIP(hipMemcpyAsync(hd.data(), done_out, nb * sizeof(int), hipMemcpyDeviceToHost, s));
     NSLX_HIP(hipStreamSynchronize(s));
+    ts.check_passes(s);
     int64_t conv = 0;
     double maxrel = 0.0, tot = 0.0;
     for (int64_t i = 0; i < nb; ++i) {

@@ -11,6 +11,7 @@ in double-double (tests/ddarith.py, pinned against exact rational
 arithmetic) from the FP64 E and phase table the engine exported, so the
 check does not trust the device's own residual kernel."""
 
+import ctypes
 import math
 
 import numpy as np
@@ -176,3 +177,63 @@ def test_tss_errors():
         fm.solve_direct(nsl.MatrixCombination.MdaggerM, y)
     with pytest.raises(nsl.NslxError):                  # nchunks must divide Nt
         fm.solve_direct(nsl.MatrixCombination.M, y, nchunks=3)
+
+
+def _prof_classes(fn):
+    """Runs fn with every launch timed; returns the kernel classes launched."""
+    L_ = nsl._lib.lib()
+    torch.cuda.synchronize()
+    L_.nslx_prof_reset()
+    L_.nslx_prof_enable(1)
+    try:
+        out = fn()
+        torch.cuda.synchronize()
+    finally:
+        L_.nslx_prof_enable(0)
+    buf = ctypes.create_string_buffer(1 << 16)
+    L_.nslx_prof_read(buf, len(buf))
+    L_.nslx_prof_reset()
+    return out, {line.split()[0] for line in buf.value.decode().splitlines() if line.strip()}
+
+
+@pytest.mark.parametrize("prec", [nsl.F64, nsl.F32])
+@pytest.mark.parametrize("case", [1, 3, 6])
+def test_chain_launch_matches_per_slice_launches(case, prec, monkeypatch):
+    """The persistent recurrence launch (one launch per pass) against one
+    batched GEMM launch per slice (NSLX_CHAINS=0) on the same fields: the same
+    M^{-1} y and M^{-dag} y up to the summation order (cond(M) < 1e4 here:
+    1e-10 in FP64, 1e-3 in FP32), and the kernel classes show which path ran."""
+    kind, size, Nt, beta, extra = CASES[case]
+    kw = {k: v for k, v in extra.items() if k != "nchunks"}
+    res = {}
+    y = None
+    ops = (nsl.MatrixCombination.M, nsl.MatrixCombination.Mdagger)
+    for chains in ("1", "0"):
+        monkeypatch.setenv("NSLX_CHAINS", chains)
+        fm, _ = make(kind, size, Nt, beta, nfield=2, seed=1100 + case, **kw)
+        if y is None:
+            y = torch.as_tensor(rng.complex_normal(1400 + case, 0, (4, Nt, fm.Nx))).to(DEV)
+        xs, classes = _prof_classes(lambda: [fm.solve_direct(op, y, precision=prec) for op in ops])
+        chain_classes = {c for c in classes if c.startswith("chain_")}
+        if chains == "1":
+            assert len(chain_classes) == 2, classes     # ph1 (M^-1) and ph2 (M^-dag) passes
+        else:
+            assert not chain_classes, classes
+        res[chains] = [x.to("cpu", torch.complex128) for x in xs]
+    tol = 1e-10 if prec == nsl.F64 else 1e-3
+    for a, b in zip(res["1"], res["0"]):
+        assert rel(a, b) < tol
+
+def test_chain_give_up_is_reported_and_recovers(monkeypatch):
+    """A wait bound of 0 makes every wait of the persistent recurrence launch
+    give up: the launch still ends normally, the solve reports
+    NSLX_E_INTERNAL, and the next solve (normal bound) is correct again."""
+    Nt, beta = 32, 6.0
+    fm, ofs = make("honeycomb", (6, 6), Nt, beta, nfield=2, seed=1500)
+    y = torch.as_tensor(rng.complex_normal(1501, 0, (4, Nt, 72)))
+    monkeypatch.setenv("NSLX_CHAIN_SPIN", "0")
+    with pytest.raises(nsl.NslxError):
+        fm.solve_direct(nsl.MatrixCombination.M, y.to(DEV))
+    monkeypatch.delenv("NSLX_CHAIN_SPIN")
+    x = fm.solve_direct(nsl.MatrixCombination.M, y.to(DEV)).cpu()
+    assert rel(oracle_apply(ofs, "M", x), y) < 1e-10
