@@ -1,11 +1,14 @@
 // Batched complex NT GEMM on CDNA4 matrix cores (see BJob in tss.hpp), the
 // engine of the time-slice solver: chunk products, the forward/back sweeps
-// of the reduced block system, the Gauss-Jordan trailing updates and the
-// in-chunk recurrences are all launches of this kernel over job lists.
+// of the reduced block system, the Gauss-Jordan trailing updates and (where
+// the persistent recurrence launch of chain.hip does not fit) the in-chunk
+// recurrences are all launches of this kernel over job lists.
 //
 // A workgroup (4 waves, 2 x 2) owns one BM x BN tile of one job; the K loop
-// stages BK = 16 columns of A (re / im planes, phase applied by the loader)
-// and of B (re [/ im] planes) in LDS and feeds v_mfma_f64_16x16x4f64 (FP64) or
+// stages BK columns of A (16 for 64 x 64 tiles, 32 for 32 x 32; re / im
+// planes, phase applied by the loader) and of B (re [/ im] planes) in LDS,
+// the next chunk's global loads in flight during the current chunk's MFMAs,
+// and feeds v_mfma_f64_16x16x4f64 (FP64) or
 // v_mfma_f32_16x16x4f32 (FP32).  Complex A x real B is two real products
 // (re, im share B); complex x complex is four (rr, ii, ri, ir), combined in
 // the epilogue.
@@ -82,22 +85,51 @@ __global__ void __launch_bounds__(256) k_bgemm(const BJob* __restrict__ jobs, co
             #pragma unroll
             for (int j = 0; j < FN; ++j) acc[a][i][j] = AT{0, 0, 0, 0};
 
+    // staging coordinates of this thread, fixed over the K loop: its element
+    // e is tile row (B: tile column) rb + e * RS at K offset kk of a chunk, so
+    // the row addresses and phase rows are computed once, not per chunk
+    constexpr int RS = 256 / BK;
+    const int kk = tid % BK, rb = tid / BK;
+    const size_t ce = 2 * sizeof(S);
+    const char* A1base = static_cast<const char*>(J.A1 ? J.A1 : J.A0);
+    const char* pa0[NA];
+    const char* pa1[NA];
+    const char* pph[NA];
+    bool ra[NA];
+    #pragma unroll
+    for (int e = 0; e < NA; ++e) {
+        const int r = r0 + rb + e * RS;
+        ra[e] = r < rows;
+        const long long rr = ra[e] ? r : 0;
+        pa0[e] = static_cast<const char*>(J.A0) + rr * J.lda0 * ce;
+        pa1[e] = A1base + rr * J.lda1 * ce;
+        if constexpr (PH == 1) pph[e] = static_cast<const char*>(J.ph) + (rr / J.phdiv) * J.ldph * ce;
+        else pph[e] = nullptr;
+    }
+    const char* pb[NB];
+    bool cv[NB];
+    #pragma unroll
+    for (int e = 0; e < NB; ++e) {
+        const int c = c0 + rb + e * RS;
+        cv[e] = c < cols;
+        pb[e] = static_cast<const char*>(J.B) + (long long)(cv[e] ? c : 0) * J.ldb * (CB ? ce : sizeof(S));
+    }
+    const int kseg = J.kseg;
+
     // register stage of one K chunk: the next chunk's global loads are issued
     // before the current chunk's MFMAs and stored to LDS after them
     C2<S> ga[NA];
     C2<S> gb[NB];
     auto load = [&](int k0) {
+        const int k = k0 + kk;
+        const bool kin = k < K;
         #pragma unroll
         for (int e = 0; e < NA; ++e) {
-            const int idx = tid + e * 256;
-            const int row = idx / BK, kk = idx % BK;
-            const int r = r0 + row, k = k0 + kk;
             C2<S> z{S(0), S(0)};
-            if (r < rows && k < K) {
-                z = k < J.kseg ? ldc<S>(J.A0, (long long)r * J.lda0 + k)
-                               : ldc<S>(J.A1, (long long)r * J.lda1 + (k - J.kseg));
+            if (ra[e] && kin) {
+                z = k < kseg ? ldc<S>(pa0[e], k) : ldc<S>(pa1[e], k - kseg);
                 if constexpr (PH == 1) {
-                    const C2<S> p = ldc<S>(J.ph, (long long)(r / J.phdiv) * J.ldph + k);
+                    const C2<S> p = ldc<S>(pph[e], k);
                     const S re = p.re * z.re - p.im * z.im;
                     const S im = p.re * z.im + p.im * z.re;
                     z.re = re;
@@ -108,13 +140,10 @@ __global__ void __launch_bounds__(256) k_bgemm(const BJob* __restrict__ jobs, co
         }
         #pragma unroll
         for (int e = 0; e < NB; ++e) {
-            const int idx = tid + e * 256;
-            const int col = idx / BK, kk = idx % BK;
-            const int c = c0 + col, k = k0 + kk;
             C2<S> z{S(0), S(0)};
-            if (c < cols && k < K) {
-                if constexpr (CB) z = ldc<S>(J.B, (long long)c * J.ldb + k);
-                else z.re = reinterpret_cast<const S*>(J.B)[(long long)c * J.ldb + k];
+            if (cv[e] && kin) {
+                if constexpr (CB) z = ldc<S>(pb[e], k);
+                else z.re = reinterpret_cast<const S*>(pb[e])[k];
             }
             gb[e] = z;
         }
@@ -122,15 +151,13 @@ __global__ void __launch_bounds__(256) k_bgemm(const BJob* __restrict__ jobs, co
     auto store = [&]() {
         #pragma unroll
         for (int e = 0; e < NA; ++e) {
-            const int idx = tid + e * 256;
-            sA[0][idx / BK][idx % BK] = ga[e].re;
-            sA[1][idx / BK][idx % BK] = ga[e].im;
+            sA[0][rb + e * RS][kk] = ga[e].re;
+            sA[1][rb + e * RS][kk] = ga[e].im;
         }
         #pragma unroll
         for (int e = 0; e < NB; ++e) {
-            const int idx = tid + e * 256;
-            sB[0][idx / BK][idx % BK] = gb[e].re;
-            if constexpr (CB) sB[1][idx / BK][idx % BK] = gb[e].im;
+            sB[0][rb + e * RS][kk] = gb[e].re;
+            if constexpr (CB) sB[1][rb + e * RS][kk] = gb[e].im;
         }
     };
     load(0);
