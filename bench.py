@@ -218,105 +218,141 @@ def cpu_baseline(phi, b, cores, iters_needed, chunk_s=6.0):
                       f"{solve_s:.0f} s", **info}
 
 
-def batching_leg(prec, nprob=64, seed=7000, tol=1e-10):
-    """The batching form: nprob independent fermion matrices of uneven size
-    (honeycomb Nx 18/72, (6,6) nanotubes Nx 96-144), Nt 16/32, beta 2-6,
-    U 1-4, one Philox field each with 1-16 Gaussian RHS, solved to 1e-10
-    per RHS (a) in one grouped launch per Krylov phase, HIP-graph replayed,
-    (b) the same without the graph, (c) one problem at a time with the
-    single-problem solver.  Returns solves/s of each."""
-    import torch
+def batch_mix_specs(nprob=64, seed=7000):
+    """The batching form's mix: nprob independent problems, half honeycomb
+    flakes with Nx in {18, 72, 288, 1152}, half (6,6) armchair nanotubes of
+    4-32 unit cells (Nx 96-768); Nt in {16, 32, 64, 128}, beta in [2, 12],
+    U in [1, 4], 1-16 Gaussian RHS, one Philox field each (all draws from
+    the library's Philox stream at seed)."""
     import nsl_amd as nsl
     u = nsl.rng_uniform(seed, 0, nprob * 6).reshape(nprob, 6)
-    lats = {}
-    probs = []
+    specs = []
     for k in range(nprob):
-        kind = int(u[k, 0] * 4)
-        key = [("h", 3), ("h", 6), ("n", 4), ("n", 6)][kind]
-        if key not in lats:
-            lats[key] = (nsl.Lattice.honeycomb(key[1], key[1]) if key[0] == "h"
-                         else nsl.Lattice.armchair_nanotube(6, key[1]))
-        lat = lats[key]
-        Nt = 16 if u[k, 1] < 0.5 else 32
-        beta = 2.0 + 4.0 * u[k, 2]
-        Uk = 1.0 + 3.0 * u[k, 3]
-        nrhs = 1 + int(u[k, 4] * 16)
-        fm = nsl.HubbardExp(lat, Nt, beta)
-        fm.populate(torch.as_tensor(nsl.real_field(seed + 1 + k, (Nt, lat.sites()),
-                                                   math.sqrt(beta / Nt * Uk))).cuda())
-        b = torch.as_tensor(nsl.complex_normal(seed + 500 + k, (nrhs, Nt, lat.sites()))).cuda()
-        probs.append((fm, b))
-    nsolves = sum(b.shape[0] for _, b in probs)
-    out = {"problems": nprob, "rhs": nsolves}
-    for name, graph in (("grouped_graph", True), ("grouped", False)):
-        nsl.batch_solve(probs[:4], tol=tol, precision=prec, use_graph=graph)        # warm-up
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        _, rep = nsl.batch_solve(probs, tol=tol, precision=prec, use_graph=graph)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        out[name] = {"solves_per_s": round(nsolves / dt, 1), "iterations": rep["iterations"],
-                     "converged": rep["converged"], "max_rel_residual": rep["max_rel_residual"],
-                     "escalations": rep["escalations"]}
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for fm, b in probs:
-        nsl.CG(fm, nsl.MMdagger, tol, 100000, precision=prec, tol_kind=nsl.TOL_REL_PER_RHS_FLOOR)(b)
-    torch.cuda.synchronize()
-    out["one_at_a_time"] = {"solves_per_s": round(nsolves / (time.perf_counter() - t0), 1)}
-    out["speedup_graph_vs_one_at_a_time"] = round(out["grouped_graph"]["solves_per_s"] /
-                                                 out["one_at_a_time"]["solves_per_s"], 2)
+        if u[k, 0] < 0.5:
+            lat = ("honeycomb", [3, 6, 12, 24][int(u[k, 5] * 4)])
+        else:
+            lat = ("nanotube", 4 + int(u[k, 5] * 29))
+        specs.append({"lattice": lat, "Nt": [16, 32, 64, 128][int(u[k, 1] * 4)], "beta": 2.0 + 10.0 * u[k, 2],
+                      "U": 1.0 + 3.0 * u[k, 3], "nrhs": 1 + int(u[k, 4] * 16), "field_seed": seed + 1 + k,
+                      "rhs_seed": seed + 500 + k})
+    return specs
+
+
+def mix_lattice(key, cache):
+    import nsl_amd as nsl
+    if key not in cache:
+        cache[key] = (nsl.Lattice.honeycomb(key[1], key[1]) if key[0] == "honeycomb"
+                      else nsl.Lattice.armchair_nanotube(6, key[1]))
+    return cache[key]
+
+
+def build_mix_problems(specs):
+    """(fm, b, phi) per spec on the device."""
+    import torch
+    import nsl_amd as nsl
+    lats, out = {}, []
+    for sp in specs:
+        lat = mix_lattice(sp["lattice"], lats)
+        Nx = lat.sites()
+        fm = nsl.HubbardExp(lat, sp["Nt"], sp["beta"])
+        phi = torch.as_tensor(nsl.real_field(sp["field_seed"], (sp["Nt"], Nx),
+                                             math.sqrt(sp["beta"] / sp["Nt"] * sp["U"]))).cuda()
+        fm.populate(phi)
+        b = torch.as_tensor(nsl.complex_normal(sp["rhs_seed"], (sp["nrhs"], sp["Nt"], Nx))).cuda()
+        out.append((fm, b, phi))
     return out
 
 
-SERVE_RATE = 40.0      # requests / s (Poisson)
+def batching_leg(nprob=64, seed=7000, tol=1e-10, krylov=True, krylov_maxit=5000):
+    """The batching form on its full mix (batch_mix_specs): every step
+    re-populates all 64 fields (so each solve factorises afresh) and solves
+    every RHS to |b - MM^dag x| <= 1e-10 |b|:
+      (a) time-slice-preconditioned CG with double-double correction, the
+          problems spread over 8 worker streams (nslx_batch_solve_tss);
+      (b) the same on one stream (one problem after another);
+      (c) the Krylov pool: unpreconditioned CG for all problems in one
+          grouped launch per iteration, HIP-graph replayed, FP32 recurrence
+          with FP64 reliable updates, capped at krylov_maxit iterations.
+    Returns solves/s (RHS / wall time) and convergence of each."""
+    import torch
+    import nsl_amd as nsl
+    specs = batch_mix_specs(nprob, seed)
+    probs = build_mix_problems(specs)
+    pb = [(fm, b) for fm, b, _ in probs]
+    nsolves = sum(b.shape[0] for b in (p[1] for p in probs))
+    sizes = sorted({(p[0].Nx, p[0].Nt) for p in probs})
+    out = {"problems": nprob, "rhs": nsolves, "tol": tol,
+           "workload": "64 fields: honeycomb Nx 18/72/288/1152 and (6,6) nanotubes of 4-32 cells (Nx 96-768), "
+                       "Nt 16/32/64/128, beta 2-12, U 1-4, 1-16 Gaussian RHS each; fields re-populated per run",
+           "distinct_shapes": len(sizes), "max_Nx": max(n for n, _ in sizes), "max_Nt": max(t for _, t in sizes)}
+    for name, ns in (("tss_streams8", 8), ("tss_one_stream", 1)):
+        nsl.batch_solve_tss(pb, tol, nsl.F64, nstreams=ns)          # warm-up: plans and buffers
+        for fm, _, phi in probs:
+            fm.populate(phi)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, rep = nsl.batch_solve_tss(pb, tol, nsl.F64, nstreams=ns)
+        dt = time.perf_counter() - t0
+        out[name] = {"solves_per_s": round(nsolves / dt, 1), "seconds": round(dt, 3),
+                     "converged_items": rep["converged_items"], "max_rel_residual": rep["max_rel_residual"],
+                     "max_iterations": rep["iterations"], "preconditioner_applies": rep["replays"]}
+    out["streams_speedup"] = round(out["tss_streams8"]["solves_per_s"] / out["tss_one_stream"]["solves_per_s"], 2)
+    if krylov:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, rep = nsl.batch_solve(pb, tol=tol, precision=nsl.F32, maxIter=krylov_maxit, use_graph=True)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out["krylov_pool_f32"] = {"solves_per_s": round(nsolves / dt, 1), "seconds": round(dt, 3),
+                                  "converged_items": rep["converged_items"], "iterations": rep["iterations"],
+                                  "max_rel_residual": rep["max_rel_residual"], "maxit": krylov_maxit,
+                                  "note": "solves/s counts every RHS, including those of the items the "
+                                          "unpreconditioned CG left unconverged at maxit"}
+        out["tss_over_krylov"] = round(out["tss_streams8"]["solves_per_s"] / out["krylov_pool_f32"]["solves_per_s"], 1)
+    return out
+
+
+SERVE_RATE = 10.0      # requests / s (Poisson), about 60 % of what the server sustains on this stream
 SERVE_N = 32           # requests per serving run
 
 
-def serve_requests(seed=9000):
-    """The serving mix: honeycomb flakes (Nx 18, 72) and (6,6) armchair
-    nanotubes of 2-4 cells (Nx 48-96), Nt in {16, 32}, beta in [2, 6],
-    U in [1, 4], a Philox-seeded phi ~ N(0, sqrt(U beta / Nt)) and Nx point
-    sources at t = 0 each."""
+def serve_requests(seed=9000, n=SERVE_N, rate=SERVE_RATE, subset=None):
+    """The serving stream: n requests drawn like the batching mix
+    (batch_mix_specs at seed: honeycomb Nx 18-1152, (6,6) nanotubes Nx
+    96-768, Nt 16-128, beta 2-12, U 1-4), each a Philox-seeded phi ~
+    N(0, sqrt(U beta / Nt)) asking for the two-point correlator with Nx
+    point sources at t = 0; Poisson arrivals at rate.  subset(Nx, Nt)
+    keeps only the requests it accepts."""
     import nsl_amd as nsl
-    u = nsl.rng_uniform(seed, 0, SERVE_N * 6).reshape(SERVE_N, 6)
-    lats = {}
-    out = []
-    for k in range(SERVE_N):
-        kind = int(u[k, 0] * 3)
-        if kind == 0:
-            key = ("h", 3)
-        elif kind == 1:
-            key = ("h", 6)
-        else:
-            key = ("n", 2 + int(u[k, 1] * 3))
-        if key not in lats:
-            lats[key] = (nsl.Lattice.honeycomb(key[1], key[1]) if key[0] == "h"
-                         else nsl.Lattice.armchair_nanotube(6, key[1]))
-        lat = lats[key]
-        Nt = 16 if u[k, 2] < 0.5 else 32
-        beta = 2.0 + 4.0 * u[k, 3]
-        Uk = 1.0 + 3.0 * u[k, 4]
-        phi = nsl.real_field(seed + 1 + k, (Nt, lat.sites()), math.sqrt(beta / Nt * Uk))
-        out.append((lat, Nt, beta, phi))
-    arrivals = -np.log(1.0 - nsl.rng_uniform(seed + 999, 0, SERVE_N)) / SERVE_RATE
+    specs = batch_mix_specs(n, seed)
+    lats, out = {}, []
+    for sp in specs:
+        lat = mix_lattice(sp["lattice"], lats)
+        if subset and not subset(lat.sites(), sp["Nt"]):
+            continue
+        phi = nsl.real_field(sp["field_seed"], (sp["Nt"], lat.sites()), math.sqrt(sp["beta"] / sp["Nt"] * sp["U"]))
+        out.append((lat, sp["Nt"], sp["beta"], phi))
+    arrivals = -np.log(1.0 - nsl.rng_uniform(seed + 999, 0, len(out))) / rate
     return out, arrivals
 
 
-def serving_leg(prec, tol=1e-10):
-    """One Poisson request stream through the correlator server; returns
-    latency percentiles and sustained solves/s (RHS solved / wall time)."""
+def serving_leg(prec, tol=1e-10, workers=0, rate=SERVE_RATE, subset=None, maxit=100000):
+    """One Poisson request stream through the correlator server: workers > 0
+    serves with the time-slice-preconditioned solver on that many streams
+    (prec = the preconditioner's precision), workers = 0 with the Krylov
+    pool (prec = the CG recurrence's).  Returns latency percentiles and
+    sustained solves/s (RHS solved / wall time from the first submit)."""
     import nsl_amd as nsl
-    reqs, gaps = serve_requests()
-    srv = nsl.Server()
+    reqs, gaps = serve_requests(rate=rate, subset=subset)
+    srv = nsl.Server(hbm_budget=200e9, precond_workers=workers)
     try:
-        warm = srv.submit(reqs[0][0], reqs[0][1], reqs[0][2], reqs[0][3], tol=tol, precision=prec)
+        warm = srv.submit(reqs[0][0], reqs[0][1], reqs[0][2], reqs[0][3], tol=tol, precision=prec, maxIter=maxit)
         srv.wait(warm)
         t0 = time.perf_counter()
         ids = []
         for (lat, Nt, beta, phi), gap in zip(reqs, gaps):
             time.sleep(float(gap))
-            ids.append(srv.submit(lat, Nt, beta, phi, tol=tol, precision=prec))
+            ids.append(srv.submit(lat, Nt, beta, phi, tol=tol, precision=prec, maxIter=maxit))
         res = [srv.wait(i)[1] for i in ids]
         wall = time.perf_counter() - t0
     finally:
@@ -324,8 +360,9 @@ def serving_leg(prec, tol=1e-10):
     lat_ms = np.array([r["latency_ms"] for r in res])
     rhs = sum(lat.sites() for lat, _, _, _ in reqs)
     return {"p50_ms": round(float(np.percentile(lat_ms, 50)), 3), "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
-            "solves_per_s": round(rhs / wall, 1), "requests": len(res), "rate_per_s": SERVE_RATE,
-            "ok": sum(int(r["status"] == 0) for r in res), "escalated": sum(r["escalated"] for r in res),
+            "solves_per_s": round(rhs / wall, 1), "requests": len(res), "rhs": rhs, "rate_per_s": rate,
+            "workers": workers, "ok": sum(int(r["status"] == 0) for r in res),
+            "escalated": sum(r["escalated"] for r in res), "max_iterations": max(r["iterations"] for r in res),
             "max_rel_residual": max(r["max_rel_residual"] for r in res)}
 
 
@@ -554,18 +591,26 @@ def main():
     if not args.no_refcrit:
         out["refcrit"] = refcrit_leg(phi_h, b_h, args.refcrit_fields)
     if not args.no_batch:
-        out["batching"] = batching_leg(nsl.F32)
+        out["batching"] = batching_leg()
     if not args.no_serve:
-        sv_mix = serving_leg(nsl.BF16X3)
-        sv_64 = serving_leg(nsl.F64)
-        best = sv_64 if sv_64["p99_ms"] <= sv_mix["p99_ms"] else sv_mix
-        out["p50_ms"] = best["p50_ms"]
-        out["p99_ms"] = best["p99_ms"]
-        out["serving"] = {"mixed": sv_mix, "fp64": sv_64,
-                          "reported": "fp64" if best is sv_64 else "mixed",
-                          "workload": f"{SERVE_N} correlator requests, Poisson {SERVE_RATE}/s, honeycomb Nx 18/72 "
-                                      "and (6,6) nanotubes Nx 48-96, Nt 16/32, beta 2-6, U 1-4, Nx point "
-                                      "sources each, per-RHS 1e-10"}
+        # the full stream through the time-slice-preconditioned server (FP64
+        # and FP32 preconditioner); the plain FP64 path (unpreconditioned CG
+        # in the Krylov pool) only on the small requests it finishes in
+        # bounded time, beside the preconditioned server on the same subset
+        small = lambda nx, nt: nx <= 96 and nt <= 32
+        sv_64 = serving_leg(nsl.F64, workers=4)
+        sv_32 = serving_leg(nsl.F32, workers=4)
+        sub_tss = serving_leg(nsl.F64, workers=4, subset=small)
+        sub_k64 = serving_leg(nsl.F64, workers=0, subset=small)
+        out["p50_ms"] = sv_64["p50_ms"]
+        out["p99_ms"] = sv_64["p99_ms"]
+        out["serving"] = {"tss_f64": sv_64, "tss_f32": sv_32, "reported": "tss_f64",
+                          "subset_nx_le_96_nt_le_32": {"tss_f64": sub_tss, "plain_fp64_krylov": sub_k64,
+                                                       "solves_per_s_ratio": round(sub_tss["solves_per_s"] /
+                                                                                   sub_k64["solves_per_s"], 2)},
+                          "workload": f"{SERVE_N} correlator requests drawn like the batching mix (honeycomb Nx "
+                                      "18-1152, (6,6) nanotubes Nx 96-768, Nt 16-128, beta 2-12, U 1-4), Poisson "
+                                      f"{SERVE_RATE}/s, Nx point sources each, per-RHS 1e-10, 4 worker streams"}
     else:
         out["p50_ms"] = None
         out["p99_ms"] = None

@@ -334,6 +334,26 @@ typedef struct {
 int nslx_batch_solve(nslx_ctx ctx, int64_t nitems, const nslx_batch_item* items, double tol, int precision,
                      int64_t maxit, int use_graph, nslx_batch_report* report, void* stream);
 
+/* Time-slice-preconditioned batch (no counterpart in the reference, whose
+ * systems are solved one CG at a time, CG.tpp:14-113): every item's MMdagger
+ * x = b is solved as nslx_solve_tss does (CG preconditioned by the item's
+ * own time-slice solve in precond_precision, double-double correction to
+ * |b - A (x + x_lo)| <= tol |b| per RHS; x receives the FP64 rounding,
+ * x_lo[i] (NULL, or an array of nitems device c128 buffers shaped like x,
+ * entries may be NULL) the remainder).
+ * Items (HubbardExp, each fermion matrix at most once) are taken largest
+ * first by nstreams host threads (<= 0: 8), each driving its own stream, so
+ * the items' factorisations and applies overlap on the device; with more
+ * than one stream the in-chunk recurrences run one launch per slice.  An
+ * item's result does not depend on the others.  report: iterations = max
+ * inner iterations over the items, replays = preconditioner applications,
+ * restarts = double-double residual evaluations, converged_items,
+ * max_rel_residual (double-double), seconds.  Synchronous (the device is
+ * synchronised at entry). */
+int nslx_batch_solve_tss(nslx_ctx ctx, int64_t nitems, const nslx_batch_item* items, void* const* x_lo,
+                         double tol, int precond_precision, int64_t maxit, int nstreams,
+                         nslx_batch_report* report);
+
 /* ---- measurements ----------------------------------------------------------------
  * TwoPointCorrelator::measure(N)
  * (src/NSL/Measurements/Hubbard/twoPointCorrelationFunction.tpp:116-162) on
@@ -417,6 +437,17 @@ typedef struct {
     double  hbm_budget;         /* bytes of device memory for admitted work     */
     int     use_graph;          /* replay the iteration loop as a HIP graph     */
     int     iters_per_replay;   /* even; <= 0: 32                               */
+    /* > 0: time-slice-preconditioned serving (no counterpart in the
+     * reference) with that many worker streams instead of the Krylov pool.
+     * Each admitted request is solved as nslx_solve_tss does (CG on MMdagger
+     * preconditioned by its own time-slice solve, double-double correction
+     * to tol per RHS; request.precision = the preconditioner's, F64 or F32;
+     * HubbardExp only), on the first free worker whose stream then runs its
+     * factorisation, solve and correlator beside the other workers'
+     * requests; admission is FIFO while the footprint (factorisation
+     * included) fits hbm_budget.  iterations in the result = inner
+     * preconditioned iterations; use_graph and iters_per_replay unused. */
+    int     precond_workers;
 } nslx_server_config;
 
 typedef struct {

@@ -469,15 +469,57 @@ def batch_solve(problems, tol=1e-12, precision=F64, maxIter=100000, use_graph=Tr
     return xs, d
 
 
+def batch_solve_tss(problems, tol=1e-12, precision=F64, maxIter=2000, nstreams=8, ctx=None, return_lo=False):
+    """Solve MMdagger x = b for many HubbardExp matrices of different Nx, Nt,
+    beta and fields, each by CG preconditioned with its own time-slice solve
+    and double-double correction (nslx_batch_solve_tss; no counterpart in the
+    reference), the items spread over nstreams worker streams.  problems: a
+    list of (fm, b), every fm at most once, b (nbatch, Nt, Nx) complex128 on
+    the device.  precision: the preconditioner's (F64 or F32).  Returns (list
+    of x, report dict), or (list of x, list of x_lo, report) with return_lo
+    (x + x_lo is the double-double solution)."""
+    torch = _torch()
+    if not problems:
+        raise ValueError("batch_solve_tss: no problems")
+    ctx = ctx or problems[0][0].ctx
+    items = (_lib.BatchItem * len(problems))()
+    keep, xs = [], []
+    for i, (fm, b) in enumerate(problems):
+        b = b.to(dtype=torch.complex128).contiguous()
+        nb = int(np.prod(b.shape[:-2])) if b.dim() > 2 else 1
+        x = torch.empty_like(b)
+        keep.append(b)
+        xs.append(x)
+        items[i].fm = fm._h
+        items[i].b = _ptr(b)
+        items[i].x = _ptr(x)
+        items[i].nbatch = nb
+    los, lo_arr = None, None
+    if return_lo:
+        los = [torch.empty_like(x) for x in xs]
+        lo_arr = (ctypes.c_void_p * len(problems))(*[_ptr(t) for t in los])
+    rep = _lib.BatchReport()
+    st = _lib.lib().nslx_batch_solve_tss(ctx._h, len(problems), items, lo_arr, float(tol), int(precision),
+                                         int(maxIter), int(nstreams), ctypes.byref(rep))
+    _lib.check(st, allow=(_lib.OK, _lib.NOT_CONVERGED))
+    d = rep.as_dict()
+    d["converged"] = st == _lib.OK
+    return (xs, los, d) if return_lo else (xs, d)
+
+
 class Server:
     """Correlator server with continuous batching (nslx_server_*): submit
     requests (lattice, Nt, beta, mu, species, phi, time sources, tol,
     precision), wait for their two-point correlators; admission by device
     footprint, requests share every Krylov iteration with what is running."""
 
-    def __init__(self, ctx=None, max_rhs=0, hbm_budget=0.0, use_graph=True, iters_per_replay=32):
+    def __init__(self, ctx=None, max_rhs=0, hbm_budget=0.0, use_graph=True, iters_per_replay=32, precond_workers=0):
+        """precond_workers > 0: serve with the time-slice-preconditioned
+        solver on that many worker streams (a request's precision is then
+        the preconditioner's, F64 or F32) instead of the Krylov pool."""
         self.ctx = ctx or Context.default()
-        cfg = _lib.ServerConfig(int(max_rhs), float(hbm_budget), int(bool(use_graph)), int(iters_per_replay))
+        cfg = _lib.ServerConfig(int(max_rhs), float(hbm_budget), int(bool(use_graph)), int(iters_per_replay),
+                                int(precond_workers))
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().nslx_server_create(self.ctx._h, ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

@@ -46,7 +46,8 @@ class BatchReport(ctypes.Structure):
 
 
 class ServerConfig(ctypes.Structure):
-    _fields_ = [("max_rhs", c_int64), ("hbm_budget", c_double), ("use_graph", c_int), ("iters_per_replay", c_int)]
+    _fields_ = [("max_rhs", c_int64), ("hbm_budget", c_double), ("use_graph", c_int), ("iters_per_replay", c_int),
+                ("precond_workers", c_int)]
 
 
 class Request(ctypes.Structure):
@@ -128,6 +129,8 @@ PROTOS = {
     "nslx_fm_export": (c_int, [c_void_p, P_double, c_void_p, c_void_p]),
     "nslx_batch_solve": (c_int, [c_void_p, c_int64, ctypes.POINTER(BatchItem), c_double, c_int, c_int64, c_int,
                                  ctypes.POINTER(BatchReport), c_void_p]),
+    "nslx_batch_solve_tss": (c_int, [c_void_p, c_int64, ctypes.POINTER(BatchItem), ctypes.POINTER(c_void_p),
+                                     c_double, c_int, c_int64, c_int, ctypes.POINTER(BatchReport)]),
     "nslx_server_create": (c_int, [c_void_p, ctypes.POINTER(ServerConfig), ctypes.POINTER(c_void_p)]),
     "nslx_server_destroy": (c_int, [c_void_p]),
     "nslx_server_submit": (c_int, [c_void_p, ctypes.POINTER(Request), ctypes.POINTER(c_int64)]),

@@ -326,7 +326,7 @@ void BPlan::gemm(const std::vector<BJob>& jobs, BVariant v, int tile) {
     uploaded_ = false;
 }
 
-void BPlan::chain(const std::vector<std::vector<BJob>>& steps, BVariant v) {
+void BPlan::chain(const std::vector<std::vector<BJob>>& steps, BVariant v, bool allow) {
     if (steps.empty()) return;
     const size_t nch = steps[0].size();
     NSLX_REQUIRE(nch >= 1, "chain plan: no jobs in a step");
@@ -348,7 +348,7 @@ void BPlan::chain(const std::vector<std::vector<BJob>>& steps, BVariant v) {
     const char* env = std::getenv("NSLX_CHAINS");   // read per plan: tests compare both paths
     const bool enabled = !(env && env[0] == '0');
     v.tile = 1;
-    if (!enabled || v.cb || !chain_supported(v, (int)steps.size(), (int)nch, j0.rows, j0.cols, j0.K)) {
+    if (!allow || !enabled || v.cb || !chain_supported(v, (int)steps.size(), (int)nch, j0.rows, j0.cols, j0.K)) {
         for (const auto& st : steps) gemm(st, v);
         return;
     }

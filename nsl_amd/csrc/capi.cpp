@@ -326,6 +326,16 @@ int nslx_batch_solve(nslx_ctx ctx, int64_t nitems, const nslx_batch_item* items,
     });
 }
 
+int nslx_batch_solve_tss(nslx_ctx ctx, int64_t nitems, const nslx_batch_item* items, void* const* x_lo,
+                         double tol, int precond_precision, int64_t maxit, int nstreams,
+                         nslx_batch_report* report) {
+    return guard([&] {
+        NSLX_REQUIRE(ctx, "null context handle");
+        DeviceGuard dg(ctx->c.device);
+        return nslx::batch_solve_tss(ctx->c, nitems, items, x_lo, tol, precond_precision, maxit, nstreams, report);
+    });
+}
+
 int nslx_server_create(nslx_ctx ctx, const nslx_server_config* cfg, nslx_server* out) {
     return guard([&] {
         NSLX_REQUIRE(ctx && out, "server_create: null argument");

@@ -84,6 +84,10 @@ struct FermionMatrix {
     // the chunk count by tss_chunk: the direct solve's bound, or, for a
     // preconditioner (precond) in FP64, the wider one.
     std::unique_ptr<TimeSliceSolver> tss[2];
+    // whether the time-slice applies may use the persistent recurrence
+    // launch (which needs the device to itself); off while several streams
+    // drive this engine at once (batched and served time-slice solves)
+    bool chains_ok = true;
     TimeSliceSolver& time_slice_solver(int prec, int nc, bool precond, hipStream_t s);
     // delta ||K|| + |delta mu| + max |Im phi|: the log of a slice's growth bound
     double slice_growth() const;
@@ -138,6 +142,11 @@ int solve_precond(FermionMatrix& fm, FermionMatrix& fp, int op, const void* b, v
 // RHS; x = high part, xlo (device c128 or null) = low part (tss_pcg.hip)
 int solve_tss(FermionMatrix& fm, int op, const void* b, void* x, void* xlo, const void* x0, int64_t nb, double tol,
               int64_t maxit, int pprec, int nchunks, nslx_solve_report* rep, hipStream_t s);
+
+// Batched time-slice-preconditioned solves over nstreams worker streams
+// (tss_batch.cpp); report fields as nslx_batch_solve_tss documents.
+int batch_solve_tss(Ctx& ctx, int64_t nitems, const nslx_batch_item* items, void* const* x_lo, double tol,
+                    int pprec, int64_t maxit, int nstreams, nslx_batch_report* rep);
 
 // log det / force (logdet.cpp)
 void fm_logdet(FermionMatrix& fm, int64_t field, int nchunks, double* out2, hipStream_t s);

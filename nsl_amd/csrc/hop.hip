@@ -1167,13 +1167,13 @@ void launch_slice_k(const HopArgs& a, int R, hipStream_t s) {
     const int ldE = e_ld(a.nx);
     const size_t planeE = (size_t)e_rows(a.nx) * ldE;
     auto kern = hop_slice<PR, FN, W, OP, MODE>;
-    static bool attr = false;
-    if (!attr) {
+    static const bool attr = [&] {   // once per kernel, thread-safe
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                   150 * 1024);
         (void)hipGetLastError();
-        attr = true;
-    }
+        return true;
+    }();
+    (void)attr;
     launch_timed(kern, dim3(grid), dim3(64 * W), lds, s, a, nrt, nct, P, R, swz, kp, ldE, planeE);
 }
 

@@ -61,6 +61,30 @@ double footprint(const nslx_request& r) {
     const double corr = nt * nx * nx * 16.0;
     return 11.0 * vec + emem + phases + corr + (64 << 20);   // S, X, G, apply work, 4 recurrence, r64, t64, slack
 }
+
+// device bytes of a request served by the time-slice-preconditioned solver:
+// the factorisation (chunk products, Gauss-Jordan blocks, conjugate
+// transposes), the apply buffers, the solver's vectors, the request's own
+// S, X, G and correlator; g = the growth per slice that fixes the chunking
+double footprint_tss(const nslx_request& r, double g) {
+    const double n = r.nx, nt = r.nt, nn = n * n;
+    const int step = std::max<int64_t>(1, r.nt / std::max<int64_t>(1, r.n_tsrc));
+    const double nb = (double)((r.nt + step - 1) / step) * n;
+    const bool f64 = r.precision == NSLX_F64;
+    const double esz = f64 ? 16.0 : 8.0;
+    const int c = tss_chunk(r.nt, g, f64 ? TSS_BOUND_PRECOND_F64 : TSS_BOUND_DIRECT,
+                            f64 ? TSS_CAP_PRECOND : TSS_CAP_DIRECT);
+    const double nc = (double)(r.nt / c);
+    const double factor = esz * (2 * nc * nn + (nc - 1) * 10 * nn + 2 * nn + (nc - 1) * 4 * nn + (nc - 1) * nn +
+                                 std::max(0.0, nc - 2) * nn + 2 * nn + 16 * 8 * n);
+    const double vec = nb * nt * n;
+    const double apply = esz * (3 * vec + 3 * nb * nc * n + 5 * nb * n);
+    const double solver = 16.0 * 8 * vec;                 // r, z, p, t, d, v, x_lo, spare
+    const double request = 16.0 * 3 * vec + 16.0 * nt * nn;
+    const double ld = (double)e_ld(r.nx), rows = (double)e_rows(r.nx);
+    const double emem = 3 * nn * 16.0 + rows * ld * 28.0;  // dense, transposed, padded copies
+    return factor + apply + solver + request + emem + nt * n * 40.0 + (64 << 20);
+}
 }  // namespace
 
 Server::Server(Ctx* ctx, const nslx_server_config& cfg) : ctx_(ctx), cfg_(cfg) {
@@ -69,6 +93,10 @@ Server::Server(Ctx* ctx, const nslx_server_config& cfg) : ctx_(ctx), cfg_(cfg) {
     if (!(cfg_.hbm_budget > 0)) cfg_.hbm_budget = 64.0 * (1ull << 30);
     NSLX_HIP(hipSetDevice(ctx_->device));
     NSLX_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+    if (cfg_.precond_workers > 0) {
+        for (int w = 0; w < cfg_.precond_workers; ++w) workers_.emplace_back([this] { run_tss(); });
+        return;
+    }
     pool_ = std::make_unique<Pool>(ctx_, s_, cfg_.max_rhs, cfg_.use_graph, cfg_.iters_per_replay);
     th_ = std::thread([this] { run(); });
 }
@@ -80,6 +108,8 @@ Server::~Server() {
     }
     cv_.notify_all();
     if (th_.joinable()) th_.join();
+    for (auto& t : workers_)
+        if (t.joinable()) t.join();
     pool_.reset();
     (void)hipStreamDestroy(s_);
 }
@@ -105,8 +135,23 @@ int64_t Server::submit(const nslx_request& r) {
     q->step = (int)std::max<int64_t>(1, r.nt / r.n_tsrc);
     q->nsrc = (r.nt + q->step - 1) / q->step;
     q->nb = (int64_t)q->nsrc * r.nx;
-    q->bytes = footprint(r);
-    NSLX_REQUIRE(q->nb <= cfg_.max_rhs, "server_submit: request needs more RHS slots than the server has");
+    if (cfg_.precond_workers > 0) {
+        NSLX_REQUIRE(r.precision == NSLX_F64 || r.precision == NSLX_F32,
+                     "server_submit: the preconditioned server takes precision F64 or F32 (the preconditioner's)");
+        NSLX_REQUIRE(r.discretisation == NSLX_DISC_EXP, "server_submit: the preconditioned server serves HubbardExp");
+        double normK = 0.0, imphi = 0.0;
+        for (int i = 0; i < r.nx; ++i) {
+            double rs = 0.0;
+            for (int j = 0; j < r.nx; ++j) rs += std::fabs(q->K[(size_t)i * r.nx + j]);
+            normK = std::max(normK, rs);
+        }
+        for (const double2& z : q->phi) imphi = std::max(imphi, std::fabs(z.y));
+        const double delta = r.beta / r.nt;
+        q->bytes = footprint_tss(r, std::fabs(delta) * normK + std::fabs(delta * r.mu) + imphi);
+    } else {
+        q->bytes = footprint(r);
+        NSLX_REQUIRE(q->nb <= cfg_.max_rhs, "server_submit: request needs more RHS slots than the server has");
+    }
     NSLX_REQUIRE(q->bytes <= cfg_.hbm_budget, "server_submit: request exceeds the server's memory budget");
     q->t_submit = Clock::now();
     int64_t id;
@@ -151,20 +196,24 @@ void Server::admit(Request& q) {
     pool_->admit(std::move(j));
 }
 
-void Server::complete(Request& q, PoolJob& j) {
+void Server::correlate(Request& q, hipStream_t s) {
     const nslx_request& r = q.r;
     const size_t n = (size_t)q.nb * r.nt * r.nx;
     q.G.alloc(n);
-    q.fm->apply(NSLX_OP_MDAGGER, q.X.p, q.G.p, q.nb, NSLX_F64, s_);
+    q.fm->apply(NSLX_OP_MDAGGER, q.X.p, q.G.p, q.nb, NSLX_F64, s);
     q.corr.alloc((size_t)r.nt * r.nx * r.nx);
-    corr_zero(q.corr.p, r.nt, r.nx, s_);
-    corr_accumulate(q.corr.p, q.G.p, q.nsrc, 0, q.step, r.nt, r.nx, s_);
-    corr_scale(q.corr.p, r.nt, r.nx, 1.0 / (double)r.n_tsrc, s_);
-    NSLX_HIP(hipStreamSynchronize(s_));
+    corr_zero(q.corr.p, r.nt, r.nx, s);
+    corr_accumulate(q.corr.p, q.G.p, q.nsrc, 0, q.step, r.nt, r.nx, s);
+    corr_scale(q.corr.p, r.nt, r.nx, 1.0 / (double)r.n_tsrc, s);
+    NSLX_HIP(hipStreamSynchronize(s));
     q.S.reset();
     q.X.reset();
     q.G.reset();
     q.fm.reset();
+}
+
+void Server::complete(Request& q, PoolJob& j) {
+    correlate(q, s_);
     q.res.status = j.status;
     q.res.iterations = j.iters;
     q.res.escalated = j.escalated ? 1 : 0;
@@ -273,6 +322,108 @@ void Server::run() {
         }
         busy_ += std::chrono::duration<double>(Clock::now() - t0).count();
     }
+}
+
+// One request on the time-slice-preconditioned path, on the worker's stream.
+void Server::solve_tss_request(Request& q, hipStream_t s) {
+    const nslx_request& r = q.r;
+    q.fm = std::make_unique<FermionMatrix>();
+    fm_init(*q.fm, ctx_, r.nx, r.nt, q.K.data(), r.beta, r.mu, r.species, DISC_EXP);
+    // the persistent recurrence launch needs the device to itself
+    q.fm->chains_ok = cfg_.precond_workers == 1;
+    DevBuf<double2> phi((size_t)r.nt * r.nx);
+    NSLX_HIP(hipMemcpyAsync(phi.p, q.phi.data(), phi.n * sizeof(double2), hipMemcpyHostToDevice, s));
+    q.fm->populate(phi.p, 1, r.species, s);
+    const size_t n = (size_t)q.nb * r.nt * r.nx;
+    q.S.alloc(n);
+    q.X.alloc(n);
+    corr_sources(q.S.p, q.nsrc, 0, q.step, r.nt, r.nx, s);
+    nslx_solve_report rep{};
+    const int st = solve_tss(*q.fm, NSLX_OP_MMDAGGER, q.S.p, q.X.p, nullptr, nullptr, q.nb, r.tol, r.maxit,
+                             r.precision, 0, &rep, s);
+    correlate(q, s);
+    q.res.status = st;
+    q.res.iterations = rep.iterations;
+    q.res.escalated = 0;
+    q.res.max_rel_residual = rep.max_rel_residual;
+    q.t_done = Clock::now();
+    q.res.queue_ms = ms_between(q.t_submit, q.t_admit);
+    q.res.latency_ms = ms_between(q.t_submit, q.t_done);
+}
+
+void Server::run_tss() {
+    hipStream_t s = nullptr;
+    if (hipSetDevice(ctx_->device) != hipSuccess || hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+        fail_all("server: could not create a worker stream");
+        return;
+    }
+    for (;;) {
+        std::shared_ptr<Request> q;
+        {
+            std::unique_lock<std::mutex> g(mu_);
+            // FIFO: the front request when its footprint fits beside the
+            // running ones (a request over the whole budget is refused at submit)
+            cv_.wait(g, [&] {
+                return stop_ || !fatal_.empty() ||
+                       (!queue_.empty() && used_bytes_ + queue_.front()->bytes <= cfg_.hbm_budget);
+            });
+            if (stop_ || !fatal_.empty()) {
+                for (auto& d : queue_) {   // drop what was never admitted
+                    d->done = true;
+                    d->error = stop_ ? "server stopped" : fatal_;
+                    d->res.status = NSLX_E_STATE;
+                }
+                queue_.clear();
+                done_cv_.notify_all();
+                break;
+            }
+            q = queue_.front();
+            queue_.pop_front();
+            used_bytes_ += q->bytes;
+            bytes_peak_ = std::max(bytes_peak_, used_bytes_);
+            q->t_admit = Clock::now();
+            running_[q->id] = q;
+            admitted_peak_ = std::max<int64_t>(admitted_peak_, (int64_t)running_.size());
+        }
+        const auto t0 = Clock::now();
+        std::string err;
+        int err_status = NSLX_OK;
+        try {
+            solve_tss_request(*q, s);
+        } catch (const Error& e) {
+            err = e.what();
+            err_status = e.status;
+        } catch (const std::exception& e) {
+            err = e.what();
+            err_status = NSLX_E_INTERNAL;
+        }
+        if (!err.empty()) {
+            q->S.reset();
+            q->X.reset();
+            q->G.reset();
+            q->fm.reset();
+        }
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            running_.erase(q->id);
+            used_bytes_ -= q->bytes;
+            q->done = true;
+            if (!err.empty()) {
+                q->error = err;
+                q->res.status = err_status;
+            } else {
+                lat_.push_back(q->res.latency_ms);
+                rhs_solved_ += q->nb;
+                ++completed_;
+                iterations_ += q->res.iterations;
+            }
+            busy_ += std::chrono::duration<double>(Clock::now() - t0).count();
+            done_cv_.notify_all();
+        }
+        cv_.notify_all();   // memory freed: the next request may fit
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
 }
 
 int Server::wait(int64_t id, double timeout_ms, void* corr, nslx_result* res) {

@@ -29,7 +29,12 @@ class Server {
     void run();
     void admit(Request& q);
     void complete(Request& q, PoolJob& j);
+    // G = M^dag X, the correlator, and the request's buffers released
+    void correlate(Request& q, hipStream_t s);
     void fail_all(const std::string& msg);
+    // time-slice-preconditioned mode: one worker per stream
+    void run_tss();
+    void solve_tss_request(Request& q, hipStream_t s);
 
     Ctx* ctx_;
     nslx_server_config cfg_;
@@ -47,6 +52,7 @@ class Server {
     std::string fatal_;
     bool stop_ = false;
     std::thread th_;
+    std::vector<std::thread> workers_;
 };
 
 }  // namespace nslx

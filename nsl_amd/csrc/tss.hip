@@ -753,6 +753,10 @@ void TimeSliceSolver::build_setup(hipStream_t s) {
 }
 
 TimeSliceSolver::ApplyBufs& TimeSliceSolver::bufs(int64_t nb, hipStream_t s) {
+    if (plans_chains_ != fm_.chains_ok) {   // rebuilt for the other launch form
+        plans_.clear();
+        plans_chains_ = fm_.chains_ok;
+    }
     auto it = plans_.find(nb);
     if (it != plans_.end()) return *it->second;
     NSLX_REQUIRE(nb >= 1 && nb % nf_ == 0, "time-slice solve: nbatch must be a positive multiple of the fields");
@@ -811,7 +815,7 @@ void TimeSliceSolver::build_minv(BPlan& p, const void* y, void* x, ApplyBufs& B,
             }
             steps.push_back(std::move(jobs));
         }
-        p.chain(steps, BVariant{prec_, false, 1, 0});
+        p.chain(steps, BVariant{prec_, false, 1, 0}, fm_.chains_ok);
     }
     const char* wsrc = c == 1 ? Y : X;
     auto w_at = [&](int j, int f) { return wsrc + ((size_t)f * rpf * lv + (size_t)(j * c + c - 1) * n) * e; };
@@ -895,7 +899,7 @@ void TimeSliceSolver::build_minv(BPlan& p, const void* y, void* x, ApplyBufs& B,
             }
             steps.push_back(std::move(jobs));
         }
-        p.chain(steps, BVariant{prec_, false, 1, 0});
+        p.chain(steps, BVariant{prec_, false, 1, 0}, fm_.chains_ok);
     }
 }
 
@@ -940,7 +944,7 @@ void TimeSliceSolver::build_mdinv(BPlan& p, const void* y, void* x, ApplyBufs& B
             }
             steps.push_back(std::move(jobs));
         }
-        p.chain(steps, BVariant{prec_, false, 2, 0});
+        p.chain(steps, BVariant{prec_, false, 2, 0}, fm_.chains_ok);
     }
     const char* wsrc = c == 1 ? Y : X;
     // what_k = w'_{k+1}, at slot f_{(k+1) mod nc}
@@ -1056,7 +1060,7 @@ void TimeSliceSolver::build_mdinv(BPlan& p, const void* y, void* x, ApplyBufs& B
             }
             steps.push_back(std::move(jobs));
         }
-        p.chain(steps, BVariant{prec_, false, 2, 0});
+        p.chain(steps, BVariant{prec_, false, 2, 0}, fm_.chains_ok);
     }
 }
 

@@ -105,9 +105,9 @@ class BPlan {
     void gemm(const std::vector<BJob>& jobs, BVariant v, int tile = -1);
     // append a recurrence: steps[s][i] = step s of chain i, where step s + 1
     // reads as A the rows step s wrote as Y (A0 == the previous Y, same
-    // stride).  One persistent launch when chain_supported() (and
-    // NSLX_CHAINS is not 0), else one batched GEMM launch per step.
-    void chain(const std::vector<std::vector<BJob>>& steps, BVariant v);
+    // stride).  One persistent launch when allowed, chain_supported() and
+    // NSLX_CHAINS is not 0, else one batched GEMM launch per step.
+    void chain(const std::vector<std::vector<BJob>>& steps, BVariant v, bool allow = true);
     // append any other launch
     void custom(std::function<void(hipStream_t)> fn);
     void upload(hipStream_t s);
@@ -189,6 +189,7 @@ class TimeSliceSolver {
     size_t setup_launches_ = 0;
     DevBuf<unsigned char> Tt_, Q_, Qfin_, PhiH_, VH_, WHcat_, DinvH_, ident_;
     std::map<int64_t, std::unique_ptr<ApplyBufs>> plans_;
+    bool plans_chains_ = true;    // the chains setting plans_ were built with
     BPlan setup_plan_;
     bool setup_built_ = false;
     const void* e_used_ = nullptr;

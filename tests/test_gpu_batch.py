@@ -116,3 +116,55 @@ def test_batch_large_lattice_grows_partials():
     assert rep["max_rel_residual"] <= 1e-8
     r = b[0] - ofs[0].MMdagger(xs[0][0].cpu())
     assert float(r.norm() / b[0].norm()) < 2e-8
+
+
+# ---- time-slice-preconditioned batch (nslx_batch_solve_tss) -----------------
+
+def test_batch_tss_full_mix():
+    """The batching form's full mix (bench.batch_mix_specs: 64 fields,
+    honeycomb Nx 18-1152, (6,6) nanotubes of 4-32 cells, Nt 16-128,
+    beta 2-12, U 1-4, 1-16 RHS) on 8 worker streams: every item converges,
+    every RHS to 1e-10 in double-double; the six smallest items' (x, x_lo)
+    are re-checked on the CPU in double-double from the exported operator."""
+    import bench
+    from tests import ddarith as dd
+    probs = bench.build_mix_problems(bench.batch_mix_specs(64, 7000))
+    xs, los, rep = nsl.batch_solve_tss([(fm, b) for fm, b, _ in probs], tol=1e-10, precision=nsl.F64, nstreams=8,
+                                       return_lo=True)
+    assert rep["converged"] and rep["converged_items"] == 64, rep
+    assert rep["max_rel_residual"] <= 1e-10, rep
+    assert rep["iterations"] <= 8, rep
+    small = sorted(range(64), key=lambda k: probs[k][0].Nx * probs[k][0].Nt)[:6]
+    for k in small:
+        fm, b, _ = probs[k]
+        E, P = fm.export()
+        _, relv = dd.mmdag_residual(E, P, b.cpu().numpy(), xs[k].cpu().numpy(), los[k].cpu().numpy(), rpf=b.shape[0])
+        assert relv.max() <= 1.05e-10, (k, relv.max())
+
+
+def test_batch_tss_item_independent_of_mix():
+    """An item's solution does not depend on the other items or on which
+    worker runs it: four items solved among the whole 16-item mix and alone
+    in a batch of four (both on several streams) are identical bit for bit."""
+    import bench
+    specs = bench.batch_mix_specs(16, 7100)
+    keep = [k for k in range(16) if specs[k]["lattice"][1] <= 12][:4]
+    full = bench.build_mix_problems(specs)
+    xs_full, rep = nsl.batch_solve_tss([(fm, b) for fm, b, _ in full], tol=1e-10, nstreams=6)
+    assert rep["converged"], rep
+    part = bench.build_mix_problems([specs[k] for k in keep])
+    xs_part, rep2 = nsl.batch_solve_tss([(fm, b) for fm, b, _ in part], tol=1e-10, nstreams=3)
+    assert rep2["converged"], rep2
+    for j, k in enumerate(keep):
+        assert torch.equal(xs_full[k], xs_part[j]), k
+
+
+def test_batch_tss_errors():
+    ctx = nsl.Context.default()
+    fm, _, b = make("honeycomb", (3, 3), 16, 4.0, 2.0, 1, 2, 100, ctx)
+    with pytest.raises(nsl.NslxError):      # the same fermion matrix twice
+        nsl.batch_solve_tss([(fm, b.to(DEV)), (fm, b.to(DEV))])
+    fd = nsl.HubbardDiag(nsl.Lattice.honeycomb(3, 3), 16, 4.0)
+    fd.populate(torch.zeros((16, 18), dtype=torch.complex128, device=DEV))
+    with pytest.raises(nsl.NslxError):      # HubbardExp only
+        nsl.batch_solve_tss([(fd, b.to(DEV))])

@@ -187,3 +187,89 @@ def test_server_admission_waits_for_contiguous_slots():
         alone.close()
     assert r1["status"] == 0
     assert torch.equal(c1, out[4][0])
+
+
+# ---- time-slice-preconditioned serving (precond_workers > 0) ----------------
+
+def test_tss_server_vs_oracle_cfg0():
+    """The preconditioned server on the configs[0] lattice (honeycomb 3x3,
+    Nt = 16, beta = 4, U = 2; 2 time sources; per-RHS 1e-12) against the
+    oracle's reference-mode correlator to 1e-9, with 3 worker streams."""
+    lat = nsl.Lattice.honeycomb(3, 3)
+    phi = field(lat, 16, 4.0, 2.0, 1000)
+    srv = nsl.Server(precond_workers=3)
+    try:
+        rid = srv.submit(lat, 16, 4.0, phi, n_tsrc=2, tol=1e-12)
+        corr, res = srv.wait(rid)
+    finally:
+        srv.close()
+    assert res["status"] == 0, res
+    assert res["max_rel_residual"] <= 1e-12
+    o = F.HubbardExp(olat.honeycomb(3, 3), 16, 4.0)
+    o.populate(phi)
+    ref, _ = ocorr.two_point(o, 16, 18, 2)
+    assert relerr(corr, ref) < 1e-9
+
+
+def _tss_stream():
+    """Six requests of the serving stream's kind: honeycomb Nx 18 / 72 and a
+    (6,6) nanotube of 4 cells, Nt 16-64, beta 2-12, U 1-4."""
+    shapes = [(nsl.Lattice.honeycomb(3, 3), 16, 11.5, 3.7), (nsl.Lattice.honeycomb(6, 6), 32, 6.0, 2.0),
+              (nsl.Lattice.armchair_nanotube(6, 4), 16, 3.0, 1.2), (nsl.Lattice.honeycomb(3, 3), 64, 8.0, 3.0),
+              (nsl.Lattice.honeycomb(6, 6), 16, 2.5, 4.0), (nsl.Lattice.armchair_nanotube(6, 4), 32, 12.0, 2.5)]
+    return [(lat, Nt, beta, field(lat, Nt, beta, U, 3100 + k)) for k, (lat, Nt, beta, U) in enumerate(shapes)]
+
+
+@pytest.mark.parametrize("prec", [nsl.F64, nsl.F32])
+def test_tss_server_alone_equals_in_mix(prec):
+    """Continuous batching on worker streams: six requests submitted at once
+    (their factorisations and solves overlap on the device) give the same
+    correlators bit for bit as each request alone on the same server, and
+    every request reaches its 1e-10 per-RHS target."""
+    reqs = _tss_stream()
+    srv = nsl.Server(precond_workers=4, hbm_budget=64e9)
+    try:
+        ids = [srv.submit(lat, Nt, beta, phi, tol=1e-10, precision=prec) for lat, Nt, beta, phi in reqs]
+        mix = [srv.wait(i) for i in ids]
+        alone = []
+        for lat, Nt, beta, phi in reqs:
+            alone.append(srv.wait(srv.submit(lat, Nt, beta, phi, tol=1e-10, precision=prec)))
+        st = srv.stats()
+    finally:
+        srv.close()
+    for (cm, rm), (ca, ra) in zip(mix, alone):
+        assert rm["status"] == 0 and ra["status"] == 0, (rm, ra)
+        assert rm["max_rel_residual"] <= 1e-10, rm
+        assert torch.equal(cm, ca)
+    assert st["completed"] == 2 * len(reqs)
+    assert st["admitted_peak"] >= 2          # the mix really ran side by side
+
+
+def test_tss_server_matches_krylov_server():
+    """The same request (honeycomb 6x6, Nt 32, beta 6, U 2) through the
+    preconditioned server and the Krylov pool, FP64, per-RHS 1e-12: the
+    correlators agree to 1e-8 (cond(M) < 1e4 here)."""
+    lat, Nt, beta, phi = _tss_stream()[1]
+    out = []
+    for workers in (2, 0):
+        srv = nsl.Server(precond_workers=workers)
+        try:
+            out.append(srv.wait(srv.submit(lat, Nt, beta, phi, tol=1e-12)))
+        finally:
+            srv.close()
+    (ct, rt), (ck, rk) = out
+    assert rt["status"] == 0 and rk["status"] == 0
+    assert relerr(ct, ck) < 1e-8
+
+
+def test_tss_server_errors():
+    lat = nsl.Lattice.honeycomb(3, 3)
+    phi = field(lat, 16, 4.0, 2.0, 1000)
+    srv = nsl.Server(precond_workers=2)
+    try:
+        with pytest.raises(nsl.NslxError):      # the preconditioner runs in F64 or F32 only
+            srv.submit(lat, 16, 4.0, phi, precision=nsl.BF16X3)
+        with pytest.raises(nsl.NslxError):      # HubbardExp only
+            srv.submit(lat, 16, 4.0, phi, fermion_matrix=nsl.HubbardDiag)
+    finally:
+        srv.close()
