@@ -116,9 +116,16 @@ int nslx_lattice_armchair_nanotube(int n, int cells, double kappa, double* K, do
  * E(s*delta) = exp(s*delta*K) is computed in FP64 on the device (Taylor with
  * scaling and squaring) and memoised per exponent, like
  * SpatialLattice::exp_hopping_matrix (lattice.tpp:51-58).
- * beta and mu are real here (the reference allows complex Type). */
-int nslx_fm_create(nslx_ctx ctx, int nx, int nt, const double* K, double beta, double mu,
-                   int species, nslx_fm* out);
+ * beta = beta_re + i beta_im and mu = mu_re + i mu_im stand for the
+ * reference's `const Type & beta, const Type & mu` with Type complex.  mu
+ * may be complex (it enters only the phases P = exp(i s phi + s delta mu),
+ * hubbardExp.hpp:116-128, so an imaginary chemical potential is a shift of
+ * the field).  beta must be real: beta_im != 0 returns NSLX_E_ARG (a complex
+ * delta makes exp(s delta K) complex and M^dag's E(s conj(delta)) differ
+ * from it, hubbardExp.tpp:62-65, which the engine's real-E kernels do not
+ * hold). */
+int nslx_fm_create(nslx_ctx ctx, int nx, int nt, const double* K, double beta_re, double beta_im,
+                   double mu_re, double mu_im, int species, nslx_fm* out);
 
 /* Replaces NSL::FermionMatrix::HubbardDiag<complex<double>, Lattice>, ctor
  * (species, lat, Nt, beta, mu): src/NSL/FermionMatrix/Impl/hubbardDiag.hpp.
@@ -132,9 +139,10 @@ int nslx_fm_create(nslx_ctx ctx, int nx, int nt, const double* K, double beta, d
  * (1 - s delta K) diag(Pinv_{t+1})), stable chunked form, no mu term like the
  * reference) and grad_logdet :193-252.  nslx_batch_solve takes HubbardDiag
  * handles too (mixed with HubbardExp ones), and a server request selects its
- * discretisation (nslx_request.discretisation). */
-int nslx_fm_create_diag(nslx_ctx ctx, int nx, int nt, const double* K, double beta, double mu,
-                        int species, nslx_fm* out);
+ * discretisation (nslx_request.discretisation).  beta and mu as in
+ * nslx_fm_create (complex mu, real beta). */
+int nslx_fm_create_diag(nslx_ctx ctx, int nx, int nt, const double* K, double beta_re, double beta_im,
+                        double mu_re, double mu_im, int species, nslx_fm* out);
 int nslx_fm_destroy(nslx_fm fm);
 
 /* populate(phi[, species]): hubbardExp.hpp:103-128.  phi: device c128,
@@ -250,7 +258,8 @@ int nslx_solve_precond(nslx_fm fm, nslx_fm fm_prec, int op, const void* b, void*
  * field by Gauss-Jordan elimination with partial pivoting (HubbardExp only).
  *
  * nslx_fm_solve_direct: out = M^{-1} in (op NSLX_OP_M), M^{-dag} in
- * (NSLX_OP_MDAGGER) or (M M^dag)^{-1} in (NSLX_OP_MMDAGGER) for nbatch RHS;
+ * (NSLX_OP_MDAGGER), (M M^dag)^{-1} in (NSLX_OP_MMDAGGER) or (M^dag M)^{-1}
+ * in (NSLX_OP_MDAGGERM) for nbatch RHS;
  * precision NSLX_F64 (c128 in/out) or NSLX_F32 (c64 in/out).  The first call
  * after a populate factorises (nchunks <= 0: chosen from the growth per slice).
  * Synchronous.  The in-chunk recurrences run as one persistent launch per
@@ -259,8 +268,9 @@ int nslx_solve_precond(nslx_fm fm, nslx_fm fm_prec, int op, const void* b, void*
 int nslx_fm_solve_direct(nslx_fm fm, int op, const void* in, void* out, int64_t nbatch, int precision,
                          int nchunks, void* stream);
 
-/* CG on MMdagger (the CG of CG.tpp:14-113 on MatrixCombination MMdagger,
- * Solver.hpp:86-96), preconditioned by the time-slice solve in
+/* CG on MMdagger or MdaggerM (op; the CG of CG.tpp:14-113 on
+ * MatrixCombination MMdagger / MdaggerM, Solver.hpp:86-96), preconditioned
+ * by the time-slice solve (of the same product) in
  * precond_precision (NSLX_F64 or NSLX_F32: the preconditioner's factorisation
  * and applies on FP32 matrix cores, the CG recurrence in FP64), wrapped in a
  * double-double defect correction: every RHS stops when
@@ -298,11 +308,13 @@ int nslx_fm_export(nslx_fm fm, double* E_host, void* P_host, void* stream);
 
 /* ---- batched ragged solve ----------------------------------------------------------
  * No single counterpart in the reference (it solves one HubbardExp at a
- * time, CG.tpp:14-113); this is the reference's CG on MMdagger for many
- * fermion matrices at once.  Each item is a populated fm (own lattice, Nt,
- * beta, fields; HubbardExp or HubbardDiag) with its own b and x (device c128,
- * nbatch x nt x nx); all fms must belong to ctx.  Every Krylov iteration is one
- * grouped persistent hop launch per phase over the tiles of all items (per
+ * time, CG.tpp:14-113); this is the reference's CG on MMdagger or MdaggerM
+ * (MatrixCombination, Solver.hpp:86-96) for many fermion matrices at once.
+ * Each item is a populated fm (own lattice, Nt, beta, fields; HubbardExp or
+ * HubbardDiag) with its own b and x (device c128, nbatch x nt x nx) and its
+ * operator (op: NSLX_OP_MMDAGGER, or NSLX_OP_MDAGGERM for HubbardExp items);
+ * all fms must belong to ctx.  Every Krylov iteration is one grouped
+ * persistent hop launch per phase over the tiles of all items (per operator /
  * discretisation / precision / tile class);
  * with use_graph the iteration loop is built once as a HIP graph and
  * replayed.  Per-RHS stop: |b - A x| <= tol |b| in FP64 or the FP64 attainable
@@ -317,6 +329,7 @@ typedef struct {
     const void* b;
     void*       x;
     int64_t     nbatch;
+    int         op;         /* NSLX_OP_MMDAGGER or NSLX_OP_MDAGGERM */
 } nslx_batch_item;
 
 typedef struct {
@@ -336,7 +349,7 @@ int nslx_batch_solve(nslx_ctx ctx, int64_t nitems, const nslx_batch_item* items,
 
 /* Time-slice-preconditioned batch (no counterpart in the reference, whose
  * systems are solved one CG at a time, CG.tpp:14-113): every item's MMdagger
- * x = b is solved as nslx_solve_tss does (CG preconditioned by the item's
+ * x = b (or MdaggerM x = b, item.op) is solved as nslx_solve_tss does (CG preconditioned by the item's
  * own time-slice solve in precond_precision, double-double correction to
  * |b - A (x + x_lo)| <= tol |b| per RHS; x receives the FP64 rounding,
  * x_lo[i] (NULL, or an array of nitems device c128 buffers shaped like x,

@@ -139,8 +139,9 @@ class Lattice:
 class HubbardExp:
     """NSL::FermionMatrix::HubbardExp on the device.
 
-    ctor (lat, Nt, beta, mu) / (species, lat, Nt, beta, mu): hubbardExp.hpp:48-76
-    (beta and mu real here).  populate(phi[, species]): hubbardExp.hpp:103-128;
+    ctor (lat, Nt, beta, mu) / (species, lat, Nt, beta, mu): hubbardExp.hpp:48-76;
+    mu may be complex, beta must be real (a complex beta raises NslxError).
+    populate(phi[, species]): hubbardExp.hpp:103-128;
     phi is a complex128 CUDA tensor (Nt, Nx), or (nfield, Nt, Nx) to hold
     several fields sharing the lattice (RHS k of a batch of nbatch uses field
     k // (nbatch // nfield)).
@@ -152,14 +153,15 @@ class HubbardExp:
         self.lat = lat
         self.Nt = int(Nt)
         self.Nx = lat.sites()
-        self.beta = float(beta)
-        self.mu = float(mu)
+        beta, mu = complex(beta), complex(mu)
+        self.beta = beta.real if beta.imag == 0 else beta
+        self.mu = mu.real if mu.imag == 0 else mu
         self._species = int(species)
         self.ctx = ctx or Context.default()
         h = ctypes.c_void_p()
         _lib.check(getattr(_lib.lib(), self._create)(self.ctx._h, self.Nx, self.Nt,
-                                                     lat.K.ctypes.data_as(_lib.P_double), self.beta, self.mu,
-                                                     self._species, ctypes.byref(h)))
+                                                     lat.K.ctypes.data_as(_lib.P_double), beta.real, beta.imag,
+                                                     mu.real, mu.imag, self._species, ctypes.byref(h)))
         self._h = h
         self.nfield = 0
         self._phi = None
@@ -246,7 +248,7 @@ class HubbardExp:
 
     # -- time-slice direct solve (no counterpart in the reference) ----------
     def solve_direct(self, op, y, out=None, precision=F64, nchunks=0):
-        """M^{-1} y (op M), M^{-dag} y (Mdagger) or (M M^dag)^{-1} y
+        """M^{-1} y (op M), M^{-dag} y (Mdagger), (M^dag M)^{-1} y (MdaggerM) or (M M^dag)^{-1} y
         (MMdagger) by the time-slice solve (nslx_fm_solve_direct); y is
         complex128 for F64, complex64 for F32."""
         torch = _torch()
@@ -385,17 +387,19 @@ class CGpreconditioned:
 
 
 class CGTimeSlice:
-    """CG on MMdagger preconditioned by the time-slice direct solve, in a
-    double-double defect correction to |b - MM^dag x| <= eps |b| per RHS
-    (nslx_solve_tss; no counterpart in the reference, whose CG on MMdagger
-    is CG.tpp:14-113).  precision: the preconditioner's (F32: factorisation
+    """CG on MMdagger (or, with matrixCombination=MdaggerM, on M^dag M)
+    preconditioned by the time-slice direct solve, in a double-double defect
+    correction to |b - A x| <= eps |b| per RHS (nslx_solve_tss; no
+    counterpart in the reference, whose CG on MMdagger is CG.tpp:14-113).  precision: the preconditioner's (F32: factorisation
     and applies on FP32 matrix cores, the CG recurrence in FP64; F64).
     After a call, .x_lo holds the low part of the double-double solution
     (x + x_lo) and .report the solver report.  nchunks 0: the chunk count
     nslx_solve_tss picks for the precision (wider chunks in F64)."""
 
-    def __init__(self, fm, eps=1e-12, maxIter=10000, precision=F32, nchunks=0):
+    def __init__(self, fm, eps=1e-12, maxIter=10000, precision=F32, nchunks=0,
+                 matrixCombination=MatrixCombination.MMdagger):
         self.fm = fm
+        self.op = int(matrixCombination)
         self.eps = float(eps)
         self.maxIter = int(maxIter)
         self.precision = int(precision)
@@ -413,7 +417,7 @@ class CGTimeSlice:
         if x0 is not None:
             x0 = x0.to(dtype=torch.complex128).contiguous()
         rep = SolveReport()
-        st = _lib.lib().nslx_solve_tss(self.fm._h, int(MatrixCombination.MMdagger), _ptr(b), _ptr(x), _ptr(lo),
+        st = _lib.lib().nslx_solve_tss(self.fm._h, self.op, _ptr(b), _ptr(x), _ptr(lo),
                                        _ptr(x0) if x0 is not None else None, nb, self.eps, self.maxIter,
                                        self.precision, self.nchunks, ctypes.byref(rep), _stream())
         _lib.check(st, allow=(_lib.OK, _lib.NOT_CONVERGED))
@@ -440,9 +444,10 @@ def dd_residual(fm, b, x, x_lo=None, op=MatrixCombination.MMdagger):
 
 
 def batch_solve(problems, tol=1e-12, precision=F64, maxIter=100000, use_graph=True, ctx=None):
-    """Solve MMdagger x = b for many fermion matrices of different Nx, Nt
-    and fields at once (nslx_batch_solve): problems is a list of (fm, b)
-    with b (nbatch, Nt, Nx) complex128 on the device.  Returns (list of x,
+    """Solve MMdagger x = b (or MdaggerM x = b) for many fermion matrices of
+    different Nx, Nt and fields at once (nslx_batch_solve): problems is a
+    list of (fm, b) or (fm, b, op) with b (nbatch, Nt, Nx) complex128 on the
+    device and op MMdagger (default) or MdaggerM.  Returns (list of x,
     report dict); raises on errors, reports non-convergence in the dict."""
     torch = _torch()
     if not problems:
@@ -450,7 +455,8 @@ def batch_solve(problems, tol=1e-12, precision=F64, maxIter=100000, use_graph=Tr
     ctx = ctx or problems[0][0].ctx
     items = (_lib.BatchItem * len(problems))()
     keep, xs = [], []
-    for i, (fm, b) in enumerate(problems):
+    for i, pr in enumerate(problems):
+        fm, b = pr[0], pr[1]
         b = b.to(dtype=torch.complex128).contiguous()
         nb = int(np.prod(b.shape[:-2])) if b.dim() > 2 else 1
         x = torch.empty_like(b)
@@ -460,6 +466,7 @@ def batch_solve(problems, tol=1e-12, precision=F64, maxIter=100000, use_graph=Tr
         items[i].b = _ptr(b)
         items[i].x = _ptr(x)
         items[i].nbatch = nb
+        items[i].op = int(pr[2]) if len(pr) > 2 else int(MatrixCombination.MMdagger)
     rep = _lib.BatchReport()
     st = _lib.lib().nslx_batch_solve(ctx._h, len(problems), items, float(tol), int(precision), int(maxIter),
                                      int(bool(use_graph)), ctypes.byref(rep), _stream())
@@ -470,12 +477,13 @@ def batch_solve(problems, tol=1e-12, precision=F64, maxIter=100000, use_graph=Tr
 
 
 def batch_solve_tss(problems, tol=1e-12, precision=F64, maxIter=2000, nstreams=8, ctx=None, return_lo=False):
-    """Solve MMdagger x = b for many HubbardExp matrices of different Nx, Nt,
-    beta and fields, each by CG preconditioned with its own time-slice solve
-    and double-double correction (nslx_batch_solve_tss; no counterpart in the
-    reference), the items spread over nstreams worker streams.  problems: a
-    list of (fm, b), every fm at most once, b (nbatch, Nt, Nx) complex128 on
-    the device.  precision: the preconditioner's (F64 or F32).  Returns (list
+    """Solve MMdagger x = b (or MdaggerM x = b) for many HubbardExp matrices
+    of different Nx, Nt, beta and fields, each by CG preconditioned with its
+    own time-slice solve and double-double correction (nslx_batch_solve_tss;
+    no counterpart in the reference), the items spread over nstreams worker
+    streams.  problems: a list of (fm, b) or (fm, b, op), every fm at most
+    once, b (nbatch, Nt, Nx) complex128 on the device, op MMdagger (default)
+    or MdaggerM.  precision: the preconditioner's (F64 or F32).  Returns (list
     of x, report dict), or (list of x, list of x_lo, report) with return_lo
     (x + x_lo is the double-double solution)."""
     torch = _torch()
@@ -484,7 +492,8 @@ def batch_solve_tss(problems, tol=1e-12, precision=F64, maxIter=2000, nstreams=8
     ctx = ctx or problems[0][0].ctx
     items = (_lib.BatchItem * len(problems))()
     keep, xs = [], []
-    for i, (fm, b) in enumerate(problems):
+    for i, pr in enumerate(problems):
+        fm, b = pr[0], pr[1]
         b = b.to(dtype=torch.complex128).contiguous()
         nb = int(np.prod(b.shape[:-2])) if b.dim() > 2 else 1
         x = torch.empty_like(b)
@@ -494,6 +503,7 @@ def batch_solve_tss(problems, tol=1e-12, precision=F64, maxIter=2000, nstreams=8
         items[i].b = _ptr(b)
         items[i].x = _ptr(x)
         items[i].nbatch = nb
+        items[i].op = int(pr[2]) if len(pr) > 2 else int(MatrixCombination.MMdagger)
     los, lo_arr = None, None
     if return_lo:
         los = [torch.empty_like(x) for x in xs]

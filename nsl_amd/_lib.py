@@ -33,7 +33,7 @@ class SolveReport(ctypes.Structure):
 
 
 class BatchItem(ctypes.Structure):
-    _fields_ = [("fm", c_void_p), ("b", c_void_p), ("x", c_void_p), ("nbatch", c_int64)]
+    _fields_ = [("fm", c_void_p), ("b", c_void_p), ("x", c_void_p), ("nbatch", c_int64), ("op", c_int)]
 
 
 class BatchReport(ctypes.Structure):
@@ -105,10 +105,10 @@ PROTOS = {
     "nslx_lattice_honeycomb": (c_int, [c_int, c_int, c_double, P_double, P_double]),
     "nslx_lattice_ring": (c_int, [c_int, c_double, P_double, P_double]),
     "nslx_lattice_armchair_nanotube": (c_int, [c_int, c_int, c_double, P_double, P_double]),
-    "nslx_fm_create": (c_int, [c_void_p, c_int, c_int, P_double, c_double, c_double, c_int,
-                               ctypes.POINTER(c_void_p)]),
-    "nslx_fm_create_diag": (c_int, [c_void_p, c_int, c_int, P_double, c_double, c_double, c_int,
-                                    ctypes.POINTER(c_void_p)]),
+    "nslx_fm_create": (c_int, [c_void_p, c_int, c_int, P_double, c_double, c_double, c_double, c_double,
+                               c_int, ctypes.POINTER(c_void_p)]),
+    "nslx_fm_create_diag": (c_int, [c_void_p, c_int, c_int, P_double, c_double, c_double, c_double, c_double,
+                                    c_int, ctypes.POINTER(c_void_p)]),
     "nslx_fm_destroy": (c_int, [c_void_p]),
     "nslx_fm_populate": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_void_p]),
     "nslx_fm_apply": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int64, c_int, c_void_p]),

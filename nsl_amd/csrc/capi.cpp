@@ -130,27 +130,41 @@ int nslx_lattice_armchair_nanotube(int n, int cells, double kappa, double* K, do
     return guard([&] { nslx::lattice_nanotube(n, cells, kappa, K, sites); return NSLX_OK; });
 }
 
-int nslx_fm_create(nslx_ctx ctx, int nx, int nt, const double* K, double beta, double mu, int species, nslx_fm* out) {
+namespace {
+// beta: real (a complex beta would make exp(s delta K) complex, which the
+// engine's real-E kernels do not hold); mu: complex
+void check_beta_mu(double beta_re, double beta_im, double mu_re, double mu_im) {
+    NSLX_REQUIRE(std::isfinite(beta_re) && std::isfinite(beta_im) && std::isfinite(mu_re) && std::isfinite(mu_im),
+                 "fm_create: beta and mu must be finite");
+    NSLX_REQUIRE(beta_im == 0.0, "fm_create: beta must be real (Im beta != 0 is not supported)");
+}
+}  // namespace
+
+int nslx_fm_create(nslx_ctx ctx, int nx, int nt, const double* K, double beta_re, double beta_im, double mu_re,
+                   double mu_im, int species, nslx_fm* out) {
     return guard([&] {
         NSLX_REQUIRE(ctx && out && K, "fm_create: null argument");
         NSLX_REQUIRE(nx >= 1 && nt >= 2, "fm_create: need nx >= 1 and nt >= 2");
         NSLX_REQUIRE(species == NSLX_PARTICLE || species == NSLX_HOLE, "fm_create: bad species");
-        NSLX_REQUIRE(std::isfinite(beta) && std::isfinite(mu), "fm_create: beta and mu must be finite");
+        check_beta_mu(beta_re, beta_im, mu_re, mu_im);
         DeviceGuard dg(ctx->c.device);
         auto f = std::make_unique<nslx_fm_s>();
-        nslx::fm_init(f->m, &ctx->c, nx, nt, K, beta, mu, species);
+        nslx::fm_init(f->m, &ctx->c, nx, nt, K, beta_re, mu_re, species);
+        f->m.mu_im = mu_im;
         *out = f.release();
         return NSLX_OK;
     });
 }
 
-int nslx_fm_create_diag(nslx_ctx ctx, int nx, int nt, const double* K, double beta, double mu, int species,
-                        nslx_fm* out) {
+int nslx_fm_create_diag(nslx_ctx ctx, int nx, int nt, const double* K, double beta_re, double beta_im,
+                        double mu_re, double mu_im, int species, nslx_fm* out) {
     return guard([&] {
         NSLX_REQUIRE(ctx && out && K, "fm_create_diag: null argument");
+        check_beta_mu(beta_re, beta_im, mu_re, mu_im);
         DeviceGuard dg(ctx->c.device);
         auto f = std::make_unique<nslx_fm_s>();
-        nslx::fm_init(f->m, &ctx->c, nx, nt, K, beta, mu, species, nslx::DISC_DIAG);
+        nslx::fm_init(f->m, &ctx->c, nx, nt, K, beta_re, mu_re, species, nslx::DISC_DIAG);
+        f->m.mu_im = mu_im;
         *out = f.release();
         return NSLX_OK;
     });

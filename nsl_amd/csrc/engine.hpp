@@ -2,6 +2,7 @@
 // (HubbardExp) with its memoised exponential hopping matrices, solvers.
 #pragma once
 
+#include <cmath>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -62,7 +63,7 @@ struct FermionMatrix {
     Ctx* ctx = nullptr;
     int disc = DISC_EXP;             // HubbardExp or HubbardDiag (hop.hpp)
     int nx = 0, nt = 0;
-    double beta = 1.0, mu = 0.0;
+    double beta = 1.0, mu = 0.0, mu_im = 0.0;   // mu = mu + i mu_im (complex chemical potential)
     int species = NSLX_PARTICLE;
     std::vector<double> K_host;
     double normK = 0.0;              // max row sum of |K|
@@ -95,6 +96,8 @@ struct FermionMatrix {
 
     int sgn() const { return species == NSLX_PARTICLE ? +1 : -1; }
     double delta() const { return beta / nt; }
+    // |delta mu|: the largest change of log |P| the chemical potential makes
+    double mut_abs() const { return std::hypot(delta() * mu, delta() * mu_im); }
     int64_t len() const { return (int64_t)nt * nx; }
 
     EEntry& E(double a, hipStream_t s);      // memoised exp(a K)

@@ -138,7 +138,7 @@ void FermionMatrix::populate(const void* phi_dev, int64_t nf, int sp, hipStream_
     ++pop_gen;
     // deep copy of phi (the reference's populate copies into its own buffer)
     NSLX_HIP(hipMemcpyAsync(phi.p, phi_dev, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
-    phase_tables(phi.p, n, sgn(), delta() * mu, P64.p, Pinv64.p, P32.p, s);
+    phase_tables(phi.p, n, sgn(), delta() * mu, delta() * mu_im, P64.p, Pinv64.p, P32.p, s);
     // max |Im phi| bounds |P| (used by the solvers' attainable-accuracy test)
     std::vector<double2> h((size_t)n);
     NSLX_HIP(hipMemcpyAsync(h.data(), phi.p, n * sizeof(double2), hipMemcpyDeviceToHost, s));
@@ -152,7 +152,7 @@ void FermionMatrix::populate(const void* phi_dev, int64_t nf, int sp, hipStream_
 }
 
 double FermionMatrix::slice_growth() const {
-    return std::fabs(delta()) * normK + std::fabs(delta() * mu) + max_im_phi;
+    return std::fabs(delta()) * normK + mut_abs() + max_im_phi;
 }
 
 TimeSliceSolver& FermionMatrix::time_slice_solver(int prec, int nc, bool precond, hipStream_t s) {

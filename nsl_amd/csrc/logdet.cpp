@@ -38,7 +38,7 @@ int auto_chunks(FermionMatrix& fm, int64_t field, hipStream_t s) {
         for (int t = 0; t < fm.nt; ++t) a += std::fabs(h[(size_t)t * fm.nx + x].y);
         im = std::max(im, a);
     }
-    const double growth = std::fabs(fm.beta) * fm.normK + im + std::fabs(fm.beta * fm.mu);
+    const double growth = std::fabs(fm.beta) * fm.normK + im + fm.mut_abs() * fm.nt;
     int n = (int)std::ceil(growth / 8.0);
     return std::max(1, std::min(n, fm.nt));
 }

@@ -65,7 +65,7 @@ inline unsigned g1(int64_t n) { return (unsigned)((n + 255) / 256); }
 // c eps ||MM^dag||: the attainable accuracy of an FP64 residual per |x|, with
 // ||MM^dag|| = ||M||^2 <= (1 + e^{|delta| ||K||} e^{|mu~| + max|Im phi|})^2.
 double floor_c(const FermionMatrix& fm) {
-    const double nM = 1.0 + std::exp(std::fabs(fm.delta()) * fm.normK + std::fabs(fm.delta() * fm.mu) + fm.max_im_phi);
+    const double nM = 1.0 + std::exp(std::fabs(fm.delta()) * fm.normK + fm.mut_abs() + fm.max_im_phi);
     return env_double("NSLX_FLOOR_C", 1.0) * 1.1102230246251565e-16 * nM * nM;
 }
 
@@ -235,7 +235,7 @@ void Pool::start_recurrence(PoolJob& j, bool fresh, bool keep_done) {
         if (tile >= 4 || !tile_fits(j.prec, tile, fm.nx)) tile = default_tile(j.prec, fm.nx, fm.nt, j.nb);
     }
     j.tile = tile;
-    j.cls = cls_of(j.prec, tile, fm.disc);
+    j.cls = cls_of(j.prec, tile, fm.disc, j.op);
     j.nparts = hop_nparts(j.prec, tile, fm.nx, fm.nt);
     j.part0 = alloc_range(part_used_, j.nb * j.nparts);
     if (j.part0 < 0) {
@@ -267,7 +267,8 @@ void Pool::admit(std::unique_ptr<PoolJob> jp) {
     NSLX_REQUIRE(j.fm && j.b && j.x && j.nb >= 1, "pool: bad job");
     j.fm->require_populated();
     NSLX_REQUIRE(j.nb % j.fm->nfield == 0, "pool: nbatch must be a multiple of the populated fields");
-    NSLX_REQUIRE(j.op == NSLX_OP_MMDAGGER, "pool: only MMdagger systems are batched");
+    NSLX_REQUIRE(j.op == NSLX_OP_MMDAGGER || j.op == NSLX_OP_MDAGGERM, "pool: the batched systems are MMdagger or MdaggerM");
+    NSLX_REQUIRE(j.op == NSLX_OP_MMDAGGER || j.fm->disc == DISC_EXP, "pool: HubbardDiag jobs are MMdagger systems");
     NSLX_REQUIRE(j.prec == NSLX_F64 || j.prec == NSLX_F32 || j.prec == NSLX_BF16X3, "pool: bad precision");
     if (j.rhs0 < 0) j.rhs0 = alloc_range(rhs_used_, j.nb);   // else reserved by reserve_rhs
     NSLX_REQUIRE(j.rhs0 >= 0, "pool: out of RHS slots");
@@ -342,9 +343,9 @@ void Pool::rebuild_class(int c) {
 // by stream capture, so that work other threads put on the legacy stream
 // meanwhile never meets a capturing stream.
 void Pool::run_iterations() {
-    unsigned mask = 0;
+    uint64_t mask = 0;
     for (int c = 0; c < NCLS; ++c)
-        if (cls_[c].njobs > 0) mask |= 1u << c;
+        if (cls_[c].njobs > 0) mask |= uint64_t(1) << c;
     if (!mask) return;
     // the launch sequence: hop(c, pass, args) per grouped launch, then advance()
     auto sequence = [&](auto&& hop, auto&& advance) {
@@ -356,7 +357,7 @@ void Pool::run_iterations() {
             const int pnew = ((K - 1) & 1) ? VS_P0 : VS_P1;
             for (int pass = 0; pass < 2; ++pass) {
                 for (int c = 0; c < NCLS; ++c) {
-                    if (!(mask & (1u << c))) continue;
+                    if (!(mask & (uint64_t(1) << c))) continue;
                     GroupArgs g;
                     g.probs = cls_[c].probs.p;
                     g.nprob = cls_[c].njobs;
@@ -370,9 +371,9 @@ void Pool::run_iterations() {
                         g.cg.ru = 1;
                         g.cg.delta2 = delta * delta;
                     }
-                    if (pass == 0) {   // CGA: Y = M^dag(p_new), p_new = r + beta p_old
+                    if (pass == 0) {   // CGA: Y = M^dag(p_new) (M(p_new) for MdaggerM), p_new = r + beta p_old
                         g.role[0] = VS_R; g.role[1] = VS_U; g.role[2] = (signed char)pold; g.role[3] = (signed char)pnew;
-                    } else {           // CGB: t = M u, x += alpha p_new, r -= alpha t
+                    } else {           // CGB: t = M u (M^dag u), x += alpha p_new, r -= alpha t
                         g.role[0] = VS_U; g.role[2] = (signed char)pnew; g.role[4] = VS_X; g.role[5] = VS_R;
                     }
                     hop(c, pass, g);
@@ -381,7 +382,8 @@ void Pool::run_iterations() {
         }
         advance();
     };
-    auto op_of = [](int pass) { return pass == 0 ? HOP_MD : HOP_M; };
+    // MMdagger = M (M^dag .): CGA applies M^dag, CGB M; MdaggerM the other way round
+    auto op_of = [](int c, int pass) { return (pass == 0) != mdagm_of(c) ? HOP_MD : HOP_M; };
     auto mode_of = [](int pass) { return pass == 0 ? MODE_CGA : MODE_CGB; };
     if (use_graph_) {
         if (!exec_ || mask != cls_mask_) {
@@ -406,7 +408,7 @@ void Pool::run_iterations() {
             sequence(
                 [&](int c, int pass, GroupArgs& g) {
                     NSLX_REQUIRE(g.nprob > 0 && cls_[c].grid > 0, "pool: empty grouped launch");
-                    const GroupedKernel gk = hop_grouped_kernel(prec_of(c), op_of(pass), mode_of(pass), tile_of(c), disc_of(c));
+                    const GroupedKernel gk = hop_grouped_kernel(prec_of(c), op_of(c, pass), mode_of(pass), tile_of(c), disc_of(c));
                     void* params[1] = {&g};
                     add(gk.fn, (unsigned)cls_[c].grid, (unsigned)gk.nthreads, params);
                 },
@@ -421,7 +423,7 @@ void Pool::run_iterations() {
     } else {
         sequence(
             [&](int c, int pass, GroupArgs& g) {
-                launch_hop_grouped(prec_of(c), op_of(pass), mode_of(pass), tile_of(c), disc_of(c), g, cls_[c].grid, s_);
+                launch_hop_grouped(prec_of(c), op_of(c, pass), mode_of(pass), tile_of(c), disc_of(c), g, cls_[c].grid, s_);
             },
             [&]() {
                 hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s_, it_dev_.p, P_);
@@ -529,7 +531,7 @@ std::vector<std::unique_ptr<PoolJob>> Pool::step() {
         }
     }
     if (replaced) read_flags();
-    unsigned dirty = 0;
+    uint64_t dirty = 0;
     std::vector<std::unique_ptr<PoolJob>> keep;
     for (auto& jp : jobs_) {
         PoolJob& j = *jp;
@@ -538,7 +540,7 @@ std::vector<std::unique_ptr<PoolJob>> Pool::step() {
         const bool budget = j.prec == NSLX_F64 ? iter_ - j.join_iter >= j.maxit - j.iters : low_over(j);
         if (!all_done && !budget) { keep.push_back(std::move(jp)); continue; }
         const int old_cls = j.cls;
-        dirty |= 1u << old_cls;
+        dirty |= uint64_t(1) << old_cls;
         free_range(part_used_, j.part0, j.nb * j.nparts);
         j.part0 = -1;
         const bool was_low = j.prec != NSLX_F64;
@@ -549,13 +551,13 @@ std::vector<std::unique_ptr<PoolJob>> Pool::step() {
             out.push_back(std::move(jp));
         } else {
             start_recurrence(j, false, was_low);
-            dirty |= 1u << j.cls;
+            dirty |= uint64_t(1) << j.cls;
             keep.push_back(std::move(jp));
         }
     }
     jobs_ = std::move(keep);
     for (int c = 0; c < NCLS; ++c)
-        if (dirty & (1u << c)) rebuild_class(c);
+        if (dirty & (uint64_t(1) << c)) rebuild_class(c);
     return out;
 }
 
@@ -587,6 +589,7 @@ int batch_solve(Ctx& ctx, int64_t nitems, const nslx_batch_item* items, double t
             j->b = items[i].b;
             j->x = items[i].x;
             j->nb = items[i].nbatch;
+            j->op = items[i].op;
             j->tol = tol;
             j->prec = prec;
             j->maxit = maxit;

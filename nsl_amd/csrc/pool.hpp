@@ -72,8 +72,9 @@ class Pool {
     const PoolStats& stats() const { return st_; }
 
    private:
-    // class = (discretisation, precision, classic tile 0-3): (disc * 3 + prec) * 4 + tile
-    static constexpr int NCLS = 24;
+    // class = (operator, discretisation, precision, classic tile 0-3):
+    // ((opb * 2 + disc) * 3 + prec) * 4 + tile, opb = 1 for MdaggerM systems
+    static constexpr int NCLS = 48;
     struct Cls {
         DevBuf<HopProblem> probs;
         DevBuf<int4> tiles;
@@ -96,15 +97,18 @@ class Pool {
     std::vector<std::unique_ptr<PoolJob>> jobs_;
     std::vector<int> done_h_, want_h_;
     int64_t iter_ = 1;                 // next global iteration (device counter mirror)
-    unsigned cls_mask_ = 0;            // classes in the graph
+    uint64_t cls_mask_ = 0;            // classes in the graph
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
     PoolStats st_;
 
-    static int cls_of(int prec, int tile, int disc) { return ((disc & 1) * 3 + prec % 3) * 4 + (tile & 3); }
+    static int cls_of(int prec, int tile, int disc, int op) {
+        return (((op == NSLX_OP_MDAGGERM ? 1 : 0) * 2 + (disc & 1)) * 3 + prec % 3) * 4 + (tile & 3);
+    }
     static int prec_of(int c) { return (c / 4) % 3; }
     static int tile_of(int c) { return c % 4; }
-    static int disc_of(int c) { return c / 12; }
+    static int disc_of(int c) { return (c / 12) % 2; }
+    static bool mdagm_of(int c) { return c >= 24; }
     int64_t alloc_range(std::vector<char>& used, int64_t n);
     void free_range(std::vector<char>& used, int64_t start, int64_t n);
     void start_recurrence(PoolJob& j, bool fresh, bool keep_done);

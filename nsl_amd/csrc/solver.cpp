@@ -387,7 +387,7 @@ int solve_cg_ru(FermionMatrix& fm, int op, int prec, const void* b, void* x, con
     w.best.ensure(nb); w.stall.ensure(nb); w.rstart.ensure(nb);
     // ||MM^dag|| = ||M||^2 <= (1 + ||E|| max|P|)^2, ||E|| <= e^{|delta| ||K||},
     // |P| <= e^{|mu~| + max|Im phi|}
-    const double nM = 1.0 + std::exp(std::fabs(fm.delta()) * fm.normK + std::fabs(fm.delta() * fm.mu) + fm.max_im_phi);
+    const double nM = 1.0 + std::exp(std::fabs(fm.delta()) * fm.normK + fm.mut_abs() + fm.max_im_phi);
     const double fc = env_double("NSLX_FLOOR_C", 1.0) * 1.1102230246251565e-16 * nM * nM;
     // double-double mode: no attainable-accuracy stop (the residual is exact
     // enough to reach tol); floor2_64 only decides when to switch to it

@@ -1077,7 +1077,8 @@ void TimeSliceSolver::check_passes(hipStream_t s) {
 }
 
 void TimeSliceSolver::apply(int op, const void* y, void* x, int64_t nb, bool io_c128, hipStream_t s) {
-    NSLX_REQUIRE(op == NSLX_OP_M || op == NSLX_OP_MDAGGER || op == NSLX_OP_MMDAGGER, "time-slice solve: bad op");
+    NSLX_REQUIRE(op == NSLX_OP_M || op == NSLX_OP_MDAGGER || op == NSLX_OP_MMDAGGER || op == NSLX_OP_MDAGGERM,
+                 "time-slice solve: bad op");
     NSLX_REQUIRE(y && x && y != x, "time-slice solve: null or aliased buffers");
     NSLX_REQUIRE(gen_ == fm_.pop_gen, "time-slice solve: the fermion matrix was re-populated since the factorisation");
     ApplyBufs& B = bufs(nb, s);
@@ -1100,11 +1101,17 @@ void TimeSliceSolver::apply(int op, const void* y, void* x, int64_t nb, bool io_
         in(B.mid.p);
         B.Mdinv.run(s);
         out(B.out.p);
-    } else {
+    } else if (op == NSLX_OP_MMDAGGER) {       // (M M^dag)^{-1} = M^{-dag} M^{-1}
         in(B.in.p);
         B.Minv.run(s);
         B.Mdinv.run(s);
         out(B.out.p);
+    } else {                                   // (M^dag M)^{-1} = M^{-1} M^{-dag}
+        in(B.mid.p);
+        B.Mdinv.run(s);
+        NSLX_HIP(hipMemcpyAsync(B.in.p, B.out.p, bytes, hipMemcpyDeviceToDevice, s));
+        B.Minv.run(s);
+        out(B.mid.p);
     }
 }
 

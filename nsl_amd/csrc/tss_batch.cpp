@@ -1,5 +1,5 @@
 // Batched time-slice-preconditioned solves (nslx_batch_solve_tss): many
-// HubbardExp systems MM^dag x = b of different Nx, Nt, beta and fields, each
+// HubbardExp systems MM^dag x = b (or M^dag M x = b) of different Nx, Nt, beta and fields, each
 // solved by solve_tss (CG preconditioned by its own time-slice solve, with
 // double-double correction), spread over several host threads that each
 // drive one HIP stream.  With 2-3 preconditioned iterations per system the
@@ -42,6 +42,7 @@ int batch_solve_tss(Ctx& ctx, int64_t nitems, const nslx_batch_item* items, void
     for (int64_t i = 0; i < nitems; ++i) {
         const nslx_batch_item& it = items[i];
         NSLX_REQUIRE(it.fm && it.b && it.x && it.nbatch >= 1, "batch_solve_tss: incomplete item");
+        NSLX_REQUIRE(it.op == NSLX_OP_MMDAGGER || it.op == NSLX_OP_MDAGGERM, "batch_solve_tss: op must be MMdagger or MdaggerM");
         FermionMatrix& fm = it.fm->m;
         NSLX_REQUIRE(fm.ctx == &ctx, "batch_solve_tss: an item's fermion matrix belongs to another context");
         NSLX_REQUIRE(fm.disc == DISC_EXP, "batch_solve_tss: HubbardExp items only");
@@ -82,7 +83,7 @@ int batch_solve_tss(Ctx& ctx, int64_t nitems, const nslx_batch_item* items, void
             const int64_t i = order[k];
             const nslx_batch_item& it = items[i];
             try {
-                status[i] = solve_tss(it.fm->m, NSLX_OP_MMDAGGER, it.b, it.x, x_lo ? x_lo[i] : nullptr, nullptr,
+                status[i] = solve_tss(it.fm->m, it.op, it.b, it.x, x_lo ? x_lo[i] : nullptr, nullptr,
                                       it.nbatch, tol, maxit, pprec, 0, &reps[i], s);
             } catch (const Error& e) {
                 status[i] = e.status;

@@ -1,10 +1,10 @@
-// CG on MM^dag preconditioned by the time-slice direct solve, inside a
-// double-double defect-correction loop (nslx_solve_tss).
+// CG on MM^dag (or M^dag M) preconditioned by the time-slice direct solve,
+// inside a double-double defect-correction loop (nslx_solve_tss).
 //
 // Outer loop (per right-hand side, until |b - A x| <= tol |b| in double-
 // double): r = b - A (x_hi + x_lo) in double-double, rounded to FP64; an inner
 // flexible PCG on A d = r from d = 0 with FP64 vectors and A applied in FP64
-// on the hop kernels, preconditioned by z = (M M^dag)^{-1} r from the
+// on the hop kernels, preconditioned by z = A^{-1} r (A = M M^dag or M^dag M) from the
 // time-slice solve in FP64 or FP32; x += d in double-double.  The inner solve
 // stops a right-hand side once its recursive residual has dropped by rho
 // (1e-6) or below 0.3 tol |b|: below that the FP64 recurrence no longer tracks
@@ -86,7 +86,8 @@ int solve_tss(FermionMatrix& fm, int op, const void* b, void* x, void* xlo, cons
     const auto t0 = std::chrono::steady_clock::now();
     NSLX_REQUIRE(b && x, "solve_tss: null buffer");
     NSLX_REQUIRE(b != x && x0 != x && (!xlo || (xlo != x && xlo != b)), "solve_tss: x must not alias b or x0");
-    NSLX_REQUIRE(op == NSLX_OP_MMDAGGER, "solve_tss: the time-slice preconditioner serves MMdagger systems");
+    NSLX_REQUIRE(op == NSLX_OP_MMDAGGER || op == NSLX_OP_MDAGGERM,
+                 "solve_tss: the time-slice preconditioner serves MMdagger and MdaggerM systems");
     NSLX_REQUIRE(pprec == NSLX_F64 || pprec == NSLX_F32, "solve_tss: preconditioner precision must be F64 or F32");
     NSLX_REQUIRE(tol > 0.0 && maxit >= 1, "solve_tss: bad tol/maxit");
     NSLX_REQUIRE(fm.disc == DISC_EXP, "solve_tss: HubbardExp only");
@@ -160,7 +161,7 @@ int solve_tss(FermionMatrix& fm, int op, const void* b, void* x, void* xlo, cons
         NSLX_HIP(hipMemcpyAsync(done_in, done_out, nb * sizeof(int), hipMemcpyDeviceToDevice, s));
         NSLX_HIP(hipMemcpyAsync(r0n2, rn2, nb * sizeof(double), hipMemcpyDeviceToDevice, s));
         NSLX_HIP(hipMemsetAsync(d, 0, bytes, s));
-        ts.apply(NSLX_OP_MMDAGGER, r, z, nb, true, s);
+        ts.apply(op, r, z, nb, true, s);
         ++napply;
         NSLX_HIP(hipMemcpyAsync(p, z, bytes, hipMemcpyDeviceToDevice, s));
         dot_perb(1, r, z, nb, len, rzn, 0, fm.vsc, s);
@@ -170,7 +171,7 @@ int solve_tss(FermionMatrix& fm, int op, const void* b, void* x, void* xlo, cons
         while (inner_active > 0 && it < max_inner && inner_total < maxit) {
             ++it;
             ++inner_total;
-            fm.apply(NSLX_OP_MMDAGGER, p, t, nb, NSLX_F64, s);
+            fm.apply(op, p, t, nb, NSLX_F64, s);
             dot_perb(1, p, t, nb, len, pt, 0, fm.vsc, s);
             hipLaunchKernelGGL(k_alpha, dim3(g1(nb)), dim3(256), 0, s, nb, rz, pt, done_in, alpha);
             NSLX_LAUNCH_CHECK();
@@ -185,7 +186,7 @@ int solve_tss(FermionMatrix& fm, int op, const void* b, void* x, void* xlo, cons
             NSLX_HIP(hipStreamSynchronize(s));
             inner_active = *nact_h;
             if (inner_active == 0) break;
-            ts.apply(NSLX_OP_MMDAGGER, r, z, nb, true, s);
+            ts.apply(op, r, z, nb, true, s);
             ++napply;
             dot_perb(1, t, z, nb, len, tz, 0, fm.vsc, s);
             dot_perb(1, r, z, nb, len, rzn, 0, fm.vsc, s);

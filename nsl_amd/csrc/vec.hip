@@ -73,12 +73,14 @@ __global__ void k_finish(const double* __restrict__ part, int nchunk, int nb, do
     }
 }
 
-__global__ void k_phase(const double2* phi, int64_t n, int sgn, double mut, double2* P, double2* Pinv, float2* P32) {
+__global__ void k_phase(const double2* phi, int64_t n, int sgn, double mut, double mut_im, double2* P, double2* Pinv,
+                        float2* P32) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double2 f = phi[i];
-    // exp(i s phi + s mu) with phi = fr + i fi: exponent = (s mu - s fi) + i s fr
-    const double er = sgn * mut - sgn * f.y, ei = sgn * f.x;
+    // exp(i s phi + s mu~) with phi = fr + i fi, mu~ = mr + i mi:
+    // exponent = (s mr - s fi) + i s (fr + mi)
+    const double er = sgn * mut - sgn * f.y, ei = sgn * (f.x + mut_im);
     const double m = exp(er);
     double sn, cs;
     sincos(ei, &sn, &cs);
@@ -291,9 +293,10 @@ static void finish(const double* part, int nchunk, int64_t nb, double* out, int 
     NSLX_LAUNCH_CHECK();
 }
 
-void phase_tables(const void* phi, int64_t n, int sgn, double mut, void* P64, void* Pinv64, void* P32, hipStream_t s) {
+void phase_tables(const void* phi, int64_t n, int sgn, double mut, double mut_im, void* P64, void* Pinv64, void* P32,
+                  hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_phase, dim3(grid1(n)), dim3(256), 0, s, (const double2*)phi, n, sgn, mut, (double2*)P64,
+    hipLaunchKernelGGL(k_phase, dim3(grid1(n)), dim3(256), 0, s, (const double2*)phi, n, sgn, mut, mut_im, (double2*)P64,
                        (double2*)Pinv64, (float2*)P32);
     NSLX_LAUNCH_CHECK();
 }

@@ -17,8 +17,8 @@ struct VecScratch {
 };
 
 // P = exp(i s phi + s mu~), Pinv = exp(-i s phi - s mu~) in FP64, plus the
-// FP32 copy of P.  phi: c128 [n].  (hubbardExp.hpp:116-128)
-void phase_tables(const void* phi, int64_t n, int sgn, double mut, void* P64, void* Pinv64,
+// FP32 copy of P; mu~ = mut + i mut_im.  phi: c128 [n].  (hubbardExp.hpp:116-128)
+void phase_tables(const void* phi, int64_t n, int sgn, double mut, double mut_im, void* P64, void* Pinv64,
                   void* P32, hipStream_t s);
 
 // out[b] = sum |v_b|^2 (double), S = double (c128) or float (c64).

@@ -167,14 +167,18 @@ def dd_Mdag(E, P, v, field_of):
     return v - ev.mul_c128(np.conj(P[field_of])).scale(sign)
 
 
-def mmdag_residual(E, P, b, x_hi, x_lo, rpf=1):
-    """r = b - M M^dag (x_hi + x_lo) in double-double; returns (r as
-    complex128, per-RHS relative residual |r| / |b|).  E: (nx, nx) float64;
-    P: (nfield, nt, nx) complex128; b, x_hi, x_lo: (nb, nt, nx) complex128."""
+def mmdag_residual(E, P, b, x_hi, x_lo, rpf=1, op="MMdagger"):
+    """r = b - M M^dag (x_hi + x_lo) (op "MMdagger") or b - M^dag M (x_hi +
+    x_lo) (op "MdaggerM") in double-double; returns (r as complex128, per-RHS
+    relative residual |r| / |b|).  E: (nx, nx) float64; P: (nfield, nt, nx)
+    complex128; b, x_hi, x_lo: (nb, nt, nx) complex128."""
     nb = b.shape[0]
     field_of = np.arange(nb) // rpf
     x = CDD.from_c128(x_hi, x_lo)
-    t = dd_M(E, P, dd_Mdag(E, P, x, field_of), field_of)
+    if op == "MMdagger":
+        t = dd_M(E, P, dd_Mdag(E, P, x, field_of), field_of)
+    else:
+        t = dd_Mdag(E, P, dd_M(E, P, x, field_of), field_of)
     r = CDD.from_c128(b) - t
     rc = r.to_c128()
     rel = np.sqrt((np.abs(rc) ** 2).sum(axis=(1, 2)) / (np.abs(b) ** 2).sum(axis=(1, 2)))
@@ -186,7 +190,7 @@ def _fc(z):
     return (Fraction(float(z.real)), Fraction(float(z.imag)))
 
 
-def exact_mmdag_residual(E, P, b, x_hi, x_lo):
+def exact_mmdag_residual(E, P, b, x_hi, x_lo, op="MMdagger"):
     """The same residual for ONE right-hand side (b, x: (nt, nx)) and one
     field (P: (nt, nx)) in exact rational arithmetic; returns complex128 r
     (each component rounded once).  Slow: tiny lattices only."""
@@ -204,26 +208,30 @@ def exact_mmdag_residual(E, P, b, x_hi, x_lo):
         return [(sum(Ef[y][x] * v[x][0] for x in range(nx)), sum(Ef[y][x] * v[x][1] for x in range(nx)))
                 for y in range(nx)]
 
-    # u = M^dag x
-    u = []
-    for t in range(nt):
-        ev = eapply(X[(t + 1) % nt])
-        sgn = -1 if t == nt - 1 else 1
-        row = []
-        for i in range(nx):
-            pc = (Pf[t][i][0], -Pf[t][i][1])
-            q = cmul(pc, ev[i])
-            row.append((X[t][i][0] - sgn * q[0], X[t][i][1] - sgn * q[1]))
-        u.append(row)
-    # t = M u
+    def mdag(v):   # (M^dag v)_t = v_t - b'_t conj(P_t) E v_{t+1}
+        out = []
+        for t in range(nt):
+            ev = eapply(v[(t + 1) % nt])
+            sgn = -1 if t == nt - 1 else 1
+            row = []
+            for i in range(nx):
+                q = cmul((Pf[t][i][0], -Pf[t][i][1]), ev[i])
+                row.append((v[t][i][0] - sgn * q[0], v[t][i][1] - sgn * q[1]))
+            out.append(row)
+        return out
+
+    def m(v):      # (M v)_t = v_t - b_t E (P_{t-1} v_{t-1})
+        out = []
+        for t in range(nt):
+            ev = eapply([cmul(Pf[(t - 1) % nt][i], v[(t - 1) % nt][i]) for i in range(nx)])
+            sgn = -1 if t == 0 else 1
+            out.append([(v[t][i][0] - sgn * ev[i][0], v[t][i][1] - sgn * ev[i][1]) for i in range(nx)])
+        return out
+
+    tv = m(mdag(X)) if op == "MMdagger" else mdag(m(X))
     r = np.zeros((nt, nx), dtype=np.complex128)
     for t in range(nt):
-        prev = [cmul(Pf[(t - 1) % nt][i], u[(t - 1) % nt][i]) for i in range(nx)]
-        ev = eapply(prev)
-        sgn = -1 if t == 0 else 1
         for i in range(nx):
-            tr = u[t][i][0] - sgn * ev[i][0]
-            ti = u[t][i][1] - sgn * ev[i][1]
             br, bi = _fc(b[t, i])
-            r[t, i] = complex(float(br - tr), float(bi - ti))
+            r[t, i] = complex(float(br - tv[t][i][0]), float(bi - tv[t][i][1]))
     return r

@@ -40,6 +40,34 @@ def test_dd_equals_exact_rational():
     assert np.abs(r[0] - re).max() <= 1e-15 * np.abs(re).max()
 
 
+def test_dd_equals_exact_rational_mdagm():
+    """The M^dag M residual (op "MdaggerM") against exact rational arithmetic,
+    at the solution of the dense M^dag M system (a residual of pure
+    cancellation), and against the dense matrix built independently:
+    M^dag M = (M M^dag with M and M^dag swapped), checked through the dense
+    M of dense_mmdag's construction."""
+    E, P, Nt, Nx = setup(seed=11)
+    g = np.random.default_rng(3)
+    b = g.standard_normal((1, Nt, Nx)) + 1j * g.standard_normal((1, Nt, Nx))
+    I = np.eye(Nt * Nx, dtype=complex).reshape(-1, Nt, Nx)
+    s = np.ones(Nt); s[0] = -1
+    M = (I - s[None, :, None] * (np.roll(I * P[0], 1, axis=1) @ E.T)).reshape(Nt * Nx, -1).T
+    A = M.conj().T @ M
+    x = np.linalg.solve(A, b.reshape(-1)).reshape(1, Nt, Nx)
+    xl = (g.standard_normal((1, Nt, Nx)) + 1j * g.standard_normal((1, Nt, Nx))) * np.abs(x).max() * 1e-18
+    r, _ = dd.mmdag_residual(E, P, b, x, xl, op="MdaggerM")
+    re = dd.exact_mmdag_residual(E, P[0], b[0], x[0], xl[0], op="MdaggerM")
+    assert np.abs(r[0] - re).max() <= 1e-15 * np.abs(re).max()
+    # at a random x the residual is b - A x to FP64 accuracy (a swapped
+    # operator order or a dropped conjugate fails this)
+    y = g.standard_normal((1, Nt, Nx)) + 1j * g.standard_normal((1, Nt, Nx))
+    r2, _ = dd.mmdag_residual(E, P, b, y, np.zeros_like(y), op="MdaggerM")
+    want = b.reshape(-1) - A @ y.reshape(-1)
+    assert np.abs(r2.reshape(-1) - want).max() <= 1e-12 * np.abs(want).max()
+    r3, _ = dd.mmdag_residual(E, P, b, y, np.zeros_like(y))
+    assert np.abs(r3.reshape(-1) - want).max() > 1e-6 * np.abs(want).max()
+
+
 def test_dd_resolves_below_fp64():
     """x_lo of size 1e-20 |x| changes the double-double residual by the
     amount A x_lo (the FP64 sum x + x_lo would not change at all)."""

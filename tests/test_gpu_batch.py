@@ -46,10 +46,10 @@ SPECS = [
 ]
 
 
-def check_residuals(ofs, b, x, target):
+def check_residuals(ofs, b, x, target, name="MMdagger"):
     rpf = b.shape[0] // len(ofs)
     for k in range(b.shape[0]):
-        r = b[k] - ofs[k // rpf].MMdagger(x[k].cpu())
+        r = b[k] - getattr(ofs[k // rpf], name)(x[k].cpu())
         assert float(r.norm() / b[k].norm()) < 2 * target, k
 
 
@@ -67,6 +67,43 @@ def test_batch_true_residual(prec, use_graph):
     assert rep["max_rel_residual"] <= target
     for (fm, ofs, b), x in zip(probs, xs):
         check_residuals(ofs, b, x, target)
+
+
+@pytest.mark.parametrize("prec", [nsl.F64, nsl.F32])
+def test_batch_mdagm_items_in_the_mix(prec):
+    """MdaggerM systems (the two-body correlator's second CG,
+    twoBodyCorrelationFunction.tpp:256-283) batched beside MMdagger ones:
+    each item checked with the oracle's own operator, and an MdaggerM item
+    bit-identical alone and in the mix."""
+    ctx = nsl.Context(0)
+    probs = [make(*s, ctx=ctx) for s in SPECS]
+    ops = [nsl.MdaggerM, nsl.MMdagger, nsl.MdaggerM, nsl.MMdagger]
+    xs, rep = nsl.batch_solve([(fm, b.to(DEV), op) for (fm, _, b), op in zip(probs, ops)], tol=1e-10,
+                              precision=prec, ctx=ctx)
+    assert rep["converged"] and rep["converged_items"] == len(SPECS), rep
+    target = max(1e-10, rep["max_floor_rel"])
+    for (fm, ofs, b), x, op in zip(probs, xs, ops):
+        check_residuals(ofs, b, x, target, "MdaggerM" if op == nsl.MdaggerM else "MMdagger")
+    alone, _ = nsl.batch_solve([(probs[0][0], probs[0][2].to(DEV), nsl.MdaggerM)], tol=1e-10, precision=prec,
+                               ctx=ctx)
+    assert torch.equal(alone[0], xs[0])
+
+
+def test_batch_tss_mdagm_items():
+    """The time-slice-preconditioned batch with MdaggerM and MMdagger items:
+    every RHS to 1e-12 in double-double, re-checked on the CPU."""
+    from tests import ddarith as dd
+    ctx = nsl.Context.default()
+    probs = [make(*s, ctx=ctx) for s in SPECS[:3]]
+    ops = [nsl.MdaggerM, nsl.MMdagger, nsl.MdaggerM]
+    xs, los, rep = nsl.batch_solve_tss([(fm, b.to(DEV), op) for (fm, _, b), op in zip(probs, ops)], tol=1e-12,
+                                       nstreams=3, return_lo=True)
+    assert rep["converged"], rep
+    for (fm, ofs, b), x, lo, op in zip(probs, xs, los, ops):
+        E, P = fm.export()
+        _, relv = dd.mmdag_residual(E, P, b.numpy(), x.cpu().numpy(), lo.cpu().numpy(), rpf=b.shape[0] // len(ofs),
+                                    op="MdaggerM" if op == nsl.MdaggerM else "MMdagger")
+        assert relv.max() <= 1.05e-12, relv
 
 
 def test_batch_alone_equals_in_mix():
@@ -101,6 +138,12 @@ def test_batch_errors():
     other = nsl.Context(0)
     with pytest.raises(nsl.NslxError):                       # fm of another context
         nsl.batch_solve([(fm, b.to(DEV))], ctx=other)
+    with pytest.raises(nsl.NslxError):                       # M is not a batched system
+        nsl.batch_solve([(fm, b.to(DEV), nsl.M)], ctx=ctx)
+    fd = nsl.HubbardDiag(nsl.Lattice.honeycomb(3, 3), 16, 4.0, ctx=ctx)
+    fd.populate(torch.zeros((16, 18), dtype=torch.complex128, device=DEV))
+    with pytest.raises(nsl.NslxError):                       # HubbardDiag items are MMdagger systems
+        nsl.batch_solve([(fd, b[:2].to(DEV), nsl.MdaggerM)], ctx=ctx)
 
 
 def test_batch_large_lattice_grows_partials():

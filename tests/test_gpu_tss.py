@@ -92,9 +92,10 @@ def test_direct_solve_residual(case, prec):
     # (M M^dag)^{-1} = M^{-dag} M^{-1}: residual of MM^dag, FP64 only (FP32's
     # error is cond(M) * eps32 amplified once more)
     if prec == nsl.F64:
-        x = fm.solve_direct(nsl.MatrixCombination.MMdagger, y.to(DEV), nchunks=nch).cpu()
-        r = rel(oracle_apply(ofs, "MMdagger", x), y)
-        assert r < 1e-8, r
+        for op, name in ((nsl.MatrixCombination.MMdagger, "MMdagger"), (nsl.MatrixCombination.MdaggerM, "MdaggerM")):
+            x = fm.solve_direct(op, y.to(DEV), nchunks=nch).cpu()
+            r = rel(oracle_apply(ofs, name, x), y)
+            assert r < 1e-8, (name, r)
 
 
 def cpu_dd_check(fm, b, x, x_lo, rpf):
@@ -143,6 +144,26 @@ def test_tss_pcg_reaches_1e12(prec):
     assert xonly.max() > 1e-11
 
 
+@pytest.mark.parametrize("prec", [nsl.F64, nsl.F32])
+def test_tss_pcg_mdagm_reaches_1e12(prec):
+    """The same fields with M^dag M (the two-body correlator's second
+    system, twoBodyCorrelationFunction.tpp:256-283): preconditioned by
+    (M^dag M)^{-1} = M^{-1} M^{-dag}, every RHS to 1e-12 in double-double,
+    re-checked on the CPU with the M^dag M double-double residual."""
+    Nt, beta, U = 64, 8.0, 3.0
+    fm, ofs = make("honeycomb", (6, 6), Nt, beta, nfield=2, seed=1000, std=math.sqrt(beta / Nt * U))
+    b = torch.as_tensor(rng.complex_normal(2100, 0, (4, Nt, 72)))
+    cg = nsl.CGTimeSlice(fm, 1e-12, 2000, precision=prec, matrixCombination=nsl.MatrixCombination.MdaggerM)
+    x = cg(b.to(DEV))
+    assert cg.converged, cg.report
+    assert cg.report["max_rel_residual"] <= 1e-12, cg.report
+    E, P = fm.export()
+    _, relv = dd.mmdag_residual(E, P, b.numpy(), x.cpu().numpy(), cg.x_lo.cpu().numpy(), rpf=2, op="MdaggerM")
+    assert relv.max() <= 1.05e-12, relv
+    dev = nsl.dd_residual(fm, b.to(DEV), x, cg.x_lo, op=nsl.MatrixCombination.MdaggerM)
+    assert np.allclose(dev, relv, rtol=1e-6, atol=1e-20), (dev, relv)
+
+
 def test_tss_full_size_bench_workload():
     """The bench workload (12x12 honeycomb, Nx = 288, Nt = 64, beta = 8, U = 3,
     16 fields x 8 RHS) with the FP32 preconditioner: every RHS to 1e-12, and
@@ -173,8 +194,8 @@ def test_tss_errors():
         fd.solve_direct(nsl.MatrixCombination.M, y)
     fm = nsl.HubbardExp(lat, 16, 4.0)
     fm.populate(torch.zeros((16, 18), dtype=torch.complex128, device=DEV))
-    with pytest.raises(nsl.NslxError):
-        fm.solve_direct(nsl.MatrixCombination.MdaggerM, y)
+    with pytest.raises(nsl.NslxError):                  # CG on M is not a product system
+        nsl.CGTimeSlice(fm, 1e-12, 100, matrixCombination=nsl.MatrixCombination.M)(y)
     with pytest.raises(nsl.NslxError):                  # nchunks must divide Nt
         fm.solve_direct(nsl.MatrixCombination.M, y, nchunks=3)
 
