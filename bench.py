@@ -96,9 +96,10 @@ def parse_prof(text):
 
 def class_roofline(name, flops, bytes_, avg_s):
     """Bound, achieved and peak of a kernel class from its algorithmic work per
-    launch: the batched GEMM on the matrix cores of its precision, the
-    double-double hop on the FP64 vector units, everything else on HBM."""
-    if name.startswith("bgemm_"):
+    launch: the GEMM-shaped classes (batched GEMM, chunk products, persistent
+    recurrences) on the matrix cores of their precision, the double-double
+    hop on the FP64 vector units, everything else on HBM."""
+    if name.startswith(("bgemm_", "chunkprod_", "chain_")):
         prec = name.split("_")[1]
         return "mfma", flops / avg_s / 1e12, PEAK[prec], "TFLOP/s"
     if name.startswith("dd_hop"):

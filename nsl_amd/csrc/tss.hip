@@ -649,8 +649,14 @@ void TimeSliceSolver::build_setup(hipStream_t s) {
     gj_binv_.alloc((size_t)nf * GJB * GJB * esz_);
     gj_C_.alloc((size_t)nf * 2 * n * GJB * esz_);
     gj_YT_.alloc((size_t)nf * W * GJB * esz_);
-    // 1. chunk products: step q writes buffer (c-1-q) & 1 (0 = Tt, 1 = ping-pong half)
-    {
+    // 1. chunk products: one launch of row blocks that keep their rows for
+    // all c steps, where they fit; else c batched GEMM launches, step q
+    // writing buffer (c-1-q) & 1 (0 = Tt, 1 = ping-pong half)
+    if (chunkprod_supported(prec_, n)) {
+        unsigned char* Tt0 = Tt_.p;
+        const int prec = prec_;
+        p.custom([=](hipStream_t st) { launch_chunkprod(prec, n, nt, c, nc, nf, Ep, ldE, Pp, Tt0, st); });
+    } else {
         unsigned char* buf[2] = {Tt_.p, Tt_.p + (size_t)nf * nc * nn * esz_};
         for (int q = 0; q < c; ++q) {
             unsigned char* dst = buf[(c - 1 - q) & 1];

@@ -96,6 +96,15 @@ std::string chain_class(const BVariant& v);
 void launch_chain(const BVariant& v, const BJob* d_jobs, int steps, int nchain, int rows, int cols, unsigned* sync,
                   int* err, hipStream_t s, double flops = 0.0, double bytes = 0.0);
 
+// Chunk products T_j (row i of Tt_j = T_j e_i) for every field and chunk in
+// one launch (chunkprod.hip): a workgroup owns 16 rows of one chain for all
+// c steps, its rows resident in LDS.  chunkprod_supported: whether the rows
+// fit (two workgroups per CU); else the per-step batched GEMM is used.
+// E: exp(s delta K) in prec (ld ldE), P: phases [nf][nt][n], Tt: [nf][nc][n][n].
+bool chunkprod_supported(int prec, int n);
+void launch_chunkprod(int prec, int n, int nt, int c, int nc, int nf, const void* E, long long ldE, const void* P,
+                      void* Tt, hipStream_t s);
+
 // A recorded sequence of launches whose job and tile arrays live in one
 // device buffer each (uploaded once); replayed by run().
 class BPlan {

@@ -56,6 +56,8 @@ def test_roofline_of_reported_work_classes():
         "bgemm_f32_cplx_ph0_64 100 10.0 50 5.0e11 1.0e10",   # 200 us, 1e10 flops per launch
         "dd_hop_m_resid 4 3.2 4 8.0e10 4.0e8",               # 800 us
         "gj_panel_f64 10 1.0 5 1e6 5.0e7",                   # 200 us
+        "chunkprod_f64 2 1.0 1 3.0e10 1.0e7",                # 1 ms, 3e10 flops
+        "chain_f32_ph1 6 0.6 3 3.0e9 1.0e8",                 # 200 us, 1e9 flops per launch
     ])
     rf, rows = bench.roofline_from_prof(text, 128, NX, NT, {})
     assert rf["kernel"] == "bgemm_f32_cplx_ph0_64"
@@ -64,6 +66,11 @@ def test_roofline_of_reported_work_classes():
     table = {d["kernel"]: d for d in bench.kernel_table(rows)}
     assert table["dd_hop_m_resid"]["bound"] == "valu"
     assert table["dd_hop_m_resid"]["achieved"] == pytest.approx(2e10 / 800e-6 / 1e12, rel=1e-3)
+    assert table["chunkprod_f64"]["bound"] == "mfma"
+    assert table["chunkprod_f64"]["achieved"] == pytest.approx(3e10 / 1e-3 / 1e12, rel=1e-3)
+    assert table["chunkprod_f64"]["frac"] == pytest.approx(30.0 / 78.6, rel=1e-3)
+    assert table["chain_f32_ph1"]["bound"] == "mfma"
+    assert table["chain_f32_ph1"]["frac"] == pytest.approx(5.0 / 157.3, rel=1e-3)
     assert table["gj_panel_f64"]["bound"] == "hbm"
     assert table["gj_panel_f64"]["achieved"] == pytest.approx(1e7 / 200e-6 / 1e9, rel=1e-3)
 

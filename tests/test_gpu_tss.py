@@ -258,3 +258,40 @@ def test_chain_give_up_is_reported_and_recovers(monkeypatch):
     monkeypatch.delenv("NSLX_CHAIN_SPIN")
     x = fm.solve_direct(nsl.MatrixCombination.M, y.to(DEV)).cpu()
     assert rel(oracle_apply(ofs, "M", x), y) < 1e-10
+
+
+@pytest.mark.parametrize("prec", [nsl.F64, nsl.F32])
+@pytest.mark.parametrize("kind,size,Nt,beta,nf", [("honeycomb", (3, 3), 16, 4.0, 2),      # 1 column block per wave
+                                                  ("honeycomb", (6, 6), 32, 6.0, 2),      # 2
+                                                  ("nanotube", (3, 4), 20, 3.0, 1),       # Nx = 48, odd block count
+                                                  ("honeycomb", (12, 12), 64, 8.0, 1)])   # Nx = 288, 5
+def test_chunk_products_row_kernel_matches_batched_gemm(kind, size, Nt, beta, nf, prec, monkeypatch):
+    """The chunk products built by row blocks that keep their rows for the
+    whole chunk (one launch) against one batched GEMM launch per slice
+    (NSLX_CHUNKPROD=0): the same M^{-1} y and M^{-dag} y up to summation order
+    (1e-10 FP64, 1e-3 FP32 relative), and the kernel classes show which path
+    built them.  The Nx = 288 case uses the bench's beta, Nt and field
+    width (FP64 only: cond(M) ~ 1e5 there, beyond what an FP32 comparison
+    at 1e-3 resolves)."""
+    if prec == nsl.F32 and size == (12, 12):
+        pytest.skip("FP32 solves of the Nx = 288, beta = 8 fields differ by eps32 * cond(M) ~ 1e-2")
+    res = {}
+    y = None
+    ops = (nsl.MatrixCombination.M, nsl.MatrixCombination.Mdagger)
+    std = math.sqrt(beta / Nt * 3.0)
+    for mode in ("1", "0"):
+        monkeypatch.setenv("NSLX_CHUNKPROD", mode)
+        fm, ofs = make(kind, size, Nt, beta, nfield=nf, seed=1600, std=std)
+        if y is None:
+            y = torch.as_tensor(rng.complex_normal(1601, 0, (2 * nf, Nt, fm.Nx)))
+        yd = y.to(DEV) if prec == nsl.F64 else y.to(torch.complex64).to(DEV)
+        xs, classes = _prof_classes(lambda: [fm.solve_direct(op, yd, precision=prec) for op in ops])
+        name = "chunkprod_f64" if prec == nsl.F64 else "chunkprod_f32"
+        assert (name in classes) == (mode == "1"), classes
+        res[mode] = [x.to("cpu", torch.complex128) for x in xs]
+        if mode == "1" and prec == nsl.F64 and fm.Nx <= 72:
+            for x, nm in zip(res[mode], ("M", "Mdagger")):
+                assert rel(oracle_apply(ofs, nm, x), y) < 1e-9, nm
+    tol = 1e-10 if prec == nsl.F64 else 1e-3
+    for a, b in zip(res["1"], res["0"]):
+        assert rel(a, b) < tol, rel(a, b)
