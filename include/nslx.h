@@ -384,6 +384,23 @@ int nslx_correlator_two_point(nslx_fm fm, int64_t n_tsrc, double eps, int64_t ma
                               int precision, int64_t max_rhs, void* corr, nslx_solve_report* report,
                               void* stream);
 
+/* The measurement loop of TwoPointCorrelator::measure()
+ * (twoPointCorrelationFunction.tpp:241-315) computes measure(N) once per
+ * configuration of an ensemble; here every populated field of a HubbardExp
+ * fm is one configuration and all of them are measured together: the point
+ * sources of every field go through one time-slice-preconditioned CG on
+ * MMdagger (nslx_solve_tss: one factorisation per field, every RHS to
+ * |b - MM^dag x| <= tol |b| in double-double), then G = Mdagger X, re-centred
+ * and averaged per field as measure(N) does.  corr: device c128 (nfield,
+ * nt, nx, nx).  max_rhs bounds the RHS of one solve (<= 0: 8192; one time
+ * source of every field at least).  precond_precision NSLX_F64 or NSLX_F32.
+ * No counterpart in the reference (which measures one configuration at a
+ * time; the loop over the stored configurations is nsl_amd.measure_ensemble).
+ * Synchronous. */
+int nslx_correlator_two_point_fields(nslx_fm fm, int64_t n_tsrc, double tol, int64_t maxit,
+                                     int precond_precision, int64_t max_rhs, void* corr,
+                                     nslx_solve_report* report, void* stream);
+
 /* TwoPointCorrelator::measureK(k, N) (twoPointCorrelationFunction.tpp:169-233)
  * for the k-th set of wall sources: wall = wallSources[k], device c128
  * (nw, nx).  For tsrc = 0, Nt/N, ... < Nt: S[s][tsrc][x] = wall[s][x],

@@ -156,6 +156,8 @@ PROTOS = {
     "nslx_ctx_set_tile": (c_int, [c_void_p, c_int, c_int, c_int, c_int64, c_int]),
     "nslx_ctx_get_tile": (c_int, [c_void_p, c_int, c_int, c_int, c_int64, ctypes.POINTER(c_int)]),
     "nslx_fm_tune": (c_int, [c_void_p, c_int, c_int64, c_int, ctypes.POINTER(c_int), P_double, c_void_p]),
+    "nslx_correlator_two_point_fields": (c_int, [c_void_p, c_int64, c_double, c_int64, c_int, c_int64, c_void_p,
+                                                 ctypes.POINTER(SolveReport), c_void_p]),
     "nslx_prof_enable": (c_int, [c_int]),
     "nslx_prof_reset": (c_int, []),
     "nslx_prof_read": (c_int, [ctypes.c_char_p, c_int64]),

@@ -403,6 +403,16 @@ int nslx_correlator_two_point(nslx_fm fm, int64_t n_tsrc, double eps, int64_t ma
     });
 }
 
+int nslx_correlator_two_point_fields(nslx_fm fm, int64_t n_tsrc, double tol, int64_t maxit, int precond_precision,
+                                     int64_t max_rhs, void* corr, nslx_solve_report* report, void* stream) {
+    return guard([&] {
+        auto& m = FM(fm);
+        DeviceGuard dg(m.ctx->device);
+        return nslx::fm_two_point_fields(m, n_tsrc, tol, maxit, precond_precision, max_rhs, corr, report,
+                                         nslx::as_stream(stream));
+    });
+}
+
 int nslx_correlator_two_point_wall(nslx_fm fm, const void* wall, int64_t nw, int64_t n_tsrc, double eps,
                                    int64_t maxit, int tol_kind, int precision, void* corrK, void* block,
                                    nslx_solve_report* report, void* stream) {

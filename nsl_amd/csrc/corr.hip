@@ -158,4 +158,67 @@ int fm_two_point(FermionMatrix& fm, int64_t ntsrc, double eps, int64_t maxit, in
     return status;
 }
 
+// The same measurement for every populated field of fm at once (an ensemble
+// of configurations): the point sources of a group of time sources are laid
+// out per field (RHS f * ng * nx + g * nx + x uses field f), all fields'
+// systems MM^dag X = S go through one time-slice-preconditioned solve (one
+// factorisation per field, double-double correction to tol per RHS), then
+// G = M^dag X and each field's re-centred G is added to its own corr.
+int fm_two_point_fields(FermionMatrix& fm, int64_t ntsrc, double tol, int64_t maxit, int pprec, int64_t max_rhs,
+                        void* corr_out, nslx_solve_report* rep, hipStream_t s) {
+    fm.require_populated();
+    NSLX_REQUIRE(corr_out, "two_point_fields: null output");
+    NSLX_REQUIRE(ntsrc >= 1 && ntsrc <= fm.nt, "two_point_fields: need 1 <= number of time sources <= Nt");
+    NSLX_REQUIRE(fm.disc == DISC_EXP, "two_point_fields: HubbardExp only (the time-slice solver)");
+    const int nt = fm.nt, nx = fm.nx;
+    const int64_t nf = fm.nfield;
+    const int step = (int)(nt / ntsrc);
+    const int nsrc = (nt + step - 1) / step;
+    const int64_t cap = max_rhs > 0 ? max_rhs : 8192;
+    const int group = (int)std::max<int64_t>(1, std::min<int64_t>(nsrc, cap / (nf * nx)));
+    const int64_t len = fm.len();
+    const size_t vbytes = (size_t)nf * group * nx * len;
+    DevBuf<double2> S(vbytes), X(vbytes), G(vbytes);
+    double2* corr = static_cast<double2*>(corr_out);
+    const int64_t ncorr = (int64_t)nt * nx * nx;
+    hipLaunchKernelGGL(k_fill0, dim3(grid_for(nf * ncorr)), dim3(256), 0, s, corr, nf * ncorr);
+    NSLX_LAUNCH_CHECK();
+    nslx_solve_report agg{};
+    int status = NSLX_OK;
+    for (int g0 = 0; g0 < nsrc; g0 += group) {
+        const int ng = std::min(group, nsrc - g0);
+        const int64_t rpf = (int64_t)ng * nx;          // RHS per field
+        const int64_t nb = nf * rpf;
+        const int tsrc0 = g0 * step;
+        for (int64_t f = 0; f < nf; ++f) {
+            const int64_t n = rpf * len;
+            hipLaunchKernelGGL(k_sources, dim3(grid_for(n)), dim3(256), 0, s, S.p + f * n, ng, tsrc0, step, nt, nx);
+            NSLX_LAUNCH_CHECK();
+        }
+        nslx_solve_report r{};
+        const int st = solve_tss(fm, NSLX_OP_MMDAGGER, S.p, X.p, nullptr, nullptr, nb, tol, maxit, pprec, 0, &r, s);
+        if (st != NSLX_OK) status = st;
+        fm.apply(NSLX_OP_MDAGGER, X.p, G.p, nb, NSLX_F64, s);
+        for (int64_t f = 0; f < nf; ++f) {
+            hipLaunchKernelGGL(k_accumulate, dim3(grid_for(ncorr)), dim3(256), 0, s, corr + f * ncorr,
+                               G.p + f * rpf * len, ng, tsrc0, step, nt, nx, 1.0);
+            NSLX_LAUNCH_CHECK();
+        }
+        agg.iterations = std::max(agg.iterations, r.iterations);
+        agg.iterations_total += r.iterations_total;
+        agg.outer_steps += r.outer_steps;
+        agg.converged_rhs += r.converged_rhs;
+        agg.rsqr += r.rsqr;
+        agg.max_rel_residual = std::max(agg.max_rel_residual, r.max_rel_residual);
+        agg.seconds += r.seconds;
+        agg.rhs_iters_low += r.rhs_iters_low;
+        agg.rhs_iters_f64 += r.rhs_iters_f64;
+    }
+    hipLaunchKernelGGL(k_scale, dim3(grid_for(nf * ncorr)), dim3(256), 0, s, corr, nf * ncorr, 1.0 / (double)ntsrc);
+    NSLX_LAUNCH_CHECK();
+    NSLX_HIP(hipStreamSynchronize(s));   // S, X, G are freed on return
+    if (rep) *rep = agg;
+    return status;
+}
+
 }  // namespace nslx

@@ -12,6 +12,13 @@ namespace nslx {
 int fm_two_point(FermionMatrix& fm, int64_t ntsrc, double eps, int64_t maxit, int tol_kind, int prec, int64_t max_rhs,
                  void* corr_out, nslx_solve_report* rep, hipStream_t s);
 
+// measure(N) for every populated field of fm (nfield configurations at
+// once), solved by the time-slice-preconditioned CG to tol per RHS:
+// corr_out (nfield, nt, nx, nx).  max_rhs bounds the RHS of one solve
+// (<= 0: 8192; at least one time source per solve).
+int fm_two_point_fields(FermionMatrix& fm, int64_t ntsrc, double tol, int64_t maxit, int pprec, int64_t max_rhs,
+                        void* corr_out, nslx_solve_report* rep, hipStream_t s);
+
 // TwoPointCorrelator::measureK(k, N) for one set of nw wall sources
 // (wall: device c128 nw x nx): corrK_out (nw, nt, nx) and block_out
 // (nt, nw, nw), either may be null (corr_wall.hip).
