@@ -367,6 +367,58 @@ def serving_leg(prec, tol=1e-10, workers=0, rate=SERVE_RATE, subset=None, maxit=
             "max_rel_residual": max(r["max_rel_residual"] for r in res)}
 
 
+def hmc_leg(ntraj=100, nforce=3):
+    """The HMC side of the path: (a) the CPU-runnable configuration (honeycomb
+    3x3, Nt 16, beta 4, U 2): ntraj leapfrog trajectories of length 1 with 10
+    steps (each 11 force evaluations of both species, log det at both ends,
+    Metropolis in FP64) from a Philox field, time per trajectory; (b) one
+    force evaluation (gradLogDetM of both species) and one log det at the
+    headline shapes (Nx 288, Nt 64, beta 8, U 3), median of nforce, with the
+    kernel classes timed."""
+    import torch
+    import nsl_amd as nsl
+    out = {}
+    A = nsl.HubbardAction(nsl.Lattice.honeycomb(3, 3), 16, 4.0, 2.0)
+    phi = torch.as_tensor(nsl.real_field(1000, (16, 18), math.sqrt(4.0 / 16 * 2.0))).cuda()
+    A.chain(phi, 3, 1.0, 10, 2999)                       # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = A.chain(phi, ntraj, 1.0, 10, 3000)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out["cfg0"] = {"trajectories": ntraj, "ms_per_trajectory": round(dt / ntraj * 1e3, 3),
+                   "acceptance": round(sum(int(r["accepted"]) for r in reps) / ntraj, 3),
+                   "workload": "honeycomb 3x3 (Nx 18), Nt 16, beta 4, U 2, trajectory length 1, 10 leapfrog steps"}
+    A = nsl.HubbardAction(nsl.Lattice.honeycomb(L, L), NT, BETA, U)
+    phi = torch.as_tensor(nsl.real_field(1000, (NT, 2 * L * L), math.sqrt(BETA / NT * U))).cuda()
+    A.force(phi)                                          # warm-up (E memo)
+    L_ = nsl._lib.lib()
+    ft, lt = [], []
+    L_.nslx_prof_reset()
+    L_.nslx_prof_enable(1)
+    for _ in range(nforce):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        A.force(phi)
+        torch.cuda.synchronize()
+        ft.append(time.perf_counter() - t0)
+    L_.nslx_prof_enable(0)
+    text = prof_text_of(L_)
+    A.fm.populate(phi)
+    for _ in range(nforce):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        A.fm.logDetM()
+        torch.cuda.synchronize()
+        lt.append(time.perf_counter() - t0)
+    out["nx288"] = {"force_ms": round(statistics.median(ft) * 1e3, 3),
+                    "logdet_ms": round(statistics.median(lt) * 1e3, 3),
+                    "force_kernels": kernel_table(parse_prof(text), top=5),
+                    "workload": "honeycomb 12x12 (Nx 288), Nt 64, beta 8, U 3; force = gradLogDetM of both species "
+                                "+ gauge force; log det of one species"}
+    return out
+
+
 def prof_text_of(L_):
     buf = ctypes.create_string_buffer(1 << 16)
     L_.nslx_prof_read(buf, len(buf))
@@ -476,6 +528,7 @@ def main():
     ap.add_argument("--no-refcrit", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-hmc", action="store_true")
     ap.add_argument("--hops", type=int, default=0)
     ap.add_argument("--tss", type=int, default=0)
     ap.add_argument("--precond", choices=["f64", "f32"], default="f64",
@@ -591,6 +644,8 @@ def main():
                              "headline_over_this": round(out["value"] / (nb * args.steps / tt), 3)}
     if not args.no_refcrit:
         out["refcrit"] = refcrit_leg(phi_h, b_h, args.refcrit_fields)
+    if not args.no_hmc:
+        out["hmc"] = hmc_leg()
     if not args.no_batch:
         out["batching"] = batching_leg()
     if not args.no_serve:

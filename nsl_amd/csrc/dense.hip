@@ -304,9 +304,65 @@ __global__ void k_dscale_rows(int n, const double2* dl, const double* E, double2
     out[i] = make_double2(d.x * e, d.y * e);
 }
 
+__global__ void k_conj(int64_t m, const double2* in, double2* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) out[i] = make_double2(in[i].x, -in[i].y);
+}
+
+__global__ void k_dscale_cols(int n, const double* E, const double2* dr, double2* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)n * n) return;
+    const double2 d = dr[i % n];
+    const double e = E[i];
+    out[i] = make_double2(d.x * e, d.y * e);
+}
+
+__global__ void k_transpose_c(int n, const double2* A, double2* AT) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)n * n) return;
+    const int r = (int)(i / n), c = (int)(i % n);
+    AT[(size_t)c * n + r] = A[i];
+}
+
+// d[x] = i * sum_k QT[k][x] G[k][x]: one thread per x, consecutive threads
+// reading consecutive elements of each row
+__global__ void k_diag_iprod_cols(int n, const double2* QT, const double2* G, double2* d) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    double r = 0.0, im = 0.0;
+    for (int k = 0; k < n; ++k) {
+        const double2 a = QT[(size_t)k * n + x], b = G[(size_t)k * n + x];
+        r += a.x * b.x - a.y * b.y;
+        im += a.x * b.y + a.y * b.x;
+    }
+    d[x] = make_double2(-im, r);
+}
+
 inline unsigned g1(int64_t m) { return (unsigned)((m + 255) / 256); }
 
 }  // namespace
+
+void conj_c(int64_t m, const void* in, void* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_conj, dim3(g1(m)), dim3(256), 0, s, m, (const double2*)in, (double2*)out);
+    NSLX_LAUNCH_CHECK();
+}
+
+void dscale_cols(int n, const double* E, const void* dr, void* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_dscale_cols, dim3(g1((int64_t)n * n)), dim3(256), 0, s, n, E, (const double2*)dr,
+                       (double2*)out);
+    NSLX_LAUNCH_CHECK();
+}
+
+void transpose_c(int n, const void* A, void* AT, hipStream_t s) {
+    hipLaunchKernelGGL(k_transpose_c, dim3(g1((int64_t)n * n)), dim3(256), 0, s, n, (const double2*)A, (double2*)AT);
+    NSLX_LAUNCH_CHECK();
+}
+
+void diag_iprod_cols(int n, const void* QT, const void* G, void* d, hipStream_t s) {
+    hipLaunchKernelGGL(k_diag_iprod_cols, dim3(g1(n)), dim3(256), 0, s, n, (const double2*)QT, (const double2*)G,
+                       (double2*)d);
+    NSLX_LAUNCH_CHECK();
+}
 
 void dscale_rows(int n, const void* dl, const double* E, void* out, hipStream_t s) {
     hipLaunchKernelGGL(k_dscale_rows, dim3(g1((int64_t)n * n)), dim3(256), 0, s, n, (const double2*)dl, E, (double2*)out);

@@ -44,5 +44,13 @@ void add_identity_c(int n, void* A, hipStream_t s);
 void scaled_plus_identity(int n, const double* K, double a, double id, double* out, hipStream_t s);
 // out[i][j] = dl[i] * E[i][j] (E real, out complex).
 void dscale_rows(int n, const void* dl, const double* E, void* out, hipStream_t s);
+// out[i][j] = E[i][j] * dr[j] (E real, out complex).
+void dscale_cols(int n, const double* E, const void* dr, void* out, hipStream_t s);
+// out = conj(in), m complex values.
+void conj_c(int64_t m, const void* in, void* out, hipStream_t s);
+// AT = A^T (complex n x n, no aliasing).
+void transpose_c(int n, const void* A, void* AT, hipStream_t s);
+// d[x] = i * sum_k QT[k][x] G[k][x]  (the diagonal of i * QT^T G), complex.
+void diag_iprod_cols(int n, const void* QT, const void* G, void* d, hipStream_t s);
 
 }  // namespace nslx

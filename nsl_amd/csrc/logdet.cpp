@@ -20,6 +20,14 @@
 // log det(1 + fk_0 fk_1 ... fk_{Nt-1}), fk_t = D diag(Pinv_{t+1}) (the
 // product built left to right from t = 0, no mu term, as the reference);
 // gradLogDetM with Fk_t = diag(Pinv_{t-1}) D.
+//
+// The Nx x Nx products run on the FP64 matrix cores as launches of the
+// batched GEMM (bgemm.hip): every chunk's product advances one slice per
+// launch (all chunks in one launch), and the force's two sequential chains
+// are one launch per slice each.  Both use that E (and D) are symmetric
+// (K is), so E[k][c] = E[c][k]: a product X E diag(p) is the GEMM's
+// Y = X B^T with B = E and the phase on the output columns.
+#include <algorithm>
 #include <cmath>
 
 #include "engine.hpp"
@@ -55,30 +63,51 @@ void fm_logdet(FermionMatrix& fm, int64_t field, int nchunks, double* out2, hipS
     EEntry& e = diag ? fm.Elin(-fm.sgn() * fm.delta(), 1.0, s) : fm.E(fm.sgn() * fm.delta(), s);
     const double2* P = (diag ? fm.Pinv64.p : fm.P64.p) + field * fm.len();
     const size_t m = (size_t)nx * nx;
-    DevBuf<double2> C(m * nchunks), tmp(m), cur(m);
+    // chunk j covers slices [t_lo_j, t_hi_j); its product starts at I and
+    // takes one slice per launch: HubbardExp cur = cur E diag(P_{t+1}) for t
+    // descending (sausage.mat_mul(prod[t])), HubbardDiag cur = cur D
+    // diag(Pinv_{t+1}) for t ascending (hubbardDiag.tpp:176-182).  The GEMM's
+    // output-column phase is conjugated, so it reads conj(P).
+    std::vector<int> tlo(nchunks), len(nchunks);
+    int maxlen = 0;
     for (int j = 0; j < nchunks; ++j) {
-        const int t_lo = (int)std::llround((double)j * nt / nchunks);
-        const int t_hi = (int)std::llround((double)(j + 1) * nt / nchunks);
-        set_identity_c(nx, cur.p, s);
-        if (diag) {
-            // cur = cur * D * diag(Pinv_{t+1}) for t ascending (hubbardDiag.tpp:176-182)
-            for (int t = t_lo; t < t_hi; ++t) {
-                const double2* Pn = P + (size_t)((t + 1) % nt) * nx;
-                gemm_sq(nx, cur.p, 1, e.dense.p, 0, tmp.p, nullptr, nullptr, Pn, 1.0, 0.0, s);
-                std::swap(cur.p, tmp.p);
-            }
-        } else {
-            for (int t = t_hi - 1; t >= t_lo; --t) {
-                // cur = cur * E * diag(P_{t+1})   (sausage.mat_mul(prod[t]))
-                const double2* Pn = P + (size_t)((t + 1) % nt) * nx;
-                gemm_sq(nx, cur.p, 1, e.dense.p, 0, tmp.p, nullptr, nullptr, Pn, 1.0, 0.0, s);
-                std::swap(cur.p, tmp.p);
-            }
+        tlo[j] = (int)std::llround((double)j * nt / nchunks);
+        len[j] = (int)std::llround((double)(j + 1) * nt / nchunks) - tlo[j];
+        maxlen = std::max(maxlen, len[j]);
+    }
+    DevBuf<double2> buf[2] = {DevBuf<double2>(m * nchunks), DevBuf<double2>(m * nchunks)};
+    DevBuf<double2> Pc((size_t)fm.len());
+    conj_c(fm.len(), P, Pc.p, s);
+    for (int j = 0; j < nchunks; ++j) set_identity_c(nx, buf[0].p + (size_t)j * m, s);
+    BPlan plan;
+    for (int q = 0; q < maxlen; ++q) {
+        std::vector<BJob> jobs;
+        for (int j = 0; j < nchunks; ++j) {
+            if (q >= len[j]) continue;
+            const int t = diag ? tlo[j] + q : tlo[j] + len[j] - 1 - q;
+            BJob b;
+            b.A0 = buf[q & 1].p + (size_t)j * m;
+            b.lda0 = nx;
+            b.rows = b.cols = b.K = nx;
+            b.B = e.dense.p;
+            b.ldb = nx;
+            b.Y = buf[(q + 1) & 1].p + (size_t)j * m;
+            b.ldy = nx;
+            b.ph = Pc.p + (size_t)((t + 1) % nt) * nx;
+            b.ldph = 0;
+            jobs.push_back(b);
         }
-        // build_cyclic forms det(1 + C_{n-1} ... C_0): the ascending Diag
-        // product C_0 ... C_{n-1} is stored in reverse chunk order
+        plan.gemm(jobs, BVariant{NSLX_F64, false, 2, 0});
+    }
+    plan.upload(s);
+    plan.run(s);
+    // build_cyclic forms det(1 + C_{n-1} ... C_0): the ascending Diag
+    // product C_0 ... C_{n-1} is stored in reverse chunk order
+    DevBuf<double2> C(m * nchunks);
+    for (int j = 0; j < nchunks; ++j) {
         const int slot = diag ? nchunks - 1 - j : j;
-        NSLX_HIP(hipMemcpyAsync(C.p + (size_t)slot * m, cur.p, m * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        NSLX_HIP(hipMemcpyAsync(C.p + (size_t)slot * m, buf[len[j] & 1].p + (size_t)j * m, m * sizeof(double2),
+                                hipMemcpyDeviceToDevice, s));
     }
     const int N = nx * nchunks;
     DevBuf<double2> R((size_t)N * N);
@@ -115,24 +144,62 @@ void fm_grad_logdet(FermionMatrix& fm, int64_t field, void* out, hipStream_t s) 
     const double2* Pi = fm.Pinv64.p + field * fm.len();
     auto Pprev = [&](int t) { return Pi + (size_t)((t - 1 + nt) % nt) * nx; };   // Pinv_{t-1}
     const size_t m = (size_t)nx * nx;
-    DevBuf<double2> Q(m * nt), G(m), tmp(m), aug(2 * m), work((size_t)nx);
-    // Q_{Nt-1} = Fk_{Nt-1}
-    dscale_rows(nx, Pprev(nt - 1), en.dense.p, Q.p + (size_t)(nt - 1) * m, s);
-    for (int t = nt - 2; t >= 0; --t)
-        gemm_sq(nx, en.dense.p, 0, Q.p + (size_t)(t + 1) * m, 1, Q.p + (size_t)t * m, Pprev(t), nullptr, nullptr, 1.0,
-                0.0, s);
-    NSLX_HIP(hipMemcpyAsync(G.p, Q.p, m * sizeof(double2), hipMemcpyDeviceToDevice, s));
-    add_identity_c(nx, G.p, s);
-    gj_inverse(nx, G.p, aug.p, work.p, s);
-    double2* pi = reinterpret_cast<double2*>(out);
-    diag_iprod(nx, Q.p, G.p, pi + (size_t)(nt - 1) * nx, s);
-    for (int t = 0; t < nt - 1; ++t) {
-        // G = G diag(Pinv_{t-1}) E(-s delta)
-        gemm_sq(nx, G.p, 1, en.dense.p, 0, tmp.p, nullptr, Pprev(t), nullptr, 1.0, 0.0, s);
-        std::swap(G.p, tmp.p);
-        diag_iprod(nx, Q.p + (size_t)(t + 1) * m, G.p, pi + (size_t)t * nx, s);
+    // The chain is kept transposed, QT_t = Q_t^T = QT_{t+1} E diag(Pinv_{t-1})
+    // (E symmetric): a GEMM with the phase on its output columns (given
+    // conjugated).  diag(Q_{t+1} G)[x] = sum_k QT_{t+1}[k][x] G[k][x].
+    DevBuf<double2> QT(m * nt), G(m), G2(m), H(m), aug(2 * m), work((size_t)nx), Pic((size_t)fm.len());
+    conj_c(fm.len(), Pi, Pic.p, s);
+    auto Pcprev = [&](int t) { return Pic.p + (size_t)((t - 1 + nt) % nt) * nx; };
+    const double* E = en.dense.p;
+    // QT_{Nt-1} = Fk_{Nt-1}^T = E diag(Pinv_{Nt-2})
+    dscale_cols(nx, E, Pprev(nt - 1), QT.p + (size_t)(nt - 1) * m, s);
+    BPlan chain_q;
+    for (int t = nt - 2; t >= 0; --t) {
+        BJob b;
+        b.A0 = QT.p + (size_t)(t + 1) * m;
+        b.lda0 = nx;
+        b.rows = b.cols = b.K = nx;
+        b.B = E;
+        b.ldb = nx;
+        b.Y = QT.p + (size_t)t * m;
+        b.ldy = nx;
+        b.ph = Pcprev(t);
+        b.ldph = 0;
+        chain_q.gemm({b}, BVariant{NSLX_F64, false, 2, 0});
     }
-    NSLX_HIP(hipStreamSynchronize(s));   // temporaries are freed on return
+    chain_q.upload(s);
+    chain_q.run(s);
+    // G = (1 + Q_0)^{-1} = ((1 + QT_0)^{-1})^T
+    NSLX_HIP(hipMemcpyAsync(H.p, QT.p, m * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    add_identity_c(nx, H.p, s);
+    gj_inverse(nx, H.p, aug.p, work.p, s);
+    transpose_c(nx, H.p, G.p, s);
+    double2* pi = reinterpret_cast<double2*>(out);
+    diag_iprod_cols(nx, QT.p, G.p, pi + (size_t)(nt - 1) * nx, s);
+    // G <- G diag(Pinv_{t-1}) E(-s delta) (the phase on the GEMM's A columns),
+    // pi_t = i diag(Q_{t+1} G)
+    BPlan chain_g;
+    double2* gb[2] = {G.p, G2.p};
+    for (int t = 0; t < nt - 1; ++t) {
+        BJob b;
+        b.A0 = gb[t & 1];
+        b.lda0 = nx;
+        b.rows = b.cols = b.K = nx;
+        b.B = E;
+        b.ldb = nx;
+        b.Y = gb[(t + 1) & 1];
+        b.ldy = nx;
+        b.ph = Pprev(t);
+        b.ldph = 0;
+        chain_g.gemm({b}, BVariant{NSLX_F64, false, 1, 0});
+        const double2* Gt = gb[(t + 1) & 1];
+        const double2* Qn = QT.p + (size_t)(t + 1) * m;
+        double2* dst = pi + (size_t)t * nx;
+        chain_g.custom([=](hipStream_t st) { diag_iprod_cols(nx, Qn, Gt, dst, st); });
+    }
+    chain_g.upload(s);
+    chain_g.run(s);
+    NSLX_HIP(hipStreamSynchronize(s));   // temporaries and the plans' device arrays are freed on return
 }
 
 }  // namespace nslx
