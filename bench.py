@@ -419,6 +419,47 @@ def hmc_leg(ntraj=100, nforce=3):
     return out
 
 
+def bicgstab_leg(nsolves=100, cpu=True):
+    """The reference's BiCGStab example (Executables/Examples/
+    example_BiCGStab.cpp:86-104) on the headline lattice: phi = 0,
+    BiCGStab(M, MMdagger, 1e-10) on a point source b at (t, x) = (0, 0),
+    nsolves solves timed; the device solver keeps every scalar on the GPU.
+    Beside it the oracle's BiCGStab (the reference's recurrence on libtorch
+    CPU, all host threads) on the same system, one solve."""
+    import torch
+    import nsl_amd as nsl
+    Nx = 2 * L * L
+    fm = nsl.HubbardExp(nsl.Lattice.honeycomb(L, L), NT, BETA)
+    fm.populate(torch.zeros((NT, Nx), dtype=torch.complex128, device="cuda"))
+    b = torch.zeros((NT, Nx), dtype=torch.complex128, device="cuda")
+    b[0, 0] = 1.0
+    bi = nsl.BiCGStab(fm, nsl.MMdagger, 1e-10, 10000)
+    x = bi(b)                                              # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nsolves):
+        x = bi(b)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / nsolves
+    r = b - fm.MMdagger(x)
+    out = {"workload": "honeycomb 12x12 (Nx 288), Nt 64, beta 8, phi = 0, point source, BiCGStab on MMdagger, "
+                       "eps 1e-10 (the reference's |r|^2 <= eps^2 stop)",
+           "ms_per_solve": round(dt * 1e3, 3), "iterations": bi.report["iterations"], "converged": bool(bi.converged),
+           "true_residual": float(r.abs().pow(2).sum().sqrt())}
+    if cpu:
+        from oracle import fermion as F, lattice as olat, solvers as osol
+        torch.set_num_threads(os.cpu_count() or 1)
+        o = F.HubbardExp(olat.honeycomb(L, L), NT, BETA)
+        o.populate(torch.zeros((NT, Nx), dtype=torch.complex128))
+        bc = b.cpu()
+        t0 = time.perf_counter()
+        xo, info = osol.bicgstab(o.MMdagger, bc, 1e-10, 10000)
+        out["oracle_cpu_ms_per_solve"] = round((time.perf_counter() - t0) * 1e3, 1)
+        out["oracle_iterations"] = info["iterations"]
+        out["max_abs_diff_vs_oracle"] = float((x.cpu() - xo).abs().max())
+    return out
+
+
 def prof_text_of(L_):
     buf = ctypes.create_string_buffer(1 << 16)
     L_.nslx_prof_read(buf, len(buf))
@@ -646,6 +687,7 @@ def main():
         out["refcrit"] = refcrit_leg(phi_h, b_h, args.refcrit_fields)
     if not args.no_hmc:
         out["hmc"] = hmc_leg()
+        out["bicgstab"] = bicgstab_leg(cpu=not args.no_cpu)
     if not args.no_batch:
         out["batching"] = batching_leg()
     if not args.no_serve:

@@ -140,6 +140,12 @@ int solve_dd(FermionMatrix& fm, int op, const void* b, void* x, void* xlo, const
 int solve_precond(FermionMatrix& fm, FermionMatrix& fp, int op, const void* b, void* x, const void* x0, int64_t nb,
                   double tol, int tol_kind, int64_t maxit, int inner_prec, nslx_solve_report* rep, hipStream_t s);
 
+// CG (for M, M^dag) and BiCGStab (any op) in FP64 with every recurrence
+// scalar on the device (krylov.hip); the reference's semantics with
+// NSLX_TOL_ABS_BATCH, per-RHS scalars and stops otherwise.
+int solve_generic_dev(FermionMatrix& fm, int op, int method, const void* b, void* x, const void* x0, int64_t nb,
+                      double tol, int tol_kind, int64_t maxit, nslx_solve_report* rep, hipStream_t s);
+
 // CG on MMdagger preconditioned by the time-slice solve in precision pprec
 // (FP64 / FP32), double-double defect correction to |b - A x| <= tol |b| per
 // RHS; x = high part, xlo (device c128 or null) = low part (tss_pcg.hip)

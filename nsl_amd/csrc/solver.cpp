@@ -512,126 +512,6 @@ inline cd cdiv(cd a, cd b) {
     return {(a.first * b.first + a.second * b.second) / d, (a.second * b.first - a.first * b.second) / d};
 }
 
-int solve_generic(FermionMatrix& fm, int op, int method, const void* b, void* x, const void* x0, int64_t nb, double tol,
-                  int tol_kind, int64_t maxit, nslx_solve_report* rep, hipStream_t s) {
-    SolverWork& w = fm.sw;
-    const int64_t len = fm.len();
-    const size_t bytes = (size_t)nb * len * 16;
-    const int global = tol_kind == NSLX_TOL_ABS_BATCH;
-    const int64_t nslot = global ? 1 : nb;
-    w.ensure_scalars(nb, 1, s);
-    w.r.ensure(bytes); w.p0.ensure(bytes); w.u.ensure(bytes);
-    w.v1.ensure(bytes); w.v2.ensure(bytes); w.v3.ensure(bytes); w.v4.ensure(bytes);
-    HostDots hd{fm, nb, global, s};
-    std::vector<double> t2(nslot);
-    if (global) t2[0] = tol * tol;
-    else {
-        auto bb = hd.dot(b, b);
-        for (int64_t i = 0; i < nb; ++i) t2[i] = tol * tol * bb[i].first;
-    }
-    copy_dev(x, x0 ? x0 : b, bytes, s);
-    // done flags on the device for axpy masking
-    std::vector<int> done(nslot, -1);
-    auto push_done = [&]() {
-        for (int64_t i = 0; i < nslot; ++i) w.int_h.p[i] = done[i];
-        NSLX_HIP(hipMemcpyAsync(w.done.p, w.int_h.p, nslot * sizeof(int), hipMemcpyHostToDevice, s));
-        NSLX_HIP(hipStreamSynchronize(s));
-    };
-    const std::vector<cd> one(nslot, cd{1.0, 0.0});
-    // r = b - A x
-    fm.apply(op, x, w.u.p, nb, NSLX_F64, s);
-    push_done();
-    hd.axpy(b, w.u.p, w.r.p, one, -1.0, nullptr);
-    auto rr = hd.dot(w.r.p, w.r.p);
-    int64_t active = 0;
-    for (int64_t i = 0; i < nslot; ++i) { if (rr[i].first <= t2[i]) done[i] = 0; else ++active; }
-    push_done();
-    int64_t it = 0;
-    if (method == NSLX_CG) {
-        void* p = w.p0.p;
-        void* t = w.u.p;
-        copy_dev(p, w.r.p, bytes, s);
-        while (active > 0 && it < maxit) {
-            ++it;
-            fm.apply(op, p, t, nb, NSLX_F64, s);
-            auto pt = hd.dot(p, t);
-            std::vector<cd> al(nslot);
-            for (int64_t i = 0; i < nslot; ++i) al[i] = cdiv(cd{rr[i].first, 0.0}, pt[i]);
-            hd.axpy(x, p, x, al, 1.0, w.done.p);
-            hd.axpy(w.r.p, t, w.r.p, al, -1.0, w.done.p);
-            auto rn = hd.dot(w.r.p, w.r.p);
-            std::vector<cd> be(nslot);
-            for (int64_t i = 0; i < nslot; ++i) {
-                if (done[i] >= 0) { be[i] = {0, 0}; continue; }
-                be[i] = {rn[i].first / rr[i].first, 0.0};
-                rr[i] = rn[i];
-                if (rn[i].first <= t2[i]) { done[i] = (int)it; --active; }
-            }
-            // p = r + beta p for still-active slots (done slots are frozen)
-            hd.axpy(w.r.p, p, p, be, 1.0, w.done.p);
-            push_done();
-        }
-    } else {
-        // BiCGStab.tpp:38-117: y = s = r; As = A s; delta = <y,r>; phi = <y,As>/delta
-        void* y = w.v1.p; void* sv = w.v2.p; void* As = w.v3.p; void* wv = w.v4.p; void* Aw = w.u.p;
-        copy_dev(y, w.r.p, bytes, s);
-        copy_dev(sv, w.r.p, bytes, s);
-        fm.apply(op, sv, As, nb, NSLX_F64, s);
-        auto delta = hd.dot(y, w.r.p);
-        auto yAs = hd.dot(y, As);
-        std::vector<cd> phi(nslot);
-        for (int64_t i = 0; i < nslot; ++i) phi[i] = cdiv(yAs[i], delta[i]);
-        while (active > 0 && it < maxit) {
-            ++it;
-            std::vector<cd> om(nslot);
-            for (int64_t i = 0; i < nslot; ++i) om[i] = cdiv(cd{1.0, 0.0}, phi[i]);
-            hd.axpy(w.r.p, As, wv, om, -1.0, w.done.p);             // w = r - omega As
-            fm.apply(op, wv, Aw, nb, NSLX_F64, s);
-            auto aww = hd.dot(Aw, wv);
-            auto awaw = hd.dot(Aw, Aw);
-            std::vector<cd> chi(nslot);
-            for (int64_t i = 0; i < nslot; ++i) chi[i] = cdiv(aww[i], cd{awaw[i].first, 0.0});
-            hd.axpy(wv, Aw, w.r.p, chi, -1.0, w.done.p);            // r = w - chi Aw
-            hd.axpy(x, sv, x, om, 1.0, w.done.p);                   // x += omega s
-            hd.axpy(x, wv, x, chi, 1.0, w.done.p);                  // x += chi w
-            auto dp1 = hd.dot(y, w.r.p);
-            std::vector<cd> psi(nslot), mpsi(nslot), pc(nslot);
-            for (int64_t i = 0; i < nslot; ++i) {
-                psi[i] = cdiv(cmul(cd{-1.0, 0.0}, cmul(om[i], dp1[i])), cmul(delta[i], chi[i]));
-                mpsi[i] = cd{-psi[i].first, -psi[i].second};
-                pc[i] = cmul(psi[i], chi[i]);
-            }
-            // s = r - psi s + psi chi As  (t = s; s = r; s -= psi t; s += psi chi As)
-            hd.axpy(w.r.p, sv, sv, mpsi, 1.0, w.done.p);
-            hd.axpy(sv, As, sv, pc, 1.0, w.done.p);
-            fm.apply(op, sv, As, nb, NSLX_F64, s);
-            auto yAs2 = hd.dot(y, As);
-            auto rn = hd.dot(w.r.p, w.r.p);
-            for (int64_t i = 0; i < nslot; ++i) {
-                if (done[i] >= 0) continue;
-                phi[i] = cdiv(yAs2[i], dp1[i]);
-                delta[i] = dp1[i];
-                rr[i] = rn[i];
-                if (rn[i].first <= t2[i]) { done[i] = (int)it; --active; }
-            }
-            push_done();
-        }
-    }
-    int64_t conv = 0, maxi = 0;
-    for (int d : done) if (d >= 0) { ++conv; maxi = std::max<int64_t>(maxi, d); }
-    const bool all = conv == nslot;
-    if (rep) {
-        rep->iterations = all ? maxi : it;
-        rep->iterations_total = rep->iterations;
-        rep->outer_steps = 0;
-        rep->converged_rhs = global ? (all ? nb : 0) : conv;
-        double t = 0.0;
-        for (auto& v : rr) t += v.first;
-        rep->rsqr = t;
-    }
-    return all ? NSLX_OK : NSLX_NOT_CONVERGED;
-}
-
 }  // namespace
 
 int solve(FermionMatrix& fm, int op, int method, const void* b, void* x, const void* x0, int64_t nb, double tol,
@@ -664,7 +544,7 @@ int solve(FermionMatrix& fm, int op, int method, const void* b, void* x, const v
     } else if (method == NSLX_CG && herm) {
         st = solve_cg_f64(fm, op, b, x, x0, nb, tol, tol_kind, maxit, rep, s);
     } else {
-        st = solve_generic(fm, op, method, b, x, x0, nb, tol, tol_kind, maxit, rep, s);
+        st = solve_generic_dev(fm, op, method, b, x, x0, nb, tol, tol_kind, maxit, rep, s);
     }
     if (check && prec == NSLX_F64) rep->max_rel_residual = true_rel_residual(fm, op, b, x, nb, s);
     NSLX_HIP(hipStreamSynchronize(s));
