@@ -584,14 +584,21 @@ int nslx_hmc_generate(nslx_fm fm, void* phi, int64_t markov_time0, double U, dou
  * variants (ids 0 .. count-1 per precision); the context keeps one choice per
  * (precision, nx, nt, power-of-two class of nbatch), filled by nslx_fm_tune
  * or nslx_ctx_set_tile, else a built-in heuristic.
- * nslx_fm_tune times reps x (M + Mdagger) plain applies of nbatch vectors on
- * fm's populated fields for every tile that fits nx, records the fastest in
- * fm's context and returns it (best_us: mean time of one M + Mdagger pair).
- * Synchronous; allocates two temporary vectors of nbatch fields. */
+ * nslx_fm_tune times, on fm's populated fields and nbatch vectors, either
+ * reps x (M + Mdagger) plain applies for every tile that fits nx
+ * (NSLX_TUNE_APPLY), or reps iterations of the fused CG on MMdagger -- the
+ * CGA and CGB launches the solvers and the batching engine run -- for the
+ * classic tiles 0-3 (NSLX_TUNE_CG), records the fastest in fm's context and
+ * returns it (best_us: mean time of one pair).  The solvers (nslx_solve's
+ * fused CG, nslx_batch_solve) tune in CG mode on their own when the table has
+ * no entry for a block of at least 512 workgroups of the 32 x 96 tile
+ * (environment NSLX_AUTOTUNE=0 turns that off).  Synchronous; allocates
+ * temporary vectors. */
+typedef enum { NSLX_TUNE_APPLY = 0, NSLX_TUNE_CG = 1 } nslx_tune_mode;
 int nslx_tile_count(int precision, int* count);
 int nslx_ctx_set_tile(nslx_ctx ctx, int precision, int nx, int nt, int64_t nbatch, int tile);
 int nslx_ctx_get_tile(nslx_ctx ctx, int precision, int nx, int nt, int64_t nbatch, int* tile);
-int nslx_fm_tune(nslx_fm fm, int precision, int64_t nbatch, int reps, int* best_tile, double* best_us,
+int nslx_fm_tune(nslx_fm fm, int precision, int64_t nbatch, int reps, int mode, int* best_tile, double* best_us,
                  void* stream);
 
 /* ---- kernel timing ----------------------------------------------------------

@@ -197,12 +197,14 @@ class HubbardExp:
         self._phi = phi
         return self
 
-    def tune(self, precision=F64, nbatch=None, reps=10):
-        """Autotune the hop tile for this lattice and nbatch (nslx_fm_tune);
-        returns (tile, microseconds per M + Mdagger pair)."""
+    def tune(self, precision=F64, nbatch=None, reps=10, cg=False):
+        """Autotune the hop tile for this lattice and nbatch (nslx_fm_tune):
+        plain M + Mdagger applies, or with cg=True one fused CG iteration on
+        MMdagger (the CGA and CGB launches, classic tiles); returns (tile,
+        microseconds per pair)."""
         nb = int(nbatch or self.nfield)
         t, us = ctypes.c_int(), ctypes.c_double()
-        _lib.check(_lib.lib().nslx_fm_tune(self._h, int(precision), nb, int(reps), ctypes.byref(t),
+        _lib.check(_lib.lib().nslx_fm_tune(self._h, int(precision), nb, int(reps), 1 if cg else 0, ctypes.byref(t),
                                            ctypes.byref(us), _stream()))
         return t.value, us.value
 

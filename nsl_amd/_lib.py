@@ -155,7 +155,7 @@ PROTOS = {
     "nslx_tile_count": (c_int, [c_int, ctypes.POINTER(c_int)]),
     "nslx_ctx_set_tile": (c_int, [c_void_p, c_int, c_int, c_int, c_int64, c_int]),
     "nslx_ctx_get_tile": (c_int, [c_void_p, c_int, c_int, c_int, c_int64, ctypes.POINTER(c_int)]),
-    "nslx_fm_tune": (c_int, [c_void_p, c_int, c_int64, c_int, ctypes.POINTER(c_int), P_double, c_void_p]),
+    "nslx_fm_tune": (c_int, [c_void_p, c_int, c_int64, c_int, c_int, ctypes.POINTER(c_int), P_double, c_void_p]),
     "nslx_correlator_two_point_fields": (c_int, [c_void_p, c_int64, c_double, c_int64, c_int, c_int64, c_void_p,
                                                  ctypes.POINTER(SolveReport), c_void_p]),
     "nslx_prof_enable": (c_int, [c_int]),

@@ -543,12 +543,12 @@ int nslx_ctx_get_tile(nslx_ctx ctx, int precision, int nx, int nt, int64_t nbatc
     });
 }
 
-int nslx_fm_tune(nslx_fm fm, int precision, int64_t nbatch, int reps, int* best_tile, double* best_us,
+int nslx_fm_tune(nslx_fm fm, int precision, int64_t nbatch, int reps, int mode, int* best_tile, double* best_us,
                  void* stream) {
     return guard([&] {
         auto& m = FM(fm);
         DeviceGuard dg(m.ctx->device);
-        return nslx::fm_tune(m, precision, nbatch, reps, best_tile, best_us, nslx::as_stream(stream));
+        return nslx::fm_tune(m, precision, nbatch, reps, mode, best_tile, best_us, nslx::as_stream(stream));
     });
 }
 

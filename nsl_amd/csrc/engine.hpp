@@ -168,7 +168,11 @@ void noalloc_init(int device);
 void apply_noalloc(FermionMatrix& fm, int op, const void* in, void* out, int64_t nb, int prec, hipStream_t s);
 
 // hop tile autotuner (tune.cpp)
-int fm_tune(FermionMatrix& fm, int prec, int64_t nb, int reps, int* best_tile, double* best_us, hipStream_t s);
+int fm_tune(FermionMatrix& fm, int prec, int64_t nb, int reps, int mode, int* best_tile, double* best_us,
+            hipStream_t s);
+// The tile the table holds for (prec, nx, nt, nb), tuned in CG mode on first
+// use when NSLX_AUTOTUNE is on and the block is large (tune.cpp).
+int tuned_tile(FermionMatrix& fm, int prec, int64_t nb, hipStream_t s);
 
 // set by NSLX_INNER_TOL etc.; see solver.cpp
 double env_double(const char* name, double dflt);

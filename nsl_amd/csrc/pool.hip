@@ -231,7 +231,7 @@ void Pool::start_recurrence(PoolJob& j, bool fresh, bool keep_done) {
     const int64_t wg_mid = (((int64_t)j.nb * fm.nt + 31) / 32) * ((fm.nx + 95) / 96);
     int tile = 2;
     if (wg_mid >= 512) {
-        tile = fm.tile(j.prec, j.nb);
+        tile = tuned_tile(fm, j.prec, j.nb, s_);
         if (tile >= 4 || !tile_fits(j.prec, tile, fm.nx)) tile = default_tile(j.prec, fm.nx, fm.nt, j.nb);
     }
     j.tile = tile;

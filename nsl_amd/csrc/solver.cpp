@@ -74,7 +74,7 @@ int64_t count_active(SolverWork& w, int64_t nslot, hipStream_t s) {
 int64_t cg_fused(FermionMatrix& fm, int prec, int inner, int outer, const void* b, void* x, int64_t nb, int global,
                  int64_t maxit, void* r, void* p0, void* p1, void* u, hipStream_t s, std::vector<int>& done_h) {
     SolverWork& w = fm.sw;
-    const int tile = fm.tile(prec, nb);
+    const int tile = tuned_tile(fm, prec, nb, s);
     const int nparts = hop_nparts(prec, tile, fm.nx, fm.nt);
     w.ensure_scalars(nb, nparts, s);
     HopArgs a = fm.hop_args(prec, nb, s);
@@ -311,7 +311,7 @@ struct RuPhase {
         const int low64 = prec == NSLX_F64;
         const size_t blo = (size_t)nb * len * (low64 ? 16 : 8);
         w.r.ensure(blo); w.p0.ensure(blo); w.p1.ensure(blo); w.u.ensure(blo); w.x32.ensure(blo);
-        const int tile = fm.tile(prec, nb);
+        const int tile = tuned_tile(fm, prec, nb, s);
         const int nparts = hop_nparts(prec, tile, fm.nx, fm.nt);
         w.ensure_scalars(nb, nparts, s);
         w.status_h2.ensure(nb + 4);

@@ -120,3 +120,7 @@ def test_tune_records_choice():
         assert 0 <= t < nsl.Context.tile_count(prec)
         assert us > 0
         assert ctx.get_tile(prec, Nx, 64, nb) == t
+        # CG mode: the fused CGA + CGB pair, classic tiles 0-3
+        t2, us2 = fm.tune(prec, nb, reps=3, cg=True)
+        assert 0 <= t2 < 4 and us2 > 0
+        assert ctx.get_tile(prec, Nx, 64, nb) == t2
