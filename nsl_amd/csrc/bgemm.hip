@@ -160,6 +160,27 @@ __global__ void __launch_bounds__(256) k_bgemm(const BJob* __restrict__ jobs, co
             if constexpr (CB) sB[1][rb + e * RS][kk] = gb[e].im;
         }
     };
+    // the Add operand (the Gauss-Jordan updates' trailing matrix, the sweeps'
+    // accumulators) is read before the K loop, so its latency overlaps the
+    // MFMAs: read in the epilogue, each load waited behind the previous
+    // element's store (Add may be Y, so the compiler keeps that order), which
+    // for the K = 16 updates was most of the launch
+    C2<S> addv[FM][FN][4];
+    const bool has_add = J.Add != nullptr;
+    if (has_add) {
+        #pragma unroll
+        for (int i = 0; i < FM; ++i)
+            #pragma unroll
+            for (int j = 0; j < FN; ++j)
+                #pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int rr = (sizeof(S) == 8) ? 4 * v + q : 4 * q + v;
+                    const int r = r0 + wm * (BM / 2) + 16 * i + rr;
+                    const int c = c0 + wn * (BN / 2) + 16 * j + r16;
+                    addv[i][j][v] = (r < rows && c < cols) ? ldc<S>(J.Add, (long long)r * J.ldadd + c)
+                                                           : C2<S>{S(0), S(0)};
+                }
+    }
     load(0);
     for (int k0 = 0; k0 < K; k0 += BK) {
         store();
@@ -225,10 +246,9 @@ __global__ void __launch_bounds__(256) k_bgemm(const BJob* __restrict__ jobs, co
                     im = ti;
                 }
                 C2<S> out{sg * re, sg * im};
-                if (J.Add) {
-                    const C2<S> a = ldc<S>(J.Add, (long long)r * J.ldadd + c);
-                    out.re = out.re + a.re;
-                    out.im = out.im + a.im;
+                if (has_add) {
+                    out.re = out.re + addv[i][j][v].re;
+                    out.im = out.im + addv[i][j][v].im;
                 }
                 stc<S>(J.Y, (long long)r * J.ldy + c, out);
             }

@@ -605,7 +605,10 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
             j.ldadd = ldq;
             j.sgn = -1.0;
         }
-        p.gemm(jobs, BVariant{prec_, true, 0, 0});
+        // tile of the rank-16 updates (K = 16, bound by reading and writing the
+        // trailing matrix): NSLX_GJ_TILE 0 (64 x 64) / 1 (32 x 32), else by count
+        static const int gj_tile = (int)env_double("NSLX_GJ_TILE", -1.0);
+        p.gemm(jobs, BVariant{prec_, true, 0, 0}, gj_tile == 0 || gj_tile == 1 ? gj_tile : -1);
     }
 }
 
