@@ -232,10 +232,15 @@ __global__ void __launch_bounds__(256) k_gj_panel(void* Qb, long long ld, long l
 
 // The same column block with the sub-panel rows [jb, rows) held in registers:
 // GJT threads, thread t owning rows jb + t + m GJT (m < RPT), all 16 columns.
-// Per pivot column: a block argmax (largest |.|^2, lowest row on ties), the
-// pivot row and the row it replaces exchanged through LDS, the pivot row
-// broadcast from LDS, the rank-1 update of the rows below in registers.
-// Results (pivots, Binv, C, the sub-panel LU) are those of k_gj_panel.
+// Per pivot column two barriers: after each wave has posted its argmax
+// (largest |.|^2, lowest row on ties) every thread reduces the 8 posts itself
+// and the owners of the pivot row and of the row it replaces post those rows;
+// after the second barrier every thread swaps and updates its own rows.  The
+// posts and the row buffers alternate between two copies by column parity,
+// so the next column never overwrites what a slower thread still reads.  The
+// pivot block's inverse is a Gauss-Jordan sweep of [B | I] in LDS, one thread
+// per element.  Results (pivots, Binv, C, the sub-panel LU) are those of
+// k_gj_panel.
 constexpr int GJT = 512;
 
 template <class S, int RPT>
@@ -248,11 +253,11 @@ __global__ void __launch_bounds__(GJT) k_gj_panel_reg(void* Qb, long long ld, lo
     char* C = reinterpret_cast<char*>(Cb) + (size_t)f * rows * GJB * sizeof(Z<S>);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     constexpr int NW = GJT / 64;
-    __shared__ double red_v[NW];
-    __shared__ int red_i[NW];
-    __shared__ Z<S> rowj[GJB], rowp[GJB];
-    __shared__ Z<S> ub[GJB][GJB + 1];   // the pivot block after the LU (L below, U on and above the diagonal)
-    __shared__ int s_p;
+    __shared__ double red_v[2][NW];
+    __shared__ int red_i[2][NW];
+    __shared__ Z<S> rowj[2][GJB], rowp[2][GJB];
+    __shared__ Z<S> ub[GJB][GJB + 1];        // the pivot block after the LU (L below, U on and above the diagonal)
+    __shared__ Z<S> aug[GJB][2 * GJB + 1];   // [B | I] -> [I | B^-1]
     Z<S> a[RPT][GJB];
     #pragma unroll
     for (int m = 0; m < RPT; ++m) {
@@ -267,8 +272,8 @@ __global__ void __launch_bounds__(GJT) k_gj_panel_reg(void* Qb, long long ld, lo
         // column index would put the sub-panel in scratch memory); w is
         // uniform over the block, so every thread takes the same path
         if (jj < w) {
+        const int par = jj & 1;
         const int j = jb + jj;
-        // argmax over rows >= j
         double best = -1.0;
         int bi = rows;
         #pragma unroll
@@ -285,30 +290,30 @@ __global__ void __launch_bounds__(GJT) k_gj_panel_reg(void* Qb, long long ld, lo
             const int oi = __shfl_xor(bi, o, 64);
             if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
         }
-        if (lane == 0) { red_v[wv] = best; red_i[wv] = bi; }
+        if (lane == 0) { red_v[par][wv] = best; red_i[par][wv] = bi; }
         __syncthreads();
+        double bv = red_v[par][0];
+        int p = red_i[par][0];
+        #pragma unroll
+        for (int k = 1; k < NW; ++k) {
+            const double v = red_v[par][k];
+            const int ix = red_i[par][k];
+            if (v > bv || (v == bv && ix < p)) { bv = v; p = ix; }
+        }
         if (tid == 0) {
-            double bv = red_v[0];
-            int bx = red_i[0];
-            for (int k = 1; k < NW; ++k)
-                if (red_v[k] > bv || (red_v[k] == bv && red_i[k] < bx)) { bv = red_v[k]; bx = red_i[k]; }
-            s_p = bx;
-            piv[jj] = bx;
+            piv[jj] = p;
             if (!(bv > 0.0)) atomicExch(singular, 1);
         }
-        __syncthreads();
-        const int p = s_p;
-        // exchange rows j and p through LDS
         const int oj = j - jb, op = p - jb;
         #pragma unroll
         for (int m = 0; m < RPT; ++m) {
             if (oj == tid + m * GJT) {
                 #pragma unroll
-                for (int c = 0; c < GJB; ++c) rowj[c] = a[m][c];
+                for (int c = 0; c < GJB; ++c) rowj[par][c] = a[m][c];
             }
             if (op == tid + m * GJT) {
                 #pragma unroll
-                for (int c = 0; c < GJB; ++c) rowp[c] = a[m][c];
+                for (int c = 0; c < GJB; ++c) rowp[par][c] = a[m][c];
             }
         }
         __syncthreads();
@@ -316,15 +321,15 @@ __global__ void __launch_bounds__(GJT) k_gj_panel_reg(void* Qb, long long ld, lo
         for (int m = 0; m < RPT; ++m) {
             if (op == tid + m * GJT && p != j) {
                 #pragma unroll
-                for (int c = 0; c < GJB; ++c) a[m][c] = rowj[c];
+                for (int c = 0; c < GJB; ++c) a[m][c] = rowj[par][c];
             }
             if (oj == tid + m * GJT) {
                 #pragma unroll
-                for (int c = 0; c < GJB; ++c) a[m][c] = rowp[c];
+                for (int c = 0; c < GJB; ++c) a[m][c] = rowp[par][c];
             }
         }
-        // rowp now holds the pivot row (row j); update the rows below
-        const Z<S> dinv = zinv<S>(rowp[jj]);
+        // rowp[par] holds the pivot row (row j); update the rows below
+        const Z<S> dinv = zinv<S>(rowp[par][jj]);
         #pragma unroll
         for (int m = 0; m < RPT; ++m) {
             const int gi = jb + tid + m * GJT;
@@ -333,10 +338,9 @@ __global__ void __launch_bounds__(GJT) k_gj_panel_reg(void* Qb, long long ld, lo
                 a[m][jj] = l;
                 #pragma unroll
                 for (int c = 0; c < GJB; ++c)
-                    if (c > jj && c < w) a[m][c] = zsub<S>(a[m][c], zmul<S>(l, rowp[c]));
+                    if (c > jj && c < w) a[m][c] = zsub<S>(a[m][c], zmul<S>(l, rowp[par][c]));
             }
         }
-        __syncthreads();   // rowj / rowp are rewritten by the next column
         }
     }
     // write the sub-panel back; the pivot block to LDS
@@ -355,28 +359,44 @@ __global__ void __launch_bounds__(GJT) k_gj_panel_reg(void* Qb, long long ld, lo
         }
     }
     __syncthreads();
-    // Binv column c (thread c < w): solve L_B y = e_c, then U_B x = y
-    if (tid < GJB) {
-        Z<S> y[GJB];
-        #pragma unroll
-        for (int i = 0; i < GJB; ++i) {
-            Z<S> sm{S(i == tid ? 1 : 0), S(0)};
-            #pragma unroll
-            for (int m = 0; m < GJB; ++m)
-                if (m < i) sm = zsub<S>(sm, zmul<S>(ub[i][m], y[m]));
-            y[i] = sm;
+    // B = L U of the pivot block (unit L) padded with the identity beyond w,
+    // beside I: thread (i, c) of the 16 x 32 augmented matrix
+    static_assert(GJT == GJB * 2 * GJB, "one thread per element of [B | I]");
+    const int gi_ = tid / (2 * GJB), gc_ = tid % (2 * GJB);
+    {
+        Z<S> v{S(0), S(0)};
+        if (gc_ < GJB) {
+            if (gi_ < w && gc_ < w) {
+                #pragma unroll
+                for (int k = 0; k < GJB; ++k) {
+                    if (k > gi_ || k > gc_) continue;
+                    const Z<S> l = k == gi_ ? Z<S>{S(1), S(0)} : ub[gi_][k];
+                    v = {v.re + (l.re * ub[k][gc_].re - l.im * ub[k][gc_].im),
+                         v.im + (l.re * ub[k][gc_].im + l.im * ub[k][gc_].re)};
+                }
+            } else {
+                v.re = S(gi_ == gc_ ? 1 : 0);
+            }
+        } else {
+            v.re = S(gc_ - GJB == gi_ ? 1 : 0);
         }
-        #pragma unroll
-        for (int i = GJB - 1; i >= 0; --i) {
-            Z<S> sm = y[i];
-            #pragma unroll
-            for (int m = 0; m < GJB; ++m)
-                if (m > i && m < w) sm = zsub<S>(sm, zmul<S>(ub[i][m], y[m]));
-            y[i] = i < w ? zmul<S>(sm, zinv<S>(ub[i][i])) : Z<S>{S(0), S(0)};
-        }
-        #pragma unroll
-        for (int i = 0; i < GJB; ++i)
-            st<S>(Binv, (long long)i * GJB + tid, (tid < w && i < w) ? y[i] : Z<S>{S(0), S(0)});
+        aug[gi_][gc_] = v;
+    }
+    __syncthreads();
+    // Gauss-Jordan on [B | I] without further pivoting: the pivots are U's
+    // diagonal (the rows are already in partial-pivoting order)
+    for (int k = 0; k < GJB; ++k) {
+        const Z<S> pk = aug[k][k], fk = aug[gi_][k], rk = aug[k][gc_];
+        __syncthreads();
+        const Z<S> pinv = zinv<S>(pk);
+        const Z<S> rn = zmul<S>(rk, pinv);
+        if (gi_ == k) aug[gi_][gc_] = rn;
+        else aug[gi_][gc_] = zsub<S>(aug[gi_][gc_], zmul<S>(fk, rn));
+        __syncthreads();
+    }
+    if (gc_ >= GJB) {
+        const int c = gc_ - GJB;
+        st<S>(Binv, (long long)gi_ * GJB + c, (gi_ < w && c < w) ? aug[gi_][gc_] : Z<S>{S(0), S(0)});
     }
     // C: rows above jb their current values, pivot rows 0, below L21 U_B
     for (int i = tid; i < jb; i += GJT) {
