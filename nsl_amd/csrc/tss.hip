@@ -237,17 +237,24 @@ __global__ void __launch_bounds__(256) k_gj_panel(void* Qb, long long ld, long l
 // and the owners of the pivot row and of the row it replaces post those rows;
 // after the second barrier every thread swaps and updates its own rows.  The
 // posts and the row buffers alternate between two copies by column parity,
-// so the next column never overwrites what a slower thread still reads.  The
-// pivot block's inverse is a Gauss-Jordan sweep of [B | I] in LDS, one thread
-// per element.  Results (pivots, Binv, C, the sub-panel LU) are those of
-// k_gj_panel.
+// so the next column never overwrites what a slower thread still reads.
+//
+// The elimination only chooses the pivots: C and the pivot block B are the
+// sub-panel's values before this block, taken in pivot order (A21 = L21 U_B
+// and B = L_B U_B hold exactly, so no product of the factors is formed and
+// both are one rounding closer than k_gj_panel's).  C is written from the
+// rows as loaded, before the elimination; afterwards the rows the swaps moved
+// (at most 2w) are rewritten from Q, whose sub-panel columns this kernel does
+// not modify (no later step reads them).  B's inverse is a Gauss-Jordan sweep
+// of [B | I] in LDS, one thread per element.  Pivots and singularity flag are
+// those of k_gj_panel; Binv and C agree with it to rounding.
 constexpr int GJT = 512;
 
 template <class S, int RPT>
 __global__ void __launch_bounds__(GJT) k_gj_panel_reg(void* Qb, long long ld, long long strideQ, int rows, int jb,
                                                        int w, int* pivb, void* Binvb, void* Cb, int* singular) {
     const int f = blockIdx.x;
-    char* Q = reinterpret_cast<char*>(Qb) + (size_t)f * strideQ * sizeof(Z<S>);
+    const char* Q = reinterpret_cast<const char*>(Qb) + (size_t)f * strideQ * sizeof(Z<S>);
     int* piv = pivb + f * GJB;
     char* Binv = reinterpret_cast<char*>(Binvb) + (size_t)f * GJB * GJB * sizeof(Z<S>);
     char* C = reinterpret_cast<char*>(Cb) + (size_t)f * rows * GJB * sizeof(Z<S>);
@@ -256,7 +263,7 @@ __global__ void __launch_bounds__(GJT) k_gj_panel_reg(void* Qb, long long ld, lo
     __shared__ double red_v[2][NW];
     __shared__ int red_i[2][NW];
     __shared__ Z<S> rowj[2][GJB], rowp[2][GJB];
-    __shared__ Z<S> ub[GJB][GJB + 1];        // the pivot block after the LU (L below, U on and above the diagonal)
+    __shared__ int s_piv[GJB];
     __shared__ Z<S> aug[GJB][2 * GJB + 1];   // [B | I] -> [I | B^-1]
     Z<S> a[RPT][GJB];
     #pragma unroll
@@ -265,6 +272,21 @@ __global__ void __launch_bounds__(GJT) k_gj_panel_reg(void* Qb, long long ld, lo
         #pragma unroll
         for (int c = 0; c < GJB; ++c)
             a[m][c] = (gi < rows && c < w) ? ldz<S>(Q, (long long)gi * ld + jb + c) : Z<S>{S(0), S(0)};
+    }
+    // C: every row's values before this block, in the order as loaded (the
+    // stores drain while the elimination runs)
+    #pragma unroll
+    for (int m = 0; m < RPT; ++m) {
+        const int gi = jb + tid + m * GJT;
+        if (gi < rows) {
+            #pragma unroll
+            for (int c = 0; c < GJB; ++c) st<S>(C, (long long)gi * GJB + c, a[m][c]);
+        }
+    }
+    for (int i = tid; i < jb; i += GJT) {
+        #pragma unroll
+        for (int c = 0; c < GJB; ++c)
+            st<S>(C, (long long)i * GJB + c, c < w ? ldz<S>(Q, (long long)i * ld + jb + c) : Z<S>{S(0), S(0)});
     }
     #pragma unroll
     for (int jj = 0; jj < GJB; ++jj) {
@@ -300,8 +322,11 @@ __global__ void __launch_bounds__(GJT) k_gj_panel_reg(void* Qb, long long ld, lo
             const int ix = red_i[par][k];
             if (v > bv || (v == bv && ix < p)) { bv = v; p = ix; }
         }
+        // no comparable entry (all NaN): no swap, the singular flag is set
+        if (p >= rows) p = j;
         if (tid == 0) {
             piv[jj] = p;
+            s_piv[jj] = p;
             if (!(bv > 0.0)) atomicExch(singular, 1);
         }
         const int oj = j - jb, op = p - jb;
@@ -343,48 +368,51 @@ __global__ void __launch_bounds__(GJT) k_gj_panel_reg(void* Qb, long long ld, lo
         }
         }
     }
-    // write the sub-panel back; the pivot block to LDS
+    __syncthreads();
+    // row of the original sub-panel now at row r: the swaps undone from the last
+    auto origin = [&](int r) {
+        int x = r;
+        for (int k = w - 1; k >= 0; --k) {
+            const int j = jb + k, pk = s_piv[k];
+            x = x == j ? pk : (x == pk ? j : x);
+        }
+        return x;
+    };
+    // C: zero at the pivot rows, the moved rows rewritten (by the thread that
+    // stored them first, so the two stores keep their order)
     #pragma unroll
     for (int m = 0; m < RPT; ++m) {
         const int gi = jb + tid + m * GJT;
-        if (gi < rows) {
+        if (gi >= rows) continue;
+        if (gi < jb + w) {
             #pragma unroll
-            for (int c = 0; c < GJB; ++c)
-                if (c < w) st<S>(Q, (long long)gi * ld + jb + c, a[m][c]);
-        }
-        const int li = tid + m * GJT;
-        if (li < w) {
-            #pragma unroll
-            for (int c = 0; c < GJB; ++c) ub[li][c] = a[m][c];
+            for (int c = 0; c < GJB; ++c) st<S>(C, (long long)gi * GJB + c, Z<S>{S(0), S(0)});
+        } else {
+            const int o = origin(gi);
+            if (o != gi) {
+                #pragma unroll
+                for (int c = 0; c < GJB; ++c)
+                    st<S>(C, (long long)gi * GJB + c, c < w ? ldz<S>(Q, (long long)o * ld + jb + c) : Z<S>{S(0), S(0)});
+            }
         }
     }
-    __syncthreads();
-    // B = L U of the pivot block (unit L) padded with the identity beyond w,
-    // beside I: thread (i, c) of the 16 x 32 augmented matrix
+    // [B | I]: B = the pivot rows' values before this block, padded with the
+    // identity beyond w; thread (i, c) of the 16 x 32 augmented matrix
     static_assert(GJT == GJB * 2 * GJB, "one thread per element of [B | I]");
     const int gi_ = tid / (2 * GJB), gc_ = tid % (2 * GJB);
     {
         Z<S> v{S(0), S(0)};
         if (gc_ < GJB) {
-            if (gi_ < w && gc_ < w) {
-                #pragma unroll
-                for (int k = 0; k < GJB; ++k) {
-                    if (k > gi_ || k > gc_) continue;
-                    const Z<S> l = k == gi_ ? Z<S>{S(1), S(0)} : ub[gi_][k];
-                    v = {v.re + (l.re * ub[k][gc_].re - l.im * ub[k][gc_].im),
-                         v.im + (l.re * ub[k][gc_].im + l.im * ub[k][gc_].re)};
-                }
-            } else {
-                v.re = S(gi_ == gc_ ? 1 : 0);
-            }
+            if (gi_ < w && gc_ < w) v = ldz<S>(Q, (long long)origin(jb + gi_) * ld + jb + gc_);
+            else v.re = S(gi_ == gc_ ? 1 : 0);
         } else {
             v.re = S(gc_ - GJB == gi_ ? 1 : 0);
         }
         aug[gi_][gc_] = v;
     }
     __syncthreads();
-    // Gauss-Jordan on [B | I] without further pivoting: the pivots are U's
-    // diagonal (the rows are already in partial-pivoting order)
+    // Gauss-Jordan on [B | I] without further pivoting: the rows are in
+    // partial-pivoting order, so the pivots are those of the LU above
     for (int k = 0; k < GJB; ++k) {
         const Z<S> pk = aug[k][k], fk = aug[gi_][k], rk = aug[k][gc_];
         __syncthreads();
@@ -397,28 +425,6 @@ __global__ void __launch_bounds__(GJT) k_gj_panel_reg(void* Qb, long long ld, lo
     if (gc_ >= GJB) {
         const int c = gc_ - GJB;
         st<S>(Binv, (long long)gi_ * GJB + c, (gi_ < w && c < w) ? aug[gi_][gc_] : Z<S>{S(0), S(0)});
-    }
-    // C: rows above jb their current values, pivot rows 0, below L21 U_B
-    for (int i = tid; i < jb; i += GJT) {
-        #pragma unroll
-        for (int c = 0; c < GJB; ++c)
-            st<S>(C, (long long)i * GJB + c, c < w ? ldz<S>(Q, (long long)i * ld + jb + c) : Z<S>{S(0), S(0)});
-    }
-    #pragma unroll
-    for (int m = 0; m < RPT; ++m) {
-        const int gi = jb + tid + m * GJT;
-        if (gi >= rows) continue;
-        #pragma unroll
-        for (int c = 0; c < GJB; ++c) {
-            Z<S> v{S(0), S(0)};
-            if (gi >= jb + w && c < w) {
-                #pragma unroll
-                for (int k = 0; k < GJB; ++k)
-                    if (k <= c) v = {v.re + (a[m][k].re * ub[k][c].re - a[m][k].im * ub[k][c].im),
-                                     v.im + (a[m][k].re * ub[k][c].im + a[m][k].im * ub[k][c].re)};
-            }
-            st<S>(C, (long long)gi * GJB + c, v);
-        }
     }
 }
 
@@ -563,17 +569,19 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
     check_extent(piv, (long long)nf * GJB - 1, sizeof(int), "Gauss-Jordan pivots");
     for (int jb = 0; jb < n_; jb += GJB) {
         const int w = std::min(GJB, n_ - jb);
-        // work of the pivot search and sub-panel LU: the sub-panel read and
-        // written once, C written; 8 flops per complex multiply-add
-        const double pbytes = (double)nf * rows * (2.0 * w + GJB) * e;
-        double pflops = 0.0;
-        for (int jj = 0; jj < w; ++jj) pflops += 8.0 * (rows - jb - jj) * (w - jj);
-        pflops = (pflops + 4.0 * rows * w * w) * nf;
         // rows of the sub-panel per thread of the register kernel (0: too many
         // for its register budget, the global-memory kernel runs instead)
         const int live = rows - jb;
         const int rpt = (live + GJT - 1) / GJT;
         const int reg = rpt <= (f64 ? 2 : 3) ? rpt : 0;
+        // work of the pivot search and sub-panel LU (8 flops per complex
+        // multiply-add): the register kernel reads the sub-panel once and
+        // writes C; the global-memory one also writes the sub-panel back and
+        // forms C = L21 U_B
+        const double pbytes = (double)nf * rows * ((reg ? 1.0 : 2.0) * w + GJB) * e;
+        double pflops = 0.0;
+        for (int jj = 0; jj < w; ++jj) pflops += 8.0 * (rows - jb - jj) * (w - jj);
+        pflops = (pflops + (reg ? 0.0 : 4.0 * rows * w * w)) * nf;
         p.custom([=](hipStream_t s) {
             ProfScope prof(prof_on() ? std::string(f64 ? "gj_panel_f64" : "gj_panel_f32") : std::string(), s);
             prof.work(pflops, pbytes);
