@@ -16,7 +16,9 @@
 // the in-chunk recurrence from the reduced solution; M^{-dag} mirrors it
 // with the backward recurrence, the chunks shifted by one slice, and R^H.
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
+#include <type_traits>
 
 #include "engine.hpp"
 #include "prof.hpp"
@@ -230,177 +232,241 @@ __global__ void __launch_bounds__(256) k_gj_panel(void* Qb, long long ld, long l
     }
 }
 
+// Pivot candidates of the register kernel as one double each: |a|^2 (capped
+// at the largest finite value) with its low 11 mantissa bits replaced by
+// 2047 - (row - jb), so that the largest key is the largest |a|^2, lowest
+// row among values equal to 2^-41 relative; -1 = no candidate.  A plain
+// maximum then reduces value and row together.
+constexpr int GJ_KEYBITS = 11;
+__device__ inline double gj_key(double v, int rel) {
+    const long long b = __double_as_longlong(fmin(v, DBL_MAX));
+    return __longlong_as_double((b & ~((1LL << GJ_KEYBITS) - 1)) | ((1 << GJ_KEYBITS) - 1 - rel));
+}
+__device__ inline int gj_key_row(double k) {
+    return (1 << GJ_KEYBITS) - 1 - (int)(__double_as_longlong(k) & ((1LL << GJ_KEYBITS) - 1));
+}
+__device__ inline double gj_key_value(double k) {
+    return __longlong_as_double(__double_as_longlong(k) & ~((1LL << GJ_KEYBITS) - 1));
+}
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline double readlane_f64(double v, int lane) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+// Maximum over the 64 lanes of the wave, uniform: within each 16-lane row by
+// DPP (lane xor 1, xor 2, then the mirrors over 8 and 16 lanes), then the
+// four rows' results.
+__device__ inline double wave_max(double k) {
+    k = fmax(k, dpp_f64<0xB1>(k));    // quad_perm [1, 0, 3, 2]
+    k = fmax(k, dpp_f64<0x4E>(k));    // quad_perm [2, 3, 0, 1]
+    k = fmax(k, dpp_f64<0x141>(k));   // row_half_mirror
+    k = fmax(k, dpp_f64<0x140>(k));   // row_mirror
+    return fmax(fmax(readlane_f64(k, 0), readlane_f64(k, 16)), fmax(readlane_f64(k, 32), readlane_f64(k, 48)));
+}
+
 // The same column block with the sub-panel rows [jb, rows) held in registers:
-// GJT threads, thread t owning rows jb + t + m GJT (m < RPT), all 16 columns.
-// Per pivot column two barriers: after each wave has posted its argmax
-// (largest |.|^2, lowest row on ties) every thread reduces the 8 posts itself
-// and the owners of the pivot row and of the row it replaces post those rows;
-// after the second barrier every thread swaps and updates its own rows.  The
-// posts and the row buffers alternate between two copies by column parity,
-// so the next column never overwrites what a slower thread still reads.
-//
+// NT threads, thread t owning rows jb + t + m NT (m < RPT), all 16 columns.
 // The elimination only chooses the pivots: C and the pivot block B are the
 // sub-panel's values before this block, taken in pivot order (A21 = L21 U_B
 // and B = L_B U_B hold exactly, so no product of the factors is formed and
-// both are one rounding closer than k_gj_panel's).  C is written from the
-// rows as loaded, before the elimination; afterwards the rows the swaps moved
-// (at most 2w) are rewritten from Q, whose sub-panel columns this kernel does
-// not modify (no later step reads them).  B's inverse is a Gauss-Jordan sweep
-// of [B | I] in LDS, one thread per element.  Pivots and singularity flag are
-// those of k_gj_panel; Binv and C agree with it to rounding.
-constexpr int GJT = 512;
-
-template <class S, int RPT>
-__global__ void __launch_bounds__(GJT) k_gj_panel_reg(void* Qb, long long ld, long long strideQ, int rows, int jb,
-                                                       int w, int* pivb, void* Binvb, void* Cb, int* singular) {
+// both are one rounding closer than k_gj_panel's).  Every instruction of the
+// column loop is issued by every wave, so the loop is kept short:
+//   - one barrier per pivot column: each wave reduces its candidates' keys
+//     (gj_key) and posts the maximum, its owner lane posts that row, and the
+//     owner of row j posts row j; after the barrier every thread takes the
+//     maximum of the posts, the winning wave's row as the pivot row, and
+//     swaps and updates its own rows.  Posts and rows alternate between two
+//     copies by column parity: a thread writes column jj + 2's copy only
+//     after the barrier of column jj + 1, which every reader of column jj
+//     has passed;
+//   - column jj is register column 0 of step jj (the columns shift left after
+//     each step), so the loop stays rolled with static register indices, and
+//     only the live columns, in bands of 4, are updated and moved (all
+//     columns with 512 threads of several rows, whose band copies spill);
+//   - the multipliers use conj(p) / |p|^2 (they only steer the pivot search);
+//   - no global memory operation inside the loop.
+// C is written from the rows as loaded, before the elimination; afterwards
+// the rows the swaps moved below the pivot rows are rewritten.  Such a row
+// came from a pivot position (a row moved up from below stays a pivot row),
+// so its values are the original pivot-position rows, kept in LDS.  B's rows
+// are read back from Q, whose sub-panel columns this kernel does not modify
+// (no later step reads them), and B's inverse is a Gauss-Jordan sweep of
+// [B | I] in LDS.  Pivots are those of k_gj_panel except between values
+// equal to 2^-41 relative; Binv and C agree with it to rounding.
+template <class S, int RPT, int NT>
+__global__ void __launch_bounds__(NT) k_gj_panel_reg(void* Qb, long long ld, long long strideQ, int rows, int jb,
+                                                      int w, int* pivb, void* Binvb, void* Cb, int* singular) {
+    static_assert(NT % 64 == 0 && NT >= 64 && (2 * GJB * GJB) % NT == 0, "whole waves, [B | I] split evenly");
+    static_assert(RPT * NT <= (1 << GJ_KEYBITS), "relative rows fit the key");
     const int f = blockIdx.x;
     const char* Q = reinterpret_cast<const char*>(Qb) + (size_t)f * strideQ * sizeof(Z<S>);
     int* piv = pivb + f * GJB;
     char* Binv = reinterpret_cast<char*>(Binvb) + (size_t)f * GJB * GJB * sizeof(Z<S>);
     char* C = reinterpret_cast<char*>(Cb) + (size_t)f * rows * GJB * sizeof(Z<S>);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    constexpr int NW = GJT / 64;
-    __shared__ double red_v[2][NW];
-    __shared__ int red_i[2][NW];
-    __shared__ Z<S> rowj[2][GJB], rowp[2][GJB];
+    constexpr int NW = NT / 64;
+    __shared__ double wkey[2][NW];
+    __shared__ Z<S> wrow[2][NW][GJB];        // each wave's candidate row
+    __shared__ Z<S> rowj[2][GJB];
+    __shared__ Z<S> orig[GJB][GJB];          // rows [jb, jb + w) before this block
     __shared__ int s_piv[GJB];
     __shared__ Z<S> aug[GJB][2 * GJB + 1];   // [B | I] -> [I | B^-1]
+    // the sub-panel (loads clamped in bounds, zero outside it)
     Z<S> a[RPT][GJB];
+    const int wl = w - 1;
     #pragma unroll
     for (int m = 0; m < RPT; ++m) {
-        const int gi = jb + tid + m * GJT;
+        const int gi = jb + tid + m * NT;
+        const long long rb = (long long)min(gi, rows - 1) * ld + jb;
+        #pragma unroll
+        for (int c = 0; c < GJB; ++c) a[m][c] = ldz<S>(Q, rb + min(c, wl));
         #pragma unroll
         for (int c = 0; c < GJB; ++c)
-            a[m][c] = (gi < rows && c < w) ? ldz<S>(Q, (long long)gi * ld + jb + c) : Z<S>{S(0), S(0)};
+            if (gi >= rows || c >= w) a[m][c] = Z<S>{S(0), S(0)};
     }
     // C: every row's values before this block, in the order as loaded (the
     // stores drain while the elimination runs)
     #pragma unroll
     for (int m = 0; m < RPT; ++m) {
-        const int gi = jb + tid + m * GJT;
+        const int gi = jb + tid + m * NT;
         if (gi < rows) {
             #pragma unroll
             for (int c = 0; c < GJB; ++c) st<S>(C, (long long)gi * GJB + c, a[m][c]);
         }
     }
-    for (int i = tid; i < jb; i += GJT) {
+    if (tid < GJB) {
         #pragma unroll
-        for (int c = 0; c < GJB; ++c)
-            st<S>(C, (long long)i * GJB + c, c < w ? ldz<S>(Q, (long long)i * ld + jb + c) : Z<S>{S(0), S(0)});
+        for (int c = 0; c < GJB; ++c) orig[tid][c] = a[0][c];
     }
-    #pragma unroll
-    for (int jj = 0; jj < GJB; ++jj) {
-        // fully unrolled with a constant jj (a guard, not a break: a dynamic
-        // column index would put the sub-panel in scratch memory); w is
-        // uniform over the block, so every thread takes the same path
-        if (jj < w) {
+    for (int i = tid; i < jb; i += NT) {
+        Z<S> v[GJB];
+        #pragma unroll
+        for (int c = 0; c < GJB; ++c) v[c] = ldz<S>(Q, (long long)i * ld + jb + min(c, wl));
+        #pragma unroll
+        for (int c = 0; c < GJB; ++c) st<S>(C, (long long)i * GJB + c, c < w ? v[c] : Z<S>{S(0), S(0)});
+    }
+    bool sing = false;
+    #pragma unroll 1
+    for (int jj = 0; jj < w; ++jj) {
         const int par = jj & 1;
         const int j = jb + jj;
+        double key[RPT];
         double best = -1.0;
-        int bi = rows;
         #pragma unroll
         for (int m = 0; m < RPT; ++m) {
-            const int gi = jb + tid + m * GJT;
-            if (gi >= j && gi < rows) {
-                const double v = (double)a[m][jj].re * a[m][jj].re + (double)a[m][jj].im * a[m][jj].im;
-                if (v > best) { best = v; bi = gi; }
+            const int gi = jb + tid + m * NT;
+            const double v = (double)a[m][0].re * a[m][0].re + (double)a[m][0].im * a[m][0].im;
+            key[m] = (gi >= j && gi < rows && v == v) ? gj_key(v, gi - jb) : -1.0;
+            best = fmax(best, key[m]);
+        }
+        best = wave_max(best);
+        if (lane == 0) wkey[par][wv] = best;
+        #pragma unroll
+        for (int m = 0; m < RPT; ++m) {
+            const int gi = jb + tid + m * NT;
+            if (key[m] == best && best >= 0.0) {
+                #pragma unroll
+                for (int c = 0; c < GJB; ++c) wrow[par][wv][c] = a[m][c];
             }
-        }
-        #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const double ov = __shfl_xor(best, o, 64);
-            const int oi = __shfl_xor(bi, o, 64);
-            if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-        }
-        if (lane == 0) { red_v[par][wv] = best; red_i[par][wv] = bi; }
-        __syncthreads();
-        double bv = red_v[par][0];
-        int p = red_i[par][0];
-        #pragma unroll
-        for (int k = 1; k < NW; ++k) {
-            const double v = red_v[par][k];
-            const int ix = red_i[par][k];
-            if (v > bv || (v == bv && ix < p)) { bv = v; p = ix; }
-        }
-        // no comparable entry (all NaN): no swap, the singular flag is set
-        if (p >= rows) p = j;
-        if (tid == 0) {
-            piv[jj] = p;
-            s_piv[jj] = p;
-            if (!(bv > 0.0)) atomicExch(singular, 1);
-        }
-        const int oj = j - jb, op = p - jb;
-        #pragma unroll
-        for (int m = 0; m < RPT; ++m) {
-            if (oj == tid + m * GJT) {
+            if (gi == j) {
                 #pragma unroll
                 for (int c = 0; c < GJB; ++c) rowj[par][c] = a[m][c];
             }
-            if (op == tid + m * GJT) {
-                #pragma unroll
-                for (int c = 0; c < GJB; ++c) rowp[par][c] = a[m][c];
-            }
         }
         __syncthreads();
+        double kmax = wkey[par][0];
+        #pragma unroll
+        for (int k = 1; k < NW; ++k) kmax = fmax(kmax, wkey[par][k]);
+        // the pivot row: the winning wave's post, or row j itself when no
+        // entry is comparable (all NaN: no swap, the singular flag is set)
+        int p;
+        const Z<S>* prow;
+        if (kmax >= 0.0) {
+            const int rel = gj_key_row(kmax);
+            p = jb + rel;
+            prow = wrow[par][(rel % NT) >> 6];
+        } else {
+            p = j;
+            prow = rowj[par];
+        }
+        sing = sing || !(kmax >= 0.0 && gj_key_value(kmax) > 0.0);
+        if (tid == 0) s_piv[jj] = p;
         #pragma unroll
         for (int m = 0; m < RPT; ++m) {
-            if (op == tid + m * GJT && p != j) {
+            const int gi = jb + tid + m * NT;
+            if (gi == p && p != j) {
                 #pragma unroll
                 for (int c = 0; c < GJB; ++c) a[m][c] = rowj[par][c];
             }
-            if (oj == tid + m * GJT) {
+            if (gi == j) {
                 #pragma unroll
-                for (int c = 0; c < GJB; ++c) a[m][c] = rowp[par][c];
+                for (int c = 0; c < GJB; ++c) a[m][c] = prow[c];
             }
         }
-        // rowp[par] holds the pivot row (row j); update the rows below
-        const Z<S> dinv = zinv<S>(rowp[par][jj]);
-        #pragma unroll
-        for (int m = 0; m < RPT; ++m) {
-            const int gi = jb + tid + m * GJT;
-            if (gi > j && gi < rows) {
-                const Z<S> l = zmul<S>(a[m][jj], dinv);
-                a[m][jj] = l;
+        // eliminate below the pivot row, then shift.  The live register
+        // columns are [0, w - jj); columns beyond them are stale and never
+        // read again.
+        const Z<S> pv = prow[0];
+        const double rn = 1.0 / ((double)pv.re * pv.re + (double)pv.im * pv.im);
+        const Z<S> dinv{S(pv.re * rn), S(-pv.im * rn)};
+        auto step = [&](auto ncol) {
+            constexpr int NC = decltype(ncol)::value;   // columns 1 .. NC updated, 0 .. NC-1 moved
+            Z<S> pr[NC + 1];
+            #pragma unroll
+            for (int c = 1; c <= NC; ++c) pr[c] = prow[c];
+            #pragma unroll
+            for (int m = 0; m < RPT; ++m) {
+                const int gi = jb + tid + m * NT;
+                if (gi >= rows) continue;
+                if (gi > j) {
+                    const Z<S> l = zmul<S>(a[m][0], dinv);
+                    #pragma unroll
+                    for (int c = 1; c <= NC; ++c) a[m][c] = zsub<S>(a[m][c], zmul<S>(l, pr[c]));
+                }
                 #pragma unroll
-                for (int c = 0; c < GJB; ++c)
-                    if (c > jj && c < w) a[m][c] = zsub<S>(a[m][c], zmul<S>(l, rowp[par][c]));
+                for (int c = 0; c < NC; ++c) a[m][c] = a[m][c + 1];
             }
-        }
+        };
+        static_assert(GJB == 16, "bands of 4 columns");
+        if constexpr (NT * RPT > 512 && NT > 256) {
+            // 512 threads with several rows each: the band copies would spill
+            step(std::integral_constant<int, 15>{});
+        } else {
+            // one uniform branch picks the band's straight-line code
+            const int live = w - jj;
+            if (live > 12) step(std::integral_constant<int, 15>{});
+            else if (live > 8) step(std::integral_constant<int, 11>{});
+            else if (live > 4) step(std::integral_constant<int, 7>{});
+            else step(std::integral_constant<int, 3>{});
         }
     }
     __syncthreads();
+    int pv_[GJB];
+    #pragma unroll
+    for (int k = 0; k < GJB; ++k) pv_[k] = k < w ? s_piv[k] : jb + k;
+    if (tid < w) piv[tid] = s_piv[tid];
+    if (tid == 0 && sing) atomicExch(singular, 1);
     // row of the original sub-panel now at row r: the swaps undone from the last
     auto origin = [&](int r) {
         int x = r;
-        for (int k = w - 1; k >= 0; --k) {
-            const int j = jb + k, pk = s_piv[k];
+        #pragma unroll
+        for (int k = GJB - 1; k >= 0; --k) {
+            const int j = jb + k, pk = pv_[k];
             x = x == j ? pk : (x == pk ? j : x);
         }
         return x;
     };
-    // C: zero at the pivot rows, the moved rows rewritten (by the thread that
-    // stored them first, so the two stores keep their order)
-    #pragma unroll
-    for (int m = 0; m < RPT; ++m) {
-        const int gi = jb + tid + m * GJT;
-        if (gi >= rows) continue;
-        if (gi < jb + w) {
-            #pragma unroll
-            for (int c = 0; c < GJB; ++c) st<S>(C, (long long)gi * GJB + c, Z<S>{S(0), S(0)});
-        } else {
-            const int o = origin(gi);
-            if (o != gi) {
-                #pragma unroll
-                for (int c = 0; c < GJB; ++c)
-                    st<S>(C, (long long)gi * GJB + c, c < w ? ldz<S>(Q, (long long)o * ld + jb + c) : Z<S>{S(0), S(0)});
-            }
-        }
-    }
     // [B | I]: B = the pivot rows' values before this block, padded with the
-    // identity beyond w; thread (i, c) of the 16 x 32 augmented matrix
-    static_assert(GJT == GJB * 2 * GJB, "one thread per element of [B | I]");
-    const int gi_ = tid / (2 * GJB), gc_ = tid % (2 * GJB);
-    {
+    // identity beyond w; element e = tid + q NT of the 16 x 32 augmented matrix
+    constexpr int EPT = 2 * GJB * GJB / NT;
+    #pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int e = tid + q * NT, gi_ = e / (2 * GJB), gc_ = e % (2 * GJB);
         Z<S> v{S(0), S(0)};
         if (gc_ < GJB) {
             if (gi_ < w && gc_ < w) v = ldz<S>(Q, (long long)origin(jb + gi_) * ld + jb + gc_);
@@ -410,21 +476,55 @@ __global__ void __launch_bounds__(GJT) k_gj_panel_reg(void* Qb, long long ld, lo
         }
         aug[gi_][gc_] = v;
     }
+    // C: zero at the pivot rows; a row moved below them holds an original
+    // pivot-position row (rewritten by the thread that stored it first, so the
+    // two stores keep their order)
+    #pragma unroll
+    for (int m = 0; m < RPT; ++m) {
+        const int gi = jb + tid + m * NT;
+        if (gi >= rows) continue;
+        if (gi < jb + w) {
+            #pragma unroll
+            for (int c = 0; c < GJB; ++c) st<S>(C, (long long)gi * GJB + c, Z<S>{S(0), S(0)});
+        } else {
+            const int o = origin(gi);
+            if (o != gi) {
+                #pragma unroll
+                for (int c = 0; c < GJB; ++c) st<S>(C, (long long)gi * GJB + c, orig[o - jb][c]);
+            }
+        }
+    }
     __syncthreads();
     // Gauss-Jordan on [B | I] without further pivoting: the rows are in
     // partial-pivoting order, so the pivots are those of the LU above
+    #pragma unroll 1
     for (int k = 0; k < GJB; ++k) {
-        const Z<S> pk = aug[k][k], fk = aug[gi_][k], rk = aug[k][gc_];
+        const Z<S> pk = aug[k][k];
+        Z<S> fk[EPT], rk[EPT];
+        #pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = tid + q * NT, gi_ = e / (2 * GJB), gc_ = e % (2 * GJB);
+            fk[q] = aug[gi_][k];
+            rk[q] = aug[k][gc_];
+        }
         __syncthreads();
         const Z<S> pinv = zinv<S>(pk);
-        const Z<S> rn = zmul<S>(rk, pinv);
-        if (gi_ == k) aug[gi_][gc_] = rn;
-        else aug[gi_][gc_] = zsub<S>(aug[gi_][gc_], zmul<S>(fk, rn));
+        #pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = tid + q * NT, gi_ = e / (2 * GJB), gc_ = e % (2 * GJB);
+            const Z<S> rn = zmul<S>(rk[q], pinv);
+            if (gi_ == k) aug[gi_][gc_] = rn;
+            else aug[gi_][gc_] = zsub<S>(aug[gi_][gc_], zmul<S>(fk[q], rn));
+        }
         __syncthreads();
     }
-    if (gc_ >= GJB) {
-        const int c = gc_ - GJB;
-        st<S>(Binv, (long long)gi_ * GJB + c, (gi_ < w && c < w) ? aug[gi_][gc_] : Z<S>{S(0), S(0)});
+    #pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int e = tid + q * NT, gi_ = e / (2 * GJB), gc_ = e % (2 * GJB);
+        if (gc_ >= GJB) {
+            const int c = gc_ - GJB;
+            st<S>(Binv, (long long)gi_ * GJB + c, (gi_ < w && c < w) ? aug[gi_][gc_] : Z<S>{S(0), S(0)});
+        }
     }
 }
 
@@ -569,11 +669,13 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
     check_extent(piv, (long long)nf * GJB - 1, sizeof(int), "Gauss-Jordan pivots");
     for (int jb = 0; jb < n_; jb += GJB) {
         const int w = std::min(GJB, n_ - jb);
-        // rows of the sub-panel per thread of the register kernel (0: too many
-        // for its register budget, the global-memory kernel runs instead)
+        // register kernel shape for the live sub-panel rows: 256 threads with
+        // up to 2 (F64) / 4 (F32) rows each, then 512 threads (register
+        // budget), else the global-memory kernel (reg 0)
         const int live = rows - jb;
-        const int rpt = (live + GJT - 1) / GJT;
-        const int reg = rpt <= (f64 ? 2 : 3) ? rpt : 0;
+        int reg = 0;
+        if (f64) reg = live <= 256 ? 1 : live <= 512 ? 2 : live <= 1024 ? 3 : 0;
+        else reg = live <= 256 ? 1 : live <= 512 ? 2 : live <= 768 ? 3 : live <= 1024 ? 4 : live <= 1536 ? 5 : 0;
         // work of the pivot search and sub-panel LU (8 flops per complex
         // multiply-add): the register kernel reads the sub-panel once and
         // writes C; the global-memory one also writes the sub-panel back and
@@ -589,13 +691,16 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
             void* bv = (void*)binv;
             void* Cv = (void*)C;
             if (f64) {
-                if (reg == 1) launch_timed(k_gj_panel_reg<double, 1>, dim3(nf), dim3(GJT), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
-                else if (reg == 2) launch_timed(k_gj_panel_reg<double, 2>, dim3(nf), dim3(GJT), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
+                if (reg == 1) launch_timed(k_gj_panel_reg<double, 1, 256>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
+                else if (reg == 2) launch_timed(k_gj_panel_reg<double, 2, 256>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
+                else if (reg == 3) launch_timed(k_gj_panel_reg<double, 2, 512>, dim3(nf), dim3(512), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
                 else launch_timed(k_gj_panel<double>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
             } else {
-                if (reg == 1) launch_timed(k_gj_panel_reg<float, 1>, dim3(nf), dim3(GJT), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
-                else if (reg == 2) launch_timed(k_gj_panel_reg<float, 2>, dim3(nf), dim3(GJT), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
-                else if (reg == 3) launch_timed(k_gj_panel_reg<float, 3>, dim3(nf), dim3(GJT), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
+                if (reg == 1) launch_timed(k_gj_panel_reg<float, 1, 256>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
+                else if (reg == 2) launch_timed(k_gj_panel_reg<float, 2, 256>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
+                else if (reg == 3) launch_timed(k_gj_panel_reg<float, 3, 256>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
+                else if (reg == 4) launch_timed(k_gj_panel_reg<float, 4, 256>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
+                else if (reg == 5) launch_timed(k_gj_panel_reg<float, 3, 512>, dim3(nf), dim3(512), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
                 else launch_timed(k_gj_panel<float>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
             }
             NSLX_LAUNCH_CHECK();
