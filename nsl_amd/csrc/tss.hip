@@ -128,6 +128,40 @@ __global__ void k_build_fin(int n, long long W, const void* prevQ, long long str
 }
 
 constexpr int GJB = 16;   // column block of the Gauss-Jordan elimination
+// Per field and column block the pivot record holds 4 x GJB ints: piv[jj] =
+// the row swapped with jb + jj, then the block's row map for k_gj_swap
+// (gj_row_map).
+constexpr int GJREC = 4 * GJB;
+
+// Row map of a column block from its w pivots spiv (LDS), written by threads
+// tid < GJB: map[m] = the row whose values end at pivot position jb + m
+// (m < w); map[GJB + k], map[2 GJB + k] = destination and source row of a row
+// that swap k moved below the pivot positions (-1: none).  The swaps only
+// exchange a pivot position with a row at or below it, so a row that ends
+// below the pivot positions, if moved, holds an original pivot-position row.
+__device__ inline void gj_row_map(int tid, int jb, int w, const int* spiv, int* map) {
+    if (tid >= GJB) return;
+    auto origin = [&](int r) {
+        int x = r;
+        for (int k = w - 1; k >= 0; --k) {
+            const int j = jb + k, pk = spiv[k];
+            x = x == j ? pk : (x == pk ? j : x);
+        }
+        return x;
+    };
+    int src = -1, dst = -1, msrc = -1;
+    if (tid < w) {
+        src = origin(jb + tid);
+        const int pk = spiv[tid];
+        if (pk >= jb + w) {
+            const int o = origin(pk);
+            if (o != pk) { dst = pk; msrc = o; }
+        }
+    }
+    map[tid] = src;
+    map[GJB + tid] = dst;
+    map[2 * GJB + tid] = msrc;
+}
 
 // One column block [jb, jb + w) of the Gauss-Jordan elimination of field
 // blockIdx.x's matrix (rows x ld, row-major): LU with partial pivoting of the
@@ -135,19 +169,21 @@ constexpr int GJB = 16;   // column block of the Gauss-Jordan elimination
 // then Binv = (L_B U_B)^{-1} of the w x w pivot block and C = the sub-panel's
 // values before this block at every non-pivot row (rows above jb untouched,
 // rows below = L21 U_B), zero at the pivot rows.  piv[jj] = row swapped with
-// jb + jj.  Only the sub-panel columns of Q are modified.
+// jb + jj, followed by the row map (gj_row_map).  Only the sub-panel columns
+// of Q are modified.
 template <class S>
 __global__ void __launch_bounds__(256) k_gj_panel(void* Qb, long long ld, long long strideQ, int rows, int jb, int w,
                                                   int* pivb, void* Binvb, void* Cb, int* singular) {
     const int f = blockIdx.x;
     char* Q = reinterpret_cast<char*>(Qb) + (size_t)f * strideQ * sizeof(Z<S>);
-    int* piv = pivb + f * GJB;
+    int* piv = pivb + f * GJREC;
     char* Binv = reinterpret_cast<char*>(Binvb) + (size_t)f * GJB * GJB * sizeof(Z<S>);
     char* C = reinterpret_cast<char*>(Cb) + (size_t)f * rows * GJB * sizeof(Z<S>);
     const int tid = threadIdx.x;
     __shared__ double rv[256];
     __shared__ int ri[256];
     __shared__ int s_p;
+    __shared__ int s_pv[GJB];
     auto at = [&](int i, int j) { return (long long)i * ld + j; };
     for (int jj = 0; jj < w; ++jj) {
         const int j = jb + jj;
@@ -172,6 +208,7 @@ __global__ void __launch_bounds__(256) k_gj_panel(void* Qb, long long ld, long l
         }
         if (tid == 0) {
             s_p = ri[0];
+            s_pv[jj] = ri[0];
             piv[jj] = ri[0];
             if (!(rv[0] > 0.0)) atomicExch(singular, 1);
         }
@@ -230,6 +267,7 @@ __global__ void __launch_bounds__(256) k_gj_panel(void* Qb, long long ld, long l
             st<S>(C, (long long)i * GJB + cc, v);
         }
     }
+    gj_row_map(tid, jb, w, s_pv, piv + GJB);
 }
 
 // Pivot candidates of the register kernel as one double each: |a|^2 (capped
@@ -305,7 +343,7 @@ __global__ void __launch_bounds__(NT) k_gj_panel_reg(void* Qb, long long ld, lon
     static_assert(RPT * NT <= (1 << GJ_KEYBITS), "relative rows fit the key");
     const int f = blockIdx.x;
     const char* Q = reinterpret_cast<const char*>(Qb) + (size_t)f * strideQ * sizeof(Z<S>);
-    int* piv = pivb + f * GJB;
+    int* piv = pivb + f * GJREC;
     char* Binv = reinterpret_cast<char*>(Binvb) + (size_t)f * GJB * GJB * sizeof(Z<S>);
     char* C = reinterpret_cast<char*>(Cb) + (size_t)f * rows * GJB * sizeof(Z<S>);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -450,6 +488,7 @@ __global__ void __launch_bounds__(NT) k_gj_panel_reg(void* Qb, long long ld, lon
     #pragma unroll
     for (int k = 0; k < GJB; ++k) pv_[k] = k < w ? s_piv[k] : jb + k;
     if (tid < w) piv[tid] = s_piv[tid];
+    gj_row_map(tid, jb, w, s_piv, piv + GJB);
     if (tid == 0 && sing) atomicExch(singular, 1);
     // row of the original sub-panel now at row r: the swaps undone from the last
     auto origin = [&](int r) {
@@ -530,28 +569,32 @@ __global__ void __launch_bounds__(NT) k_gj_panel_reg(void* Qb, long long ld, lon
 
 // Columns [c0, width) of field blockIdx.y: the block's row swaps, then the
 // pivot rows become Binv x (also written transposed to YT, ld 16, for the
-// rank-w update of the other rows).
+// rank-w update of the other rows).  The swaps are applied through the row
+// map (gj_row_map): every value a column needs is read first, then the moved
+// rows and the pivot rows are written, one memory round trip per column.
+constexpr int GJS = 64;   // threads (columns) per block of k_gj_swap
 template <class S>
-__global__ void __launch_bounds__(256) k_gj_swap(void* Qb, long long ld, long long strideQ, int jb, int w, int c0,
+__global__ void __launch_bounds__(GJS) k_gj_swap(void* Qb, long long ld, long long strideQ, int jb, int w, int c0,
                                                  int width, const int* pivb, const void* Binvb, void* YTb) {
     const int f = blockIdx.y;
-    const int c = c0 + blockIdx.x * 256 + threadIdx.x;
-    if (c >= width) return;
+    const int c = c0 + blockIdx.x * GJS + threadIdx.x;
     char* Q = reinterpret_cast<char*>(Qb) + (size_t)f * strideQ * sizeof(Z<S>);
-    const int* piv = pivb + f * GJB;
+    const int* map = pivb + f * GJREC + GJB;
     const char* Binv = reinterpret_cast<const char*>(Binvb) + (size_t)f * GJB * GJB * sizeof(Z<S>);
     char* YT = reinterpret_cast<char*>(YTb) + (size_t)f * ld * GJB * sizeof(Z<S>);
-    for (int jj = 0; jj < w; ++jj) {
-        const int p = piv[jj], j = jb + jj;
-        if (p != j) {
-            const Z<S> a = ldz<S>(Q, (long long)j * ld + c), b = ldz<S>(Q, (long long)p * ld + c);
-            st<S>(Q, (long long)j * ld + c, b);
-            st<S>(Q, (long long)p * ld + c, a);
-        }
-    }
-    Z<S> x[GJB];
+    __shared__ Z<S> sb[GJB][GJB];
+    for (int e = threadIdx.x; e < GJB * GJB; e += GJS) sb[e / GJB][e % GJB] = ldz<S>(Binv, e);
+    __syncthreads();
+    if (c >= width) return;
+    Z<S> x[GJB], y[GJB];
     #pragma unroll
-    for (int m = 0; m < GJB; ++m) x[m] = m < w ? ldz<S>(Q, (long long)(jb + m) * ld + c) : Z<S>{S(0), S(0)};
+    for (int m = 0; m < GJB; ++m) {
+        x[m] = m < w ? ldz<S>(Q, (long long)map[m] * ld + c) : Z<S>{S(0), S(0)};
+        y[m] = (m < w && map[GJB + m] >= 0) ? ldz<S>(Q, (long long)map[2 * GJB + m] * ld + c) : Z<S>{S(0), S(0)};
+    }
+    #pragma unroll
+    for (int m = 0; m < GJB; ++m)
+        if (m < w && map[GJB + m] >= 0) st<S>(Q, (long long)map[GJB + m] * ld + c, y[m]);
     // Binv is zero outside its w x w block and x beyond w, so the full 16 x 16
     // product is exact; rows m >= w of YT are written as zeros
     #pragma unroll
@@ -559,7 +602,7 @@ __global__ void __launch_bounds__(256) k_gj_swap(void* Qb, long long ld, long lo
         Z<S> s{S(0), S(0)};
         #pragma unroll
         for (int l = 0; l < GJB; ++l) {
-            const Z<S> b = ldz<S>(Binv, (long long)m * GJB + l);
+            const Z<S> b = sb[m][l];
             s = {s.re + (b.re * x[l].re - b.im * x[l].im), s.im + (b.re * x[l].im + b.im * x[l].re)};
         }
         if (m < w) st<S>(Q, (long long)(jb + m) * ld + c, s);
@@ -666,7 +709,7 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
     check_extent(C, (long long)nf * rows * GJB - 1, e, "Gauss-Jordan C");
     check_extent(YT, (long long)nf * ldq * GJB - 1, e, "Gauss-Jordan YT");
     check_extent(binv, (long long)nf * GJB * GJB - 1, e, "Gauss-Jordan Binv");
-    check_extent(piv, (long long)nf * GJB - 1, sizeof(int), "Gauss-Jordan pivots");
+    check_extent(piv, (long long)nf * GJREC - 1, sizeof(int), "Gauss-Jordan pivot records");
     for (int jb = 0; jb < n_; jb += GJB) {
         const int w = std::min(GJB, n_ - jb);
         // register kernel shape for the live sub-panel rows: 256 threads with
@@ -707,18 +750,20 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
         });
         const int c0 = jb + w;
         const int ncol = width - c0;
-        // swaps (2w reads and writes), the pivot rows read and written, YT written
-        const double sbytes = (double)nf * ncol * 7.0 * w * e;
+        // the pivot rows' sources read and the pivot rows written (w each),
+        // YT written (16), the moved rows read and written (at most w each,
+        // counted in full)
+        const double sbytes = (double)nf * ncol * (4.0 * w + GJB) * e;
         const double sflops = (double)nf * ncol * 8.0 * GJB * GJB;
         p.custom([=](hipStream_t s) {
             ProfScope prof(prof_on() ? std::string(f64 ? "gj_swap_f64" : "gj_swap_f32") : std::string(), s);
             prof.work(sflops, sbytes);
-            dim3 g(blocks(ncol), nf);
+            dim3 g((ncol + GJS - 1) / GJS, nf);
             if (f64)
-                launch_timed(k_gj_swap<double>, g, dim3(256), 0, s, (void*)Q, ldq, strideQ, jb, w, c0, width,
+                launch_timed(k_gj_swap<double>, g, dim3(GJS), 0, s, (void*)Q, ldq, strideQ, jb, w, c0, width,
                              (const int*)piv, (const void*)binv, (void*)YT);
             else
-                launch_timed(k_gj_swap<float>, g, dim3(256), 0, s, (void*)Q, ldq, strideQ, jb, w, c0, width,
+                launch_timed(k_gj_swap<float>, g, dim3(GJS), 0, s, (void*)Q, ldq, strideQ, jb, w, c0, width,
                              (const int*)piv, (const void*)binv, (void*)YT);
             NSLX_LAUNCH_CHECK();
         });
@@ -780,7 +825,7 @@ void TimeSliceSolver::build_setup(hipStream_t s) {
     BPlan& p = setup_plan_;
     p = BPlan();
     const long long W = 5LL * n;
-    gj_piv_.alloc((size_t)nf * GJB);
+    gj_piv_.alloc((size_t)nf * GJREC);
     gj_sing_.alloc(1);
     gj_binv_.alloc((size_t)nf * GJB * GJB * esz_);
     gj_C_.alloc((size_t)nf * 2 * n * GJB * esz_);
