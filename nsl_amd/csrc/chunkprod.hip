@@ -196,6 +196,208 @@ __global__ void __launch_bounds__(CPT) k_chunkprod(int n, int nt, int c, const S
     }
 }
 
+// FP64 chunk products by 8-row blocks (grid (ceil(n / 8), nc, nf), 2
+// workgroups per CU): the re and im planes of the block's 8 rows are stacked
+// as the 16 rows of the MFMA's A operand, so one product serves both planes
+// and the grid has twice the blocks of k_chunkprod (1152 at n = 288, 2 chunks,
+// 16 fields: the last round of workgroups is a smaller share of the launch).
+// In the f64 16x16x4 output layout lane (q, r16) holds rows 4v + q: v = 0, 1
+// are the real parts of rows q and 4 + q, v = 2, 3 their imaginary parts, so
+// the phase of the next step is applied without a lane exchange.  Column
+// blocks are balanced over the 4 waves: wave w takes blocks w + 4b for
+// b < nblk / 4, and the nblk % 4 remaining blocks are split by K into
+// quarters, wave w taking quarter w of each; the quarters' partial sums meet
+// in LDS (added in wave order).
+// NF = nblk / 4 and RB = nblk % 4 are template arguments, so every loop
+// over blocks is static.
+constexpr int R8 = 8;
+template <int NF, int RB>
+__global__ void __launch_bounds__(CPT, 2) k_chunkprod8(int n, int nt, int c, const double* __restrict__ E,
+                                                        long long ldE, const double* __restrict__ P,
+                                                        double* __restrict__ Tt) {
+    using Acc = d4;
+    constexpr int ld = 64 * NF + 16 * RB + LPAD;
+    constexpr int RBS = RB > 0 ? RB : 1;
+    __shared__ __align__(16) double rows_lds[2][R8 * ld];
+    __shared__ __align__(16) Acc red[4][RBS][64];
+    const int np = (n + 15) & ~15;
+    double* sre = rows_lds[0];
+    double* sim = rows_lds[1];
+    const int r0 = blockIdx.x * R8, j = blockIdx.y, f = blockIdx.z;
+    const int nc = gridDim.y;
+    const int t0 = j * c;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int q = lane >> 4, r16 = lane & 15;
+    constexpr int full = NF, rem = RB;
+    const int nk = np / 16;                                         // 16-wide K chunks
+    const int klo = wave * nk / 4, khi = (wave + 1) * nk / 4;       // this wave's quarter
+    const double* Pf = P + 2 * (size_t)f * nt * n;
+
+    // v = e_{r0 + r} with the first step's phase P_{t0 - 1} applied
+    for (int e = tid; e < R8 * ld; e += CPT) {
+        const int r = e / ld, x = e % ld;
+        double vr = 0.0, vi = 0.0;
+        if (x == r0 + r && x < n) {
+            const int tp = (t0 - 1 + nt) % nt;
+            vr = Pf[2 * ((size_t)tp * n + x)];
+            vi = Pf[2 * ((size_t)tp * n + x) + 1];
+        }
+        sre[e] = vr;
+        sim[e] = vi;
+    }
+    __syncthreads();
+
+    // E rows of this lane: full block b covers columns 16 (wave + 4b) + r16
+    // (64 b rows past e0), remaining block r covers 16 (4 full + r) + r16
+    // (16 r rows past e1)
+    const double* e0 = E + (size_t)(16 * wave + r16) * ldE + 4 * q;
+    const double* e1 = E + (size_t)(16 * 4 * full + r16) * ldE + 4 * q;
+    const long long sb = 64 * ldE, sr = 16 * ldE;
+    // A operand: MFMA row r16 = real part of row r16 (< 8) or imaginary part of row r16 - 8
+    const double* arow = (r16 < R8 ? sre + r16 * ld : sim + (r16 - R8) * ld) + 4 * q;
+
+    for (int s = 0; s < c; ++s) {
+        const int t = t0 + s;
+        Acc acc[NF > 0 ? NF : 1], accr[RBS];
+        #pragma unroll
+        for (int b = 0; b < NF; ++b) acc[b] = Acc{0, 0, 0, 0};
+        #pragma unroll
+        for (int r = 0; r < RBS; ++r) accr[r] = Acc{0, 0, 0, 0};
+        if (s == 0) {
+            // the first step maps e_i to column i of E times P_{t0-1}[i]: no
+            // product needed (a remaining block's value goes to wave 0's part)
+            auto first = [&](int y, Acc& a) {
+                #pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int rr = (4 * v + q) & 7;
+                    const int i = r0 + rr;
+                    if (i < n) a[v] = (v < 2 ? sre : sim)[rr * ld + i] * E[(size_t)y * ldE + i];
+                }
+            };
+            #pragma unroll
+            for (int b = 0; b < NF; ++b)
+                if (b < full) first(16 * (wave + 4 * b) + r16, acc[b]);
+            if (wave == 0) {
+                #pragma unroll
+                for (int r = 0; r < RB; ++r)
+                    if (r < rem) first(16 * (4 * full + r) + r16, accr[r]);
+            }
+        } else {
+            // full blocks over all of K; the next chunk's E and A are loaded
+            // while the current chunk's MFMAs run (two register sets used in
+            // turn: no copies between chunks, so the loads of one set are
+            // never waited on by the products of the other)
+            constexpr int NFS = NF > 0 ? NF : 1;
+            double ea[NFS][4], eb[NFS][4], xa[4], xb[4];
+            auto load = [&](double (&e)[NFS][4], double (&x)[4], int kc) {
+                #pragma unroll
+                for (int b = 0; b < NF; ++b) ld4<double>(e0 + b * sb + 16 * kc, e[b]);
+                ld4<double>(arow + 16 * kc, x);
+            };
+            auto prod = [&](const double (&e)[NFS][4], const double (&x)[4]) {
+                #pragma unroll
+                for (int ks = 0; ks < 4; ++ks)
+                    #pragma unroll
+                    for (int b = 0; b < NF; ++b) acc[b] = CP<double>::mfma(x[ks], e[b][ks], acc[b]);
+            };
+            if (NF > 0) {
+                // chunk pairs with unconditional loads (the last pair loads
+                // chunk nk - 1 again), so the wait counts are exact on the
+                // loop's single path
+                load(ea, xa, 0);
+                #pragma unroll 1
+                for (int pp = 0; pp < nk / 2; ++pp) {
+                    load(eb, xb, 2 * pp + 1);
+                    prod(ea, xa);
+                    load(ea, xa, min(2 * pp + 2, nk - 1));
+                    prod(eb, xb);
+                }
+                if (nk & 1) prod(ea, xa);
+            }
+            double xc[4];
+            // remaining blocks over this wave's quarter of K (their E values
+            // loaded at the top of each chunk: few chunks, and the other
+            // workgroup on the CU covers the wait)
+            if (RB > 0 && klo < khi) {
+                #pragma unroll 1
+                for (int kc = klo; kc < khi; ++kc) {
+                    double rc[RBS][4];
+                    #pragma unroll
+                    for (int r = 0; r < RB; ++r) ld4<double>(e1 + r * sr + 16 * kc, rc[r]);
+                    ld4<double>(arow + 16 * kc, xc);
+                    #pragma unroll
+                    for (int ks = 0; ks < 4; ++ks)
+                        #pragma unroll
+                        for (int r = 0; r < RB; ++r) accr[r] = CP<double>::mfma(xc[ks], rc[r][ks], accr[r]);
+                }
+            }
+        }
+        // the remaining blocks' quarter sums; every wave is done reading this
+        // step's rows after the barrier
+        if (rem > 0) {
+            #pragma unroll
+            for (int r = 0; r < RB; ++r)
+                if (r < rem) red[wave][r][lane] = accr[r];
+        }
+        __syncthreads();
+        Acc accw = Acc{0, 0, 0, 0};
+        if (wave < rem) {
+            #pragma unroll
+            for (int w2 = 0; w2 < 4; ++w2) accw += red[w2][wave][lane];
+        }
+        const double sg = t == 0 ? -1.0 : 1.0;
+        const bool last = s == c - 1;
+        auto out = [&](int y, const Acc& a) {
+            double pr = 0.0, pi = 0.0;
+            if (!last && y < n) {
+                pr = Pf[2 * ((size_t)t * n + y)];
+                pi = Pf[2 * ((size_t)t * n + y) + 1];
+            }
+            #pragma unroll
+            for (int v = 0; v < 2; ++v) {
+                const int rr = 4 * v + q;
+                const double vr = sg * a[v], vi = sg * a[v + 2];
+                if (last) {
+                    const int r = r0 + rr;
+                    if (r < n && y < n) {
+                        double* o = Tt + 2 * ((((size_t)f * nc + j) * n + r) * n + y);
+                        o[0] = vr;
+                        o[1] = vi;
+                    }
+                } else {
+                    // the next step's A: P_t . v (zero beyond column n)
+                    sre[rr * ld + y] = pr * vr - pi * vi;
+                    sim[rr * ld + y] = pr * vi + pi * vr;
+                }
+            }
+        };
+        #pragma unroll
+        for (int b = 0; b < NF; ++b)
+            if (b < full) out(16 * (wave + 4 * b) + r16, acc[b]);
+        if (wave < rem) out(16 * (4 * full + wave) + r16, accw);
+        __syncthreads();
+    }
+}
+
+template <int NF, int RB>
+void launch_k8(int n, int nt, int c, int nc, int nf, const void* E, long long ldE, const void* P, void* Tt,
+               hipStream_t s) {
+    const dim3 grid((n + R8 - 1) / R8, nc, nf);
+    launch_timed(k_chunkprod8<NF, RB>, grid, dim3(CPT), 0, s, n, nt, c, static_cast<const double*>(E), ldE,
+                 static_cast<const double*>(P), static_cast<double*>(Tt));
+}
+
+template <int NF>
+void launch_k8r(int rb, int n, int nt, int c, int nc, int nf, const void* E, long long ldE, const void* P, void* Tt,
+                hipStream_t s) {
+    switch (rb) {
+        case 0: launch_k8<NF, 0>(n, nt, c, nc, nf, E, ldE, P, Tt, s); break;
+        case 1: launch_k8<NF, 1>(n, nt, c, nc, nf, E, ldE, P, Tt, s); break;
+        case 2: launch_k8<NF, 2>(n, nt, c, nc, nf, E, ldE, P, Tt, s); break;
+        default: launch_k8<NF, 3>(n, nt, c, nc, nf, E, ldE, P, Tt, s); break;
+    }
+}
+
 template <class S, int MAXB>
 void launch_k(int n, int nt, int c, int nc, int nf, const void* E, long long ldE, const void* P, void* Tt,
               hipStream_t s) {
@@ -207,6 +409,29 @@ void launch_k(int n, int nt, int c, int nc, int nf, const void* E, long long ldE
 template <class S>
 void launch_s(int n, int nt, int c, int nc, int nf, const void* E, long long ldE, const void* P, void* Tt,
               hipStream_t s) {
+    if constexpr (sizeof(S) == 8) {
+        static const bool rows8 = [] {
+            const char* env = std::getenv("NSLX_CHUNKPROD8");
+            return !(env && std::atoi(env) == 0);
+        }();
+        // the 8-row kernel where its registers fit two workgroups per CU
+        // without spilling: up to 18 column blocks (n <= 288)
+        const int nblk = (n + 15) / 16, full = nblk / 4, rb = nblk % 4;   // full column blocks per wave
+        if (rows8 && nblk <= 18) {
+            switch (full) {
+                case 0: launch_k8r<0>(rb, n, nt, c, nc, nf, E, ldE, P, Tt, s); break;
+                case 1: launch_k8r<1>(rb, n, nt, c, nc, nf, E, ldE, P, Tt, s); break;
+                case 2: launch_k8r<2>(rb, n, nt, c, nc, nf, E, ldE, P, Tt, s); break;
+                case 3: launch_k8r<3>(rb, n, nt, c, nc, nf, E, ldE, P, Tt, s); break;
+                default:
+                    if (rb == 0) launch_k8<4, 0>(n, nt, c, nc, nf, E, ldE, P, Tt, s);
+                    else if (rb == 1) launch_k8<4, 1>(n, nt, c, nc, nf, E, ldE, P, Tt, s);
+                    else launch_k8<4, 2>(n, nt, c, nc, nf, E, ldE, P, Tt, s);
+                    break;
+            }
+            return;
+        }
+    }
     const int need = ((n + 15) / 16 + 3) / 4;   // column blocks per wave
     if (need <= 1) launch_k<S, 1>(n, nt, c, nc, nf, E, ldE, P, Tt, s);
     else if (need <= 2) launch_k<S, 2>(n, nt, c, nc, nf, E, ldE, P, Tt, s);
