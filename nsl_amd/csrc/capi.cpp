@@ -267,6 +267,7 @@ int nslx_fm_solve_direct(nslx_fm fm, int op, const void* in, void* out, int64_t 
         auto s = nslx::as_stream(stream);
         auto& ts = m.time_slice_solver(precision, nchunks, false, s);
         ts.apply(op, in, out, nbatch, false, s);
+        ts.check_singular(s, false);
         ts.check_passes(s);
         return NSLX_OK;
     });

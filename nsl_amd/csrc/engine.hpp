@@ -72,6 +72,8 @@ struct FermionMatrix {
 
     int64_t nfield = 0;
     double max_im_phi = 0.0;        // max |Im phi| over the populated fields
+    DevBuf<double> mim_part;        // its device reduction (256 block maxima + result)
+    PinBuf<double> mim_h;
     DevBuf<double2> phi;            // [nfield][nt][nx]
     DevBuf<double2> P64, Pinv64;    // phases
     DevBuf<float2> P32;

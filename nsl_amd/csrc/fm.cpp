@@ -139,13 +139,14 @@ void FermionMatrix::populate(const void* phi_dev, int64_t nf, int sp, hipStream_
     // deep copy of phi (the reference's populate copies into its own buffer)
     NSLX_HIP(hipMemcpyAsync(phi.p, phi_dev, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
     phase_tables(phi.p, n, sgn(), delta() * mu, delta() * mu_im, P64.p, Pinv64.p, P32.p, s);
-    // max |Im phi| bounds |P| (used by the solvers' attainable-accuracy test)
-    std::vector<double2> h((size_t)n);
-    NSLX_HIP(hipMemcpyAsync(h.data(), phi.p, n * sizeof(double2), hipMemcpyDeviceToHost, s));
+    // max |Im phi| bounds |P| (used by the solvers' attainable-accuracy test
+    // and the time-slice chunk rule): reduced on the device, one value back
+    mim_part.ensure(257);
+    mim_h.ensure(1);
+    max_abs_im(phi.p, n, mim_part.p, mim_part.p + 256, s);
+    NSLX_HIP(hipMemcpyAsync(mim_h.p, mim_part.p + 256, sizeof(double), hipMemcpyDeviceToHost, s));
     NSLX_HIP(hipStreamSynchronize(s));
-    double m = 0.0;
-    for (const double2& z : h) m = std::max(m, std::fabs(z.y));
-    max_im_phi = m;
+    max_im_phi = mim_h.p[0];
     // build (or find) the hop matrix of this species now, so that applies
     // after populate allocate nothing (nslx_fm_apply_noalloc relies on it)
     (void)Eptr(NSLX_F64, s);

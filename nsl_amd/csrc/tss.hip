@@ -806,11 +806,19 @@ void TimeSliceSolver::setup(hipStream_t s) {
     if (!setup_built_) build_setup(s);
     NSLX_HIP(hipMemsetAsync(gj_sing_.p, 0, sizeof(int), s));
     setup_plan_.run(s);
-    int singular = 0;
-    NSLX_HIP(hipMemcpyAsync(&singular, gj_sing_.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    NSLX_HIP(hipStreamSynchronize(s));
-    if (singular) throw Error(NSLX_E_STATE, "time-slice solve: the reduced block system is singular");
+    // the singular flag goes to pinned memory without a wait: the caller
+    // checks it at its next synchronisation (check_singular)
+    sing_h_.ensure(1);
+    NSLX_HIP(hipMemcpyAsync(sing_h_.p, gj_sing_.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    sing_pending_ = true;
     gen_ = fm_.pop_gen;
+}
+
+void TimeSliceSolver::check_singular(hipStream_t s, bool synced) {
+    if (!sing_pending_) return;
+    if (!synced) NSLX_HIP(hipStreamSynchronize(s));
+    sing_pending_ = false;
+    if (sing_h_.p[0]) throw Error(NSLX_E_STATE, "time-slice solve: the reduced block system is singular");
 }
 
 void TimeSliceSolver::build_setup(hipStream_t s) {

@@ -184,6 +184,10 @@ class TimeSliceSolver {
     // throws when a persistent recurrence launch of an apply since the last
     // check gave up waiting (synchronises s)
     void check_passes(hipStream_t s);
+    // throws when the last factorisation found the reduced block system
+    // singular.  The factorisation only queues the flag's copy to the host;
+    // synced: s has been synchronised since (else this synchronises it).
+    void check_singular(hipStream_t s, bool synced);
 
    private:
     struct ApplyBufs {
@@ -204,6 +208,8 @@ class TimeSliceSolver {
     const void* e_used_ = nullptr;
     const void* p_used_ = nullptr;
     DevBuf<int> gj_piv_, gj_sing_;
+    PinBuf<int> sing_h_;
+    bool sing_pending_ = false;
     DevBuf<unsigned char> gj_binv_, gj_C_, gj_YT_;
     void setup(hipStream_t s);
     void build_setup(hipStream_t s);

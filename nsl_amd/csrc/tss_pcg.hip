@@ -36,14 +36,31 @@ __global__ void k_alpha(int64_t nb, const double* rz, const double* pt, const in
     alpha[2 * b + 1] = ai;
 }
 
-// done_b = it once |r_b|^2 <= max(rho2 |r0_b|^2, abs2 |b_b|^2); counts the rest
-__global__ void k_inner_check(int64_t nb, const double* rn2, const double* r0n2, const double* bn2, double rho2,
-                              double abs2, int it, int* done, int* nactive) {
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    if (done[b] >= 0) return;
-    if (rn2[b] <= rho2 * r0n2[b] || rn2[b] <= abs2 * bn2[b] || !(rn2[b] == rn2[b])) done[b] = it;
-    else atomicAdd(nactive, 1);
+// *nactive = the count of c over one block of 256 threads (no memset, no
+// atomics: the one block writes it)
+__device__ inline void block_count(int c, int* nactive) {
+    __shared__ int red[256];
+    red[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *nactive = red[0];
+}
+
+// done_b = it once |r_b|^2 <= max(rho2 |r0_b|^2, abs2 |b_b|^2); counts the
+// rest (one block)
+__global__ void __launch_bounds__(256) k_inner_check(int64_t nb, const double* rn2, const double* r0n2,
+                                                     const double* bn2, double rho2, double abs2, int it, int* done,
+                                                     int* nactive) {
+    int c = 0;
+    for (int64_t b = threadIdx.x; b < nb; b += 256) {
+        if (done[b] >= 0) continue;
+        if (rn2[b] <= rho2 * r0n2[b] || rn2[b] <= abs2 * bn2[b] || !(rn2[b] == rn2[b])) done[b] = it;
+        else ++c;
+    }
+    block_count(c, nactive);
 }
 
 // beta_b = Re(-conj(alpha_b) <t, z_new>_b) / rz_old_b; rz_old <- Re <r, z_new>
@@ -62,14 +79,33 @@ __global__ void k_beta(int64_t nb, const double* alpha, const double* tz, const 
     rz[b] = rzn[2 * b];
 }
 
-// outer test on the double-double residual: done_b = round when |r|^2 <= tol2 |b|^2
-__global__ void k_outer_check(int64_t nb, const double* rn2, const double* bn2, double tol2, int round, int* done,
-                              int* nactive) {
+// outer test on the double-double residual: done_b = round when |r|^2 <=
+// tol2 |b|^2; counts the rest (one block)
+__global__ void __launch_bounds__(256) k_outer_check(int64_t nb, const double* rn2, const double* bn2, double tol2,
+                                                     int round, int* done, int* nactive) {
+    int c = 0;
+    for (int64_t b = threadIdx.x; b < nb; b += 256) {
+        if (done[b] >= 0) continue;
+        if (rn2[b] <= tol2 * bn2[b]) done[b] = round;
+        else ++c;
+    }
+    block_count(c, nactive);
+}
+
+// start of an outer step: done_in = done_out, r0n2 = rn2
+__global__ void k_outer_begin(int64_t nb, const int* done_out, int* done_in, const double* rn2, double* r0n2) {
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
-    if (done[b] >= 0) return;
-    if (rn2[b] <= tol2 * bn2[b]) done[b] = round;
-    else atomicAdd(nactive, 1);
+    done_in[b] = done_out[b];
+    r0n2[b] = rn2[b];
+}
+
+// start of the solve: every RHS open; rn2 = bn2 when x starts at zero
+__global__ void k_solve_begin(int64_t nb, const double* bn2, double* rn2, int* done_out) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    done_out[b] = -1;
+    if (bn2) rn2[b] = bn2[b];
 }
 
 __global__ void k_rz_init(int64_t nb, const double* rzn, double* rz) {
@@ -140,26 +176,33 @@ int solve_tss(FermionMatrix& fm, int op, const void* b, void* x, void* xlo, cons
         NSLX_HIP(hipMemsetAsync(x, 0, bytes, s));
         NSLX_HIP(hipMemsetAsync(xl, 0, bytes, s));
         NSLX_HIP(hipMemcpyAsync(r, b, bytes, hipMemcpyDeviceToDevice, s));
-        NSLX_HIP(hipMemcpyAsync(rn2, bn2, nb * sizeof(double), hipMemcpyDeviceToDevice, s));
     }
-    NSLX_HIP(hipMemsetAsync(done_out, 0xFF, nb * sizeof(int), s));
-    auto count_outer = [&](int round) {
-        NSLX_HIP(hipMemsetAsync(nact, 0, sizeof(int), s));
-        hipLaunchKernelGGL(k_outer_check, dim3(g1(nb)), dim3(256), 0, s, nb, rn2, bn2, tol * tol, round, done_out, nact);
+    hipLaunchKernelGGL(k_solve_begin, dim3(g1(nb)), dim3(256), 0, s, nb, x0 ? nullptr : (const double*)bn2, rn2,
+                       done_out);
+    NSLX_LAUNCH_CHECK();
+    auto check_outer = [&](int round) {
+        hipLaunchKernelGGL(k_outer_check, dim3(1), dim3(256), 0, s, nb, rn2, bn2, tol * tol, round, done_out, nact);
         NSLX_LAUNCH_CHECK();
+    };
+    auto count_outer = [&](int round) {
+        check_outer(round);
         NSLX_HIP(hipMemcpyAsync(nact_h, nact, sizeof(int), hipMemcpyDeviceToHost, s));
         NSLX_HIP(hipStreamSynchronize(s));
+        ts.check_singular(s, true);
         return *nact_h;
     };
-    int active = count_outer(0);
+    // round 0 marks the RHS already solved on the device; the host takes
+    // every RHS as open until the first inner test's count (no wait here)
+    check_outer(0);
+    int active = (int)nb;
     const double rho = env_double("NSLX_TSS_RHO", 1e-6);
     const int max_outer = (int)env_double("NSLX_TSS_OUTER", 24.0);
     const int max_inner = (int)env_double("NSLX_TSS_INNER", 64.0);
     int64_t inner_total = 0, inner_max = 0, nouter = 0, napply = 0;
     while (active > 0 && nouter < max_outer && inner_total < maxit) {
         ++nouter;
-        NSLX_HIP(hipMemcpyAsync(done_in, done_out, nb * sizeof(int), hipMemcpyDeviceToDevice, s));
-        NSLX_HIP(hipMemcpyAsync(r0n2, rn2, nb * sizeof(double), hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_outer_begin, dim3(g1(nb)), dim3(256), 0, s, nb, done_out, done_in, rn2, r0n2);
+        NSLX_LAUNCH_CHECK();
         NSLX_HIP(hipMemsetAsync(d, 0, bytes, s));
         ts.apply(op, r, z, nb, true, s);
         ++napply;
@@ -178,12 +221,12 @@ int solve_tss(FermionMatrix& fm, int op, const void* b, void* x, void* xlo, cons
             axpy_perb(1, d, p, d, alpha, 1.0, nb, len, 0, done_in, s);     // d += alpha p
             axpy_perb(1, r, t, r, alpha, -1.0, nb, len, 0, done_in, s);    // r -= alpha t
             norm2_perb(1, r, nb, len, rn2, fm.vsc, s);
-            NSLX_HIP(hipMemsetAsync(nact, 0, sizeof(int), s));
-            hipLaunchKernelGGL(k_inner_check, dim3(g1(nb)), dim3(256), 0, s, nb, rn2, r0n2, bn2, rho * rho,
+            hipLaunchKernelGGL(k_inner_check, dim3(1), dim3(256), 0, s, nb, rn2, r0n2, bn2, rho * rho,
                                0.09 * tol * tol, it, done_in, nact);
             NSLX_LAUNCH_CHECK();
             NSLX_HIP(hipMemcpyAsync(nact_h, nact, sizeof(int), hipMemcpyDeviceToHost, s));
             NSLX_HIP(hipStreamSynchronize(s));
+            ts.check_singular(s, true);
             inner_active = *nact_h;
             if (inner_active == 0) break;
             ts.apply(op, r, z, nb, true, s);
@@ -206,6 +249,7 @@ int solve_tss(FermionMatrix& fm, int op, const void* b, void* x, void* xlo, cons
     NSLX_HIP(hipMemcpyAsync(hb.data(), bn2, nb * sizeof(double), hipMemcpyDeviceToHost, s));
     NSLX_HIP(hipMemcpyAsync(hd.data(), done_out, nb * sizeof(int), hipMemcpyDeviceToHost, s));
     NSLX_HIP(hipStreamSynchronize(s));
+    ts.check_singular(s, true);
     ts.check_passes(s);
     int64_t conv = 0;
     double maxrel = 0.0, tot = 0.0;

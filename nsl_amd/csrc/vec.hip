@@ -1,4 +1,6 @@
 // Vector kernels with deterministic per-RHS reductions (see vec.hpp).
+#include <algorithm>
+
 #include "vec.hpp"
 
 namespace nslx {
@@ -290,6 +292,43 @@ void VecScratch::ensure(int64_t nb, int64_t len, hipStream_t s) {
 static void finish(const double* part, int nchunk, int64_t nb, double* out, int mode, hipStream_t s) {
     const unsigned g = mode == 2 ? 1u : grid1(nb, VNT);
     hipLaunchKernelGGL(k_finish, dim3(g), dim3(VNT), 0, s, part, nchunk, (int)nb, out, mode);
+    NSLX_LAUNCH_CHECK();
+}
+
+// max |Im phi| over n values: block maxima, then one block over those
+// (no atomics, no memset: every slot is written)
+constexpr int MIM_BLOCKS = 256;
+__global__ void __launch_bounds__(256) k_max_abs_im(const double2* __restrict__ phi, int64_t n, double* part) {
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        m = fmax(m, fabs(phi[i].y));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+__global__ void __launch_bounds__(256) k_max_part(const double* __restrict__ part, int np, double* out) {
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < np; i += 256) m = fmax(m, part[i]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+void max_abs_im(const void* phi, int64_t n, double* part, double* out, hipStream_t s) {
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(MIM_BLOCKS, (n + 255) / 256));
+    hipLaunchKernelGGL(k_max_abs_im, dim3(g), dim3(256), 0, s, (const double2*)phi, n, part);
+    NSLX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_max_part, dim3(1), dim3(256), 0, s, (const double*)part, (int)g, out);
     NSLX_LAUNCH_CHECK();
 }
 

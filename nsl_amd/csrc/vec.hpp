@@ -21,6 +21,9 @@ struct VecScratch {
 void phase_tables(const void* phi, int64_t n, int sgn, double mut, double mut_im, void* P64, void* Pinv64,
                   void* P32, hipStream_t s);
 
+// out[0] = max |Im phi| over n c128 values (device); part: 256 doubles of scratch.
+void max_abs_im(const void* phi, int64_t n, double* part, double* out, hipStream_t s);
+
 // out[b] = sum |v_b|^2 (double), S = double (c128) or float (c64).
 void norm2_perb(int prec_is_f64, const void* v, int64_t nb, int64_t len, double* out,
                 VecScratch& sc, hipStream_t s);
