@@ -80,7 +80,9 @@ struct Acc {
     __device__ inline dd value() const { return quick(s, c); }
 };
 
-constexpr int DCOLS = 128;   // threads per block: one output column each
+// threads per block, one output column each: one wave, so that Nx = 288
+// fills 4.5 blocks of columns (90 % of the threads busy; 75 % with 128)
+constexpr int DCOLS = 64;
 constexpr int DROWS = 8;     // output rows (rhs, t) per block
 constexpr int DKB = 32;      // K chunk staged in LDS
 
@@ -172,14 +174,17 @@ __global__ void __launch_bounds__(DCOLS) k_dd_hop(int n, int nt, long long nrows
         }
     }
     if (FINAL) {
-        __shared__ double red[DROWS][DCOLS];
+        // the row sums reuse the staging buffer (free since the K loop's
+        // last barrier): no extra LDS, so as many blocks fit a CU as without
+        static_assert(DROWS * DCOLS <= 4 * DROWS * (DKB + 1), "row sums fit the staging buffer");
+        double* red = &sh[0][0][0];
         #pragma unroll
-        for (int rr = 0; rr < DROWS; ++rr) red[rr][tid] = rsq[rr];
+        for (int rr = 0; rr < DROWS; ++rr) red[rr * DCOLS + tid] = rsq[rr];
         __syncthreads();
         if (tid < DROWS) {
             const long long g = g0 + tid;
             double sum = 0.0;
-            for (int k = 0; k < DCOLS; ++k) sum += red[tid][k];
+            for (int k = 0; k < DCOLS; ++k) sum += red[tid * DCOLS + k];
             if (g < nrows) part[g * gridDim.x + blockIdx.x] = sum;
         }
     }

@@ -243,11 +243,15 @@ int solve_tss(FermionMatrix& fm, int op, const void* b, void* x, void* xlo, cons
         active = count_outer((int)nouter);
     }
     // report
-    std::vector<double> hr(nb), hb(nb);
-    std::vector<int> hd(nb);
-    NSLX_HIP(hipMemcpyAsync(hr.data(), rn2, nb * sizeof(double), hipMemcpyDeviceToHost, s));
-    NSLX_HIP(hipMemcpyAsync(hb.data(), bn2, nb * sizeof(double), hipMemcpyDeviceToHost, s));
-    NSLX_HIP(hipMemcpyAsync(hd.data(), done_out, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+    // per-RHS residuals, norms and flags into pinned memory, one wait
+    w.dbl_h.ensure(2 * (size_t)nb);
+    w.int_h.ensure((size_t)nb);
+    double* hr = w.dbl_h.p;
+    double* hb = w.dbl_h.p + nb;
+    int* hd = w.int_h.p;
+    NSLX_HIP(hipMemcpyAsync(hr, rn2, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+    NSLX_HIP(hipMemcpyAsync(hb, bn2, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+    NSLX_HIP(hipMemcpyAsync(hd, done_out, nb * sizeof(int), hipMemcpyDeviceToHost, s));
     NSLX_HIP(hipStreamSynchronize(s));
     ts.check_singular(s, true);
     ts.check_passes(s);
