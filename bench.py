@@ -34,6 +34,10 @@ Prints one JSON line.  Profiling helpers (no JSON contract line):
                                     --precond precision), for rocprofv3 runs
     python bench.py --copy-calib M  three device copies of M MiB (scales the
                                     FETCH_SIZE / WRITE_SIZE counters)
+    python bench.py --dgemm-ceiling N  the FP64 matrix-core rate the platform
+                                    library reaches (torch.matmul, N x N x N
+                                    float64): the practical ceiling beside the
+                                    published 78.6 TFLOP/s
 """
 
 import argparse
@@ -525,6 +529,24 @@ def copy_calib(mib):
     print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
 
 
+def dgemm_ceiling(n):
+    """FP64 GEMM rate of the platform library (rocBLAS / hipBLASLt through
+    torch.matmul) on n x n x n: 2 warm-up products, then 5 timed."""
+    import torch
+    a = torch.randn(n, n, dtype=torch.float64, device="cuda")
+    b = torch.randn(n, n, dtype=torch.float64, device="cuda")
+    for _ in range(2):
+        c = a @ b
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        c = a @ b
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 5
+    print(json.dumps({"dgemm_n": n, "ms": round(dt * 1e3, 3), "tflops": round(2.0 * n ** 3 / dt / 1e12, 2),
+                      "checksum": float(c[0, 0])}), flush=True)
+
+
 def refcrit_leg(phi_h, b_h, nfields):
     """The reference's criterion in FP64: per field, CG on that field's 8 RHS
     with x0 = b, one alpha/beta for the batch and sum |r|^2 <= 1e-24 on the
@@ -575,11 +597,14 @@ def main():
     ap.add_argument("--precond", choices=["f64", "f32"], default="f64",
                     help="precision of the time-slice preconditioner of the headline (and of --tss)")
     ap.add_argument("--copy-calib", type=int, default=0)
+    ap.add_argument("--dgemm-ceiling", type=int, default=0)
     ap.add_argument("--prof-every", type=int, default=PROF_EVERY,
                     help="time every n-th launch of a kernel class with HIP events (0: none)")
     args = ap.parse_args()
     if args.copy_calib:
         return copy_calib(args.copy_calib)
+    if args.dgemm_ceiling:
+        return dgemm_ceiling(args.dgemm_ceiling)
     if args.hops:
         return hop_profile(args.hops)
     if args.tss:
