@@ -258,6 +258,17 @@ __global__ void __launch_bounds__(CPT, 2) k_chunkprod8(int n, int nt, int c, con
 
     for (int s = 0; s < c; ++s) {
         const int t = t0 + s;
+        const bool last = s == c - 1;
+        // this step's phases P_t of the lane's output columns, loaded now so
+        // that the epilogue does not wait for them
+        double pph[NF + 1][2];
+        #pragma unroll
+        for (int b = 0; b <= NF; ++b) {
+            const int y = b < NF ? 16 * (wave + 4 * b) + r16 : 16 * (4 * full + wave) + r16;
+            const bool on = !last && y < n && (b < NF || wave < rem);
+            pph[b][0] = on ? Pf[2 * ((size_t)t * n + y)] : 0.0;
+            pph[b][1] = on ? Pf[2 * ((size_t)t * n + y) + 1] : 0.0;
+        }
         Acc acc[NF > 0 ? NF : 1], accr[RBS];
         #pragma unroll
         for (int b = 0; b < NF; ++b) acc[b] = Acc{0, 0, 0, 0};
@@ -288,6 +299,25 @@ __global__ void __launch_bounds__(CPT, 2) k_chunkprod8(int n, int nt, int c, con
             // turn: no copies between chunks, so the loads of one set are
             // never waited on by the products of the other)
             constexpr int NFS = NF > 0 ? NF : 1;
+            // the remaining blocks' first chunk is loaded before the full
+            // blocks' products, so its wait is hidden behind them
+            const bool quarter = RB > 0 && klo < khi;
+            double ra[RBS][4], rb2[RBS][4], qa[4], qb[4];
+            auto loadr = [&](double (&e)[RBS][4], double (&x)[4], int kc) {
+                #pragma unroll
+                for (int r = 0; r < RB; ++r) ld4<double>(e1 + r * sr + 16 * kc, e[r]);
+                ld4<double>(arow + 16 * kc, x);
+            };
+            auto prodr = [&](const double (&e)[RBS][4], const double (&x)[4]) {
+                #pragma unroll
+                for (int ks = 0; ks < 4; ++ks)
+                    #pragma unroll
+                    for (int r = 0; r < RB; ++r) accr[r] = CP<double>::mfma(x[ks], e[r][ks], accr[r]);
+            };
+            if (quarter) {
+                #pragma unroll
+                for (int r = 0; r < RB; ++r) ld4<double>(e1 + r * sr + 16 * klo, ra[r]);
+            }
             double ea[NFS][4], eb[NFS][4], xa[4], xb[4];
             auto load = [&](double (&e)[NFS][4], double (&x)[4], int kc) {
                 #pragma unroll
@@ -314,22 +344,19 @@ __global__ void __launch_bounds__(CPT, 2) k_chunkprod8(int n, int nt, int c, con
                 }
                 if (nk & 1) prod(ea, xa);
             }
-            double xc[4];
-            // remaining blocks over this wave's quarter of K (their E values
-            // loaded at the top of each chunk: few chunks, and the other
-            // workgroup on the CU covers the wait)
-            if (RB > 0 && klo < khi) {
+            // remaining blocks over this wave's quarter of K, in the same
+            // alternation of two register sets
+            if (quarter) {
+                const int nq = khi - klo;
+                ld4<double>(arow + 16 * klo, qa);
                 #pragma unroll 1
-                for (int kc = klo; kc < khi; ++kc) {
-                    double rc[RBS][4];
-                    #pragma unroll
-                    for (int r = 0; r < RB; ++r) ld4<double>(e1 + r * sr + 16 * kc, rc[r]);
-                    ld4<double>(arow + 16 * kc, xc);
-                    #pragma unroll
-                    for (int ks = 0; ks < 4; ++ks)
-                        #pragma unroll
-                        for (int r = 0; r < RB; ++r) accr[r] = CP<double>::mfma(xc[ks], rc[r][ks], accr[r]);
+                for (int pp = 0; pp < nq / 2; ++pp) {
+                    loadr(rb2, qb, klo + 2 * pp + 1);
+                    prodr(ra, qa);
+                    loadr(ra, qa, min(klo + 2 * pp + 2, khi - 1));
+                    prodr(rb2, qb);
                 }
+                if (nq & 1) prodr(ra, qa);
             }
         }
         // the remaining blocks' quarter sums; every wave is done reading this
@@ -346,13 +373,8 @@ __global__ void __launch_bounds__(CPT, 2) k_chunkprod8(int n, int nt, int c, con
             for (int w2 = 0; w2 < 4; ++w2) accw += red[w2][wave][lane];
         }
         const double sg = t == 0 ? -1.0 : 1.0;
-        const bool last = s == c - 1;
-        auto out = [&](int y, const Acc& a) {
-            double pr = 0.0, pi = 0.0;
-            if (!last && y < n) {
-                pr = Pf[2 * ((size_t)t * n + y)];
-                pi = Pf[2 * ((size_t)t * n + y) + 1];
-            }
+        auto out = [&](int y, const Acc& a, const double (&ph)[2]) {
+            const double pr = ph[0], pi = ph[1];
             #pragma unroll
             for (int v = 0; v < 2; ++v) {
                 const int rr = 4 * v + q;
@@ -372,9 +394,8 @@ __global__ void __launch_bounds__(CPT, 2) k_chunkprod8(int n, int nt, int c, con
             }
         };
         #pragma unroll
-        for (int b = 0; b < NF; ++b)
-            if (b < full) out(16 * (wave + 4 * b) + r16, acc[b]);
-        if (wave < rem) out(16 * (4 * full + wave) + r16, accw);
+        for (int b = 0; b < NF; ++b) out(16 * (wave + 4 * b) + r16, acc[b], pph[b]);
+        if (wave < rem) out(16 * (4 * full + wave) + r16, accw, pph[NF]);
         __syncthreads();
     }
 }
