@@ -34,10 +34,10 @@ Prints one JSON line.  Profiling helpers (no JSON contract line):
                                     --precond precision), for rocprofv3 runs
     python bench.py --copy-calib M  three device copies of M MiB (scales the
                                     FETCH_SIZE / WRITE_SIZE counters)
-    python bench.py --dgemm-ceiling N  the FP64 matrix-core rate the platform
-                                    library reaches (torch.matmul, N x N x N
-                                    float64): the practical ceiling beside the
-                                    published 78.6 TFLOP/s
+    python bench.py --dgemm-ceiling S  the FP64 matrix-core rate the platform
+                                    library reaches (torch.matmul, float64) on
+                                    S = N (N x N x N) or MxNxK: the practical
+                                    ceiling beside the published 78.6 TFLOP/s
 """
 
 import argparse
@@ -529,21 +529,25 @@ def copy_calib(mib):
     print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
 
 
-def dgemm_ceiling(n):
+def dgemm_ceiling(shape):
     """FP64 GEMM rate of the platform library (rocBLAS / hipBLASLt through
-    torch.matmul) on n x n x n: 2 warm-up products, then 5 timed."""
+    torch.matmul) on M x N x K (shape "N" or "MxNxK"): 2 warm-up products,
+    then 5 timed."""
     import torch
-    a = torch.randn(n, n, dtype=torch.float64, device="cuda")
-    b = torch.randn(n, n, dtype=torch.float64, device="cuda")
+    dims = [int(v) for v in shape.lower().split("x")]
+    m, n, k = dims * 3 if len(dims) == 1 else dims
+    a = torch.randn(m, k, dtype=torch.float64, device="cuda")
+    b = torch.randn(k, n, dtype=torch.float64, device="cuda")
     for _ in range(2):
         c = a @ b
     torch.cuda.synchronize()
+    reps = 5 if m * n * k >= 1 << 33 else 50
     t0 = time.perf_counter()
-    for _ in range(5):
+    for _ in range(reps):
         c = a @ b
     torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / 5
-    print(json.dumps({"dgemm_n": n, "ms": round(dt * 1e3, 3), "tflops": round(2.0 * n ** 3 / dt / 1e12, 2),
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"dgemm": [m, n, k], "us": round(dt * 1e6, 1), "tflops": round(2.0 * m * n * k / dt / 1e12, 2),
                       "checksum": float(c[0, 0])}), flush=True)
 
 
@@ -597,7 +601,7 @@ def main():
     ap.add_argument("--precond", choices=["f64", "f32"], default="f64",
                     help="precision of the time-slice preconditioner of the headline (and of --tss)")
     ap.add_argument("--copy-calib", type=int, default=0)
-    ap.add_argument("--dgemm-ceiling", type=int, default=0)
+    ap.add_argument("--dgemm-ceiling", default="")
     ap.add_argument("--prof-every", type=int, default=PROF_EVERY,
                     help="time every n-th launch of a kernel class with HIP events (0: none)")
     args = ap.parse_args()
