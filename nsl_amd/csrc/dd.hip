@@ -105,22 +105,34 @@ __global__ void __launch_bounds__(DCOLS) k_dd_hop(int n, int nt, long long nrows
     const int y = blockIdx.x * DCOLS + tid;
     const long long g0 = (long long)blockIdx.y * DROWS;
     Acc ar[DROWS], ai[DROWS];
+    // this thread's staged elements: rows rr = (tid + i DCOLS) / DKB at K
+    // offset kk = tid % DKB of every chunk (DCOLS a multiple of DKB), so the
+    // input and phase row offsets are computed once, not per chunk
+    static_assert(DCOLS % DKB == 0, "staging rows fixed per thread");
+    constexpr int NSTG = DROWS * DKB / DCOLS;
+    const int kk = tid % DKB;
+    long long in_off[NSTG], ph_off[NSTG];
+    bool g_ok[NSTG];
+    #pragma unroll
+    for (int i = 0; i < NSTG; ++i) {
+        const long long g = g0 + (tid + i * DCOLS) / DKB;
+        g_ok[i] = g < nrows;
+        const long long gg = g_ok[i] ? g : 0;
+        const long long b = gg / nt;
+        const int t = (int)(gg % nt);
+        const int tin = OP == 0 ? (t == 0 ? nt - 1 : t - 1) : (t + 1 == nt ? 0 : t + 1);
+        in_off[i] = (b * nt + tin) * n;
+        ph_off[i] = OP == 0 ? ((b / rpf) * nt + tin) * n : 0;
+    }
     for (int x0 = 0; x0 < n; x0 += DKB) {
-        for (int e = tid; e < DROWS * DKB; e += DCOLS) {
-            const int rr = e / DKB, kk = e % DKB;
-            const long long g = g0 + rr;
-            const int x = x0 + kk;
+        const int x = x0 + kk;
+        #pragma unroll
+        for (int i = 0; i < NSTG; ++i) {
+            const int rr = (tid + i * DCOLS) / DKB;
             cdd v{{0.0, 0.0}, {0.0, 0.0}};
-            if (g < nrows && x < n) {
-                const long long b = g / nt;
-                const int t = (int)(g % nt);
-                const int tin = OP == 0 ? (t == 0 ? nt - 1 : t - 1) : (t + 1 == nt ? 0 : t + 1);
-                const long long idx = (b * nt + tin) * n + x;
-                v = load(inh, inl, idx);
-                if (OP == 0) {
-                    const long long f = b / rpf;
-                    v = cmul_d(v, P[(f * nt + tin) * n + x], false);
-                }
+            if (g_ok[i] && x < n) {
+                v = load(inh, inl, in_off[i] + x);
+                if (OP == 0) v = cmul_d(v, P[ph_off[i] + x], false);
             }
             sh[0][rr][kk] = v.re.h;
             sh[1][rr][kk] = v.re.l;

@@ -328,24 +328,24 @@ __device__ inline double wave_max(double k) {
 //     columns with 512 threads of several rows, whose band copies spill);
 //   - the multipliers use conj(p) / |p|^2 (they only steer the pivot search);
 //   - no global memory operation inside the loop.
-// C is written from the rows as loaded, before the elimination; afterwards
-// the rows the swaps moved below the pivot rows are rewritten.  Such a row
-// came from a pivot position (a row moved up from below stays a pivot row),
-// so its values are the original pivot-position rows, kept in LDS.  B's rows
-// are read back from Q, whose sub-panel columns this kernel does not modify
-// (no later step reads them), and B's inverse is a Gauss-Jordan sweep of
-// [B | I] in LDS.  Pivots are those of k_gj_panel except between values
-// equal to 2^-41 relative; Binv and C agree with it to rounding.
+// C is left in the block's columns of Q itself (no later step reads them):
+// every row already holds its values before the block, so only the pivot
+// rows are zeroed and the rows the swaps moved below the pivot rows are
+// rewritten.  Such a row came from a pivot position (a row moved up from
+// below stays a pivot row), so its values are the original pivot-position
+// rows, kept in LDS.  B's rows are read back from Q before those writes, and
+// B's inverse is a Gauss-Jordan sweep of [B | I] in LDS.  Pivots are those of
+// k_gj_panel except between values equal to 2^-41 relative; Binv and C agree
+// with it to rounding.
 template <class S, int RPT, int NT>
 __global__ void __launch_bounds__(NT) k_gj_panel_reg(void* Qb, long long ld, long long strideQ, int rows, int jb,
-                                                      int w, int* pivb, void* Binvb, void* Cb, int* singular) {
+                                                      int w, int* pivb, void* Binvb, int* singular) {
     static_assert(NT % 64 == 0 && NT >= 64 && (2 * GJB * GJB) % NT == 0, "whole waves, [B | I] split evenly");
     static_assert(RPT * NT <= (1 << GJ_KEYBITS), "relative rows fit the key");
     const int f = blockIdx.x;
-    const char* Q = reinterpret_cast<const char*>(Qb) + (size_t)f * strideQ * sizeof(Z<S>);
+    char* Q = reinterpret_cast<char*>(Qb) + (size_t)f * strideQ * sizeof(Z<S>);
     int* piv = pivb + f * GJREC;
     char* Binv = reinterpret_cast<char*>(Binvb) + (size_t)f * GJB * GJB * sizeof(Z<S>);
-    char* C = reinterpret_cast<char*>(Cb) + (size_t)f * rows * GJB * sizeof(Z<S>);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     constexpr int NW = NT / 64;
     __shared__ double wkey[2][NW];
@@ -367,26 +367,9 @@ __global__ void __launch_bounds__(NT) k_gj_panel_reg(void* Qb, long long ld, lon
         for (int c = 0; c < GJB; ++c)
             if (gi >= rows || c >= w) a[m][c] = Z<S>{S(0), S(0)};
     }
-    // C: every row's values before this block, in the order as loaded (the
-    // stores drain while the elimination runs)
-    #pragma unroll
-    for (int m = 0; m < RPT; ++m) {
-        const int gi = jb + tid + m * NT;
-        if (gi < rows) {
-            #pragma unroll
-            for (int c = 0; c < GJB; ++c) st<S>(C, (long long)gi * GJB + c, a[m][c]);
-        }
-    }
     if (tid < GJB) {
         #pragma unroll
         for (int c = 0; c < GJB; ++c) orig[tid][c] = a[0][c];
-    }
-    for (int i = tid; i < jb; i += NT) {
-        Z<S> v[GJB];
-        #pragma unroll
-        for (int c = 0; c < GJB; ++c) v[c] = ldz<S>(Q, (long long)i * ld + jb + min(c, wl));
-        #pragma unroll
-        for (int c = 0; c < GJB; ++c) st<S>(C, (long long)i * GJB + c, c < w ? v[c] : Z<S>{S(0), S(0)});
     }
     bool sing = false;
     #pragma unroll 1
@@ -515,25 +498,28 @@ __global__ void __launch_bounds__(NT) k_gj_panel_reg(void* Qb, long long ld, lon
         }
         aug[gi_][gc_] = v;
     }
-    // C: zero at the pivot rows; a row moved below them holds an original
-    // pivot-position row (rewritten by the thread that stored it first, so the
-    // two stores keep their order)
+    // every read of the sub-panel columns of Q (B above) is done after the
+    // barrier: they become C in place, zero at the pivot rows, and a row moved
+    // below the pivot rows holds an original pivot-position row (every other
+    // row already holds its values before the block)
+    __syncthreads();
     #pragma unroll
     for (int m = 0; m < RPT; ++m) {
         const int gi = jb + tid + m * NT;
         if (gi >= rows) continue;
         if (gi < jb + w) {
             #pragma unroll
-            for (int c = 0; c < GJB; ++c) st<S>(C, (long long)gi * GJB + c, Z<S>{S(0), S(0)});
+            for (int c = 0; c < GJB; ++c)
+                if (c < w) st<S>(Q, (long long)gi * ld + jb + c, Z<S>{S(0), S(0)});
         } else {
             const int o = origin(gi);
             if (o != gi) {
                 #pragma unroll
-                for (int c = 0; c < GJB; ++c) st<S>(C, (long long)gi * GJB + c, orig[o - jb][c]);
+                for (int c = 0; c < GJB; ++c)
+                    if (c < w) st<S>(Q, (long long)gi * ld + jb + c, orig[o - jb][c]);
             }
         }
     }
-    __syncthreads();
     // Gauss-Jordan on [B | I] without further pivoting: the rows are in
     // partial-pivoting order, so the pivots are those of the LU above
     #pragma unroll 1
@@ -721,9 +707,10 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
         else reg = live <= 256 ? 1 : live <= 512 ? 2 : live <= 768 ? 3 : live <= 1024 ? 4 : live <= 1536 ? 5 : 0;
         // work of the pivot search and sub-panel LU (8 flops per complex
         // multiply-add): the register kernel reads the sub-panel once and
-        // writes C; the global-memory one also writes the sub-panel back and
-        // forms C = L21 U_B
-        const double pbytes = (double)nf * rows * ((reg ? 1.0 : 2.0) * w + GJB) * e;
+        // rewrites at most 2w of its rows (C in place); the global-memory one
+        // also writes the sub-panel back and forms C = L21 U_B in its buffer
+        const double pbytes = reg ? (double)nf * (rows + 2.0 * GJB) * w * e
+                                  : (double)nf * rows * (2.0 * w + GJB) * e;
         double pflops = 0.0;
         for (int jj = 0; jj < w; ++jj) pflops += 8.0 * (rows - jb - jj) * (w - jj);
         pflops = (pflops + (reg ? 0.0 : 4.0 * rows * w * w)) * nf;
@@ -734,16 +721,16 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
             void* bv = (void*)binv;
             void* Cv = (void*)C;
             if (f64) {
-                if (reg == 1) launch_timed(k_gj_panel_reg<double, 1, 256>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
-                else if (reg == 2) launch_timed(k_gj_panel_reg<double, 2, 256>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
-                else if (reg == 3) launch_timed(k_gj_panel_reg<double, 2, 512>, dim3(nf), dim3(512), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
+                if (reg == 1) launch_timed(k_gj_panel_reg<double, 1, 256>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, sing);
+                else if (reg == 2) launch_timed(k_gj_panel_reg<double, 2, 256>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, sing);
+                else if (reg == 3) launch_timed(k_gj_panel_reg<double, 2, 512>, dim3(nf), dim3(512), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, sing);
                 else launch_timed(k_gj_panel<double>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
             } else {
-                if (reg == 1) launch_timed(k_gj_panel_reg<float, 1, 256>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
-                else if (reg == 2) launch_timed(k_gj_panel_reg<float, 2, 256>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
-                else if (reg == 3) launch_timed(k_gj_panel_reg<float, 3, 256>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
-                else if (reg == 4) launch_timed(k_gj_panel_reg<float, 4, 256>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
-                else if (reg == 5) launch_timed(k_gj_panel_reg<float, 3, 512>, dim3(nf), dim3(512), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
+                if (reg == 1) launch_timed(k_gj_panel_reg<float, 1, 256>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, sing);
+                else if (reg == 2) launch_timed(k_gj_panel_reg<float, 2, 256>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, sing);
+                else if (reg == 3) launch_timed(k_gj_panel_reg<float, 3, 256>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, sing);
+                else if (reg == 4) launch_timed(k_gj_panel_reg<float, 4, 256>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, sing);
+                else if (reg == 5) launch_timed(k_gj_panel_reg<float, 3, 512>, dim3(nf), dim3(512), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, sing);
                 else launch_timed(k_gj_panel<float>, dim3(nf), dim3(256), 0, s, Qv, ldq, strideQ, rows, jb, w, piv, bv, Cv, sing);
             }
             NSLX_LAUNCH_CHECK();
@@ -770,8 +757,10 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
         std::vector<BJob> jobs(nf);
         for (int f = 0; f < nf; ++f) {
             BJob& j = jobs[f];
-            j.A0 = C + (size_t)f * rows * GJB * e;
-            j.lda0 = GJB;
+            // C: the block's columns of Q (register kernel) or the C buffer
+            j.A0 = reg ? static_cast<const void*>(Q + ((size_t)f * strideQ + jb) * e)
+                       : static_cast<const void*>(C + (size_t)f * rows * GJB * e);
+            j.lda0 = reg ? ldq : GJB;
             j.rows = rows;
             j.cols = ncol;
             j.K = w;
