@@ -13,6 +13,8 @@
 // all a residual needs).
 #pragma clang fp contract(off)
 
+#include <cfloat>
+
 #include "dd.hpp"
 #include "engine.hpp"
 #include "prof.hpp"
@@ -65,16 +67,22 @@ __device__ inline cdd load(const double2* h, const double2* l, long long i) {
     return {quick(a.x, b.x), quick(a.y, b.y)};
 }
 
-// accumulator: sum of exact products e * v.h kept as (s, c) with TwoSum, the
-// products' rounding errors and e * v.l summed in c
+// Accumulator of one output element on a fixed grid (an error-free
+// extraction, as in Rump, Ogita and Oishi's ExtractScalar): g = 1.5 * 2^m with
+// 2^(m-1) above the row's bound B on sum |e| |v.h| (k_dd_rowsig, B >= 2^(m-2)).
+// fma(e, v.h, g) lies in [2^m, 2^(m+1)], so ph = fma(e, v.h, g) - g is the
+// exact product rounded to a multiple of 2^(m-52), and the ph sum exactly in s
+// (every partial sum below 2^m).  The rest, e v.h - ph (below 2^(m-53)), is
+// added to c rounded once (error 2^(m-106) per product, a factor n below the
+// rounding of c itself), and e * v.l after it.  Six FP64 operations per
+// product against eleven with a TwoProduct and a TwoSum; the error bound is of
+// the same order, n^2 2^-104 B, from the rounding in c.
 struct Acc {
     double s = 0.0, c = 0.0;
-    __device__ inline void fma(double e, dd v) {
-        const double p = e * v.h;
-        const double pe = __builtin_fma(e, v.h, -p);
-        const dd t = two_sum(s, p);
-        s = t.h;
-        c = c + t.l + pe;
+    __device__ inline void fma(double e, dd v, double g) {
+        const double ph = __builtin_fma(e, v.h, g) - g;
+        s = s + ph;
+        c = c + __builtin_fma(e, v.h, -ph);
         c = __builtin_fma(e, v.l, c);
     }
     __device__ inline dd value() const { return quick(s, c); }
@@ -98,13 +106,17 @@ constexpr int DKB = 32;      // K chunk staged in LDS
 template <int OP, bool FINAL>
 __global__ void __launch_bounds__(DCOLS) k_dd_hop(int n, int nt, long long nrows, int rpf, const double* __restrict__ ET,
                                                   const double2* __restrict__ P, const double2* inh, const double2* inl,
-                                                  double2* outh, double2* outl, const double2* bvec, double2* r,
-                                                  double* part) {
+                                                  const double* __restrict__ sig, double2* outh, double2* outl,
+                                                  const double2* bvec, double2* r, double* part) {
     __shared__ double sh[4][DROWS][DKB + 1];   // re.h, re.l, im.h, im.l of the staged input rows
     const int tid = threadIdx.x;
     const int y = blockIdx.x * DCOLS + tid;
     const long long g0 = (long long)blockIdx.y * DROWS;
     Acc ar[DROWS], ai[DROWS];
+    // the accumulation grid of each output row (uniform over the block)
+    double gr[DROWS];
+    #pragma unroll
+    for (int rr = 0; rr < DROWS; ++rr) gr[rr] = g0 + rr < nrows ? sig[g0 + rr] : 1.5;
     // this thread's staged elements: rows rr = (tid + i DCOLS) / DKB at K
     // offset kk = tid % DKB of every chunk (DCOLS a multiple of DKB), so the
     // input and phase row offsets are computed once, not per chunk
@@ -146,8 +158,8 @@ __global__ void __launch_bounds__(DCOLS) k_dd_hop(int n, int nt, long long nrows
                 const double e = ET[(long long)(x0 + kk) * n + y];
                 #pragma unroll
                 for (int rr = 0; rr < DROWS; ++rr) {
-                    ar[rr].fma(e, dd{sh[0][rr][kk], sh[1][rr][kk]});
-                    ai[rr].fma(e, dd{sh[2][rr][kk], sh[3][rr][kk]});
+                    ar[rr].fma(e, dd{sh[0][rr][kk], sh[1][rr][kk]}, gr[rr]);
+                    ai[rr].fma(e, dd{sh[2][rr][kk], sh[3][rr][kk]}, gr[rr]);
                 }
             }
         }
@@ -228,6 +240,64 @@ __global__ void k_dd_accumulate(double2* xh, double2* xl, const double2* d, long
     xl[i] = make_double2(re.l, im.l);
 }
 
+// *out = max over rows y of sum_x |E[y][x]| (one block)
+__global__ void __launch_bounds__(256) k_dd_enorm(int n, const double* E, double* out) {
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int y = threadIdx.x; y < n; y += 256) {
+        double sum = 0.0;
+        for (int x = 0; x < n; ++x) sum += fabs(E[(long long)y * n + x]);
+        m = fmax(m, sum);
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+// The accumulation grid of each row g of a hop's input (one wave per row):
+// sig[g] = 1.5 * 2^m with 2^(m-2) <= B < 2^(m-1), B = ||E||_inf * max_x
+// (|v_x.re| + |v_x.im|) * (1 + 2^-40), v the row the hop stages
+// (P_{t-1} . in_{t-1} for M, in_{t+1} for M^dag; |v| bounded through the
+// double-double parts' magnitudes), so that B bounds sum_x |E[y][x]| |v_x|
+// for every output column y, rounding of the bound included.
+template <int OP>
+__global__ void __launch_bounds__(256) k_dd_rowsig(int n, int nt, long long nrows, int rpf,
+                                                   const double2* __restrict__ P, const double2* inh,
+                                                   const double2* inl, const double* enorm, double* sig) {
+    const long long g = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (g >= nrows) return;
+    const long long b = g / nt;
+    const int t = (int)(g % nt);
+    const int tin = OP == 0 ? (t == 0 ? nt - 1 : t - 1) : (t + 1 == nt ? 0 : t + 1);
+    const long long base = (b * nt + tin) * n;
+    const long long pbase = OP == 0 ? ((b / rpf) * nt + tin) * n : 0;
+    double m = 0.0;
+    for (int x = lane; x < n; x += 64) {
+        const double2 h = inh[base + x];
+        const double2 l = inl ? inl[base + x] : make_double2(0.0, 0.0);
+        double a = fabs(h.x) + fabs(l.x) + fabs(h.y) + fabs(l.y);
+        if (OP == 0) {
+            const double2 ph = P[pbase + x];
+            a *= fabs(ph.x) + fabs(ph.y);
+        }
+        m = fmax(m, a);
+    }
+    #pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if (lane == 0) {
+        const double bound = (*enorm) * m * (1.0 + 0x1p-40);
+        // an all-zero row takes any grid; a non-finite bound makes the row
+        // non-finite, as its products would
+        sig[g] = !(bound > 0.0) ? (bound == 0.0 ? 1.5 : bound)
+                 : bound <= DBL_MAX ? 1.5 * ldexp(1.0, ilogb(bound) + 2) : bound;
+    }
+}
+
 __global__ void k_transpose(int n, const double* E, double* ET) {
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (long long)n * n) return;
@@ -245,6 +315,8 @@ void DdWork::ensure(int64_t nb, int64_t len, int n, int nt) {
     ul.ensure(v);
     const int ncol = (n + DCOLS - 1) / DCOLS;
     part.ensure((size_t)nb * nt * ncol);
+    sig.ensure((size_t)nb * nt);
+    enorm.ensure(1);
 }
 
 void dd_residual(FermionMatrix& fm, int op, const void* b, const void* xh, const void* xl, void* r, double* rn2,
@@ -262,6 +334,8 @@ void dd_residual(FermionMatrix& fm, int op, const void* b, const void* xh, const
         w.et.ensure((size_t)n * n);
         hipLaunchKernelGGL(k_transpose, dim3(blocks1((long long)n * n)), dim3(256), 0, s, n, E, w.et.p);
         NSLX_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_dd_enorm, dim3(1), dim3(256), 0, s, n, E, w.enorm.p);
+        NSLX_LAUNCH_CHECK();
         w.et_src = E;
     }
     const double* ET = w.et.p;
@@ -271,6 +345,17 @@ void dd_residual(FermionMatrix& fm, int op, const void* b, const void* xh, const
     dim3 g(ncol, (unsigned)((nrows + DROWS - 1) / DROWS));
     const double2* ih = static_cast<const double2*>(xh);
     const double2* il = static_cast<const double2*>(xl);
+    // the accumulation grids of a hop's rows, from its input
+    const dim3 gsig((unsigned)((nrows + 3) / 4));
+    auto rowsig = [&](int hop_op, const double2* inh, const double2* inl) {
+        if (hop_op == 0)
+            hipLaunchKernelGGL(k_dd_rowsig<0>, gsig, dim3(256), 0, s, n, nt, nrows, rpf, P, inh, inl, w.enorm.p,
+                               w.sig.p);
+        else
+            hipLaunchKernelGGL(k_dd_rowsig<1>, gsig, dim3(256), 0, s, n, nt, nrows, rpf, P, inh, inl, w.enorm.p,
+                               w.sig.p);
+        NSLX_LAUNCH_CHECK();
+    };
     // one hop's algorithmic work: per output element and site two real
     // double-double multiply-adds (TwoProduct with FMA + TwoSum + the low
     // parts, 10 flops each); bytes: the input and output fields in
@@ -281,12 +366,13 @@ void dd_residual(FermionMatrix& fm, int op, const void* b, const void* xh, const
     {
         ProfScope prof(prof_on() ? std::string(op == NSLX_OP_MMDAGGER ? "dd_hop_mdag" : "dd_hop_m") : std::string(), s);
         prof.work(flops, bytes);
+        rowsig(op == NSLX_OP_MMDAGGER ? 1 : 0, ih, il);
         if (op == NSLX_OP_MMDAGGER)
-            launch_timed(k_dd_hop<1, false>, g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, ih, il, w.uh.p, w.ul.p,
-                         (const double2*)nullptr, (double2*)nullptr, (double*)nullptr);
+            launch_timed(k_dd_hop<1, false>, g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, ih, il, w.sig.p, w.uh.p,
+                         w.ul.p, (const double2*)nullptr, (double2*)nullptr, (double*)nullptr);
         else
-            launch_timed(k_dd_hop<0, false>, g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, ih, il, w.uh.p, w.ul.p,
-                         (const double2*)nullptr, (double2*)nullptr, (double*)nullptr);
+            launch_timed(k_dd_hop<0, false>, g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, ih, il, w.sig.p, w.uh.p,
+                         w.ul.p, (const double2*)nullptr, (double2*)nullptr, (double*)nullptr);
         NSLX_LAUNCH_CHECK();
     }
     {
@@ -295,12 +381,15 @@ void dd_residual(FermionMatrix& fm, int op, const void* b, const void* xh, const
         prof.work(flops, bytes);
         const double2* uh = w.uh.p;
         const double2* ul = w.ul.p;
+        rowsig(op == NSLX_OP_MMDAGGER ? 0 : 1, uh, ul);
         if (op == NSLX_OP_MMDAGGER)
-            launch_timed(k_dd_hop<0, true>, g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, uh, ul, (double2*)nullptr,
-                         (double2*)nullptr, static_cast<const double2*>(b), static_cast<double2*>(r), w.part.p);
+            launch_timed(k_dd_hop<0, true>, g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, uh, ul, w.sig.p,
+                         (double2*)nullptr, (double2*)nullptr, static_cast<const double2*>(b),
+                         static_cast<double2*>(r), w.part.p);
         else
-            launch_timed(k_dd_hop<1, true>, g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, uh, ul, (double2*)nullptr,
-                         (double2*)nullptr, static_cast<const double2*>(b), static_cast<double2*>(r), w.part.p);
+            launch_timed(k_dd_hop<1, true>, g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, uh, ul, w.sig.p,
+                         (double2*)nullptr, (double2*)nullptr, static_cast<const double2*>(b),
+                         static_cast<double2*>(r), w.part.p);
         NSLX_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(k_dd_rsum, dim3(blocks1(nb)), dim3(256), 0, s, w.part.p, nt, ncol, (long long)nb, rn2);

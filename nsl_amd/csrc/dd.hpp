@@ -11,6 +11,8 @@ struct DdWork {
     DevBuf<double2> uh, ul;    // the first hop's double-double output
     DevBuf<double> part;       // per (row, column block) |r|^2 partials
     DevBuf<double> et;         // transpose of the E it was built from
+    DevBuf<double> enorm;      // max row sum of |E| (one value)
+    DevBuf<double> sig;        // per input row: the accumulation grid of its hop
     const double* et_src = nullptr;
     void ensure(int64_t nb, int64_t len, int n, int nt);
 };
