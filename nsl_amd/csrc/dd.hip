@@ -88,11 +88,20 @@ struct Acc {
     __device__ inline dd value() const { return quick(s, c); }
 };
 
-// threads per block, one output column each: one wave, so that Nx = 288
-// fills 4.5 blocks of columns (90 % of the threads busy; 75 % with 128)
-constexpr int DCOLS = 64;
-constexpr int DROWS = 8;     // output rows (rhs, t) per block
-constexpr int DKB = 32;      // K chunk staged in LDS
+// A block is one wave computing a tile of DTR output rows (rhs, t) by DTY
+// output columns y; lane (rg, cl) = (tid / 8, tid % 8) owns rows rg + 8 i
+// (i < DRL) and columns 4 cl + j (j < DCL).  Every input value read from LDS
+// serves DCL products and every E value DRL: a third of the LDS bytes per
+// product of one output column per lane, and Nx = 288 fills 9 column tiles
+// with no idle lane.
+constexpr int DTY = 32;             // output columns per block
+constexpr int DTR = 16;             // output rows per block
+constexpr int DCL = 4;              // columns per lane
+constexpr int DRL = 2;              // rows per lane
+constexpr int DKB = 8;              // K chunk staged in LDS
+constexpr int DRS = 4 * DKB + 2;    // staged row stride in doubles (rows 16 B apart modulo 128 B)
+static_assert((DTY / DCL) * (DTR / DRL) == 64, "one wave per tile");
+static_assert(DTR * DKB % 64 == 0 && DKB * DTY % 64 == 0, "staging evenly split");
 
 // One hop of M^dag (OP 1) or M (OP 0) on a double-double field, rows
 // g = b*nt + t of a batch:
@@ -100,34 +109,40 @@ constexpr int DKB = 32;      // K chunk staged in LDS
 //   M:     out_t = in_t - b_t E (P_{t-1} in_{t-1})          b_0 = -1
 // FINAL (M only): r = bvec - out, rounded; part[g * ncol + blockIdx.x] = |r_g|^2
 // over this block's columns.  ET: the transpose of the hop's E (ET[x][y] =
-// E[y][x], so that a block's threads read consecutive columns).  E is not
+// E[y][x], so that a block stages rows of consecutive columns).  E is not
 // used as its own transpose: the FP64 exp(s delta K) is symmetric only to an
 // ulp, and at |x| / |b| ~ 1e8 that ulp is a 1e-9 change of the residual.
+// sig: the accumulation grid of each row (k_dd_rowsig).
 template <int OP, bool FINAL>
-__global__ void __launch_bounds__(DCOLS) k_dd_hop(int n, int nt, long long nrows, int rpf, const double* __restrict__ ET,
-                                                  const double2* __restrict__ P, const double2* inh, const double2* inl,
-                                                  const double* __restrict__ sig, double2* outh, double2* outl,
-                                                  const double2* bvec, double2* r, double* part) {
-    __shared__ double sh[4][DROWS][DKB + 1];   // re.h, re.l, im.h, im.l of the staged input rows
+__global__ void __launch_bounds__(64) k_dd_hop(int n, int nt, long long nrows, int rpf, const double* __restrict__ ET,
+                                               const double2* __restrict__ P, const double2* inh, const double2* inl,
+                                               const double* __restrict__ sig, double2* outh, double2* outl,
+                                               const double2* bvec, double2* r, double* part) {
+    // staged input rows (re.h, re.l, im.h, im.l per x) and the ET tile
+    __shared__ __align__(16) double sv[DTR * DRS];
+    __shared__ __align__(16) double se[DKB * DTY];
     const int tid = threadIdx.x;
-    const int y = blockIdx.x * DCOLS + tid;
-    const long long g0 = (long long)blockIdx.y * DROWS;
-    Acc ar[DROWS], ai[DROWS];
-    // the accumulation grid of each output row (uniform over the block)
-    double gr[DROWS];
+    const int cl = tid % (DTY / DCL), rg = tid / (DTY / DCL);
+    const int y0 = blockIdx.x * DTY;
+    const long long g0 = (long long)blockIdx.y * DTR;
+    Acc ar[DRL][DCL], ai[DRL][DCL];
+    double gr[DRL];
     #pragma unroll
-    for (int rr = 0; rr < DROWS; ++rr) gr[rr] = g0 + rr < nrows ? sig[g0 + rr] : 1.5;
-    // this thread's staged elements: rows rr = (tid + i DCOLS) / DKB at K
-    // offset kk = tid % DKB of every chunk (DCOLS a multiple of DKB), so the
-    // input and phase row offsets are computed once, not per chunk
-    static_assert(DCOLS % DKB == 0, "staging rows fixed per thread");
-    constexpr int NSTG = DROWS * DKB / DCOLS;
+    for (int i = 0; i < DRL; ++i) {
+        const long long g = g0 + rg + 8 * i;
+        gr[i] = g < nrows ? sig[g] : 1.5;
+    }
+    // this thread's staged input elements: rows (tid + 64 s) / DKB at K
+    // offset kk = tid % DKB of every chunk; its ET elements: (k, c) =
+    // ((tid + 64 s) / DTY, (tid + 64 s) % DTY)
+    constexpr int NSTG = DTR * DKB / 64;
+    constexpr int NSE = DKB * DTY / 64;
     const int kk = tid % DKB;
     long long in_off[NSTG], ph_off[NSTG];
     bool g_ok[NSTG];
     #pragma unroll
     for (int i = 0; i < NSTG; ++i) {
-        const long long g = g0 + (tid + i * DCOLS) / DKB;
+        const long long g = g0 + (tid + i * 64) / DKB;
         g_ok[i] = g < nrows;
         const long long gg = g_ok[i] ? g : 0;
         const long long b = gg / nt;
@@ -137,79 +152,95 @@ __global__ void __launch_bounds__(DCOLS) k_dd_hop(int n, int nt, long long nrows
         ph_off[i] = OP == 0 ? ((b / rpf) * nt + tin) * n : 0;
     }
     for (int x0 = 0; x0 < n; x0 += DKB) {
-        const int x = x0 + kk;
         #pragma unroll
         for (int i = 0; i < NSTG; ++i) {
-            const int rr = (tid + i * DCOLS) / DKB;
+            const int row = (tid + i * 64) / DKB;
+            const int x = x0 + kk;
             cdd v{{0.0, 0.0}, {0.0, 0.0}};
             if (g_ok[i] && x < n) {
                 v = load(inh, inl, in_off[i] + x);
                 if (OP == 0) v = cmul_d(v, P[ph_off[i] + x], false);
             }
-            sh[0][rr][kk] = v.re.h;
-            sh[1][rr][kk] = v.re.l;
-            sh[2][rr][kk] = v.im.h;
-            sh[3][rr][kk] = v.im.l;
+            double2* d = reinterpret_cast<double2*>(&sv[row * DRS + kk * 4]);
+            d[0] = make_double2(v.re.h, v.re.l);
+            d[1] = make_double2(v.im.h, v.im.l);
+        }
+        #pragma unroll
+        for (int i = 0; i < NSE; ++i) {
+            const int q = tid + i * 64;
+            const int x = x0 + q / DTY, y = y0 + q % DTY;
+            se[q] = x < n && y < n ? ET[(long long)x * n + y] : 0.0;
         }
         __syncthreads();
-        if (y < n) {
-            const int kmax = min(DKB, n - x0);
-            for (int kk = 0; kk < kmax; ++kk) {
-                const double e = ET[(long long)(x0 + kk) * n + y];
+        // padded x (beyond n) hold zeros in both operands: exact zero
+        // products.  Unrolled by 2: fully unrolled, the hoisted LDS reads
+        // take the registers of a fourth wave per SIMD
+        #pragma unroll 2
+        for (int k = 0; k < DKB; ++k) {
+            const double2 e01 = *reinterpret_cast<const double2*>(&se[k * DTY + DCL * cl]);
+            const double2 e23 = *reinterpret_cast<const double2*>(&se[k * DTY + DCL * cl + 2]);
+            const double e[DCL] = {e01.x, e01.y, e23.x, e23.y};
+            #pragma unroll
+            for (int i = 0; i < DRL; ++i) {
+                const double* vr = &sv[(rg + 8 * i) * DRS + 4 * k];
+                const double2 vre = *reinterpret_cast<const double2*>(vr);
+                const double2 vim = *reinterpret_cast<const double2*>(vr + 2);
                 #pragma unroll
-                for (int rr = 0; rr < DROWS; ++rr) {
-                    ar[rr].fma(e, dd{sh[0][rr][kk], sh[1][rr][kk]}, gr[rr]);
-                    ai[rr].fma(e, dd{sh[2][rr][kk], sh[3][rr][kk]}, gr[rr]);
+                for (int j = 0; j < DCL; ++j) {
+                    ar[i][j].fma(e[j], dd{vre.x, vre.y}, gr[i]);
+                    ai[i][j].fma(e[j], dd{vim.x, vim.y}, gr[i]);
                 }
             }
         }
         __syncthreads();
     }
-    double rsq[DROWS];
+    double rsq[DRL];
     #pragma unroll
-    for (int rr = 0; rr < DROWS; ++rr) {
-        rsq[rr] = 0.0;
-        const long long g = g0 + rr;
-        if (g >= nrows || y >= n) continue;
+    for (int i = 0; i < DRL; ++i) {
+        rsq[i] = 0.0;
+        const long long g = g0 + rg + 8 * i;
+        if (g >= nrows) continue;
         const long long b = g / nt;
         const int t = (int)(g % nt);
-        const long long idx = g * n + y;
-        cdd res{ar[rr].value(), ai[rr].value()};
-        double sg;
-        if (OP == 1) {
-            const long long f = b / rpf;
-            res = cmul_d(res, P[(f * nt + t) * n + y], true);
-            sg = t == nt - 1 ? -1.0 : 1.0;
-        } else {
-            sg = t == 0 ? -1.0 : 1.0;
-        }
-        const cdd in = load(inh, inl, idx);
-        const cdd out = cadd(in, neg_c(cscale(res, sg)));
-        if (FINAL) {
-            const double2 bv = bvec[idx];
-            const cdd bb{{bv.x, 0.0}, {bv.y, 0.0}};
-            const cdd rd = cadd(bb, neg_c(out));
-            const double rr_ = rd.re.h + rd.re.l, ri = rd.im.h + rd.im.l;
-            r[idx] = make_double2(rr_, ri);
-            rsq[rr] = rr_ * rr_ + ri * ri;
-        } else {
-            outh[idx] = make_double2(out.re.h, out.im.h);
-            outl[idx] = make_double2(out.re.l, out.im.l);
+        #pragma unroll
+        for (int j = 0; j < DCL; ++j) {
+            const int y = y0 + DCL * cl + j;
+            if (y >= n) continue;
+            const long long idx = g * n + y;
+            cdd res{ar[i][j].value(), ai[i][j].value()};
+            double sg;
+            if (OP == 1) {
+                const long long f = b / rpf;
+                res = cmul_d(res, P[(f * nt + t) * n + y], true);
+                sg = t == nt - 1 ? -1.0 : 1.0;
+            } else {
+                sg = t == 0 ? -1.0 : 1.0;
+            }
+            const cdd in = load(inh, inl, idx);
+            const cdd out = cadd(in, neg_c(cscale(res, sg)));
+            if (FINAL) {
+                const double2 bv = bvec[idx];
+                const cdd bb{{bv.x, 0.0}, {bv.y, 0.0}};
+                const cdd rd = cadd(bb, neg_c(out));
+                const double rr_ = rd.re.h + rd.re.l, ri = rd.im.h + rd.im.l;
+                r[idx] = make_double2(rr_, ri);
+                rsq[i] += rr_ * rr_ + ri * ri;
+            } else {
+                outh[idx] = make_double2(out.re.h, out.im.h);
+                outl[idx] = make_double2(out.re.l, out.im.l);
+            }
         }
     }
     if (FINAL) {
-        // the row sums reuse the staging buffer (free since the K loop's
-        // last barrier): no extra LDS, so as many blocks fit a CU as without
-        static_assert(DROWS * DCOLS <= 4 * DROWS * (DKB + 1), "row sums fit the staging buffer");
-        double* red = &sh[0][0][0];
+        // a row's partial over the block's columns: the 8 lanes of its row
+        // group (consecutive lanes), in a fixed tree order
         #pragma unroll
-        for (int rr = 0; rr < DROWS; ++rr) red[rr * DCOLS + tid] = rsq[rr];
-        __syncthreads();
-        if (tid < DROWS) {
-            const long long g = g0 + tid;
-            double sum = 0.0;
-            for (int k = 0; k < DCOLS; ++k) sum += red[tid * DCOLS + k];
-            if (g < nrows) part[g * gridDim.x + blockIdx.x] = sum;
+        for (int i = 0; i < DRL; ++i) {
+            double v = rsq[i];
+            #pragma unroll
+            for (int o = 1; o < DTY / DCL; o <<= 1) v += __shfl_xor(v, o, 64);
+            const long long g = g0 + rg + 8 * i;
+            if (cl == 0 && g < nrows) part[g * gridDim.x + blockIdx.x] = v;
         }
     }
 }
@@ -313,7 +344,7 @@ void DdWork::ensure(int64_t nb, int64_t len, int n, int nt) {
     const size_t v = (size_t)nb * len;
     uh.ensure(v);
     ul.ensure(v);
-    const int ncol = (n + DCOLS - 1) / DCOLS;
+    const int ncol = (n + DTY - 1) / DTY;
     part.ensure((size_t)nb * nt * ncol);
     sig.ensure((size_t)nb * nt);
     enorm.ensure(1);
@@ -341,8 +372,9 @@ void dd_residual(FermionMatrix& fm, int op, const void* b, const void* xh, const
     const double* ET = w.et.p;
     const double2* P = fm.P64.p;
     const long long nrows = (long long)nb * nt;
-    const int ncol = (n + DCOLS - 1) / DCOLS;
-    dim3 g(ncol, (unsigned)((nrows + DROWS - 1) / DROWS));
+    const int ncol = (n + DTY - 1) / DTY;
+    NSLX_REQUIRE((nrows + DTR - 1) / DTR <= 65535, "dd residual: at most 65535 * 16 rows (nbatch * nt) per call");
+    dim3 g(ncol, (unsigned)((nrows + DTR - 1) / DTR));
     const double2* ih = static_cast<const double2*>(xh);
     const double2* il = static_cast<const double2*>(xl);
     // the accumulation grids of a hop's rows, from its input
@@ -368,10 +400,10 @@ void dd_residual(FermionMatrix& fm, int op, const void* b, const void* xh, const
         prof.work(flops, bytes);
         rowsig(op == NSLX_OP_MMDAGGER ? 1 : 0, ih, il);
         if (op == NSLX_OP_MMDAGGER)
-            launch_timed(k_dd_hop<1, false>, g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, ih, il, w.sig.p, w.uh.p,
+            launch_timed(k_dd_hop<1, false>, g, dim3(64), 0, s, n, nt, nrows, rpf, ET, P, ih, il, w.sig.p, w.uh.p,
                          w.ul.p, (const double2*)nullptr, (double2*)nullptr, (double*)nullptr);
         else
-            launch_timed(k_dd_hop<0, false>, g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, ih, il, w.sig.p, w.uh.p,
+            launch_timed(k_dd_hop<0, false>, g, dim3(64), 0, s, n, nt, nrows, rpf, ET, P, ih, il, w.sig.p, w.uh.p,
                          w.ul.p, (const double2*)nullptr, (double2*)nullptr, (double*)nullptr);
         NSLX_LAUNCH_CHECK();
     }
@@ -383,11 +415,11 @@ void dd_residual(FermionMatrix& fm, int op, const void* b, const void* xh, const
         const double2* ul = w.ul.p;
         rowsig(op == NSLX_OP_MMDAGGER ? 0 : 1, uh, ul);
         if (op == NSLX_OP_MMDAGGER)
-            launch_timed(k_dd_hop<0, true>, g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, uh, ul, w.sig.p,
+            launch_timed(k_dd_hop<0, true>, g, dim3(64), 0, s, n, nt, nrows, rpf, ET, P, uh, ul, w.sig.p,
                          (double2*)nullptr, (double2*)nullptr, static_cast<const double2*>(b),
                          static_cast<double2*>(r), w.part.p);
         else
-            launch_timed(k_dd_hop<1, true>, g, dim3(DCOLS), 0, s, n, nt, nrows, rpf, ET, P, uh, ul, w.sig.p,
+            launch_timed(k_dd_hop<1, true>, g, dim3(64), 0, s, n, nt, nrows, rpf, ET, P, uh, ul, w.sig.p,
                          (double2*)nullptr, (double2*)nullptr, static_cast<const double2*>(b),
                          static_cast<double2*>(r), w.part.p);
         NSLX_LAUNCH_CHECK();
