@@ -13,6 +13,7 @@
 // all a residual needs).
 #pragma clang fp contract(off)
 
+#include <algorithm>
 #include <cfloat>
 
 #include "dd.hpp"
@@ -88,20 +89,22 @@ struct Acc {
     __device__ inline dd value() const { return quick(s, c); }
 };
 
-// A block is one wave computing a tile of DTR output rows (rhs, t) by DTY
-// output columns y; lane (rg, cl) = (tid / 8, tid % 8) owns rows rg + 8 i
-// (i < DRL) and columns 4 cl + j (j < DCL).  Every input value read from LDS
-// serves DCL products and every E value DRL: a third of the LDS bytes per
-// product of one output column per lane, and Nx = 288 fills 9 column tiles
-// with no idle lane.
-constexpr int DTY = 32;             // output columns per block
+// A block is one wave computing a tile of DTR output rows (rhs, t) by
+// DTY = 8 DCL output columns y; lane (rg, cl) = (tid / 8, tid % 8) owns rows
+// rg + 8 i (i < DRL) and columns DCL cl + j (j < DCL).  Every input value
+// read from LDS serves DCL products and every E value DRL.  DCL = 4 (32
+// columns, 128 registers: 4 waves per SIMD) or 6 (48 columns, 3 waves per
+// SIMD); dd_residual picks the one whose tiles fill the device's wave slots
+// with the shorter last round (at Nx = 288 and 8192 rows: 3072 tiles of 48
+// columns, one round at 3 waves per SIMD, against 4608 tiles of 32 columns,
+// 1.125 rounds at 4).
 constexpr int DTR = 16;             // output rows per block
-constexpr int DCL = 4;              // columns per lane
 constexpr int DRL = 2;              // rows per lane
 constexpr int DKB = 8;              // K chunk staged in LDS
 constexpr int DRS = 4 * DKB + 2;    // staged row stride in doubles (rows 16 B apart modulo 128 B)
-static_assert((DTY / DCL) * (DTR / DRL) == 64, "one wave per tile");
-static_assert(DTR * DKB % 64 == 0 && DKB * DTY % 64 == 0, "staging evenly split");
+constexpr int DTY_MIN = 32;         // the narrowest tile (partial sums per row)
+static_assert(8 * (DTR / DRL) == 64, "one wave per tile");
+static_assert(DTR * DKB % 64 == 0, "staging evenly split");
 
 // One hop of M^dag (OP 1) or M (OP 0) on a double-double field, rows
 // g = b*nt + t of a batch:
@@ -113,11 +116,13 @@ static_assert(DTR * DKB % 64 == 0 && DKB * DTY % 64 == 0, "staging evenly split"
 // used as its own transpose: the FP64 exp(s delta K) is symmetric only to an
 // ulp, and at |x| / |b| ~ 1e8 that ulp is a 1e-9 change of the residual.
 // sig: the accumulation grid of each row (k_dd_rowsig).
-template <int OP, bool FINAL>
-__global__ void __launch_bounds__(64) k_dd_hop(int n, int nt, long long nrows, int rpf, const double* __restrict__ ET,
+template <int OP, bool FINAL, int DCL>
+__global__ void __launch_bounds__(64, DCL == 4 ? 4 : 3) k_dd_hop(int n, int nt, long long nrows, int rpf, const double* __restrict__ ET,
                                                const double2* __restrict__ P, const double2* inh, const double2* inl,
                                                const double* __restrict__ sig, double2* outh, double2* outl,
                                                const double2* bvec, double2* r, double* part) {
+    constexpr int DTY = 8 * DCL;
+    static_assert(DCL % 2 == 0 && DKB * DTY % 64 == 0, "E tile read in pairs, staged evenly");
     // staged input rows (re.h, re.l, im.h, im.l per x) and the ET tile
     __shared__ __align__(16) double sv[DTR * DRS];
     __shared__ __align__(16) double se[DKB * DTY];
@@ -177,9 +182,13 @@ __global__ void __launch_bounds__(64) k_dd_hop(int n, int nt, long long nrows, i
         // take the registers of a fourth wave per SIMD
         #pragma unroll 2
         for (int k = 0; k < DKB; ++k) {
-            const double2 e01 = *reinterpret_cast<const double2*>(&se[k * DTY + DCL * cl]);
-            const double2 e23 = *reinterpret_cast<const double2*>(&se[k * DTY + DCL * cl + 2]);
-            const double e[DCL] = {e01.x, e01.y, e23.x, e23.y};
+            double e[DCL];
+            #pragma unroll
+            for (int j = 0; j < DCL; j += 2) {
+                const double2 ep = *reinterpret_cast<const double2*>(&se[k * DTY + DCL * cl + j]);
+                e[j] = ep.x;
+                e[j + 1] = ep.y;
+            }
             #pragma unroll
             for (int i = 0; i < DRL; ++i) {
                 const double* vr = &sv[(rg + 8 * i) * DRS + 4 * k];
@@ -245,14 +254,18 @@ __global__ void __launch_bounds__(64) k_dd_hop(int n, int nt, long long nrows, i
     }
 }
 
-// rn2[b] = sum over its nt rows and ncol column blocks, in index order
-__global__ void k_dd_rsum(const double* part, int nt, int ncol, long long nb, double* rn2) {
-    const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// rn2[b] = sum over its nt rows and ncol column blocks (one wave per b: lane
+// l sums partials l, l + 64, ... in order, then a fixed xor tree)
+__global__ void __launch_bounds__(256) k_dd_rsum(const double* part, int nt, int ncol, long long nb, double* rn2) {
+    const long long b = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (b >= nb) return;
     double s = 0.0;
     const long long m = (long long)nt * ncol;
-    for (long long k = 0; k < m; ++k) s += part[b * m + k];
-    rn2[b] = s;
+    for (long long k = lane; k < m; k += 64) s += part[b * m + k];
+    #pragma unroll
+    for (int o = 1; o < 64; o <<= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) rn2[b] = s;
 }
 
 // (xh, xl) += d for RHS b with done[b] < 0 (done may be null)
@@ -338,13 +351,49 @@ __global__ void k_transpose(int n, const double* E, double* ET) {
 
 inline unsigned blocks1(long long n) { return (unsigned)((n + 255) / 256); }
 
+template <int OP, bool FINAL>
+void launch_hop(int dcl, int n, int nt, long long nrows, int rpf, const double* ET, const double2* P,
+                const double2* inh, const double2* inl, const double* sig, double2* outh, double2* outl,
+                const double2* bvec, double2* r, double* part, hipStream_t s) {
+    const dim3 g((unsigned)((n + 8 * dcl - 1) / (8 * dcl)), (unsigned)((nrows + DTR - 1) / DTR));
+    if (dcl == 6)
+        launch_timed(k_dd_hop<OP, FINAL, 6>, g, dim3(64), 0, s, n, nt, nrows, rpf, ET, P, inh, inl, sig, outh, outl,
+                     bvec, r, part);
+    else
+        launch_timed(k_dd_hop<OP, FINAL, 4>, g, dim3(64), 0, s, n, nt, nrows, rpf, ET, P, inh, inl, sig, outh, outl,
+                     bvec, r, part);
+    NSLX_LAUNCH_CHECK();
+}
+
+// Columns per lane of the hop tiles: each variant's last round of waves
+// (tiles over the device's resident waves) weighed by the round's length
+// (resident waves per SIMD x work per wave); the resident counts come from
+// the runtime's occupancy query for the kernels as compiled.
+int hop_columns(int n, long long nrows) {
+    static int occ4 = 0, occ6 = 0, ncu = 0;
+    if (!ncu) {
+        int dev = 0;
+        NSLX_HIP(hipGetDevice(&dev));
+        NSLX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        NSLX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ4, k_dd_hop<0, true, 4>, 64, 0));
+        NSLX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ6, k_dd_hop<0, true, 6>, 64, 0));
+    }
+    const long long rt = (nrows + DTR - 1) / DTR;
+    auto cost = [&](int dcl, int occ) {
+        const long long tiles = rt * ((n + 8 * dcl - 1) / (8 * dcl));
+        const long long slots = (long long)std::max(ncu * occ, 1);
+        return (double)((tiles + slots - 1) / slots) * occ * dcl;
+    };
+    return cost(6, occ6) < cost(4, occ4) ? 6 : 4;
+}
+
 }  // namespace
 
 void DdWork::ensure(int64_t nb, int64_t len, int n, int nt) {
     const size_t v = (size_t)nb * len;
     uh.ensure(v);
     ul.ensure(v);
-    const int ncol = (n + DTY - 1) / DTY;
+    const int ncol = (n + DTY_MIN - 1) / DTY_MIN;
     part.ensure((size_t)nb * nt * ncol);
     sig.ensure((size_t)nb * nt);
     enorm.ensure(1);
@@ -372,9 +421,9 @@ void dd_residual(FermionMatrix& fm, int op, const void* b, const void* xh, const
     const double* ET = w.et.p;
     const double2* P = fm.P64.p;
     const long long nrows = (long long)nb * nt;
-    const int ncol = (n + DTY - 1) / DTY;
     NSLX_REQUIRE((nrows + DTR - 1) / DTR <= 65535, "dd residual: at most 65535 * 16 rows (nbatch * nt) per call");
-    dim3 g(ncol, (unsigned)((nrows + DTR - 1) / DTR));
+    const int dcl = hop_columns(n, nrows);
+    const int ncol = (n + 8 * dcl - 1) / (8 * dcl);
     const double2* ih = static_cast<const double2*>(xh);
     const double2* il = static_cast<const double2*>(xl);
     // the accumulation grids of a hop's rows, from its input
@@ -400,12 +449,11 @@ void dd_residual(FermionMatrix& fm, int op, const void* b, const void* xh, const
         prof.work(flops, bytes);
         rowsig(op == NSLX_OP_MMDAGGER ? 1 : 0, ih, il);
         if (op == NSLX_OP_MMDAGGER)
-            launch_timed(k_dd_hop<1, false>, g, dim3(64), 0, s, n, nt, nrows, rpf, ET, P, ih, il, w.sig.p, w.uh.p,
-                         w.ul.p, (const double2*)nullptr, (double2*)nullptr, (double*)nullptr);
+            launch_hop<1, false>(dcl, n, nt, nrows, rpf, ET, P, ih, il, w.sig.p, w.uh.p, w.ul.p, nullptr, nullptr,
+                                 nullptr, s);
         else
-            launch_timed(k_dd_hop<0, false>, g, dim3(64), 0, s, n, nt, nrows, rpf, ET, P, ih, il, w.sig.p, w.uh.p,
-                         w.ul.p, (const double2*)nullptr, (double2*)nullptr, (double*)nullptr);
-        NSLX_LAUNCH_CHECK();
+            launch_hop<0, false>(dcl, n, nt, nrows, rpf, ET, P, ih, il, w.sig.p, w.uh.p, w.ul.p, nullptr, nullptr,
+                                 nullptr, s);
     }
     {
         ProfScope prof(prof_on() ? std::string(op == NSLX_OP_MMDAGGER ? "dd_hop_m_resid" : "dd_hop_mdag_resid")
@@ -415,16 +463,14 @@ void dd_residual(FermionMatrix& fm, int op, const void* b, const void* xh, const
         const double2* ul = w.ul.p;
         rowsig(op == NSLX_OP_MMDAGGER ? 0 : 1, uh, ul);
         if (op == NSLX_OP_MMDAGGER)
-            launch_timed(k_dd_hop<0, true>, g, dim3(64), 0, s, n, nt, nrows, rpf, ET, P, uh, ul, w.sig.p,
-                         (double2*)nullptr, (double2*)nullptr, static_cast<const double2*>(b),
-                         static_cast<double2*>(r), w.part.p);
+            launch_hop<0, true>(dcl, n, nt, nrows, rpf, ET, P, uh, ul, w.sig.p, nullptr, nullptr,
+                                static_cast<const double2*>(b), static_cast<double2*>(r), w.part.p, s);
         else
-            launch_timed(k_dd_hop<1, true>, g, dim3(64), 0, s, n, nt, nrows, rpf, ET, P, uh, ul, w.sig.p,
-                         (double2*)nullptr, (double2*)nullptr, static_cast<const double2*>(b),
-                         static_cast<double2*>(r), w.part.p);
-        NSLX_LAUNCH_CHECK();
+            launch_hop<1, true>(dcl, n, nt, nrows, rpf, ET, P, uh, ul, w.sig.p, nullptr, nullptr,
+                                static_cast<const double2*>(b), static_cast<double2*>(r), w.part.p, s);
     }
-    hipLaunchKernelGGL(k_dd_rsum, dim3(blocks1(nb)), dim3(256), 0, s, w.part.p, nt, ncol, (long long)nb, rn2);
+    hipLaunchKernelGGL(k_dd_rsum, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, w.part.p, nt, ncol, (long long)nb,
+                       rn2);
     NSLX_LAUNCH_CHECK();
 }
 

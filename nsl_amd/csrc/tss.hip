@@ -1253,6 +1253,10 @@ void TimeSliceSolver::check_passes(hipStream_t s) {
     for (auto& kv : plans_) {
         failed = kv.second->Minv.chain_failed(s) || failed;
         failed = kv.second->Mdinv.chain_failed(s) || failed;
+        for (auto& q : kv.second->bound) {
+            failed = q->first.chain_failed(s) || failed;
+            failed = q->second.chain_failed(s) || failed;
+        }
     }
     if (failed)
         throw Error(NSLX_E_INTERNAL,
@@ -1277,6 +1281,42 @@ void TimeSliceSolver::apply(int op, const void* y, void* x, int64_t nb, bool io_
         if (conv) c64_to_c128(src, x, n, s);
         else NSLX_HIP(hipMemcpyAsync(x, src, bytes, hipMemcpyDeviceToDevice, s));
     };
+    // a product solve between caller buffers (no conversion) runs plans bound
+    // to them: the caller's y is only read, x only holds this solve's
+    // recurrences and result.  A few pairs are kept (the solvers of this
+    // library call with one fixed pair); others go through the copies.
+    constexpr size_t MAX_BOUND = 4;
+    const bool product = op == NSLX_OP_MMDAGGER || op == NSLX_OP_MDAGGERM;
+    auto ours = [&](const void* p) {
+        return p == B.in.p || p == B.mid.p || p == B.out.p;
+    };
+    if (product && !conv && !ours(y) && !ours(x)) {
+        BoundPlans* bp = nullptr;
+        for (auto& q : B.bound)
+            if (q->op == op && q->y == y && q->x == x) bp = q.get();
+        if (!bp && B.bound.size() < MAX_BOUND) {
+            auto q = std::make_unique<BoundPlans>();
+            q->op = op;
+            q->y = y;
+            q->x = x;
+            if (op == NSLX_OP_MMDAGGER) {       // (M M^dag)^{-1} = M^{-dag} M^{-1}
+                build_minv(q->first, y, B.mid.p, B, nb);
+                build_mdinv(q->second, B.mid.p, x, B, nb);
+            } else {                            // (M^dag M)^{-1} = M^{-1} M^{-dag}
+                build_mdinv(q->first, y, B.mid.p, B, nb);
+                build_minv(q->second, B.mid.p, x, B, nb);
+            }
+            q->first.upload(s);
+            q->second.upload(s);
+            bp = q.get();
+            B.bound.push_back(std::move(q));
+        }
+        if (bp) {
+            bp->first.run(s);
+            bp->second.run(s);
+            return;
+        }
+    }
     if (op == NSLX_OP_M) {
         in(B.in.p);
         B.Minv.run(s);

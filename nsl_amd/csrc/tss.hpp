@@ -190,10 +190,19 @@ class TimeSliceSolver {
     void check_singular(hipStream_t s, bool synced);
 
    private:
+    // plans of a product solve bound to a caller's input and output
+    // buffers (through mid): no copies in or out
+    struct BoundPlans {
+        int op = 0;
+        const void* y = nullptr;
+        void* x = nullptr;
+        BPlan first, second;
+    };
     struct ApplyBufs {
         DevBuf<unsigned char> mid, atil, v[2], z, q, cp[2], sb;
         BPlan Minv, Mdinv;     // M^{-1}: y -> x; M^{-dag}: y -> x (both via bound buffers)
         DevBuf<unsigned char> in, out;
+        std::vector<std::unique_ptr<BoundPlans>> bound;
     };
     FermionMatrix& fm_;
     int prec_, n_, nt_, nc_, c_, nf_;
