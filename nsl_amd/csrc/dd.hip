@@ -93,11 +93,9 @@ struct Acc {
 // DTY = 8 DCL output columns y; lane (rg, cl) = (tid / 8, tid % 8) owns rows
 // rg + 8 i (i < DRL) and columns DCL cl + j (j < DCL).  Every input value
 // read from LDS serves DCL products and every E value DRL.  DCL = 4 (32
-// columns, 128 registers: 4 waves per SIMD) or 6 (48 columns, 3 waves per
-// SIMD); dd_residual picks the one whose tiles fill the device's wave slots
-// with the shorter last round (at Nx = 288 and 8192 rows: 3072 tiles of 48
-// columns, one round at 3 waves per SIMD, against 4608 tiles of 32 columns,
-// 1.125 rounds at 4).
+// columns, up to 168 registers: 3 waves per SIMD) or 6 (48 columns, 2 waves
+// per SIMD); dd_residual picks by how the tiles fill the device's wave
+// slots (hop_columns).
 constexpr int DTR = 16;             // output rows per block
 constexpr int DRL = 2;              // rows per lane
 constexpr int DKB = 8;              // K chunk staged in LDS
@@ -117,7 +115,7 @@ static_assert(DTR * DKB % 64 == 0, "staging evenly split");
 // ulp, and at |x| / |b| ~ 1e8 that ulp is a 1e-9 change of the residual.
 // sig: the accumulation grid of each row (k_dd_rowsig).
 template <int OP, bool FINAL, int DCL>
-__global__ void __launch_bounds__(64, DCL == 4 ? 4 : 3) k_dd_hop(int n, int nt, long long nrows, int rpf, const double* __restrict__ ET,
+__global__ void __launch_bounds__(64, DCL == 4 ? 3 : 2) k_dd_hop(int n, int nt, long long nrows, int rpf, const double* __restrict__ ET,
                                                const double2* __restrict__ P, const double2* inh, const double2* inl,
                                                const double* __restrict__ sig, double2* outh, double2* outl,
                                                const double2* bvec, double2* r, double* part) {
@@ -156,15 +154,40 @@ __global__ void __launch_bounds__(64, DCL == 4 ? 4 : 3) k_dd_hop(int n, int nt, 
         in_off[i] = (b * nt + tin) * n;
         ph_off[i] = OP == 0 ? ((b / rpf) * nt + tin) * n : 0;
     }
-    for (int x0 = 0; x0 < n; x0 += DKB) {
+    // One chunk's global loads go to registers a chunk ahead of their use,
+    // so their latency overlaps the previous chunk's products.  The loads
+    // are unconditional (addresses clamped into the arrays, out-of-range
+    // values replaced when staged): the compiler then waits for each load
+    // only where its value is used.
+    const bool has_l = inl != nullptr;
+    const double2* inl_ = has_l ? inl : inh;
+    double2 fh[NSTG], fl[NSTG], fp[NSTG];
+    double fe[NSE];
+    auto fetch = [&](int x0) {
+        const int x = min(x0 + kk, n - 1);
+        #pragma unroll
+        for (int i = 0; i < NSTG; ++i) {
+            fh[i] = inh[in_off[i] + x];
+            fl[i] = inl_[in_off[i] + x];
+            if (OP == 0) fp[i] = P[ph_off[i] + x];
+        }
+        #pragma unroll
+        for (int i = 0; i < NSE; ++i) {
+            const int q = tid + i * 64;
+            const int xe = min(x0 + q / DTY, n - 1), ye = min(y0 + q % DTY, n - 1);
+            fe[i] = ET[(long long)xe * n + ye];
+        }
+    };
+    auto stage = [&](int x0) {
+        const int x = x0 + kk;
         #pragma unroll
         for (int i = 0; i < NSTG; ++i) {
             const int row = (tid + i * 64) / DKB;
-            const int x = x0 + kk;
             cdd v{{0.0, 0.0}, {0.0, 0.0}};
             if (g_ok[i] && x < n) {
-                v = load(inh, inl, in_off[i] + x);
-                if (OP == 0) v = cmul_d(v, P[ph_off[i] + x], false);
+                const double2 l = has_l ? fl[i] : make_double2(0.0, 0.0);
+                v = cdd{quick(fh[i].x, l.x), quick(fh[i].y, l.y)};
+                if (OP == 0) v = cmul_d(v, fp[i], false);
             }
             double2* d = reinterpret_cast<double2*>(&sv[row * DRS + kk * 4]);
             d[0] = make_double2(v.re.h, v.re.l);
@@ -173,10 +196,15 @@ __global__ void __launch_bounds__(64, DCL == 4 ? 4 : 3) k_dd_hop(int n, int nt, 
         #pragma unroll
         for (int i = 0; i < NSE; ++i) {
             const int q = tid + i * 64;
-            const int x = x0 + q / DTY, y = y0 + q % DTY;
-            se[q] = x < n && y < n ? ET[(long long)x * n + y] : 0.0;
+            const int xe = x0 + q / DTY, ye = y0 + q % DTY;
+            se[q] = xe < n && ye < n ? fe[i] : 0.0;
         }
+    };
+    fetch(0);
+    for (int x0 = 0; x0 < n; x0 += DKB) {
+        stage(x0);
         __syncthreads();
+        fetch(x0 + DKB);    // past the end: clamped re-reads, never staged
         // padded x (beyond n) hold zeros in both operands: exact zero
         // products.  Unrolled by 2: fully unrolled, the hoisted LDS reads
         // take the registers of a fourth wave per SIMD
@@ -365,10 +393,11 @@ void launch_hop(int dcl, int n, int nt, long long nrows, int rpf, const double* 
     NSLX_LAUNCH_CHECK();
 }
 
-// Columns per lane of the hop tiles: each variant's last round of waves
-// (tiles over the device's resident waves) weighed by the round's length
-// (resident waves per SIMD x work per wave); the resident counts come from
-// the runtime's occupancy query for the kernels as compiled.
+// Columns per lane of the hop tiles: each variant's rounds of waves (tiles
+// over the device's resident waves) weighed by the round's length (resident
+// waves per SIMD x work per wave); the resident counts come from the
+// runtime's occupancy query for the kernels as compiled.  Ties go to the
+// wider tile (fewer LDS bytes per product).
 int hop_columns(int n, long long nrows) {
     static int occ4 = 0, occ6 = 0, ncu = 0;
     if (!ncu) {
@@ -384,7 +413,7 @@ int hop_columns(int n, long long nrows) {
         const long long slots = (long long)std::max(ncu * occ, 1);
         return (double)((tiles + slots - 1) / slots) * occ * dcl;
     };
-    return cost(6, occ6) < cost(4, occ4) ? 6 : 4;
+    return cost(6, occ6) <= cost(4, occ4) ? 6 : 4;
 }
 
 }  // namespace
