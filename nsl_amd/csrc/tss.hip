@@ -558,38 +558,64 @@ __global__ void __launch_bounds__(NT) k_gj_panel_reg(void* Qb, long long ld, lon
 // rank-w update of the other rows).  The swaps are applied through the row
 // map (gj_row_map): every value a column needs is read first, then the moved
 // rows and the pivot rows are written, one memory round trip per column.
-constexpr int GJS = 64;   // threads (columns) per block of k_gj_swap
+// GJSQ threads share a column, each reading and writing GJB / GJSQ of its
+// rows; the x values meet in LDS.  The row loads are issued before Binv's,
+// and the block's barrier separates every read of a column from its writes.
+constexpr int GJS = 64;   // columns per block of k_gj_swap
+constexpr int GJSQ = 4;   // threads per column
 template <class S>
-__global__ void __launch_bounds__(GJS) k_gj_swap(void* Qb, long long ld, long long strideQ, int jb, int w, int c0,
-                                                 int width, const int* pivb, const void* Binvb, void* YTb) {
+__global__ void __launch_bounds__(GJS * GJSQ) k_gj_swap(void* Qb, long long ld, long long strideQ, int jb, int w,
+                                                        int c0, int width, const int* pivb, const void* Binvb,
+                                                        void* YTb) {
+    constexpr int MR = GJB / GJSQ;    // rows per thread
     const int f = blockIdx.y;
-    const int c = c0 + blockIdx.x * GJS + threadIdx.x;
+    const int tc = threadIdx.x % GJS, mq = threadIdx.x / GJS;
+    const int c = c0 + blockIdx.x * GJS + tc;
+    const bool live = c < width;
     char* Q = reinterpret_cast<char*>(Qb) + (size_t)f * strideQ * sizeof(Z<S>);
     const int* map = pivb + f * GJREC + GJB;
     const char* Binv = reinterpret_cast<const char*>(Binvb) + (size_t)f * GJB * GJB * sizeof(Z<S>);
     char* YT = reinterpret_cast<char*>(YTb) + (size_t)f * ld * GJB * sizeof(Z<S>);
     __shared__ Z<S> sb[GJB][GJB];
-    for (int e = threadIdx.x; e < GJB * GJB; e += GJS) sb[e / GJB][e % GJB] = ldz<S>(Binv, e);
-    __syncthreads();
-    if (c >= width) return;
-    Z<S> x[GJB], y[GJB];
+    __shared__ Z<S> sx[GJB][GJS];
+    // all loads to registers first (rows, then Binv), so their latencies overlap
+    constexpr int NB = GJB * GJB / (GJS * GJSQ);
+    static_assert(NB * GJS * GJSQ == GJB * GJB, "Binv split evenly");
+    Z<S> xv[MR], y[MR], bv[NB];
     #pragma unroll
-    for (int m = 0; m < GJB; ++m) {
-        x[m] = m < w ? ldz<S>(Q, (long long)map[m] * ld + c) : Z<S>{S(0), S(0)};
-        y[m] = (m < w && map[GJB + m] >= 0) ? ldz<S>(Q, (long long)map[2 * GJB + m] * ld + c) : Z<S>{S(0), S(0)};
+    for (int i = 0; i < MR; ++i) {
+        const int m = mq * MR + i;
+        xv[i] = (live && m < w) ? ldz<S>(Q, (long long)map[m] * ld + c) : Z<S>{S(0), S(0)};
+        y[i] = (live && m < w && map[GJB + m] >= 0) ? ldz<S>(Q, (long long)map[2 * GJB + m] * ld + c)
+                                                    : Z<S>{S(0), S(0)};
     }
     #pragma unroll
-    for (int m = 0; m < GJB; ++m)
-        if (m < w && map[GJB + m] >= 0) st<S>(Q, (long long)map[GJB + m] * ld + c, y[m]);
+    for (int i = 0; i < NB; ++i) bv[i] = ldz<S>(Binv, threadIdx.x + i * GJS * GJSQ);
+    #pragma unroll
+    for (int i = 0; i < MR; ++i) sx[mq * MR + i][tc] = xv[i];
+    #pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const int e = threadIdx.x + i * GJS * GJSQ;
+        sb[e / GJB][e % GJB] = bv[i];
+    }
+    __syncthreads();
+    if (!live) return;
+    #pragma unroll
+    for (int i = 0; i < MR; ++i) {
+        const int m = mq * MR + i;
+        if (m < w && map[GJB + m] >= 0) st<S>(Q, (long long)map[GJB + m] * ld + c, y[i]);
+    }
     // Binv is zero outside its w x w block and x beyond w, so the full 16 x 16
     // product is exact; rows m >= w of YT are written as zeros
     #pragma unroll
-    for (int m = 0; m < GJB; ++m) {
+    for (int i = 0; i < MR; ++i) {
+        const int m = mq * MR + i;
         Z<S> s{S(0), S(0)};
         #pragma unroll
         for (int l = 0; l < GJB; ++l) {
             const Z<S> b = sb[m][l];
-            s = {s.re + (b.re * x[l].re - b.im * x[l].im), s.im + (b.re * x[l].im + b.im * x[l].re)};
+            const Z<S> x = sx[l][tc];
+            s = {s.re + (b.re * x.re - b.im * x.im), s.im + (b.re * x.im + b.im * x.re)};
         }
         if (m < w) st<S>(Q, (long long)(jb + m) * ld + c, s);
         st<S>(YT, (long long)(c - c0) * GJB + m, s);
@@ -747,11 +773,11 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
             prof.work(sflops, sbytes);
             dim3 g((ncol + GJS - 1) / GJS, nf);
             if (f64)
-                launch_timed(k_gj_swap<double>, g, dim3(GJS), 0, s, (void*)Q, ldq, strideQ, jb, w, c0, width,
-                             (const int*)piv, (const void*)binv, (void*)YT);
+                launch_timed(k_gj_swap<double>, g, dim3(GJS * GJSQ), 0, s, (void*)Q, ldq, strideQ, jb, w, c0,
+                             width, (const int*)piv, (const void*)binv, (void*)YT);
             else
-                launch_timed(k_gj_swap<float>, g, dim3(GJS), 0, s, (void*)Q, ldq, strideQ, jb, w, c0, width,
-                             (const int*)piv, (const void*)binv, (void*)YT);
+                launch_timed(k_gj_swap<float>, g, dim3(GJS * GJSQ), 0, s, (void*)Q, ldq, strideQ, jb, w, c0,
+                             width, (const int*)piv, (const void*)binv, (void*)YT);
             NSLX_LAUNCH_CHECK();
         });
         std::vector<BJob> jobs(nf);
