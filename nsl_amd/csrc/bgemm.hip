@@ -73,6 +73,12 @@ __global__ void __launch_bounds__(256) k_bgemm(const BJob* __restrict__ jobs, co
     const BJob& J = jobs[tl.x];
     const int rows = J.rows, cols = J.cols, K = J.K;
     const int r0 = tl.y * BM, c0 = tl.z * BN;
+    if (J.zf) {
+        // the tile's rows of B all zero: Y = Add = Y (uniform over the block)
+        bool nz = false;
+        for (int t = c0 / 32; t * 32 < min(c0 + BN, cols); ++t) nz = nz || J.zf[t] != 0;
+        if (!nz) return;
+    }
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int r16 = lane & 15, q = lane >> 4;

@@ -561,12 +561,15 @@ __global__ void __launch_bounds__(NT) k_gj_panel_reg(void* Qb, long long ld, lon
 // GJSQ threads share a column, each reading and writing GJB / GJSQ of its
 // rows; the x values meet in LDS.  The row loads are issued before Binv's,
 // and the block's barrier separates every read of a column from its writes.
+// zf[f * ((ld + 31) / 32) + t] = whether YT has a nonzero in columns
+// [c0 + 32 t, c0 + 32 t + 32) (read by the update, BJob::zf).
 constexpr int GJS = 64;   // columns per block of k_gj_swap
 constexpr int GJSQ = 4;   // threads per column
+static_assert(GJS % 32 == 0, "zero flags per 32 columns");
 template <class S>
 __global__ void __launch_bounds__(GJS * GJSQ) k_gj_swap(void* Qb, long long ld, long long strideQ, int jb, int w,
                                                         int c0, int width, const int* pivb, const void* Binvb,
-                                                        void* YTb) {
+                                                        void* YTb, int* zf) {
     constexpr int MR = GJB / GJSQ;    // rows per thread
     const int f = blockIdx.y;
     const int tc = threadIdx.x % GJS, mq = threadIdx.x / GJS;
@@ -578,6 +581,8 @@ __global__ void __launch_bounds__(GJS * GJSQ) k_gj_swap(void* Qb, long long ld, 
     char* YT = reinterpret_cast<char*>(YTb) + (size_t)f * ld * GJB * sizeof(Z<S>);
     __shared__ Z<S> sb[GJB][GJB];
     __shared__ Z<S> sx[GJB][GJS];
+    __shared__ int snz[GJS / 32];
+    if (threadIdx.x < GJS / 32) snz[threadIdx.x] = 0;
     // all loads to registers first (rows, then Binv), so their latencies overlap
     constexpr int NB = GJB * GJB / (GJS * GJSQ);
     static_assert(NB * GJS * GJSQ == GJB * GJB, "Binv split evenly");
@@ -599,26 +604,35 @@ __global__ void __launch_bounds__(GJS * GJSQ) k_gj_swap(void* Qb, long long ld, 
         sb[e / GJB][e % GJB] = bv[i];
     }
     __syncthreads();
-    if (!live) return;
-    #pragma unroll
-    for (int i = 0; i < MR; ++i) {
-        const int m = mq * MR + i;
-        if (m < w && map[GJB + m] >= 0) st<S>(Q, (long long)map[GJB + m] * ld + c, y[i]);
-    }
-    // Binv is zero outside its w x w block and x beyond w, so the full 16 x 16
-    // product is exact; rows m >= w of YT are written as zeros
-    #pragma unroll
-    for (int i = 0; i < MR; ++i) {
-        const int m = mq * MR + i;
-        Z<S> s{S(0), S(0)};
+    bool nz = false;
+    if (live) {
         #pragma unroll
-        for (int l = 0; l < GJB; ++l) {
-            const Z<S> b = sb[m][l];
-            const Z<S> x = sx[l][tc];
-            s = {s.re + (b.re * x.re - b.im * x.im), s.im + (b.re * x.im + b.im * x.re)};
+        for (int i = 0; i < MR; ++i) {
+            const int m = mq * MR + i;
+            if (m < w && map[GJB + m] >= 0) st<S>(Q, (long long)map[GJB + m] * ld + c, y[i]);
         }
-        if (m < w) st<S>(Q, (long long)(jb + m) * ld + c, s);
-        st<S>(YT, (long long)(c - c0) * GJB + m, s);
+        // Binv is zero outside its w x w block and x beyond w, so the full
+        // 16 x 16 product is exact; rows m >= w of YT are written as zeros
+        #pragma unroll
+        for (int i = 0; i < MR; ++i) {
+            const int m = mq * MR + i;
+            Z<S> s{S(0), S(0)};
+            #pragma unroll
+            for (int l = 0; l < GJB; ++l) {
+                const Z<S> b = sb[m][l];
+                const Z<S> x = sx[l][tc];
+                s = {s.re + (b.re * x.re - b.im * x.im), s.im + (b.re * x.im + b.im * x.re)};
+            }
+            if (m < w) st<S>(Q, (long long)(jb + m) * ld + c, s);
+            st<S>(YT, (long long)(c - c0) * GJB + m, s);
+            nz = nz || s.re != S(0) || s.im != S(0);    // NaN counts as nonzero
+        }
+    }
+    if (nz) snz[tc / 32] = 1;
+    __syncthreads();
+    if (threadIdx.x < GJS / 32) {
+        const int t = blockIdx.x * (GJS / 32) + threadIdx.x;
+        if (32 * t < width - c0) zf[(long long)f * ((ld + 31) / 32) + t] = snz[threadIdx.x];
     }
 }
 
@@ -711,6 +725,7 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
     const bool f64 = prec_ == NSLX_F64;
     int* piv = gj_piv_.p;
     int* sing = gj_sing_.p;
+    int* zf = gj_zf_.p;
     unsigned char* binv = gj_binv_.p;
     unsigned char* C = gj_C_.p;
     unsigned char* YT = gj_YT_.p;
@@ -722,6 +737,7 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
     check_extent(YT, (long long)nf * ldq * GJB - 1, e, "Gauss-Jordan YT");
     check_extent(binv, (long long)nf * GJB * GJB - 1, e, "Gauss-Jordan Binv");
     check_extent(piv, (long long)nf * GJREC - 1, sizeof(int), "Gauss-Jordan pivot records");
+    check_extent(zf, (long long)nf * ((ldq + 31) / 32) - 1, sizeof(int), "Gauss-Jordan zero flags");
     for (int jb = 0; jb < n_; jb += GJB) {
         const int w = std::min(GJB, n_ - jb);
         // register kernel shape for the live sub-panel rows: 256 threads with
@@ -774,10 +790,10 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
             dim3 g((ncol + GJS - 1) / GJS, nf);
             if (f64)
                 launch_timed(k_gj_swap<double>, g, dim3(GJS * GJSQ), 0, s, (void*)Q, ldq, strideQ, jb, w, c0,
-                             width, (const int*)piv, (const void*)binv, (void*)YT);
+                             width, (const int*)piv, (const void*)binv, (void*)YT, zf);
             else
                 launch_timed(k_gj_swap<float>, g, dim3(GJS * GJSQ), 0, s, (void*)Q, ldq, strideQ, jb, w, c0,
-                             width, (const int*)piv, (const void*)binv, (void*)YT);
+                             width, (const int*)piv, (const void*)binv, (void*)YT, zf);
             NSLX_LAUNCH_CHECK();
         });
         std::vector<BJob> jobs(nf);
@@ -797,6 +813,9 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
             j.Add = j.Y;
             j.ldadd = ldq;
             j.sgn = -1.0;
+            // columns whose pivot-row values are all zero (the augmented
+            // identity's columns of rows not yet pivots) are left untouched
+            j.zf = zf + (size_t)f * ((ldq + 31) / 32);
         }
         // tile of the rank-16 updates (K = 16, bound by reading and writing the
         // trailing matrix): NSLX_GJ_TILE 0 (64 x 64) / 1 (32 x 32), else by count
@@ -850,6 +869,7 @@ void TimeSliceSolver::build_setup(hipStream_t s) {
     const long long W = 5LL * n;
     gj_piv_.alloc((size_t)nf * GJREC);
     gj_sing_.alloc(1);
+    gj_zf_.alloc((size_t)nf * ((W + 31) / 32));
     gj_binv_.alloc((size_t)nf * GJB * GJB * esz_);
     gj_C_.alloc((size_t)nf * 2 * n * GJB * esz_);
     gj_YT_.alloc((size_t)nf * W * GJB * esz_);

@@ -45,6 +45,9 @@ struct FermionMatrix;
 // A1 + r*lda1 (shifted by kseg) beyond; ph_r = ph + (r / phdiv) * ldph.  All
 // strides count complex elements (B: real elements for a real B).  Add may
 // be null, and may equal Y (each element is read and written by one thread).
+// zf (jobs with Add == Y only): per 32 output columns, whether any of those
+// rows of B is nonzero; a tile whose groups are all zero is left as it is
+// (Y = Add exactly) without reading or writing it.
 struct BJob {
     const void* A0 = nullptr;
     const void* A1 = nullptr;
@@ -61,6 +64,7 @@ struct BJob {
     long long ldph = 0;
     int phdiv = 1 << 30;
     double sgn = 1.0;
+    const int* zf = nullptr;
 };
 
 // Kernel variant of a launch: element precision (NSLX_F64 / NSLX_F32),
@@ -216,7 +220,7 @@ class TimeSliceSolver {
     bool setup_built_ = false;
     const void* e_used_ = nullptr;
     const void* p_used_ = nullptr;
-    DevBuf<int> gj_piv_, gj_sing_;
+    DevBuf<int> gj_piv_, gj_sing_, gj_zf_;
     PinBuf<int> sing_h_;
     bool sing_pending_ = false;
     DevBuf<unsigned char> gj_binv_, gj_C_, gj_YT_;
