@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <functional>
+#include <type_traits>
 
 #include "prof.hpp"
 #include "tss.hpp"
@@ -166,6 +167,52 @@ __global__ void __launch_bounds__(256) k_bgemm(const BJob* __restrict__ jobs, co
             if constexpr (CB) sB[1][rb + e * RS][kk] = gb[e].im;
         }
     };
+    // A 32 x 32 tile whose live rows fit its upper half (the reduced sweeps'
+    // 8 right-hand sides per field) splits each chunk's K between the wave
+    // rows instead: both wave rows compute rows 0-15, over ks halves, and the
+    // halves are added through LDS before the epilogue.  The 16 x 16 x 4 FP64
+    // MFMAs are what such a launch waits for; this halves them per wave.
+    const bool ksplit = BM == 32 && rows - r0 <= BM / 2;
+    auto compute_ks = [&](auto ks0c, auto ks1c, int wrow) {
+        constexpr int KS0 = decltype(ks0c)::value, KS1 = decltype(ks1c)::value;
+        #pragma unroll
+        for (int ks = KS0; ks < KS1; ++ks) {
+            const int k = 4 * ks + q;
+            S ar[FM], ai[FM], br[FN], bi[FN];
+            #pragma unroll
+            for (int i = 0; i < FM; ++i) {
+                const int row = wrow + 16 * i + r16;
+                ar[i] = sA[0][row][k];
+                ai[i] = sA[1][row][k];
+            }
+            #pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                const int col = wn * (BN / 2) + 16 * j + r16;
+                br[j] = sB[0][col][k];
+                if constexpr (CB) bi[j] = sB[1][col][k];
+            }
+            #pragma unroll
+            for (int i = 0; i < FM; ++i)
+                #pragma unroll
+                for (int j = 0; j < FN; ++j) {
+                    acc[0][i][j] = mfma<S>(ar[i], br[j], acc[0][i][j]);
+                    acc[1][i][j] = mfma<S>(ai[i], br[j], acc[1][i][j]);
+                    if constexpr (CB) {
+                        acc[2][i][j] = mfma<S>(ai[i], bi[j], acc[2][i][j]);
+                        acc[3][i][j] = mfma<S>(ar[i], bi[j], acc[3][i][j]);
+                    }
+                }
+        }
+    };
+    using std::integral_constant;
+    auto compute = [&]() {
+        if (ksplit) {
+            if (wm == 0) compute_ks(integral_constant<int, 0>{}, integral_constant<int, BK / 8>{}, 0);
+            else compute_ks(integral_constant<int, BK / 8>{}, integral_constant<int, BK / 4>{}, 0);
+        } else {
+            compute_ks(integral_constant<int, 0>{}, integral_constant<int, BK / 4>{}, wm * (BM / 2));
+        }
+    };
     // the Add operand (the Gauss-Jordan updates' trailing matrix, the sweeps'
     // accumulators) is read before the K loop, so its latency overlaps the
     // MFMAs: read in the epilogue, each load waited behind the previous
@@ -192,37 +239,39 @@ __global__ void __launch_bounds__(256) k_bgemm(const BJob* __restrict__ jobs, co
         store();
         __syncthreads();
         if (k0 + BK < K) load(k0 + BK);
-        #pragma unroll
-        for (int ks = 0; ks < BK / 4; ++ks) {
-            const int k = 4 * ks + q;
-            S ar[FM], ai[FM], br[FN], bi[FN];
-            #pragma unroll
-            for (int i = 0; i < FM; ++i) {
-                const int row = wm * (BM / 2) + 16 * i + r16;
-                ar[i] = sA[0][row][k];
-                ai[i] = sA[1][row][k];
-            }
-            #pragma unroll
-            for (int j = 0; j < FN; ++j) {
-                const int col = wn * (BN / 2) + 16 * j + r16;
-                br[j] = sB[0][col][k];
-                if constexpr (CB) bi[j] = sB[1][col][k];
-            }
-            #pragma unroll
-            for (int i = 0; i < FM; ++i)
-                #pragma unroll
-                for (int j = 0; j < FN; ++j) {
-                    acc[0][i][j] = mfma<S>(ar[i], br[j], acc[0][i][j]);
-                    acc[1][i][j] = mfma<S>(ai[i], br[j], acc[1][i][j]);
-                    if constexpr (CB) {
-                        acc[2][i][j] = mfma<S>(ai[i], bi[j], acc[2][i][j]);
-                        acc[3][i][j] = mfma<S>(ar[i], bi[j], acc[3][i][j]);
-                    }
-                }
-        }
+        compute();
         __syncthreads();
     }
 
+    if (ksplit) {
+        // the second wave row's K halves to the first (LDS free since the K
+        // loop's last barrier); the second wave row is then done
+        constexpr int NACC = (CB ? 4 : 2) * FM * FN * 4;
+        static_assert(BM != 32 || 2 * NACC * 64 <= 2 * BM * (BK + PAD), "partials fit sA");
+        S* red = &sA[0][0][0];
+        if (wm == 1) {
+            #pragma unroll
+            for (int a = 0; a < (CB ? 4 : 2); ++a)
+                #pragma unroll
+                for (int i = 0; i < FM; ++i)
+                    #pragma unroll
+                    for (int j = 0; j < FN; ++j)
+                        #pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                            red[(wn * NACC + ((a * FM + i) * FN + j) * 4 + v) * 64 + lane] = acc[a][i][j][v];
+        }
+        __syncthreads();
+        if (wm == 1) return;
+        #pragma unroll
+        for (int a = 0; a < (CB ? 4 : 2); ++a)
+            #pragma unroll
+            for (int i = 0; i < FM; ++i)
+                #pragma unroll
+                for (int j = 0; j < FN; ++j)
+                    #pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        acc[a][i][j][v] += red[(wn * NACC + ((a * FM + i) * FN + j) * 4 + v) * 64 + lane];
+    }
     // epilogue: register v of lane (q, r16) holds row 4v + q (FP64) or
     // 4q + v (FP32) of its 16 x 16 block, column r16
     const S sg = S(J.sgn);
