@@ -20,6 +20,7 @@
 #include <cmath>
 #include <type_traits>
 
+#include "bgemm_tile.hpp"
 #include "engine.hpp"
 #include "prof.hpp"
 #include "tss.hpp"
@@ -338,11 +339,10 @@ __device__ inline double wave_max(double k) {
 // k_gj_panel except between values equal to 2^-41 relative; Binv and C agree
 // with it to rounding.
 template <class S, int RPT, int NT>
-__global__ void __launch_bounds__(NT) k_gj_panel_reg(void* Qb, long long ld, long long strideQ, int rows, int jb,
-                                                      int w, int* pivb, void* Binvb, int* singular) {
+__device__ __attribute__((always_inline)) inline void gj_panel_body(int f, void* Qb, long long ld, long long strideQ, int rows, int jb, int w,
+                                     int* pivb, void* Binvb, int* singular) {
     static_assert(NT % 64 == 0 && NT >= 64 && (2 * GJB * GJB) % NT == 0, "whole waves, [B | I] split evenly");
     static_assert(RPT * NT <= (1 << GJ_KEYBITS), "relative rows fit the key");
-    const int f = blockIdx.x;
     char* Q = reinterpret_cast<char*>(Qb) + (size_t)f * strideQ * sizeof(Z<S>);
     int* piv = pivb + f * GJREC;
     char* Binv = reinterpret_cast<char*>(Binvb) + (size_t)f * GJB * GJB * sizeof(Z<S>);
@@ -553,6 +553,101 @@ __global__ void __launch_bounds__(NT) k_gj_panel_reg(void* Qb, long long ld, lon
     }
 }
 
+template <class S, int RPT, int NT>
+__global__ void __launch_bounds__(NT) k_gj_panel_reg(void* Qb, long long ld, long long strideQ, int rows, int jb,
+                                                      int w, int* pivb, void* Binvb, int* singular) {
+    gj_panel_body<S, RPT, NT>(blockIdx.x, Qb, ld, strideQ, rows, jb, w, pivb, Binvb, singular);
+}
+
+// One launch for block k's trailing update and block k + 1's pivot panel.
+// Workgroups f < nf first apply block k's rank-wp update to block k + 1's
+// columns [jb, jb + w) in every row (multipliers: Q's columns [jbp, jbp + wp),
+// pivot-row values: YT rows 0 .. w - 1, since block k's update starts at
+// column jbp + wp = jb), then factor the panel (gj_panel_body, which reads
+// those columns back after the barrier: the workgroup's own writes); the
+// other workgroups are the 64 x 64 tiles of block k's update of the columns
+// from jb + w on.  The two parts touch disjoint columns of Q, so block
+// k + 1's panel runs beside the update instead of after it.
+template <class S, int RPT>
+__global__ void __launch_bounds__(256) k_gj_fused(const BJob* __restrict__ jobs, const int4* __restrict__ tiles,
+                                                  int nf, void* Qb, long long ld, long long strideQ, int rows, int jb,
+                                                  int w, int jbp, int wp, const void* YTb, int* pivb, void* Binvb,
+                                                  int* singular) {
+    if ((int)blockIdx.x >= nf) {
+        bg::bgemm_tile<S, true, 0, 64, 64>(jobs, tiles, (int)blockIdx.x - nf);
+        return;
+    }
+    const int f = blockIdx.x;
+    char* Q = reinterpret_cast<char*>(Qb) + (size_t)f * strideQ * sizeof(Z<S>);
+    const char* YT = reinterpret_cast<const char*>(YTb) + (size_t)f * ld * GJB * sizeof(Z<S>);
+    // Rows in chunks of 64: the chunk's multipliers (64 x 16) staged in LDS,
+    // the next chunk's loads in flight during this chunk's products.  Thread
+    // (rg, c) updates columns jb + c and jb + c + 8 of the chunk's rows rg and
+    // rg + 32, with those columns' pivot-row values in registers.  Loads are
+    // unconditional (rows clamped; rows beyond are not stored).
+    constexpr int PR = 64;
+    __shared__ Z<S> sa[PR][GJB + 1];
+    const int tid = threadIdx.x;
+    const int c = tid % 8, rg = tid / 8;
+    Z<S> y0[GJB], y1[GJB];
+    #pragma unroll
+    for (int m = 0; m < GJB; ++m) {
+        y0[m] = (c < w && m < wp) ? ldz<S>(YT, (long long)c * GJB + m) : Z<S>{S(0), S(0)};
+        y1[m] = (c + 8 < w && m < wp) ? ldz<S>(YT, (long long)(c + 8) * GJB + m) : Z<S>{S(0), S(0)};
+    }
+    const int wl = w - 1, wpl = wp - 1;
+    Z<S> fa[4], fq[2][2];
+    auto fetch = [&](int r0) {
+        #pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u, r = min(r0 + e / GJB, rows - 1);
+            fa[u] = ldz<S>(Q, (long long)r * ld + jbp + min(e % GJB, wpl));
+        }
+        #pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const long long rb = (long long)min(r0 + rg + 32 * i, rows - 1) * ld + jb;
+            fq[i][0] = ldz<S>(Q, rb + min(c, wl));
+            fq[i][1] = ldz<S>(Q, rb + min(c + 8, wl));
+        }
+    };
+    fetch(0);
+    for (int r0 = 0; r0 < rows; r0 += PR) {
+        #pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u;
+            sa[e / GJB][e % GJB] = fa[u];
+        }
+        Z<S> v[2][2];
+        #pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            v[i][0] = fq[i][0];
+            v[i][1] = fq[i][1];
+        }
+        __syncthreads();
+        fetch(r0 + PR);
+        #pragma unroll
+        for (int m = 0; m < GJB; ++m) {
+            if (m < wp) {
+                #pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    const Z<S> a = sa[rg + 32 * i][m];
+                    v[i][0] = zsub<S>(v[i][0], zmul<S>(a, y0[m]));
+                    v[i][1] = zsub<S>(v[i][1], zmul<S>(a, y1[m]));
+                }
+            }
+        }
+        #pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int r = r0 + rg + 32 * i;
+            if (r >= rows) continue;
+            if (c < w) st<S>(Q, (long long)r * ld + jb + c, v[i][0]);
+            if (c + 8 < w) st<S>(Q, (long long)r * ld + jb + c + 8, v[i][1]);
+        }
+        __syncthreads();
+    }
+    gj_panel_body<S, RPT, 256>(f, Qb, ld, strideQ, rows, jb, w, pivb, Binvb, singular);
+}
+
 // Columns [c0, width) of field blockIdx.y: the block's row swaps, then the
 // pivot rows become Binv x (also written transposed to YT, ld 16, for the
 // rank-w update of the other rows).  The swaps are applied through the row
@@ -738,6 +833,27 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
     check_extent(binv, (long long)nf * GJB * GJB - 1, e, "Gauss-Jordan Binv");
     check_extent(piv, (long long)nf * GJREC - 1, sizeof(int), "Gauss-Jordan pivot records");
     check_extent(zf, (long long)nf * ((ldq + 31) / 32) - 1, sizeof(int), "Gauss-Jordan zero flags");
+    // tile of the rank-16 updates (K = 16, bound by reading and writing the
+    // trailing matrix): NSLX_GJ_TILE 0 (64 x 64) / 1 (32 x 32), else by count
+    static const int gj_tile = (int)env_double("NSLX_GJ_TILE", -1.0);
+    // NSLX_GJ_FUSE=0: every panel and update in launches of their own
+    static const bool gj_fuse = env_double("NSLX_GJ_FUSE", 1.0) != 0.0;
+    auto update_tile = [&](const std::vector<BJob>& jobs) {
+        if (gj_tile == 0 || gj_tile == 1) return gj_tile;
+        std::vector<int4> t;
+        bgemm_tiles(jobs, 0, t);
+        return t.size() >= 512 ? 0 : 1;
+    };
+    // a block's update is planned with the next block: in one launch with
+    // that block's panel (k_gj_fused) where both run register kernels of 256
+    // threads and the update takes 64 x 64 tiles, else on its own
+    std::vector<BJob> pend;
+    int pend_jb = 0, pend_w = 0;
+    bool pend_reg = false;
+    auto flush = [&]() {
+        if (!pend.empty()) p.gemm(pend, BVariant{prec_, true, 0, 0}, update_tile(pend));
+        pend.clear();
+    };
     for (int jb = 0; jb < n_; jb += GJB) {
         const int w = std::min(GJB, n_ - jb);
         // register kernel shape for the live sub-panel rows: 256 threads with
@@ -756,6 +872,59 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
         double pflops = 0.0;
         for (int jj = 0; jj < w; ++jj) pflops += 8.0 * (rows - jb - jj) * (w - jj);
         pflops = (pflops + (reg ? 0.0 : 4.0 * rows * w * w)) * nf;
+        const bool fusable = f64 ? (reg == 1 || reg == 2) : (reg >= 1 && reg <= 4);
+        if (gj_fuse && !pend.empty() && pend_reg && fusable && update_tile(pend) == 0) {
+            // block k's update without this panel's columns (those are
+            // updated by the panel's workgroups)
+            std::vector<BJob> rest = pend;
+            for (BJob& j : rest) {
+                j.Y = static_cast<char*>(j.Y) + (size_t)w * e;
+                j.Add = j.Y;
+                j.B = static_cast<const char*>(j.B) + (size_t)w * GJB * e;
+                j.cols -= w;
+                j.zfoff = w;
+            }
+            const int jbp = pend_jb, wp = pend_w;
+            double ufl = 0.0, uby = 0.0;
+            bgemm_work(rest, BVariant{prec_, true, 0, 0}, &ufl, &uby);
+            // the panel columns' update: read (and write) them, read the
+            // multipliers, 8 flops per complex multiply-add
+            const double ffl = pflops + ufl + 8.0 * nf * rows * w * wp;
+            const double fby = pbytes + uby + (double)nf * rows * (2.0 * w + wp) * e;
+            p.gemm_with(rest, BVariant{prec_, true, 0, 0}, 0,
+                        [=](hipStream_t s, const BJob* dj, const int4* dt, int ntl, const BVariant&) {
+                ProfScope prof(prof_on() ? std::string(f64 ? "gj_fused_f64" : "gj_fused_f32") : std::string(), s);
+                prof.work(ffl, fby);
+                void* Qv = (void*)Q;
+                const void* yt = (const void*)YT;
+                void* bv = (void*)binv;
+                const dim3 g((unsigned)(nf + ntl));
+                if (f64) {
+                    if (reg == 1)
+                        launch_timed(k_gj_fused<double, 1>, g, dim3(256), 0, s, dj, dt, nf, Qv, ldq, strideQ, rows,
+                                     jb, w, jbp, wp, yt, piv, bv, sing);
+                    else
+                        launch_timed(k_gj_fused<double, 2>, g, dim3(256), 0, s, dj, dt, nf, Qv, ldq, strideQ, rows,
+                                     jb, w, jbp, wp, yt, piv, bv, sing);
+                } else {
+                    if (reg == 1)
+                        launch_timed(k_gj_fused<float, 1>, g, dim3(256), 0, s, dj, dt, nf, Qv, ldq, strideQ, rows, jb,
+                                     w, jbp, wp, yt, piv, bv, sing);
+                    else if (reg == 2)
+                        launch_timed(k_gj_fused<float, 2>, g, dim3(256), 0, s, dj, dt, nf, Qv, ldq, strideQ, rows, jb,
+                                     w, jbp, wp, yt, piv, bv, sing);
+                    else if (reg == 3)
+                        launch_timed(k_gj_fused<float, 3>, g, dim3(256), 0, s, dj, dt, nf, Qv, ldq, strideQ, rows, jb,
+                                     w, jbp, wp, yt, piv, bv, sing);
+                    else
+                        launch_timed(k_gj_fused<float, 4>, g, dim3(256), 0, s, dj, dt, nf, Qv, ldq, strideQ, rows, jb,
+                                     w, jbp, wp, yt, piv, bv, sing);
+                }
+                NSLX_LAUNCH_CHECK();
+            });
+            pend.clear();
+        } else {
+        flush();
         p.custom([=](hipStream_t s) {
             ProfScope prof(prof_on() ? std::string(f64 ? "gj_panel_f64" : "gj_panel_f32") : std::string(), s);
             prof.work(pflops, pbytes);
@@ -777,6 +946,7 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
             }
             NSLX_LAUNCH_CHECK();
         });
+        }
         const int c0 = jb + w;
         const int ncol = width - c0;
         // the pivot rows' sources read and the pivot rows written (w each),
@@ -817,11 +987,12 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
             // identity's columns of rows not yet pivots) are left untouched
             j.zf = zf + (size_t)f * ((ldq + 31) / 32);
         }
-        // tile of the rank-16 updates (K = 16, bound by reading and writing the
-        // trailing matrix): NSLX_GJ_TILE 0 (64 x 64) / 1 (32 x 32), else by count
-        static const int gj_tile = (int)env_double("NSLX_GJ_TILE", -1.0);
-        p.gemm(jobs, BVariant{prec_, true, 0, 0}, gj_tile == 0 || gj_tile == 1 ? gj_tile : -1);
+        pend = std::move(jobs);
+        pend_jb = jb;
+        pend_w = w;
+        pend_reg = reg != 0;
     }
+    flush();
 }
 
 void TimeSliceSolver::refactor(hipStream_t s) {

@@ -45,9 +45,10 @@ struct FermionMatrix;
 // A1 + r*lda1 (shifted by kseg) beyond; ph_r = ph + (r / phdiv) * ldph.  All
 // strides count complex elements (B: real elements for a real B).  Add may
 // be null, and may equal Y (each element is read and written by one thread).
-// zf (jobs with Add == Y only): per 32 output columns, whether any of those
-// rows of B is nonzero; a tile whose groups are all zero is left as it is
-// (Y = Add exactly) without reading or writing it.
+// zf (jobs with Add == Y only): per 32 output columns, counted from zfoff
+// columns before the job's first, whether any of those rows of B is nonzero;
+// a tile whose groups are all zero is left as it is (Y = Add exactly)
+// without reading or writing it.
 struct BJob {
     const void* A0 = nullptr;
     const void* A1 = nullptr;
@@ -65,6 +66,7 @@ struct BJob {
     int phdiv = 1 << 30;
     double sgn = 1.0;
     const int* zf = nullptr;
+    int zfoff = 0;
 };
 
 // Kernel variant of a launch: element precision (NSLX_F64 / NSLX_F32),
@@ -121,6 +123,11 @@ class BPlan {
     // stride).  One persistent launch when allowed, chain_supported() and
     // NSLX_CHAINS is not 0, else one batched GEMM launch per step.
     void chain(const std::vector<std::vector<BJob>>& steps, BVariant v, bool allow = true);
+    // append a batched GEMM launched by launch (given the uploaded jobs and
+    // tiles and the resolved variant) instead of launch_bgemm: a launch that
+    // runs other work beside the tiles
+    using GemmLauncher = std::function<void(hipStream_t, const BJob*, const int4*, int, const BVariant&)>;
+    void gemm_with(const std::vector<BJob>& jobs, BVariant v, int tile, GemmLauncher launch);
     // append any other launch
     void custom(std::function<void(hipStream_t)> fn);
     void upload(hipStream_t s);
@@ -139,6 +146,7 @@ class BPlan {
         int steps = 0, nchain = 0, rows = 0, cols = 0;
         double flops = 0.0, bytes = 0.0;
         std::function<void(hipStream_t)> fn;
+        GemmLauncher fused;
     };
     std::vector<Op> ops_;
     std::vector<BJob> jobs_;
