@@ -845,8 +845,9 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
         return t.size() >= 512 ? 0 : 1;
     };
     // a block's update is planned with the next block: in one launch with
-    // that block's panel (k_gj_fused) where both run register kernels of 256
-    // threads and the update takes 64 x 64 tiles, else on its own
+    // that block's panel (k_gj_fused, 64 x 64 update tiles: hiding the panel
+    // is worth more than the smaller tiles of a small update) where both run
+    // register kernels of 256 threads, else on its own
     std::vector<BJob> pend;
     int pend_jb = 0, pend_w = 0;
     bool pend_reg = false;
@@ -873,7 +874,7 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
         for (int jj = 0; jj < w; ++jj) pflops += 8.0 * (rows - jb - jj) * (w - jj);
         pflops = (pflops + (reg ? 0.0 : 4.0 * rows * w * w)) * nf;
         const bool fusable = f64 ? (reg == 1 || reg == 2) : (reg >= 1 && reg <= 4);
-        if (gj_fuse && !pend.empty() && pend_reg && fusable && update_tile(pend) == 0) {
+        if (gj_fuse && !pend.empty() && pend_reg && fusable && gj_tile != 1) {
             // block k's update without this panel's columns (those are
             // updated by the panel's workgroups)
             std::vector<BJob> rest = pend;
