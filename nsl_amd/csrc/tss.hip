@@ -873,7 +873,10 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
         double pflops = 0.0;
         for (int jj = 0; jj < w; ++jj) pflops += 8.0 * (rows - jb - jj) * (w - jj);
         pflops = (pflops + (reg ? 0.0 : 4.0 * rows * w * w)) * nf;
-        const bool fusable = f64 ? (reg == 1 || reg == 2) : (reg >= 1 && reg <= 4);
+        // (the fused launch has 256 threads: FP64 sub-panels of up to 768
+        // rows take 3 rows per thread there, with the update tile's spare
+        // accumulation registers)
+        const bool fusable = f64 ? (reg >= 1 && reg <= 2) || (reg == 3 && live <= 3 * 256) : (reg >= 1 && reg <= 4);
         if (gj_fuse && !pend.empty() && pend_reg && fusable && gj_tile != 1) {
             // block k's update without this panel's columns (those are
             // updated by the panel's workgroups)
@@ -901,7 +904,10 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
                 void* bv = (void*)binv;
                 const dim3 g((unsigned)(nf + ntl));
                 if (f64) {
-                    if (reg == 1)
+                    if (reg == 3)
+                        launch_timed(k_gj_fused<double, 3>, g, dim3(256), 0, s, dj, dt, nf, Qv, ldq, strideQ, rows,
+                                     jb, w, jbp, wp, yt, piv, bv, sing);
+                    else if (reg == 1)
                         launch_timed(k_gj_fused<double, 1>, g, dim3(256), 0, s, dj, dt, nf, Qv, ldq, strideQ, rows,
                                      jb, w, jbp, wp, yt, piv, bv, sing);
                     else

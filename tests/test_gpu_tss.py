@@ -164,10 +164,13 @@ def test_tss_pcg_mdagm_reaches_1e12(prec):
     assert np.allclose(dev, relv, rtol=1e-6, atol=1e-20), (dev, relv)
 
 
-def test_tss_full_size_bench_workload():
+@pytest.mark.parametrize("prec", [nsl.F32, nsl.F64])
+def test_tss_full_size_bench_workload(prec):
     """The bench workload (12x12 honeycomb, Nx = 288, Nt = 64, beta = 8, U = 3,
-    16 fields x 8 RHS) with the FP32 preconditioner: every RHS to 1e-12, and
-    one RHS of the first field checked on the CPU in double-double."""
+    16 fields x 8 RHS) with either preconditioner (FP64: 2 chunks, so the
+    first block system has 576 rows and its early pivot panels more than 512
+    live rows): every RHS to 1e-12, and one RHS of the first field checked on
+    the CPU in double-double."""
     L, Nt, beta, U = 12, 64, 8.0, 3.0
     Nx = 2 * L * L
     std = math.sqrt(beta / Nt * U)
@@ -176,7 +179,7 @@ def test_tss_full_size_bench_workload():
     fm.populate(phi.to(torch.complex128).to(DEV))
     b = torch.stack([torch.as_tensor(rng.complex_normal(2000 + 16 * f + k, 0, (Nt, Nx)))
                      for f in range(16) for k in range(8)])
-    cg = nsl.CGTimeSlice(fm, 1e-12, 2000, precision=nsl.F32)
+    cg = nsl.CGTimeSlice(fm, 1e-12, 2000, precision=prec)
     x = cg(b.to(DEV))
     assert cg.converged, cg.report
     assert cg.report["max_rel_residual"] <= 1e-12
