@@ -561,13 +561,15 @@ __global__ void __launch_bounds__(NT) k_gj_panel_reg(void* Qb, long long ld, lon
 
 // One launch for block k's trailing update and block k + 1's pivot panel.
 // Workgroups f < nf first apply block k's rank-wp update to block k + 1's
-// columns [jb, jb + w) in every row (multipliers: Q's columns [jbp, jbp + wp),
-// pivot-row values: YT rows 0 .. w - 1, since block k's update starts at
-// column jbp + wp = jb), then factor the panel (gj_panel_body, which reads
-// those columns back after the barrier: the workgroup's own writes); the
-// other workgroups are the 64 x 64 tiles of block k's update of the columns
-// from jb + w on.  The two parts touch disjoint columns of Q, so block
-// k + 1's panel runs beside the update instead of after it.
+// columns [jb, jb + w) in the panel's rows [jb, rows) (multipliers: Q's
+// columns [jbp, jbp + wp), pivot-row values: YT rows 0 .. w - 1, since block
+// k's update starts at column jbp + wp = jb), then factor the panel
+// (gj_panel_body, which reads those columns back after the barrier: the
+// workgroup's own writes); the other workgroups are the 64 x 64 tiles of
+// block k's update of the columns from jb + w on and of the panel columns'
+// rows above jb (the panel reads no row above jb).  The parts write
+// disjoint parts of Q, so block k + 1's panel runs beside the update
+// instead of after it.
 template <class S, int RPT>
 __global__ void __launch_bounds__(256) k_gj_fused(const BJob* __restrict__ jobs, const int4* __restrict__ tiles,
                                                   int nf, void* Qb, long long ld, long long strideQ, int rows, int jb,
@@ -610,8 +612,8 @@ __global__ void __launch_bounds__(256) k_gj_fused(const BJob* __restrict__ jobs,
             fq[i][1] = ldz<S>(Q, rb + min(c + 8, wl));
         }
     };
-    fetch(0);
-    for (int r0 = 0; r0 < rows; r0 += PR) {
+    fetch(jb);
+    for (int r0 = jb; r0 < rows; r0 += PR) {
         #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int e = tid + 256 * u;
@@ -888,13 +890,22 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
                 j.cols -= w;
                 j.zfoff = w;
             }
+            // and the panel columns' rows above the panel (block k's own
+            // update job restricted to them)
+            if (jb > 0)
+                for (int f = 0; f < nf; ++f) {
+                    BJob j = pend[f];
+                    j.rows = jb;
+                    j.cols = w;
+                    rest.push_back(j);
+                }
             const int jbp = pend_jb, wp = pend_w;
             double ufl = 0.0, uby = 0.0;
             bgemm_work(rest, BVariant{prec_, true, 0, 0}, &ufl, &uby);
-            // the panel columns' update: read (and write) them, read the
-            // multipliers, 8 flops per complex multiply-add
-            const double ffl = pflops + ufl + 8.0 * nf * rows * w * wp;
-            const double fby = pbytes + uby + (double)nf * rows * (2.0 * w + wp) * e;
+            // the panel columns' update of the panel's rows: read (and write)
+            // them, read the multipliers, 8 flops per complex multiply-add
+            const double ffl = pflops + ufl + 8.0 * nf * (rows - jb) * w * wp;
+            const double fby = pbytes + uby + (double)nf * (rows - jb) * (2.0 * w + wp) * e;
             p.gemm_with(rest, BVariant{prec_, true, 0, 0}, 0,
                         [=](hipStream_t s, const BJob* dj, const int4* dt, int ntl, const BVariant&) {
                 ProfScope prof(prof_on() ? std::string(f64 ? "gj_fused_f64" : "gj_fused_f32") : std::string(), s);
