@@ -165,10 +165,15 @@ bool BPlan::chain_failed(hipStream_t s) {
     NSLX_HIP(hipMemcpyAsync(&e, err_.p, sizeof(int), hipMemcpyDeviceToHost, s));
     NSLX_HIP(hipStreamSynchronize(s));
     if (!e) return false;
+    chain_reset(s);
+    return true;
+}
+
+void BPlan::chain_reset(hipStream_t s) {
+    if (!err_.p) return;
     NSLX_HIP(hipMemsetAsync(err_.p, 0, sizeof(int), s));
     NSLX_HIP(hipMemsetAsync(sync_.p, 0, sync_words_ * sizeof(unsigned), s));
     NSLX_HIP(hipStreamSynchronize(s));
-    return true;
 }
 
 void BPlan::gemm_with(const std::vector<BJob>& jobs, BVariant v, int tile, GemmLauncher launch) {

@@ -1483,6 +1483,36 @@ void TimeSliceSolver::build_mdinv(BPlan& p, const void* y, void* x, ApplyBufs& B
     }
 }
 
+void TimeSliceSolver::queue_pass_checks(hipStream_t s) {
+    pass_plans_.clear();
+    for (auto& kv : plans_) {
+        ApplyBufs& B = *kv.second;
+        for (BPlan* p : {&B.Minv, &B.Mdinv}) pass_plans_.push_back(p);
+        for (auto& q : B.bound)
+            for (BPlan* p : {&q->first, &q->second}) pass_plans_.push_back(p);
+    }
+    pass_plans_.erase(std::remove_if(pass_plans_.begin(), pass_plans_.end(),
+                                     [](BPlan* p) { return p->chain_err() == nullptr; }),
+                      pass_plans_.end());
+    pass_h_.ensure(std::max<size_t>(pass_plans_.size(), 1));
+    for (size_t i = 0; i < pass_plans_.size(); ++i)
+        NSLX_HIP(hipMemcpyAsync(pass_h_.p + i, pass_plans_[i]->chain_err(), sizeof(int), hipMemcpyDeviceToHost, s));
+}
+
+void TimeSliceSolver::check_passes_synced(hipStream_t s) {
+    bool failed = false;
+    for (size_t i = 0; i < pass_plans_.size(); ++i)
+        if (pass_h_.p[i]) {
+            pass_plans_[i]->chain_reset(s);
+            failed = true;
+        }
+    pass_plans_.clear();
+    if (failed)
+        throw Error(NSLX_E_INTERNAL,
+                    "time-slice solve: a recurrence launch stopped waiting for its workgroups (results invalid; "
+                    "set NSLX_CHAINS=0 for one launch per slice)");
+}
+
 void TimeSliceSolver::check_passes(hipStream_t s) {
     bool failed = false;
     for (auto& kv : plans_) {

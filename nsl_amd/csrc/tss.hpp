@@ -136,6 +136,10 @@ class BPlan {
     // whether a chain launch of the plan gave up waiting (clears the flag
     // and the counters; synchronises s)
     bool chain_failed(hipStream_t s);
+    // the plan's give-up flag on the device (null without chain launches),
+    // and the reset of flag and counters after one was seen (synchronises s)
+    const int* chain_err() const { return err_.p; }
+    void chain_reset(hipStream_t s);
 
    private:
     struct Op {
@@ -196,6 +200,10 @@ class TimeSliceSolver {
     // throws when a persistent recurrence launch of an apply since the last
     // check gave up waiting (synchronises s)
     void check_passes(hipStream_t s);
+    // the same in two parts around a wait the caller does anyway: the flags'
+    // copies queued on s, then (after s was synchronised) the test
+    void queue_pass_checks(hipStream_t s);
+    void check_passes_synced(hipStream_t s);
     // throws when the last factorisation found the reduced block system
     // singular.  The factorisation only queues the flag's copy to the host;
     // synced: s has been synchronised since (else this synchronises it).
@@ -231,6 +239,8 @@ class TimeSliceSolver {
     DevBuf<int> gj_piv_, gj_sing_, gj_zf_;
     PinBuf<int> sing_h_;
     bool sing_pending_ = false;
+    PinBuf<int> pass_h_;                 // queued give-up flags
+    std::vector<BPlan*> pass_plans_;     // the plans they belong to
     DevBuf<unsigned char> gj_binv_, gj_C_, gj_YT_;
     void setup(hipStream_t s);
     void build_setup(hipStream_t s);

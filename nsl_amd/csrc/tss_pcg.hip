@@ -252,9 +252,10 @@ int solve_tss(FermionMatrix& fm, int op, const void* b, void* x, void* xlo, cons
     NSLX_HIP(hipMemcpyAsync(hr, rn2, nb * sizeof(double), hipMemcpyDeviceToHost, s));
     NSLX_HIP(hipMemcpyAsync(hb, bn2, nb * sizeof(double), hipMemcpyDeviceToHost, s));
     NSLX_HIP(hipMemcpyAsync(hd, done_out, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+    ts.queue_pass_checks(s);
     NSLX_HIP(hipStreamSynchronize(s));
     ts.check_singular(s, true);
-    ts.check_passes(s);
+    ts.check_passes_synced(s);
     int64_t conv = 0;
     double maxrel = 0.0, tot = 0.0;
     for (int64_t i = 0; i < nb; ++i) {
