@@ -838,8 +838,9 @@ void TimeSliceSolver::gj(BPlan& p, unsigned char* Q, long long ldq, long long st
     // tile of the rank-16 updates (K = 16, bound by reading and writing the
     // trailing matrix): NSLX_GJ_TILE 0 (64 x 64) / 1 (32 x 32), else by count
     static const int gj_tile = (int)env_double("NSLX_GJ_TILE", -1.0);
-    // NSLX_GJ_FUSE=0: every panel and update in launches of their own
-    static const bool gj_fuse = env_double("NSLX_GJ_FUSE", 1.0) != 0.0;
+    // NSLX_GJ_FUSE=0: every panel and update in launches of their own (read
+    // when a plan is built: tests compare both forms)
+    const bool gj_fuse = env_double("NSLX_GJ_FUSE", 1.0) != 0.0;
     auto update_tile = [&](const std::vector<BJob>& jobs) {
         if (gj_tile == 0 || gj_tile == 1) return gj_tile;
         std::vector<int4> t;

@@ -98,6 +98,27 @@ def test_direct_solve_residual(case, prec):
             assert r < 1e-8, (name, r)
 
 
+@pytest.mark.parametrize("prec", [nsl.F64, nsl.F32])
+def test_fused_gauss_jordan_matches_separate_launches(monkeypatch, prec):
+    """The factorisation with each block's trailing update and the next
+    block's pivot panel in one launch (the default) against the form with
+    every panel and update in launches of their own (NSLX_GJ_FUSE=0, read
+    when the plan is built): the same direct solves to rounding (the chunk
+    products' condition, up to ~1e8 for FP64's 16-slice chunks, sets the
+    tolerance).  Honeycomb 9x9 (Nx = 162), Nt = 32, 2 fields: 324-row block
+    systems (panels of 257-324 and <= 256 live rows) and a 162-row last one."""
+    Nt, beta, U = 32, 6.0, 3.0
+    y = torch.as_tensor(rng.complex_normal(1401, 0, (4, Nt, 162)))
+    nch = 2 if prec == nsl.F64 else 0
+    xs = {}
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("NSLX_GJ_FUSE", fuse)
+        fm, _ = make("honeycomb", (9, 9), Nt, beta, nfield=2, seed=1400, std=math.sqrt(beta / Nt * U))
+        xs[fuse] = fm.solve_direct(nsl.MatrixCombination.MMdagger, y.to(DEV), precision=prec, nchunks=nch).cpu()
+    tol = 1e-7 if prec == nsl.F64 else 1e-3
+    assert rel(xs["1"], xs["0"]) < tol, rel(xs["1"], xs["0"])
+
+
 def cpu_dd_check(fm, b, x, x_lo, rpf):
     E, P = fm.export()
     _, relv = dd.mmdag_residual(E, P, b.cpu().numpy(), x.cpu().numpy(), x_lo.cpu().numpy(), rpf=rpf)
