@@ -22,9 +22,15 @@ those pools; their exchange structure is pinned by Fermi antisymmetry
 (tests/test_oracle_core.py::test_two_body_pauli: for any field the eight
 I = 1, S = 1 channels vanish where the two sink or the two source labels
 coincide, the others do not), which a wrong exchange sign, a transposed
-exchange index or a swapped pool pair fails.  The prefactors alpha remain
-parity unpinned (no independent value of a channel exists without running
-the reference).
+exchange index or a swapped pool pair fails.  The prefactors alpha are
+pinned pairwise by particle-hole symmetry
+(tests/test_oracle_core.py::test_two_body_particle_hole_pairs: at phi = 0,
+mu = 0 on the bipartite honeycomb each Iz = +1 channel equals its Iz = -1
+partner times the sublattice signs of its four band labels, and the Iz = 0
+channels with Sz = +1 and -1 are equal; the six pairs cover all twelve
+channels, so a wrong alpha in any one channel fails).  A factor common to
+both channels of a pair remains parity unpinned (no independent value of a
+channel exists without running the reference).
 """
 
 import math

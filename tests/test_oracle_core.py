@@ -475,3 +475,45 @@ def test_two_body_pauli():
     # away from the coincident labels no channel vanishes
     for name in anti:
         assert max(float(c.abs().max()) for c in res[name].values()) > 1e-6 * scale, name
+
+
+def test_two_body_particle_hole_pairs():
+    """Prefactor pin of the twelve two-body channels.  At phi = 0 and mu = 0
+    on the bipartite honeycomb the free theory is symmetric under the
+    particle-hole transformation c_x -> (-1)^x c_x^dagger, which exchanges the
+    particle and hole pools and multiplies each band label by its sublattice
+    sign.  So each Iz = +1 channel equals its Iz = -1 partner times
+    s_i s_j s_l s_k (s = +1 on sublattice A, -1 on B), and the two Iz = 0
+    channels of each multiplet (Sz = +1 and -1) are equal.  These six pairs
+    cover all twelve channels, so a wrong value or sign of any one alpha
+    fails (every channel is nonzero here).  A factor common to both members
+    of a pair stays unpinned."""
+    L, Nt, beta = 3, 8, 2.0
+    lat, w, mom = honeycomb_walls(L, [(0, 0), (1, 0), (2, 0)])
+    Nx = lat.sites()
+    phi = torch.zeros((Nt, Nx), dtype=torch.float64)
+
+    def make(species):
+        o = F.HubbardExp(lat, Nt, beta, species=species)
+        o.populate(phi)
+        return o
+
+    res = correlator.two_body(make, phi, w, mom, 2)
+    b = w.shape[1]
+    s = torch.tensor([1.0, -1.0], dtype=torch.float64)
+    sub = torch.einsum("i,j,l,k->ijlk", s, s, s, s).reshape(b * b, b * b).to(torch.complex128)
+    pairs = [("I1S1Iz1Sz1", "I1S1Izn1Sz1", sub), ("I1S1Iz1Sz0", "I1S1Izn1Sz0", sub),
+             ("I1S1Iz1Szn1", "I1S1Izn1Szn1", sub), ("I1S0Iz1Sz0", "I1S0Izn1Sz0", sub),
+             ("I1S1Iz0Sz1", "I1S1Iz0Szn1", None), ("I0S1Iz0Sz1", "I0S1Iz0Szn1", None)]
+    covered = sorted(n for p in pairs for n in p[:2])
+    assert covered == sorted(n for n, _, _ in correlator.TWO_BODY_TABLE)
+    scale = max(float(c.abs().max()) for ch in res.values() for c in ch.values())
+    for a, bname, sign in pairs:
+        assert max(float(c.abs().max()) for c in res[a].values()) > 0.1 * scale, a
+        for q, c in res[a].items():
+            other = res[bname][q] if sign is None else res[bname][q] * sign
+            assert float((c - other).abs().max()) < 1e-12 * scale, (a, bname, q)
+    # the relation is not an identity of the contraction: the Iz = +1 and
+    # Iz = -1 channels differ without the sublattice sign
+    d = max(float((res["I1S1Iz1Sz0"][q] - res["I1S1Izn1Sz0"][q]).abs().max()) for q in res["I1S1Iz1Sz0"])
+    assert d > 0.1 * scale
