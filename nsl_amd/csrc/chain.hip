@@ -6,10 +6,11 @@
 //   * a workgroup owns the 16-row block rt and the 32-column tile m of one
 //     chain (chunk); its slice of B (= exp(delta K), the same in every step)
 //     stays in LDS for the whole launch;
-//   * per step it stages its 16 A rows (all K columns, phase applied) in LDS,
-//     runs v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32 over K split in
-//     quarters across its 8 waves (2 column blocks x 4 quarters), reduces the
-//     quarters through LDS and writes its 16 x 32 output tile;
+//   * per step each of its 8 waves loads its share of the 16 A rows (K steps
+//     of 4 columns, every 8th one) straight into registers, applies the phase
+//     there and runs v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32 against
+//     both 16-column blocks of the tile; the 8 partial tiles are summed
+//     through LDS and the 16 x 32 output tile is written;
 //   * the column tiles of one (chain, row block) -- a group -- meet at a
 //     counter in device memory between steps.  Rows of different groups
 //     never meet, so there is no grid-wide barrier.
@@ -90,7 +91,7 @@ __global__ void __launch_bounds__(CT) k_chain(const BJob* __restrict__ jobs, int
     using Acc = typename ChainT<S>::Acc;
     constexpr int KP = ChainT<S>::KMAX + 1;
     __shared__ S sE[CBN][KP];
-    __shared__ S sA[2][CBM][KP];
+    __shared__ S sRed[8 * 2 * CBM * CBM * 2];   // per-wave partial products
     __shared__ int s_abort;
     __shared__ unsigned s_cnt;
     const int ngroups = nchain * nrt;
@@ -100,7 +101,6 @@ __global__ void __launch_bounds__(CT) k_chain(const BJob* __restrict__ jobs, int
     if (g >= ngroups) return;
     const int chain = g / nrt, rt = g % nrt;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int cb = w & 1, kq = w >> 1;
     const int r16 = lane & 15, q = lane >> 4;
     const BJob& J0 = jobs[chain];
     const int rows = J0.rows, cols = J0.cols, K = J0.K;
@@ -120,22 +120,23 @@ __global__ void __launch_bounds__(CT) k_chain(const BJob* __restrict__ jobs, int
     const bool placed = (unsigned)__builtin_amdgcn_s_getreg(HWREG_XCC_ID) == (unsigned)(b & 7);
     bool fenced = true;     // L2 write-back before each signal, until the group is known to share one L2
 
-    constexpr int NK = (ChainT<S>::KMAX + 31) / 32;    // A elements per thread per step
-    const int row = tid >> 5, kl = tid & 31;           // staging: 32 lanes per A row
-    const int orow = tid >> 5, ocol = tid & 31;        // epilogue: one output per thread
+    __syncthreads();   // sE complete before the first step's products
+
+    constexpr int JN = (ChainT<S>::KMAX / 4 + 7) / 8;  // K steps of 4 per wave per step
+    const int r = r0 + r16;                              // this lane's A row
+    const bool rlive = r < rows;
+    const int orow = tid >> 5, ocol = tid & 31;          // epilogue: one output per thread
     for (int s = 0; s < steps; ++s) {
         const BJob& J = jobs[(long long)s * nchain + chain];
         // everything the step reads that no step writes, loaded before the
         // wait: the A-row phases (PH 1), the Add element and its phase (PH 2)
-        const int r = r0 + row;
-        const bool rlive = r < rows;
-        Z2<S> ph[NK];
+        Z2<S> ph[JN];
         if constexpr (PH == 1) {
             const long long prow = rlive ? (long long)(r / J.phdiv) * J.ldph : 0;
             #pragma unroll
-            for (int i = 0; i < NK; ++i) {
-                const int k = kl + 32 * i;
-                ph[i] = (rlive && k < K) ? ldz2<S>(J.ph, prow + k) : Z2<S>{S(0), S(0)};
+            for (int j = 0; j < JN; ++j) {
+                const int k = 4 * (w + 8 * j) + q;
+                ph[j] = (rlive && k < K) ? ldz2<S>(J.ph, prow + k) : Z2<S>{S(0), S(0)};
             }
         }
         const int orr = r0 + orow, occ = c0 + ocol;
@@ -175,57 +176,57 @@ __global__ void __launch_bounds__(CT) k_chain(const BJob* __restrict__ jobs, int
             }
             __syncthreads();
         }
-        // the step's 16 A rows, all loads in flight at once, phase applied (PH 1)
-        {
-            Z2<S> z[NK];
-            #pragma unroll
-            for (int i = 0; i < NK; ++i) {
-                const int k = kl + 32 * i;
-                z[i] = Z2<S>{S(0), S(0)};
-                if (rlive && k < K)
-                    z[i] = k < J.kseg ? ldz2<S>(J.A0, (long long)r * J.lda0 + k)
-                                      : ldz2<S>(J.A1, (long long)r * J.lda1 + (k - J.kseg));
-            }
-            #pragma unroll
-            for (int i = 0; i < NK; ++i) {
-                const int k = kl + 32 * i;
-                if (k < Kp) {
-                    Z2<S> v = z[i];
-                    if constexpr (PH == 1) {
-                        v.re = ph[i].re * z[i].re - ph[i].im * z[i].im;
-                        v.im = ph[i].re * z[i].im + ph[i].im * z[i].re;
-                    }
-                    sA[0][row][k] = v.re;
-                    sA[1][row][k] = v.im;
+        // this wave's K steps ks = w + 8 j of the 16 A rows, straight into
+        // registers (all loads in flight at once; each product waits only for
+        // its own operand), phase applied (PH 1), against both column blocks
+        Z2<S> z[JN];
+        #pragma unroll
+        for (int j = 0; j < JN; ++j) {
+            const int k = 4 * (w + 8 * j) + q;
+            z[j] = Z2<S>{S(0), S(0)};
+            if (rlive && k < K)
+                z[j] = k < J.kseg ? ldz2<S>(J.A0, (long long)r * J.lda0 + k)
+                                  : ldz2<S>(J.A1, (long long)r * J.lda1 + (k - J.kseg));
+        }
+        Acc are0 = {0, 0, 0, 0}, aim0 = {0, 0, 0, 0}, are1 = {0, 0, 0, 0}, aim1 = {0, 0, 0, 0};
+        #pragma unroll
+        for (int j = 0; j < JN; ++j) {
+            const int ks = w + 8 * j;
+            if (ks < nks) {
+                const int k = 4 * ks + q;
+                Z2<S> v = z[j];
+                if constexpr (PH == 1) {
+                    v.re = ph[j].re * z[j].re - ph[j].im * z[j].im;
+                    v.im = ph[j].re * z[j].im + ph[j].im * z[j].re;
                 }
+                const S b0 = sE[r16][k], b1 = sE[16 + r16][k];
+                are0 = cmfma<S>(v.re, b0, are0);
+                aim0 = cmfma<S>(v.im, b0, aim0);
+                are1 = cmfma<S>(v.re, b1, are1);
+                aim1 = cmfma<S>(v.im, b1, aim1);
             }
         }
-        __syncthreads();
-        Acc are = {0, 0, 0, 0}, aim = {0, 0, 0, 0};
-        for (int ks = kq; ks < nks; ks += 4) {
-            const int k = 4 * ks + q;
-            const S bv = sE[16 * cb + r16][k];
-            are = cmfma<S>(sA[0][r16][k], bv, are);
-            aim = cmfma<S>(sA[1][r16][k], bv, aim);
-        }
-        __syncthreads();   // sA is read no more this step: reuse it for the partials
-        S* red = &sA[0][0][0];   // [kq][cb][row][col][re, im]: 4096 elements
+        // partials [w][cb][row][col][re, im]; the previous step's reads of
+        // sRed ended at its closing barrier
         #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int row = (sizeof(S) == 8) ? 4 * v + q : 4 * q + v;
-            const int o = ((((kq * 2 + cb) * 16 + row) * 16) + r16) * 2;
-            red[o] = are[v];
-            red[o + 1] = aim[v];
+            const int o0 = ((((w * 2 + 0) * 16 + row) * 16) + r16) * 2;
+            const int o1 = ((((w * 2 + 1) * 16 + row) * 16) + r16) * 2;
+            sRed[o0] = are0[v];
+            sRed[o0 + 1] = aim0[v];
+            sRed[o1] = are1[v];
+            sRed[o1 + 1] = aim1[v];
         }
         __syncthreads();
         {
             const int cbx = ocol >> 4, cc = ocol & 15;
             S re = S(0), im = S(0);
             #pragma unroll
-            for (int k4 = 0; k4 < 4; ++k4) {
-                const int o = ((((k4 * 2 + cbx) * 16 + orow) * 16) + cc) * 2;
-                re += red[o];
-                im += red[o + 1];
+            for (int k8 = 0; k8 < 8; ++k8) {
+                const int o = ((((k8 * 2 + cbx) * 16 + orow) * 16) + cc) * 2;
+                re += sRed[o];
+                im += sRed[o + 1];
             }
             if (olive) {
                 if constexpr (PH == 2) {
@@ -240,7 +241,7 @@ __global__ void __launch_bounds__(CT) k_chain(const BJob* __restrict__ jobs, int
             }
         }
         // publish the tile: every wave's stores complete, then one signal
-        // (the barrier also ends this step's reads of red)
+        // (the barrier also ends this step's reads of sRed)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
         __syncthreads();
