@@ -324,18 +324,38 @@ __global__ void k_transpose_c(int n, const double2* A, double2* AT) {
     AT[(size_t)c * n + r] = A[i];
 }
 
-// d[x] = i * sum_k QT[k][x] G[k][x]: one thread per x, consecutive threads
-// reading consecutive elements of each row
-__global__ void k_diag_iprod_cols(int n, const double2* QT, const double2* G, double2* d) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= n) return;
+// d_b[x] = i * sum_k QT_b[k][x] G_b[k][x] for the nb matrix pairs b (QT_b =
+// QT + b * stride, G_b likewise, d_b = d + b * dstride): a workgroup per 16
+// columns of one pair, its 16 thread rows taking every 16th k (consecutive
+// threads read consecutive elements of a row), the 16 partial sums of a
+// column added in a fixed order
+constexpr int DIP = 16;
+__global__ void __launch_bounds__(256) k_diag_iprod_cols(int n, const double2* __restrict__ QT,
+                                                         const double2* __restrict__ G, int64_t stride,
+                                                         double2* __restrict__ d, int64_t dstride) {
+    __shared__ double2 part[DIP][DIP + 1];
+    const int c = threadIdx.x % DIP, ks = threadIdx.x / DIP;
+    const int x = blockIdx.x * DIP + c;
+    const int64_t off = (int64_t)blockIdx.y * stride;
     double r = 0.0, im = 0.0;
-    for (int k = 0; k < n; ++k) {
-        const double2 a = QT[(size_t)k * n + x], b = G[(size_t)k * n + x];
-        r += a.x * b.x - a.y * b.y;
-        im += a.x * b.y + a.y * b.x;
+    if (x < n) {
+        #pragma unroll 4
+        for (int k = ks; k < n; k += DIP) {
+            const double2 a = QT[off + (int64_t)k * n + x], b = G[off + (int64_t)k * n + x];
+            r += a.x * b.x - a.y * b.y;
+            im += a.x * b.y + a.y * b.x;
+        }
     }
-    d[x] = make_double2(-im, r);
+    part[ks][c] = make_double2(r, im);
+    __syncthreads();
+    if (ks == 0 && x < n) {
+        double sr = 0.0, si = 0.0;
+        for (int j = 0; j < DIP; ++j) {
+            sr += part[j][c].x;
+            si += part[j][c].y;
+        }
+        d[(int64_t)blockIdx.y * dstride + x] = make_double2(-si, sr);
+    }
 }
 
 inline unsigned g1(int64_t m) { return (unsigned)((m + 255) / 256); }
@@ -359,8 +379,14 @@ void transpose_c(int n, const void* A, void* AT, hipStream_t s) {
 }
 
 void diag_iprod_cols(int n, const void* QT, const void* G, void* d, hipStream_t s) {
-    hipLaunchKernelGGL(k_diag_iprod_cols, dim3(g1(n)), dim3(256), 0, s, n, (const double2*)QT, (const double2*)G,
-                       (double2*)d);
+    diag_iprod_cols_batched(n, 1, QT, G, 0, d, 0, s);
+}
+
+void diag_iprod_cols_batched(int n, int nb, const void* QT, const void* G, int64_t stride, void* d, int64_t dstride,
+                             hipStream_t s) {
+    if (n <= 0 || nb <= 0) return;
+    hipLaunchKernelGGL(k_diag_iprod_cols, dim3((n + DIP - 1) / DIP, nb), dim3(DIP * DIP), 0, s, n,
+                       (const double2*)QT, (const double2*)G, stride, (double2*)d, dstride);
     NSLX_LAUNCH_CHECK();
 }
 

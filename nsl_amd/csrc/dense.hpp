@@ -52,5 +52,9 @@ void conj_c(int64_t m, const void* in, void* out, hipStream_t s);
 void transpose_c(int n, const void* A, void* AT, hipStream_t s);
 // d[x] = i * sum_k QT[k][x] G[k][x]  (the diagonal of i * QT^T G), complex.
 void diag_iprod_cols(int n, const void* QT, const void* G, void* d, hipStream_t s);
+// the same for nb pairs: QT_b = QT + b * stride, G_b = G + b * stride, d_b = d + b * dstride
+// (strides in complex elements)
+void diag_iprod_cols_batched(int n, int nb, const void* QT, const void* G, int64_t stride, void* d, int64_t dstride,
+                             hipStream_t s);
 
 }  // namespace nslx

@@ -24,7 +24,9 @@
 // The Nx x Nx products run on the FP64 matrix cores as launches of the
 // batched GEMM (bgemm.hip): every chunk's product advances one slice per
 // launch (all chunks in one launch), and the force's two sequential chains
-// are one launch per slice each.  Both use that E (and D) are symmetric
+// are one persistent recurrence launch each (chain.hip; one launch per slice
+// where that launch does not fit the device or is not allowed), their
+// diagonals one batched launch.  Both use that E (and D) are symmetric
 // (K is), so E[k][c] = E[c][k]: a product X E diag(p) is the GEMM's
 // Y = X B^T with B = E and the phase on the output columns.
 #include <algorithm>
@@ -147,13 +149,16 @@ void fm_grad_logdet(FermionMatrix& fm, int64_t field, void* out, hipStream_t s) 
     // The chain is kept transposed, QT_t = Q_t^T = QT_{t+1} E diag(Pinv_{t-1})
     // (E symmetric): a GEMM with the phase on its output columns (given
     // conjugated).  diag(Q_{t+1} G)[x] = sum_k QT_{t+1}[k][x] G[k][x].
-    DevBuf<double2> QT(m * nt), G(m), G2(m), H(m), aug(2 * m), work((size_t)nx), Pic((size_t)fm.len());
+    // QT holds every QT_t; Gs every G_t (slot t: G after t steps), so that
+    // both recurrences are one persistent launch each and the diagonals one
+    // batched launch after them
+    DevBuf<double2> QT(m * nt), Gs(m * nt), H(m), aug(2 * m), work((size_t)nx), Pic((size_t)fm.len());
     conj_c(fm.len(), Pi, Pic.p, s);
     auto Pcprev = [&](int t) { return Pic.p + (size_t)((t - 1 + nt) % nt) * nx; };
     const double* E = en.dense.p;
     // QT_{Nt-1} = Fk_{Nt-1}^T = E diag(Pinv_{Nt-2})
     dscale_cols(nx, E, Pprev(nt - 1), QT.p + (size_t)(nt - 1) * m, s);
-    BPlan chain_q;
+    std::vector<std::vector<BJob>> steps_q, steps_g;
     for (int t = nt - 2; t >= 0; --t) {
         BJob b;
         b.A0 = QT.p + (size_t)(t + 1) * m;
@@ -165,40 +170,50 @@ void fm_grad_logdet(FermionMatrix& fm, int64_t field, void* out, hipStream_t s) 
         b.ldy = nx;
         b.ph = Pcprev(t);
         b.ldph = 0;
-        chain_q.gemm({b}, BVariant{NSLX_F64, false, 2, 0});
+        steps_q.push_back({b});
     }
-    chain_q.upload(s);
-    chain_q.run(s);
-    // G = (1 + Q_0)^{-1} = ((1 + QT_0)^{-1})^T
-    NSLX_HIP(hipMemcpyAsync(H.p, QT.p, m * sizeof(double2), hipMemcpyDeviceToDevice, s));
-    add_identity_c(nx, H.p, s);
-    gj_inverse(nx, H.p, aug.p, work.p, s);
-    transpose_c(nx, H.p, G.p, s);
-    double2* pi = reinterpret_cast<double2*>(out);
-    diag_iprod_cols(nx, QT.p, G.p, pi + (size_t)(nt - 1) * nx, s);
     // G <- G diag(Pinv_{t-1}) E(-s delta) (the phase on the GEMM's A columns),
-    // pi_t = i diag(Q_{t+1} G)
-    BPlan chain_g;
-    double2* gb[2] = {G.p, G2.p};
+    // pi_t = i diag(Q_{t+1} G_{t+1})
     for (int t = 0; t < nt - 1; ++t) {
         BJob b;
-        b.A0 = gb[t & 1];
+        b.A0 = Gs.p + (size_t)t * m;
         b.lda0 = nx;
         b.rows = b.cols = b.K = nx;
         b.B = E;
         b.ldb = nx;
-        b.Y = gb[(t + 1) & 1];
+        b.Y = Gs.p + (size_t)(t + 1) * m;
         b.ldy = nx;
         b.ph = Pprev(t);
         b.ldph = 0;
-        chain_g.gemm({b}, BVariant{NSLX_F64, false, 1, 0});
-        const double2* Gt = gb[(t + 1) & 1];
-        const double2* Qn = QT.p + (size_t)(t + 1) * m;
-        double2* dst = pi + (size_t)t * nx;
-        chain_g.custom([=](hipStream_t st) { diag_iprod_cols(nx, Qn, Gt, dst, st); });
+        steps_g.push_back({b});
     }
-    chain_g.upload(s);
-    chain_g.run(s);
+    double2* pi = reinterpret_cast<double2*>(out);
+    // a recurrence launch that gave up waiting for its workgroups leaves its
+    // input (QT_{Nt-1}, G_0) intact: the recurrence is run again one launch
+    // per slice
+    auto run_chain = [&](const std::vector<std::vector<BJob>>& steps, int ph) {
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            BPlan plan;
+            plan.chain(steps, BVariant{NSLX_F64, false, ph, 0}, attempt == 0 && fm.chains_ok);
+            plan.upload(s);
+            plan.run(s);
+            if (!plan.chain_failed(s)) {
+                NSLX_HIP(hipStreamSynchronize(s));   // the plan's device arrays are freed on return
+                return;
+            }
+        }
+    };
+    if (!steps_q.empty()) run_chain(steps_q, 2);
+    // G_0 = (1 + Q_0)^{-1} = ((1 + QT_0)^{-1})^T
+    NSLX_HIP(hipMemcpyAsync(H.p, QT.p, m * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    add_identity_c(nx, H.p, s);
+    gj_inverse(nx, H.p, aug.p, work.p, s);
+    transpose_c(nx, H.p, Gs.p, s);
+    diag_iprod_cols(nx, QT.p, Gs.p, pi + (size_t)(nt - 1) * nx, s);
+    if (!steps_g.empty()) {
+        run_chain(steps_g, 1);
+        diag_iprod_cols_batched(nx, nt - 1, QT.p + m, Gs.p + m, (int64_t)m, pi, nx, s);
+    }
     NSLX_HIP(hipStreamSynchronize(s));   // temporaries and the plans' device arrays are freed on return
 }
 

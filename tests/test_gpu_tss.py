@@ -319,3 +319,28 @@ def test_chunk_products_row_kernel_matches_batched_gemm(kind, size, Nt, beta, nf
     tol = 1e-10 if prec == nsl.F64 else 1e-3
     for a, b in zip(res["1"], res["0"]):
         assert rel(a, b) < tol, rel(a, b)
+
+
+def test_force_chains_match_per_slice_launches(monkeypatch):
+    """gradLogDetM with its two recurrences as persistent launches against one
+    launch per slice (NSLX_CHAINS=0), and with every wait of the persistent
+    launch giving up (NSLX_CHAIN_SPIN=0: the recurrence is run again one
+    launch per slice): the same force up to the summation order, and the
+    kernel classes show which path ran."""
+    fm, ofs = make("honeycomb", (6, 6), 16, 4.0, nfield=1, seed=1600)
+    res = {}
+    for mode in ("chains", "per_slice", "give_up"):
+        monkeypatch.setenv("NSLX_CHAINS", "0" if mode == "per_slice" else "1")
+        if mode == "give_up":
+            monkeypatch.setenv("NSLX_CHAIN_SPIN", "0")
+        g, classes = _prof_classes(lambda: fm.gradLogDetM())
+        chain_classes = {c for c in classes if c.startswith("chain_")}
+        assert bool(chain_classes) == (mode != "per_slice"), (mode, classes)
+        res[mode] = g.to("cpu")
+        monkeypatch.delenv("NSLX_CHAIN_SPIN", raising=False)
+    ref = ofs[0].gradLogDetM()
+    # the recurrences grow by up to exp(delta ||K|| Nt) = exp(12): rounding of
+    # a different summation order reaches ~1e-11 of the force
+    assert rel(res["chains"], ref) < 1e-10
+    assert rel(res["chains"], res["per_slice"]) < 1e-10
+    assert rel(res["give_up"], res["per_slice"]) < 1e-10
