@@ -176,7 +176,16 @@ __global__ void k_lu_update(int n, int k, double2* A) {
 }
 
 // ---- Gauss-Jordan on the augmented n x 2n matrix ----
-__global__ void k_gj_pivot(int n, int k, double2* M, double2* f) {
+// One Gauss-Jordan step on the augmented n x 2n matrix, from Min into Mout
+// (so that no workgroup reads what another writes): every workgroup finds the
+// pivot row p of column k itself (largest modulus among rows >= k, ties to
+// the lowest row, no swap when none is found), then writes its share of
+// columns j >= k (the columns left of k are unit columns that no later step
+// reads): row k = row p / pivot, and every other row i, taken from row k
+// where i = p, less its column-k element times the new row k.
+constexpr int GJ_PER = 8;   // elements per thread
+__global__ void __launch_bounds__(256) k_gj_step(int n, int k, const double2* __restrict__ Min,
+                                                 double2* __restrict__ Mout) {
     __shared__ double bv[256];
     __shared__ int bi[256];
     const int tid = threadIdx.x;
@@ -184,7 +193,7 @@ __global__ void k_gj_pivot(int n, int k, double2* M, double2* f) {
     double best = -1.0;
     int besti = n;
     for (int i = k + tid; i < n; i += 256) {
-        const double2 v = M[(size_t)i * w + k];
+        const double2 v = Min[(size_t)i * w + k];
         const double mm = hypot(v.x, v.y);
         if (mm > best || (mm == best && i < besti)) { best = mm; besti = i; }
     }
@@ -199,35 +208,28 @@ __global__ void k_gj_pivot(int n, int k, double2* M, double2* f) {
         }
         __syncthreads();
     }
-    const int p = bi[0];
-    if (p != k && p < n) {
-        for (int j = tid; j < w; j += 256) {
-            const double2 a = M[(size_t)k * w + j];
-            M[(size_t)k * w + j] = M[(size_t)p * w + j];
-            M[(size_t)p * w + j] = a;
-        }
-    }
-    __syncthreads();
-    const double2 piv = M[(size_t)k * w + k];
+    const int p = bi[0] < n ? bi[0] : k;
+    const double2 piv = Min[(size_t)p * w + k];
     const double den = piv.x * piv.x + piv.y * piv.y;
     const double2 inv = make_double2(piv.x / den, -piv.y / den);
-    __syncthreads();
-    for (int j = tid; j < w; j += 256) M[(size_t)k * w + j] = cm(M[(size_t)k * w + j], inv);
-    for (int i = tid; i < n; i += 256) f[i] = (i == k) ? make_double2(0.0, 0.0) : M[(size_t)i * w + k];
-}
-
-__global__ void k_gj_elim(int n, int k, double2* M, const double2* f) {
-    const int w = 2 * n;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)n * w) return;
-    const int i = (int)(idx / w), j = (int)(idx % w);
-    if (i == k) return;
-    const double2 fi = f[i];
-    const double2 u = M[(size_t)k * w + j];
-    double2 a = M[idx];
-    a.x -= fi.x * u.x - fi.y * u.y;
-    a.y -= fi.x * u.y + fi.y * u.x;
-    M[idx] = a;
+    const int wc = w - k;
+    const int64_t total = (int64_t)n * wc;
+    #pragma unroll
+    for (int e = 0; e < GJ_PER; ++e) {
+        const int64_t idx = ((int64_t)blockIdx.x * GJ_PER + e) * 256 + tid;
+        if (idx >= total) break;
+        const int i = (int)(idx / wc), j = k + (int)(idx % wc);
+        const double2 u = cm(Min[(size_t)p * w + j], inv);
+        double2 out = u;
+        if (i != k) {
+            const int src = (i == p) ? k : i;
+            const double2 f = Min[(size_t)src * w + k];
+            out = Min[(size_t)src * w + j];
+            out.x -= f.x * u.x - f.y * u.y;
+            out.y -= f.x * u.y + f.y * u.x;
+        }
+        Mout[(size_t)i * w + j] = out;
+    }
 }
 
 __global__ void k_aug_init(int n, const double2* A, double2* M) {
@@ -442,11 +444,14 @@ void lu_logdet(int n, void* A, double* acc, hipStream_t s) {
 void gj_inverse(int n, void* A, void* aug, void* work, hipStream_t s) {
     const int64_t m2 = (int64_t)n * 2 * n;
     hipLaunchKernelGGL(k_aug_init, dim3(g1(m2)), dim3(256), 0, s, n, (const double2*)A, (double2*)aug);
+    double2* buf[2] = {(double2*)aug, (double2*)work};
     for (int k = 0; k < n; ++k) {
-        hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(256), 0, s, n, k, (double2*)aug, (double2*)work);
-        hipLaunchKernelGGL(k_gj_elim, dim3(g1(m2)), dim3(256), 0, s, n, k, (double2*)aug, (const double2*)work);
+        const int64_t total = (int64_t)n * (2 * n - k);
+        const unsigned blocks = (unsigned)((total + 256 * GJ_PER - 1) / (256 * GJ_PER));
+        hipLaunchKernelGGL(k_gj_step, dim3(blocks), dim3(256), 0, s, n, k, buf[k & 1], buf[(k + 1) & 1]);
     }
-    hipLaunchKernelGGL(k_aug_extract, dim3(g1((int64_t)n * n)), dim3(256), 0, s, n, (const double2*)aug, (double2*)A);
+    hipLaunchKernelGGL(k_aug_extract, dim3(g1((int64_t)n * n)), dim3(256), 0, s, n, (const double2*)buf[n & 1],
+                       (double2*)A);
     NSLX_LAUNCH_CHECK();
 }
 

@@ -25,7 +25,8 @@ void expm_real_scaled(int n, const double* K, double a, int sq, double* E, doubl
 void lu_logdet(int n, void* A, double* acc, hipStream_t s);
 
 // In-place inverse of a complex n x n matrix by Gauss-Jordan with partial
-// pivoting.  work: n c128 entries.
+// pivoting, one launch per column.  aug and work: n x 2n c128 entries each
+// (the steps alternate between them).
 void gj_inverse(int n, void* A, void* aug, void* work, hipStream_t s);
 
 // d[x] = i * sum_k Q[x][k] G[k][x]   (diagonal of i*Q*G), complex.

@@ -152,7 +152,7 @@ void fm_grad_logdet(FermionMatrix& fm, int64_t field, void* out, hipStream_t s) 
     // QT holds every QT_t; Gs every G_t (slot t: G after t steps), so that
     // both recurrences are one persistent launch each and the diagonals one
     // batched launch after them
-    DevBuf<double2> QT(m * nt), Gs(m * nt), H(m), aug(2 * m), work((size_t)nx), Pic((size_t)fm.len());
+    DevBuf<double2> QT(m * nt), Gs(m * nt), H(m), aug(2 * m), work(2 * m), Pic((size_t)fm.len());
     conj_c(fm.len(), Pi, Pic.p, s);
     auto Pcprev = [&](int t) { return Pic.p + (size_t)((t - 1 + nt) % nt) * nx; };
     const double* E = en.dense.p;
