@@ -1,6 +1,8 @@
 // Dense FP64 kernels (see dense.hpp).  These serve E = exp(delta K) (once per
 // exponent), log det M and the force; they work on Nx x Nx matrices where
 // the hop kernels work on the (nbatch*Nt) x Nx field block.
+#include <algorithm>
+
 #include "dense.hpp"
 
 #include <cmath>
@@ -118,17 +120,27 @@ __global__ void k_scaled_plus_identity(int n, const double* K, double a, double 
     if (i < (int64_t)n * n) out[i] = a * K[i] + ((i / n == i % n) ? id : 0.0);
 }
 
-// ---- LU with partial pivoting, one column per pair of launches ----
-__global__ void k_lu_pivot(int n, int k, double2* A, double* acc) {
+// ---- LU with partial pivoting, one launch per column ----
+// One LU step of the log determinant, from Ain into Aout (so that no
+// workgroup reads what another writes): every workgroup finds the pivot row p
+// of column k itself (largest modulus among rows >= k, ties to the lowest
+// row, no swap when none is found); workgroup 0 adds log|pivot| and its phase
+// (+ pi for a row swap) to acc; the workgroups write the trailing rows and
+// columns > k: row i (row k where i = p) less l_i times row p, with l_i its
+// column-k element over the pivot.  Rows and columns <= k are not written:
+// no later step reads them, and the determinant needs only the pivots.
+constexpr int LU_PER = 8;   // elements per thread
+__global__ void __launch_bounds__(256) k_lu_step(int n, int k, const double2* __restrict__ Ain,
+                                                 double2* __restrict__ Aout, double* acc) {
     __shared__ double bv[256];
     __shared__ int bi[256];
     const int tid = threadIdx.x;
     double best = -1.0;
     int besti = n;
     for (int i = k + tid; i < n; i += 256) {
-        const double2 v = A[(size_t)i * n + k];
-        const double m = hypot(v.x, v.y);
-        if (m > best || (m == best && i < besti)) { best = m; besti = i; }
+        const double2 v = Ain[(size_t)i * n + k];
+        const double mm = hypot(v.x, v.y);
+        if (mm > best || (mm == best && i < besti)) { best = mm; besti = i; }
     }
     bv[tid] = best;
     bi[tid] = besti;
@@ -141,38 +153,28 @@ __global__ void k_lu_pivot(int n, int k, double2* A, double* acc) {
         }
         __syncthreads();
     }
-    const int p = bi[0];
-    if (p != k && p < n) {
-        for (int j = tid; j < n; j += 256) {
-            const double2 a = A[(size_t)k * n + j];
-            A[(size_t)k * n + j] = A[(size_t)p * n + j];
-            A[(size_t)p * n + j] = a;
-        }
-    }
-    __syncthreads();
-    const double2 piv = A[(size_t)k * n + k];
-    if (tid == 0) {
+    const int p = bi[0] < n ? bi[0] : k;
+    const double2 piv = Ain[(size_t)p * n + k];
+    if (blockIdx.x == 0 && tid == 0) {
         acc[0] += log(hypot(piv.x, piv.y));
         acc[1] += atan2(piv.y, piv.x) + ((p != k) ? M_PI : 0.0);
     }
     const double den = piv.x * piv.x + piv.y * piv.y;
     const double2 inv = make_double2(piv.x / den, -piv.y / den);
-    for (int i = k + 1 + tid; i < n; i += 256) {
-        const double2 v = A[(size_t)i * n + k];
-        A[(size_t)i * n + k] = cm(v, inv);
-    }
-}
-
-__global__ void k_lu_update(int n, int k, double2* A) {
     const int m = n - k - 1;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)m * m) return;
-    const int i = k + 1 + (int)(idx / m), j = k + 1 + (int)(idx % m);
-    const double2 l = A[(size_t)i * n + k], u = A[(size_t)k * n + j];
-    double2 a = A[(size_t)i * n + j];
-    a.x -= l.x * u.x - l.y * u.y;
-    a.y -= l.x * u.y + l.y * u.x;
-    A[(size_t)i * n + j] = a;
+    const int64_t total = (int64_t)m * m;
+    #pragma unroll
+    for (int e = 0; e < LU_PER; ++e) {
+        const int64_t idx = ((int64_t)blockIdx.x * LU_PER + e) * 256 + tid;
+        if (idx >= total) break;
+        const int i = k + 1 + (int)(idx / m), j = k + 1 + (int)(idx % m);
+        const int src = (i == p) ? k : i;
+        const double2 l = cm(Ain[(size_t)src * n + k], inv), u = Ain[(size_t)p * n + j];
+        double2 a = Ain[(size_t)src * n + j];
+        a.x -= l.x * u.x - l.y * u.y;
+        a.y -= l.x * u.y + l.y * u.x;
+        Aout[(size_t)i * n + j] = a;
+    }
 }
 
 // ---- Gauss-Jordan on the augmented n x 2n matrix ----
@@ -432,11 +434,12 @@ void expm_real_scaled(int n, const double* K, double a, int sq, double* E, doubl
     }
 }
 
-void lu_logdet(int n, void* A, double* acc, hipStream_t s) {
+void lu_logdet(int n, void* A, void* work, double* acc, hipStream_t s) {
+    double2* buf[2] = {(double2*)A, (double2*)work};
     for (int k = 0; k < n; ++k) {
-        hipLaunchKernelGGL(k_lu_pivot, dim3(1), dim3(256), 0, s, n, k, (double2*)A, acc);
-        const int64_t mm = (int64_t)(n - k - 1) * (n - k - 1);
-        if (mm > 0) hipLaunchKernelGGL(k_lu_update, dim3(g1(mm)), dim3(256), 0, s, n, k, (double2*)A);
+        const int64_t m = n - k - 1;
+        const unsigned blocks = (unsigned)std::max<int64_t>(1, (m * m + 256 * LU_PER - 1) / (256 * LU_PER));
+        hipLaunchKernelGGL(k_lu_step, dim3(blocks), dim3(256), 0, s, n, k, buf[k & 1], buf[(k + 1) & 1], acc);
     }
     NSLX_LAUNCH_CHECK();
 }

@@ -21,8 +21,9 @@ void expm_real_scaled(int n, const double* K, double a, int sq, double* E, doubl
                       hipStream_t s);
 
 // log det of a complex n x n matrix (destroyed) by LU with partial pivoting:
-// acc[0] += sum log|u_kk|, acc[1] += sum arg(u_kk) + pi * (#swaps).
-void lu_logdet(int n, void* A, double* acc, hipStream_t s);
+// acc[0] += sum log|u_kk|, acc[1] += sum arg(u_kk) + pi * (#swaps).  work: a
+// second n x n c128 buffer (the steps alternate between A and work).
+void lu_logdet(int n, void* A, void* work, double* acc, hipStream_t s);
 
 // In-place inverse of a complex n x n matrix by Gauss-Jordan with partial
 // pivoting, one launch per column.  aug and work: n x 2n c128 entries each

@@ -112,11 +112,11 @@ void fm_logdet(FermionMatrix& fm, int64_t field, int nchunks, double* out2, hipS
                                 hipMemcpyDeviceToDevice, s));
     }
     const int N = nx * nchunks;
-    DevBuf<double2> R((size_t)N * N);
+    DevBuf<double2> R((size_t)N * N), Rw((size_t)N * N);
     build_cyclic(nx, nchunks, C.p, R.p, s);
     DevBuf<double> acc(2);
     NSLX_HIP(hipMemsetAsync(acc.p, 0, 2 * sizeof(double), s));
-    lu_logdet(N, R.p, acc.p, s);
+    lu_logdet(N, R.p, Rw.p, acc.p, s);
     double h[2];
     NSLX_HIP(hipMemcpyAsync(h, acc.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     NSLX_HIP(hipStreamSynchronize(s));
