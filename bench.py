@@ -615,7 +615,7 @@ def main():
         return tss_profile(args.tss, args.precond)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "the path shards by whole requests/fields with no communication, see DESIGN.md",
+              "the path would shard by whole requests/fields with no communication",
               file=sys.stderr)
         sys.exit(2)
 

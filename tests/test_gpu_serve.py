@@ -192,7 +192,7 @@ def test_server_admission_waits_for_contiguous_slots():
 # ---- time-slice-preconditioned serving (precond_workers > 0) ----------------
 
 def test_tss_server_vs_oracle_cfg0():
-    """The preconditioned server on the configs[0] lattice (honeycomb 3x3,
+    """The preconditioned server on the CPU-runnable lattice (honeycomb 3x3,
     Nt = 16, beta = 4, U = 2; 2 time sources; per-RHS 1e-12) against the
     oracle's reference-mode correlator to 1e-9, with 3 worker streams."""
     lat = nsl.Lattice.honeycomb(3, 3)
