@@ -120,16 +120,24 @@ int64_t cg_fused(FermionMatrix& fm, int prec, int inner, int outer, const void* 
     NSLX_HIP(hipMemcpyAsync(w.int_h.p, w.done.p, nslot * sizeof(int), hipMemcpyDeviceToHost, s));
     NSLX_HIP(hipStreamSynchronize(s));
     for (int64_t i = 0; i < nslot; ++i) done_h[i] = w.int_h.p[i];
-    // the stop test of iteration `it` is made by the next CGA; make it here
+    // |r_j|^2 of the iteration j a slot stopped at, from the partials buffer
+    // of j's parity (a stopped slot's later launches write no partials), or
+    // of iteration `it` for a slot still running, whose stop test (made by
+    // the next CGA) is made here
     const size_t np = (size_t)nb * nparts;
-    std::vector<double> rp(np), t2(nslot);
-    NSLX_HIP(hipMemcpyAsync(rp.data(), (it & 1) ? w.rpart1.p : w.rpart0.p, np * sizeof(double),
-                            hipMemcpyDeviceToHost, s));
+    std::vector<double> rp[2] = {std::vector<double>(np), std::vector<double>(np)};
+    std::vector<double> t2(nslot);
+    NSLX_HIP(hipMemcpyAsync(rp[0].data(), w.rpart0.p, np * sizeof(double), hipMemcpyDeviceToHost, s));
+    NSLX_HIP(hipMemcpyAsync(rp[1].data(), w.rpart1.p, np * sizeof(double), hipMemcpyDeviceToHost, s));
     NSLX_HIP(hipMemcpyAsync(t2.data(), w.tol2.p, nslot * sizeof(double), hipMemcpyDeviceToHost, s));
     NSLX_HIP(hipStreamSynchronize(s));
     w.last_rr.assign(nslot, 0.0);
-    for (int64_t b2 = 0; b2 < nb; ++b2)
-        for (int k = 0; k < nparts; ++k) w.last_rr[global ? 0 : b2] += rp[(size_t)b2 * nparts + k];
+    for (int64_t b2 = 0; b2 < nb; ++b2) {
+        const int64_t slot = global ? 0 : b2;
+        const int64_t j = done_h[slot] >= 0 ? done_h[slot] : it;
+        const std::vector<double>& part = rp[j & 1];
+        for (int k = 0; k < nparts; ++k) w.last_rr[slot] += part[(size_t)b2 * nparts + k];
+    }
     for (int64_t i = 0; i < nslot; ++i)
         if (done_h[i] < 0 && w.last_rr[i] <= t2[i]) done_h[i] = (int)it;
     return it;
